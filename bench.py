@@ -1,0 +1,123 @@
+"""Throughput of the 3D Ising parallel-tempering path on one MI355X.
+
+Workload (the reference's config-2 PT run at full size): 3D Ising, L in
+{8,16,32,64,128}, 32 replicas per L on T = linspace(4.40, 4.62, 32), int8
+spins from a random +-1 start, Philox per (seed, replica, sweep, site),
+exchange every 10 sweeps, measure every 5.  One step = one PT sweep of every
+lattice (76,693,504 spin updates), measurement and exchange included.
+
+Prints one JSON line (the driver's contract) with a `roofline` object for the
+dominant kernel (ising_half_kernel) and a `cpu_baseline` from the oracle's
+restatement of the reference's random-site Metropolis (metropolis.rs:8-27)
+timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ising-rs_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+SIZES = [8, 16, 32, 64, 128]
+R = 32
+
+
+def linspace(a, b, n):
+    return [a + (b - a) * i / (n - 1) for i in range(n)]
+
+
+def cpu_baseline(seconds):
+    import ctypes as C
+
+    import oracle_lib as o
+    wall = C.c_double()
+    L = 128
+    upd = o.lib.o_bench_ref_metropolis(L, 1.0 / 4.5115, 42, seconds, C.byref(wall))
+    return {"value": upd / (wall.value * 1e9), "unit": "updates/ns", "cores": 1, "kind": "port",
+            "sample": "oracle restatement of metropolis.rs:8-27 (random-site Metropolis), one L=128 replica at "
+                      "T=4.5115, %d updates in %.1f s on 1 core" % (upd, wall.value)}
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "ising_half_kernel_pt_L8-128_R32_traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        return json.load(open(p)).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--seed", type=int, default=42)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import ising_hip
+    if ising_hip.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    temps = linspace(4.40, 4.62, R)
+    me, ee = 5, 10
+    pt = ising_hip.PTEnsemble("ising", SIZES, R, temps, args.seed, exchange_every=ee, measure_every=me,
+                              record_capacity=args.steps // me + 8)
+    pt.warmup(args.warmup)
+    pt.sync()
+
+    t0 = time.perf_counter()
+    pt.sample(args.steps)
+    ev_ms = pt.sync()
+    wall = time.perf_counter() - t0
+
+    tidx, bond, mag = pt.read_ising()
+    assert tidx.shape[0] == args.steps // me
+
+    upd_per_sweep = sum(L ** 3 for L in SIZES) * R
+    updates = upd_per_sweep * args.steps
+    value = updates / (wall * 1e9)
+    launches = 2 * args.steps
+    avg_launch_s = ev_ms * 1e-3 / launches
+    alg_bytes_per_launch = 3 * upd_per_sweep // 2  # 3 B per update (read c, write c, read other colour)
+    achieved = alg_bytes_per_launch / avg_launch_s / 1e9
+    line = {
+        "metric": "spin updates/ns",
+        "value": round(value, 3),
+        "unit": "updates/ns",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "3D Ising PT (config 2): L=8,16,32,64,128 x 32 replicas, T=linspace(4.40,4.62,32), "
+                               "exchange every 10, measure every 5, int8 spins, Philox4x32-10",
+                   "sizes": SIZES, "replicas": R, "updates_per_step": upd_per_sweep},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(),
+                     "kernel": "ising_half_kernel", "alg_bytes_per_launch": alg_bytes_per_launch,
+                     "avg_launch_us": round(avg_launch_s * 1e6, 3),
+                     "device_updates_per_ns": round(updates / (ev_ms * 1e6), 3)},
+    }
+    if args.cpu_seconds > 0:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,176 @@
+/*
+ * libising_hip -- MI355X (gfx950) parallel-tempering engine for 3D Ising, XY
+ * and Heisenberg spins.  C ABI: plain pointers and sizes, no C++ types.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative ISING_E_* code on
+ *    failure; ising_last_error() then gives a message (thread-local).
+ *  - No C++ exception crosses the ABI.
+ *  - The library owns device memory; callers own every host buffer.
+ *  - One HIP stream per handle.  A handle is not thread-safe; distinct
+ *    handles are.
+ *  - Spin arrays exchanged with the caller use the reference's layout:
+ *    z-major, index (z*L + y)*L + x; Ising as int8 +-1, continuous spins
+ *    component-interleaved float (x, y[, z]) per site.
+ *
+ * Reference interfaces replaced (faulknco/ising-rs):
+ *  - LatticeGpu                     src/cuda/lattice_gpu.rs:9-306
+ *  - ContinuousGpuLattice           src/cuda/gpu_lattice_continuous.rs:9-250
+ *  - reduce_gpu::*                  src/cuda/reduce_gpu.rs:8-178
+ *  - parallel_tempering::*          src/parallel_tempering.rs:5-46
+ *  - the PT loop of gpu_fss         src/bin/gpu_fss.rs:166-582 (ensemble API)
+ *  - jackknife_observables          src/bin/gpu_fss.rs:327-412
+ */
+#ifndef ISING_HIP_H
+#define ISING_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ISING_OK 0
+#define ISING_E_ARG (-1)     /* invalid argument */
+#define ISING_E_HIP (-2)     /* HIP runtime error (device, allocation, launch) */
+#define ISING_E_STATE (-3)   /* call not valid in the handle's current state */
+#define ISING_E_FULL (-4)    /* measurement buffer full: drain records first */
+
+#define ISING_MODEL_ISING 0
+#define ISING_MODEL_XY 1
+#define ISING_MODEL_HEISENBERG 2
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* ising_last_error(void);
+/* Library version string. */
+const char* ising_version(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int ising_device_count(void);
+
+/* ------------------------------------------------------------------------
+ * Single Ising lattice: mirror of LatticeGpu (src/cuda/lattice_gpu.rs).
+ * ------------------------------------------------------------------------ */
+typedef struct ising_lattice ising_lattice_t;
+
+/* LatticeGpu::new(n, seed) (lattice_gpu.rs:22).  n even, 2 <= n <= 2046.
+ * Hot start from the library's counter-based stream (the reference draws an
+ * unseeded host start, lattice_gpu.rs:38-40). */
+int ising_lattice_create(ising_lattice_t** out, int n, uint64_t seed);
+void ising_lattice_destroy(ising_lattice_t* lat);
+int ising_lattice_size(const ising_lattice_t* lat);
+/* LatticeGpu::step(beta, j, h) (lattice_gpu.rs:85): one sweep, colour 0 then 1. Async. */
+int ising_lattice_step(ising_lattice_t* lat, float beta, float j, float h);
+/* LatticeGpu::warm_up (lattice_gpu.rs:184). */
+int ising_lattice_warm_up(ising_lattice_t* lat, float beta, float j, float h, size_t sweeps);
+/* LatticeGpu::get_spins (lattice_gpu.rs:110): out holds n^3 int8 (+-1). */
+int ising_lattice_get_spins(ising_lattice_t* lat, int8_t* out);
+/* Not in the reference: upload a configuration (n^3 int8 +-1). */
+int ising_lattice_set_spins(ising_lattice_t* lat, const int8_t* in);
+/* LatticeGpu::measure_gpu(j) (lattice_gpu.rs:116): E per spin (forward bonds,
+ * h excluded as in reduce_energy_ising) and |M| per spin. */
+int ising_lattice_measure_gpu(ising_lattice_t* lat, float j, double* e_per_spin, double* m_abs_per_spin);
+/* LatticeGpu::energy_gpu(j) (lattice_gpu.rs:159): total energy. */
+int ising_lattice_energy_gpu(ising_lattice_t* lat, float j, double* e_total);
+/* Exact integer observables: forward-bond sum B (E = -J B) and sum of spins. */
+int ising_lattice_measure_raw(ising_lattice_t* lat, int64_t* bond, int64_t* mag);
+
+typedef struct {
+    double temperature;
+    double energy;          /* <E> per spin */
+    double magnetisation;   /* <|M|> per spin */
+    double heat_capacity;
+    double susceptibility;  /* from the signed magnetisation, lattice_gpu.rs:240 */
+    double m2;
+    double m4;
+} ising_observables_t;
+
+/* LatticeGpu::measure(T, j, h, samples) (lattice_gpu.rs:193-251); E includes -h M
+ * as energy_magnetisation_host does (lattice_gpu.rs:285-306). */
+int ising_lattice_measure(ising_lattice_t* lat, double temperature, double j, double h, size_t samples,
+                          ising_observables_t* out);
+/* LatticeGpu::domain_wall_density (lattice_gpu.rs:254-280). */
+int ising_lattice_domain_wall_density(ising_lattice_t* lat, double* out);
+/* Global sweep counter (number of sweeps applied so far). */
+int ising_lattice_sweeps(const ising_lattice_t* lat, int64_t* out);
+
+/* ------------------------------------------------------------------------
+ * Parallel-tempering ensemble: every lattice size and replica of a run in one
+ * set of launches; exchange, measurement and the sweep counter on the device.
+ * Replaces the host loop of gpu_fss (gpu_fss.rs:186-286 / 441-545).
+ * ------------------------------------------------------------------------ */
+typedef struct ising_pt ising_pt_t;
+
+typedef struct {
+    double j;                /* coupling J (gpu_fss uses 1.0) */
+    double h;                /* field, Ising only (gpu_fss uses 0.0) */
+    int32_t exchange_every;  /* --exchange-every, >= 1 */
+    int32_t measure_every;   /* --measure-every, >= 1 */
+    double delta;            /* --delta: continuous proposal size (radians) */
+    int32_t n_overrelax;     /* --overrelax: OR passes per sweep (continuous) */
+    int32_t renorm_every;    /* continuous: renormalise every k sweeps (0 = never) */
+    int32_t record_capacity; /* measured sweeps buffered on the device before a drain */
+    int32_t reserved;
+} ising_pt_opts_t;
+
+/* Defaults: j=1, h=0, exchange_every=10, measure_every=1, delta=0.5,
+ * n_overrelax=5, renorm_every=100, record_capacity=4096. */
+void ising_pt_default_opts(ising_pt_opts_t* opts);
+
+/* model: ISING_MODEL_*.  sizes: n_sizes even L values.  temps: `replicas`
+ * temperatures shared by every size (replica r starts at temperature r).
+ * Replica r of size L uses stream (L << 16) | r under key `seed`. */
+int ising_pt_create(ising_pt_t** out, int model, const int32_t* sizes, int32_t n_sizes, int32_t replicas,
+                    const double* temps, uint64_t seed, const ising_pt_opts_t* opts);
+void ising_pt_destroy(ising_pt_t* pt);
+/* Warm-up sweeps: no exchange, no measurement (gpu_fss.rs:199-210). */
+int ising_pt_warmup(ising_pt_t* pt, int64_t sweeps);
+/* Sample sweeps: measure when (s+1) % measure_every == 0, exchange when
+ * (s+1) % exchange_every == 0 at round s / exchange_every (gpu_fss.rs:235-280).
+ * Fails with ISING_E_FULL if the records would not fit. */
+int ising_pt_sample(ising_pt_t* pt, int64_t sweeps);
+/* Wait for queued work; return the device time of the last warmup/sample call (ms, HIP events). */
+int ising_pt_sync(ising_pt_t* pt, double* last_ms);
+/* Number of measured sweeps waiting in the device buffer. */
+int ising_pt_pending(ising_pt_t* pt, int64_t* n_measured_sweeps);
+/* Drain Ising records: for each measured sweep, one record per replica in
+ * (size, replica) order: temperature index, bond sum B, magnetisation sum M.
+ * Arrays hold max_sweeps * total_replicas entries; *n_out = sweeps drained. */
+int ising_pt_read_ising(ising_pt_t* pt, int64_t max_sweeps, int32_t* tidx, int64_t* bond, int64_t* mag,
+                        int64_t* n_out);
+/* Drain continuous records: E total (FP64), Mx, My, Mz totals, temperature index. */
+int ising_pt_read_cont(ising_pt_t* pt, int64_t max_sweeps, int32_t* tidx, double* e, double* mx, double* my,
+                       double* mz, int64_t* n_out);
+/* Permutations of one size: r2t[replica] -> temperature index, t2r[temp] -> replica. */
+int ising_pt_get_perm(ising_pt_t* pt, int32_t size_index, int32_t* r2t, int32_t* t2r);
+/* Exchange statistics since creation. */
+int ising_pt_exchange_stats(ising_pt_t* pt, int64_t* tries, int64_t* accepts);
+/* Configuration of one replica: Ising int8 +-1 (L^3) or float (L^3 * ncomp). */
+int ising_pt_get_spins_i8(ising_pt_t* pt, int32_t size_index, int32_t replica, int8_t* out);
+int ising_pt_set_spins_i8(ising_pt_t* pt, int32_t size_index, int32_t replica, const int8_t* in);
+int ising_pt_get_spins_f32(ising_pt_t* pt, int32_t size_index, int32_t replica, float* out);
+int ising_pt_set_spins_f32(ising_pt_t* pt, int32_t size_index, int32_t replica, const float* in);
+/* Sweeps applied so far (warm-up + sample). */
+int ising_pt_sweeps(const ising_pt_t* pt, int64_t* out);
+/* Bytes of spin storage on the device. */
+int ising_pt_device_bytes(const ising_pt_t* pt, int64_t* out);
+
+/* ------------------------------------------------------------------------
+ * Host-side statistics and PT helpers (no GPU needed).
+ * ------------------------------------------------------------------------ */
+/* linspace_temperatures (parallel_tempering.rs:39-46). */
+int ising_linspace_temperatures(double t_min, double t_max, int32_t n, double* out);
+/* replica_exchange (parallel_tempering.rs:5-36) on host arrays; one caller-
+ * supplied uniform per attempted pair (uniforms[k] for the k-th pair).
+ * Returns accepted swaps in *accepted. */
+int ising_replica_exchange(const double* temps, const double* energies, int32_t n, int32_t* r2t, int32_t* t2r,
+                           const double* uniforms, int32_t even_odd, int32_t* accepted);
+/* jackknife_observables (gpu_fss.rs:327-412): out[12] = E, E_err, M, M_err, M2,
+ * M2_err, M4, M4_err, Cv, Cv_err, chi, chi_err. */
+int ising_jackknife(const double* e, const double* m, int64_t n, double beta, double n3, int32_t n_blocks,
+                    double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

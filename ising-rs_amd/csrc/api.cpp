@@ -1,0 +1,716 @@
+// Host side of libising_hip.so: the C ABI declared in include/ising_hip.h.
+//
+// The ensemble object owns every lattice of a parallel-tempering run on one
+// device: spin pool, per-temperature acceptance tables, temperature
+// permutations, measurement records and the sweep control block.  A sweep is
+// two launches (colour 0, colour 1); measurement, replica exchange and the
+// sweep counter live on the device, so the host only enqueues launches and
+// drains records.  Host code never computes spins or observables: there is no
+// CPU fallback, and every entry point that needs a device fails loudly without one.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/ising_hip.h"
+#include "kernels.h"
+
+using namespace ising;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(ISING_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+// 33-bit threshold T = floor(p * 2^32) (T = 2^32 when p >= 1), split in halves.
+void threshold_pair(double p, uint32_t* hi, uint32_t* lo) {
+    uint64_t t;
+    if (!(p < 1.0)) t = 1ull << 32;
+    else if (p <= 0.0) t = 0;
+    else t = (uint64_t)std::ldexp(p, 32);
+    *hi = (uint32_t)(t >> 16);
+    *lo = (uint32_t)(t & 0xffffu);
+}
+
+// Acceptance table for k = b*7 + n_up: dE = 2 s (J (2 n_up - 6) + h).
+void ising_table(double beta, double J, double h, uint32_t out[28]) {
+    for (int b = 0; b <= 1; ++b) {
+        const double s = b ? 1.0 : -1.0;
+        for (int n = 0; n <= 6; ++n) {
+            const double dE = 2.0 * s * (J * (double)(2 * n - 6) + h);
+            const double p = dE <= 0.0 ? 1.0 : std::exp(-beta * dE);
+            threshold_pair(p, &out[b * 7 + n], &out[14 + b * 7 + n]);
+        }
+    }
+}
+
+bool valid_L(int L) { return L >= 2 && L <= 2046 && (L % 2) == 0; }
+
+}  // namespace
+
+struct ising_pt {
+    int model = ISING_MODEL_ISING;
+    int ncomp = 0;
+    int R = 0;
+    int total_reps = 0;
+    int nblocks = 0;
+    uint64_t seed = 0;
+    ising_pt_opts_t opts{};
+    std::vector<DevGroup> groups;
+    std::vector<double> temps;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+    DevGroup* d_groups = nullptr;
+    uint8_t* d_pool = nullptr;
+    int64_t pool_bytes = 0;
+    uint32_t* d_thr = nullptr;
+    int32_t* d_r2t = nullptr;
+    int32_t* d_t2r = nullptr;
+    double* d_betas = nullptr;
+    DevCtl* d_ctl = nullptr;
+    int64_t* d_acc = nullptr;
+    void* d_rec = nullptr;
+    int64_t t_host = 0;        // sweeps enqueued
+    int64_t s_host = -1;       // sample sweeps enqueued (-1 before sampling starts)
+    int64_t pending_host = 0;  // measured sweeps the device will have written
+    bool inline_thr = false;   // single-lattice mode
+    uint32_t thr_inline[28];
+
+    ~ising_pt() {
+        if (st) (void)hipStreamSynchronize(st);
+        (void)hipFree(d_groups);
+        (void)hipFree(d_pool);
+        (void)hipFree(d_thr);
+        (void)hipFree(d_r2t);
+        (void)hipFree(d_t2r);
+        (void)hipFree(d_betas);
+        (void)hipFree(d_ctl);
+        (void)hipFree(d_acc);
+        (void)hipFree(d_rec);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    IsingArgs ising_args() const {
+        IsingArgs a{};
+        a.groups = d_groups;
+        a.ngroups = (int32_t)groups.size();
+        a.total_reps = total_reps;
+        a.pool = d_pool;
+        a.thr = d_thr;
+        a.r2t = d_r2t;
+        a.t2r = d_t2r;
+        a.betas = d_betas;
+        a.ctl = d_ctl;
+        a.acc = d_acc;
+        a.rec = (IsingRec*)d_rec;
+        a.k0 = (uint32_t)seed;
+        a.k1 = (uint32_t)(seed >> 32);
+        a.J = opts.j;
+        a.inline_thr = inline_thr ? 1 : 0;
+        if (inline_thr) std::memcpy(a.thr_inline, thr_inline, sizeof(thr_inline));
+        return a;
+    }
+
+    int enqueue_sweeps(int64_t n) {
+        if (model == ISING_MODEL_ISING) {
+            const IsingArgs a = ising_args();
+            for (int64_t i = 0; i < n; ++i) {
+                HIP_TRY(launch_ising_half(a, nblocks, 0, st));
+                HIP_TRY(launch_ising_half(a, nblocks, 1, st));
+            }
+        } else {
+            return fail(ISING_E_ARG, "continuous models are not available in this build");
+        }
+        t_host += n;
+        return ISING_OK;
+    }
+
+    int read_ctl(DevCtl* out) {
+        HIP_TRY(hipStreamSynchronize(st));
+        HIP_TRY(hipMemcpy(out, d_ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
+        return ISING_OK;
+    }
+};
+
+struct ising_lattice {
+    ising_pt* pt = nullptr;
+    int L = 0;
+    float beta = NAN, j = NAN, h = NAN;
+    ~ising_lattice() { delete pt; }
+};
+
+namespace {
+
+int64_t measured_in(int64_t s0, int64_t n, int every) {
+    // count s in [s0, s0+n) with (s+1) % every == 0
+    return (s0 + n) / every - s0 / every;
+}
+
+int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, int R, const double* temps,
+                   uint64_t seed, const ising_pt_opts_t* opts) {
+    if (model != ISING_MODEL_ISING && model != ISING_MODEL_XY && model != ISING_MODEL_HEISENBERG)
+        return fail(ISING_E_ARG, "model must be ISING_MODEL_ISING, _XY or _HEISENBERG");
+    if (n_sizes < 1 || n_sizes > 64 || !sizes) return fail(ISING_E_ARG, "need 1..64 lattice sizes");
+    if (R < 1 || R > 65535 || !temps) return fail(ISING_E_ARG, "need 1..65535 replicas and their temperatures");
+    for (int i = 0; i < n_sizes; ++i)
+        if (!valid_L(sizes[i])) return fail(ISING_E_ARG, "lattice sizes must be even, 2..2046");
+    for (int i = 0; i < R; ++i)
+        if (!(temps[i] > 0.0) || !std::isfinite(temps[i])) return fail(ISING_E_ARG, "temperatures must be positive");
+    ising_pt_opts_t o;
+    if (opts) o = *opts;
+    else ising_pt_default_opts(&o);
+    if (o.exchange_every < 1 || o.measure_every < 1) return fail(ISING_E_ARG, "exchange_every and measure_every must be >= 1");
+    if (o.record_capacity < 1) return fail(ISING_E_ARG, "record_capacity must be >= 1");
+    if (model != ISING_MODEL_ISING) return fail(ISING_E_ARG, "continuous models are not available in this build");
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(ISING_E_HIP, "no HIP device visible");
+
+    p->model = model;
+    p->ncomp = model == ISING_MODEL_ISING ? 0 : (model == ISING_MODEL_XY ? 2 : 3);
+    p->R = R;
+    p->seed = seed;
+    p->opts = o;
+    p->temps.assign(temps, temps + R);
+    p->total_reps = n_sizes * R;
+
+    const int64_t elem = 1;  // bytes per site (Ising)
+    int64_t off = 0;
+    int blk = 0;
+    for (int i = 0; i < n_sizes; ++i) {
+        DevGroup G{};
+        G.L = sizes[i];
+        G.Lh = sizes[i] / 2;
+        G.R = R;
+        G.fast = (sizes[i] % 16) == 0 ? 1 : 0;
+        G.rep0 = i * R;
+        G.n_half = (int64_t)sizes[i] * sizes[i] * sizes[i] / 2;
+        G.stride = ((G.n_half * elem + 255) / 256) * 256 / elem;
+        G.off = off;
+        G.tiles = (int32_t)((G.n_half + kIsingTile - 1) / kIsingTile);
+        G.blk0 = blk;
+        blk += G.tiles * R;
+        off += 2 * (int64_t)R * G.stride;
+        p->groups.push_back(G);
+    }
+    p->nblocks = blk;
+    p->pool_bytes = off * elem;
+
+    HIP_TRY(hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&p->ev0));
+    HIP_TRY(hipEventCreate(&p->ev1));
+    HIP_TRY(hipMalloc(&p->d_groups, sizeof(DevGroup) * p->groups.size()));
+    HIP_TRY(hipMalloc(&p->d_pool, p->pool_bytes));
+    HIP_TRY(hipMalloc(&p->d_thr, sizeof(uint32_t) * 28 * p->total_reps));
+    HIP_TRY(hipMalloc(&p->d_r2t, sizeof(int32_t) * p->total_reps));
+    HIP_TRY(hipMalloc(&p->d_t2r, sizeof(int32_t) * p->total_reps));
+    HIP_TRY(hipMalloc(&p->d_betas, sizeof(double) * p->total_reps));
+    HIP_TRY(hipMalloc(&p->d_ctl, sizeof(DevCtl)));
+    HIP_TRY(hipMalloc(&p->d_acc, sizeof(int64_t) * 8 * p->total_reps));
+    HIP_TRY(hipMalloc(&p->d_rec, sizeof(IsingRec) * (size_t)o.record_capacity * p->total_reps));
+
+    std::vector<uint32_t> thr(28 * (size_t)p->total_reps);
+    std::vector<int32_t> ident(p->total_reps);
+    std::vector<double> betas(p->total_reps);
+    for (int g = 0; g < n_sizes; ++g)
+        for (int i = 0; i < R; ++i) {
+            const double beta = 1.0 / temps[i];
+            betas[g * R + i] = beta;
+            ident[g * R + i] = i;
+            ising_table(beta, o.j, o.h, &thr[28 * (size_t)(g * R + i)]);
+        }
+    DevCtl ctl{};
+    ctl.t = 0;
+    ctl.sample_start = INT64_MAX;
+    ctl.measure_every = o.measure_every;
+    ctl.exchange_every = o.exchange_every;
+    ctl.rec_cap = o.record_capacity;
+    HIP_TRY(hipMemcpy(p->d_groups, p->groups.data(), sizeof(DevGroup) * p->groups.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_thr, thr.data(), sizeof(uint32_t) * thr.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_r2t, ident.data(), sizeof(int32_t) * ident.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_t2r, ident.data(), sizeof(int32_t) * ident.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_betas, betas.data(), sizeof(double) * betas.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_ctl, &ctl, sizeof(DevCtl), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(p->d_acc, 0, sizeof(int64_t) * 8 * p->total_reps));
+    HIP_TRY(hipMemset(p->d_pool, 0, p->pool_bytes));
+    HIP_TRY(launch_ising_init(p->d_groups, (int)p->groups.size(), p->d_pool, p->nblocks, (uint32_t)seed,
+                              (uint32_t)(seed >> 32), p->st));
+    HIP_TRY(hipStreamSynchronize(p->st));
+    return ISING_OK;
+}
+
+const DevGroup* group_of(ising_pt* p, int size_index) {
+    if (size_index < 0 || size_index >= (int)p->groups.size()) return nullptr;
+    return &p->groups[size_index];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ising_last_error(void) { return g_err.c_str(); }
+const char* ising_version(void) { return "libising_hip 0.1 (gfx950)"; }
+int ising_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void ising_pt_default_opts(ising_pt_opts_t* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->j = 1.0;
+    o->h = 0.0;
+    o->exchange_every = 10;
+    o->measure_every = 1;
+    o->delta = 0.5;
+    o->n_overrelax = 5;
+    o->renorm_every = 100;
+    o->record_capacity = 4096;
+}
+
+int ising_pt_create(ising_pt_t** out, int model, const int32_t* sizes, int32_t n_sizes, int32_t replicas,
+                    const double* temps, uint64_t seed, const ising_pt_opts_t* opts) {
+    if (!out) return fail(ISING_E_ARG, "out is null");
+    *out = nullptr;
+    ising_pt* p = new (std::nothrow) ising_pt();
+    if (!p) return fail(ISING_E_HIP, "host allocation failed");
+    const int rc = build_ensemble(p, model, sizes, n_sizes, replicas, temps, seed, opts);
+    if (rc != ISING_OK) {
+        const std::string keep = g_err;
+        delete p;
+        g_err = keep;
+        return rc;
+    }
+    *out = p;
+    return ISING_OK;
+}
+
+void ising_pt_destroy(ising_pt_t* pt) { delete pt; }
+
+int ising_pt_warmup(ising_pt_t* p, int64_t sweeps) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    if (sweeps < 0) return fail(ISING_E_ARG, "sweeps must be >= 0");
+    if (p->s_host >= 0) return fail(ISING_E_STATE, "warm-up after sampling started");
+    HIP_TRY(hipEventRecord(p->ev0, p->st));
+    const int rc = p->enqueue_sweeps(sweeps);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(p->ev1, p->st));
+    p->timed = true;
+    return ISING_OK;
+}
+
+int ising_pt_sample(ising_pt_t* p, int64_t sweeps) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    if (sweeps < 0) return fail(ISING_E_ARG, "sweeps must be >= 0");
+    if (p->s_host < 0) {
+        // sampling starts at the next sweep; the device reads sample_start from its control block
+        const int64_t start = p->t_host;
+        HIP_TRY(hipStreamSynchronize(p->st));
+        HIP_TRY(hipMemcpy(&p->d_ctl->sample_start, &start, sizeof(int64_t), hipMemcpyHostToDevice));
+        p->s_host = 0;
+    }
+    const int64_t add = measured_in(p->s_host, sweeps, p->opts.measure_every);
+    if (p->pending_host + add > p->opts.record_capacity)
+        return fail(ISING_E_FULL, "measurement buffer would overflow: drain records before sampling more");
+    HIP_TRY(hipEventRecord(p->ev0, p->st));
+    const int rc = p->enqueue_sweeps(sweeps);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(p->ev1, p->st));
+    p->timed = true;
+    p->s_host += sweeps;
+    p->pending_host += add;
+    return ISING_OK;
+}
+
+int ising_pt_sync(ising_pt_t* p, double* last_ms) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    HIP_TRY(hipStreamSynchronize(p->st));
+    if (last_ms) {
+        float ms = 0.f;
+        if (p->timed) HIP_TRY(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+        *last_ms = ms;
+    }
+    return ISING_OK;
+}
+
+int ising_pt_pending(ising_pt_t* p, int64_t* n) {
+    if (!p || !n) return fail(ISING_E_ARG, "null argument");
+    DevCtl c;
+    const int rc = p->read_ctl(&c);
+    if (rc) return rc;
+    if (c.overflow) return fail(ISING_E_FULL, "measurement records were dropped (device buffer overflow)");
+    *n = c.n_rec;
+    return ISING_OK;
+}
+
+static int drain(ising_pt* p, int64_t max_sweeps, size_t rec_size, void* host, int64_t* n_out) {
+    DevCtl c;
+    int rc = p->read_ctl(&c);
+    if (rc) return rc;
+    if (c.overflow) return fail(ISING_E_FULL, "measurement records were dropped (device buffer overflow)");
+    const int64_t n = std::min<int64_t>(c.n_rec, std::max<int64_t>(max_sweeps, 0));
+    const size_t per = rec_size * (size_t)p->total_reps;
+    if (n > 0) HIP_TRY(hipMemcpy(host, p->d_rec, per * n, hipMemcpyDeviceToHost));
+    const int64_t rest = c.n_rec - n;
+    if (rest > 0 && n > 0)
+        HIP_TRY(hipMemcpy(p->d_rec, (char*)p->d_rec + per * n, per * rest, hipMemcpyDeviceToDevice));
+    const int32_t nr = (int32_t)rest;
+    HIP_TRY(hipMemcpy(&p->d_ctl->n_rec, &nr, sizeof(int32_t), hipMemcpyHostToDevice));
+    p->pending_host -= n;
+    *n_out = n;
+    return ISING_OK;
+}
+
+int ising_pt_read_ising(ising_pt_t* p, int64_t max_sweeps, int32_t* tidx, int64_t* bond, int64_t* mag,
+                        int64_t* n_out) {
+    if (!p || !n_out) return fail(ISING_E_ARG, "null argument");
+    if (p->model != ISING_MODEL_ISING) return fail(ISING_E_STATE, "not an Ising ensemble");
+    std::vector<IsingRec> buf((size_t)std::max<int64_t>(0, max_sweeps) * p->total_reps);
+    int64_t n = 0;
+    const int rc = drain(p, max_sweeps, sizeof(IsingRec), buf.data(), &n);
+    if (rc) return rc;
+    for (int64_t i = 0; i < n * p->total_reps; ++i) {
+        if (tidx) tidx[i] = buf[i].tidx;
+        if (bond) bond[i] = buf[i].bond;
+        if (mag) mag[i] = buf[i].mag;
+    }
+    *n_out = n;
+    return ISING_OK;
+}
+
+int ising_pt_read_cont(ising_pt_t* p, int64_t, int32_t*, double*, double*, double*, double*, int64_t*) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    return fail(ISING_E_STATE, "not a continuous-spin ensemble");
+}
+
+int ising_pt_get_perm(ising_pt_t* p, int32_t size_index, int32_t* r2t, int32_t* t2r) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    const DevGroup* G = group_of(p, size_index);
+    if (!G) return fail(ISING_E_ARG, "size_index out of range");
+    HIP_TRY(hipStreamSynchronize(p->st));
+    if (r2t) HIP_TRY(hipMemcpy(r2t, p->d_r2t + G->rep0, sizeof(int32_t) * G->R, hipMemcpyDeviceToHost));
+    if (t2r) HIP_TRY(hipMemcpy(t2r, p->d_t2r + G->rep0, sizeof(int32_t) * G->R, hipMemcpyDeviceToHost));
+    return ISING_OK;
+}
+
+int ising_pt_exchange_stats(ising_pt_t* p, int64_t* tries, int64_t* accepts) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    DevCtl c;
+    const int rc = p->read_ctl(&c);
+    if (rc) return rc;
+    if (tries) *tries = c.exch_tries;
+    if (accepts) *accepts = c.exch_accepts;
+    return ISING_OK;
+}
+
+int ising_pt_get_spins_i8(ising_pt_t* p, int32_t size_index, int32_t replica, int8_t* out) {
+    if (!p || !out) return fail(ISING_E_ARG, "null argument");
+    if (p->model != ISING_MODEL_ISING) return fail(ISING_E_STATE, "not an Ising ensemble");
+    const DevGroup* G = group_of(p, size_index);
+    if (!G || replica < 0 || replica >= G->R) return fail(ISING_E_ARG, "size_index or replica out of range");
+    HIP_TRY(hipStreamSynchronize(p->st));
+    std::vector<uint8_t> half(2 * (size_t)G->n_half);
+    for (int c = 0; c < 2; ++c)
+        HIP_TRY(hipMemcpy(half.data() + c * G->n_half, p->d_pool + G->off + (2 * (int64_t)replica + c) * G->stride,
+                          G->n_half, hipMemcpyDeviceToHost));
+    const int L = G->L, Lh = G->Lh;
+    for (int z = 0; z < L; ++z)
+        for (int y = 0; y < L; ++y)
+            for (int x = 0; x < L; ++x) {
+                const int c = (x + y + z) & 1;
+                const int64_t h = ((int64_t)z * L + y) * Lh + (x >> 1);
+                out[((int64_t)z * L + y) * L + x] = half[c * G->n_half + h] ? 1 : -1;
+            }
+    return ISING_OK;
+}
+
+int ising_pt_set_spins_i8(ising_pt_t* p, int32_t size_index, int32_t replica, const int8_t* in) {
+    if (!p || !in) return fail(ISING_E_ARG, "null argument");
+    if (p->model != ISING_MODEL_ISING) return fail(ISING_E_STATE, "not an Ising ensemble");
+    const DevGroup* G = group_of(p, size_index);
+    if (!G || replica < 0 || replica >= G->R) return fail(ISING_E_ARG, "size_index or replica out of range");
+    const int L = G->L, Lh = G->Lh;
+    std::vector<uint8_t> half(2 * (size_t)G->n_half);
+    for (int z = 0; z < L; ++z)
+        for (int y = 0; y < L; ++y)
+            for (int x = 0; x < L; ++x) {
+                const int8_t v = in[((int64_t)z * L + y) * L + x];
+                if (v != 1 && v != -1) return fail(ISING_E_ARG, "Ising spins must be +1 or -1");
+                const int c = (x + y + z) & 1;
+                half[c * G->n_half + ((int64_t)z * L + y) * Lh + (x >> 1)] = v > 0 ? 1 : 0;
+            }
+    HIP_TRY(hipStreamSynchronize(p->st));
+    for (int c = 0; c < 2; ++c)
+        HIP_TRY(hipMemcpy(p->d_pool + G->off + (2 * (int64_t)replica + c) * G->stride, half.data() + c * G->n_half,
+                          G->n_half, hipMemcpyHostToDevice));
+    return ISING_OK;
+}
+
+int ising_pt_get_spins_f32(ising_pt_t* p, int32_t, int32_t, float*) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    return fail(ISING_E_STATE, "not a continuous-spin ensemble");
+}
+int ising_pt_set_spins_f32(ising_pt_t* p, int32_t, int32_t, const float*) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    return fail(ISING_E_STATE, "not a continuous-spin ensemble");
+}
+
+int ising_pt_sweeps(const ising_pt_t* p, int64_t* out) {
+    if (!p || !out) return fail(ISING_E_ARG, "null argument");
+    *out = p->t_host;
+    return ISING_OK;
+}
+
+int ising_pt_device_bytes(const ising_pt_t* p, int64_t* out) {
+    if (!p || !out) return fail(ISING_E_ARG, "null argument");
+    *out = p->pool_bytes;
+    return ISING_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Single lattice (LatticeGpu mirror)
+// ---------------------------------------------------------------------------
+
+int ising_lattice_create(ising_lattice_t** out, int n, uint64_t seed) {
+    if (!out) return fail(ISING_E_ARG, "out is null");
+    *out = nullptr;
+    if (!valid_L(n)) return fail(ISING_E_ARG, "n must be even, 2..2046");
+    ising_lattice* l = new (std::nothrow) ising_lattice();
+    if (!l) return fail(ISING_E_HIP, "host allocation failed");
+    l->L = n;
+    const int32_t sizes[1] = {n};
+    const double temps[1] = {1.0};
+    ising_pt_opts_t o;
+    ising_pt_default_opts(&o);
+    o.record_capacity = 1;
+    const int rc = ising_pt_create(&l->pt, ISING_MODEL_ISING, sizes, 1, 1, temps, seed, &o);
+    if (rc) {
+        const std::string keep = g_err;
+        delete l;
+        g_err = keep;
+        return rc;
+    }
+    l->pt->inline_thr = true;
+    *out = l;
+    return ISING_OK;
+}
+
+void ising_lattice_destroy(ising_lattice_t* lat) { delete lat; }
+int ising_lattice_size(const ising_lattice_t* lat) { return lat ? lat->L : 0; }
+
+int ising_lattice_step(ising_lattice_t* l, float beta, float j, float h) {
+    if (!l) return fail(ISING_E_ARG, "null handle");
+    if (!(beta >= 0.0f) || !std::isfinite(beta) || !std::isfinite(j) || !std::isfinite(h))
+        return fail(ISING_E_ARG, "beta must be finite and >= 0; j, h finite");
+    if (!(beta == l->beta && j == l->j && h == l->h)) {
+        ising_table((double)beta, (double)j, (double)h, l->pt->thr_inline);
+        l->beta = beta;
+        l->j = j;
+        l->h = h;
+    }
+    return l->pt->enqueue_sweeps(1);
+}
+
+int ising_lattice_warm_up(ising_lattice_t* l, float beta, float j, float h, size_t sweeps) {
+    for (size_t i = 0; i < sweeps; ++i) {
+        const int rc = ising_lattice_step(l, beta, j, h);
+        if (rc) return rc;
+    }
+    return l ? ISING_OK : fail(ISING_E_ARG, "null handle");
+}
+
+int ising_lattice_get_spins(ising_lattice_t* l, int8_t* out) {
+    if (!l) return fail(ISING_E_ARG, "null handle");
+    return ising_pt_get_spins_i8(l->pt, 0, 0, out);
+}
+
+int ising_lattice_set_spins(ising_lattice_t* l, const int8_t* in) {
+    if (!l) return fail(ISING_E_ARG, "null handle");
+    return ising_pt_set_spins_i8(l->pt, 0, 0, in);
+}
+
+int ising_lattice_measure_raw(ising_lattice_t* l, int64_t* bond, int64_t* mag) {
+    if (!l) return fail(ISING_E_ARG, "null handle");
+    ising_pt* p = l->pt;
+    int64_t* acc = p->d_acc + 4;  // slots 4,5 of the accumulator block: standalone measurements
+    HIP_TRY(hipMemsetAsync(acc, 0, 2 * sizeof(int64_t), p->st));
+    HIP_TRY(launch_ising_measure(p->d_groups, 1, p->d_pool, acc - 0, p->nblocks, p->st));
+    int64_t v[2];
+    HIP_TRY(hipMemcpyAsync(v, acc, sizeof(v), hipMemcpyDeviceToHost, p->st));
+    HIP_TRY(hipStreamSynchronize(p->st));
+    if (bond) *bond = v[0];
+    if (mag) *mag = v[1];
+    return ISING_OK;
+}
+
+int ising_lattice_measure_gpu(ising_lattice_t* l, float j, double* e, double* m) {
+    int64_t B, M;
+    const int rc = ising_lattice_measure_raw(l, &B, &M);
+    if (rc) return rc;
+    const double n3 = (double)l->L * l->L * l->L;
+    if (e) *e = -(double)j * (double)B / n3;
+    if (m) *m = std::fabs((double)M) / n3;
+    return ISING_OK;
+}
+
+int ising_lattice_energy_gpu(ising_lattice_t* l, float j, double* e_total) {
+    int64_t B;
+    const int rc = ising_lattice_measure_raw(l, &B, nullptr);
+    if (rc) return rc;
+    if (e_total) *e_total = -(double)j * (double)B;
+    return ISING_OK;
+}
+
+int ising_lattice_measure(ising_lattice_t* l, double T, double j, double h, size_t samples,
+                          ising_observables_t* out) {
+    if (!l || !out) return fail(ISING_E_ARG, "null argument");
+    if (!(T > 0.0) || samples == 0) return fail(ISING_E_ARG, "temperature must be > 0 and samples >= 1");
+    const double beta = 1.0 / T, n3 = (double)l->L * l->L * l->L;
+    double se = 0, se2 = 0, sm = 0, sm2 = 0, sm4 = 0, sms = 0, sms2 = 0;
+    for (size_t i = 0; i < samples; ++i) {
+        int rc = ising_lattice_step(l, (float)beta, (float)j, (float)h);
+        if (rc) return rc;
+        int64_t B, M;
+        rc = ising_lattice_measure_raw(l, &B, &M);
+        if (rc) return rc;
+        const double e = (-j * (double)B - h * (double)M) / n3;
+        const double ms = (double)M / n3, ma = std::fabs(ms);
+        se += e;
+        se2 += e * e;
+        sm += ma;
+        sm2 += ma * ma;
+        sm4 += ma * ma * ma * ma;
+        sms += ms;
+        sms2 += ms * ms;
+    }
+    const double s = (double)samples;
+    const double ae = se / s, ae2 = se2 / s, am = sm / s, am2 = sm2 / s, am4 = sm4 / s, ams = sms / s,
+                 ams2 = sms2 / s;
+    out->temperature = T;
+    out->energy = ae;
+    out->magnetisation = am;
+    out->heat_capacity = beta * beta * (ae2 - ae * ae) * n3;
+    out->susceptibility = beta * (ams2 - ams * ams) * n3;
+    out->m2 = am2;
+    out->m4 = am4;
+    return ISING_OK;
+}
+
+int ising_lattice_domain_wall_density(ising_lattice_t* l, double* out) {
+    int64_t B;
+    const int rc = ising_lattice_measure_raw(l, &B, nullptr);
+    if (rc) return rc;
+    // 3N forward bonds; B = (#equal - #opposite) = 3N - 2 #walls
+    const double bonds = 3.0 * (double)l->L * l->L * l->L;
+    if (out) *out = (bonds - (double)B) / (2.0 * bonds);
+    return ISING_OK;
+}
+
+int ising_lattice_sweeps(const ising_lattice_t* l, int64_t* out) {
+    if (!l || !out) return fail(ISING_E_ARG, "null argument");
+    *out = l->pt->t_host;
+    return ISING_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Host statistics
+// ---------------------------------------------------------------------------
+
+int ising_linspace_temperatures(double t_min, double t_max, int32_t n, double* out) {
+    if (n < 1 || !out) return fail(ISING_E_ARG, "n must be >= 1 and out non-null");
+    if (n == 1) {
+        out[0] = t_min;
+        return ISING_OK;
+    }
+    for (int i = 0; i < n; ++i) out[i] = t_min + (t_max - t_min) * (double)i / (double)(n - 1);
+    return ISING_OK;
+}
+
+int ising_replica_exchange(const double* temps, const double* energies, int32_t n, int32_t* r2t, int32_t* t2r,
+                           const double* uniforms, int32_t even_odd, int32_t* accepted) {
+    if (!temps || !energies || !r2t || !t2r || !uniforms || n < 0) return fail(ISING_E_ARG, "null argument");
+    int acc = 0, k = 0;
+    for (int i = even_odd % 2; i + 1 < n; i += 2, ++k) {
+        const int ri = t2r[i], rj = t2r[i + 1];
+        const double bi = 1.0 / temps[i], bj = 1.0 / temps[i + 1];
+        const double delta = (bi - bj) * (energies[rj] - energies[ri]);
+        if (delta < 0.0 || uniforms[k] < std::exp(-delta)) {
+            t2r[i] = rj;
+            t2r[i + 1] = ri;
+            r2t[ri] = i + 1;
+            r2t[rj] = i;
+            ++acc;
+        }
+    }
+    if (accepted) *accepted = acc;
+    return ISING_OK;
+}
+
+int ising_jackknife(const double* e, const double* m, int64_t n, double beta, double n3, int32_t n_blocks,
+                    double* out) {
+    if (!out || (n > 0 && (!e || !m))) return fail(ISING_E_ARG, "null argument");
+    for (int i = 0; i < 12; ++i) out[i] = 0.0;
+    if (n < n_blocks || n_blocks < 2) return ISING_OK;
+    const int64_t bs = n / n_blocks, used = bs * n_blocks;
+    double se = 0, se2 = 0, sm = 0, sm2 = 0, sm4 = 0;
+    for (int64_t i = 0; i < used; ++i) {
+        const double q = m[i] * m[i];
+        se += e[i];
+        se2 += e[i] * e[i];
+        sm += m[i];
+        sm2 += q;
+        sm4 += q * q;
+    }
+    const double s = (double)used;
+    const double full[6] = {se / s, sm / s, sm2 / s, sm4 / s,
+                            beta * beta * (se2 / s - (se / s) * (se / s)) * n3,
+                            beta * (sm2 / s - (sm / s) * (sm / s)) * n3};
+    double var[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < n_blocks; ++b) {
+        // leave block b out, summing the remaining samples in index order
+        const int64_t lo = b * bs, hi = lo + bs;
+        double be = 0, be2 = 0, bm = 0, bm2 = 0, bm4 = 0;
+        for (int64_t i = 0; i < used; ++i) {
+            if (i >= lo && i < hi) continue;
+            const double q = m[i] * m[i];
+            be += e[i];
+            be2 += e[i] * e[i];
+            bm += m[i];
+            bm2 += q;
+            bm4 += q * q;
+        }
+        const double sj = (double)(used - bs);
+        const double je = be / sj, je2 = be2 / sj, jm = bm / sj, jm2 = bm2 / sj, jm4 = bm4 / sj;
+        const double jk[6] = {je, jm, jm2, jm4, beta * beta * (je2 - je * je) * n3, beta * (jm2 - jm * jm) * n3};
+        for (int k = 0; k < 6; ++k) var[k] += (jk[k] - full[k]) * (jk[k] - full[k]);
+    }
+    const double nb = (double)n_blocks;
+    for (int k = 0; k < 6; ++k) {
+        out[2 * k] = full[k];
+        out[2 * k + 1] = std::sqrt(var[k] * (nb - 1.0) / nb);
+    }
+    return ISING_OK;
+}
+
+}  // extern "C"

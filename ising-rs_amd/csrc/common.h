@@ -1,0 +1,149 @@
+// Shared device-side building blocks of libising_hip.so (gfx950 only).
+//
+// Counter-based random numbers: Philox4x32-10 (Salmon, Moraes, Dror, Shaw,
+// "Parallel random numbers: as easy as 1, 2, 3", SC'11).  Every draw is a pure
+// function of (seed, counter), so results do not depend on tiling, launch
+// shape or scheduling.  Counter words:
+//   c0 = site / site-group / pair index
+//   c1 = half-sweep id 2t + colour (t = global sweep index), or exchange round
+//   c2 = stream id (L << 16) | replica
+//   c3 = (purpose << 8) | model
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ising {
+
+enum : uint32_t { MODEL_ISING = 0, MODEL_XY = 1, MODEL_HEIS = 2 };
+enum : uint32_t { P_INIT = 1, P_ISING = 2, P_TIE = 3, P_EXCH = 4, P_CONT = 5 };
+
+__host__ __device__ constexpr uint32_t tag(uint32_t purpose, uint32_t model) { return (purpose << 8) | model; }
+__host__ __device__ constexpr uint32_t stream_id(uint32_t L, uint32_t replica) { return (L << 16) | replica; }
+
+struct u32x4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u32x4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                        uint32_t k1) {
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return {c0, c1, c2, c3};
+}
+
+// 53-bit uniform in [0,1) from two words.
+__device__ __forceinline__ double u01(uint32_t a, uint32_t b) {
+    const uint64_t m = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);
+    return (double)m * (1.0 / 9007199254740992.0);
+}
+// 24-bit uniform in [0,1), exact in FP32.
+__device__ __forceinline__ float u24f(uint32_t w) { return (float)(w >> 8) * (1.0f / 16777216.0f); }
+
+// FP64 exp from explicit fma, rint and ldexp only (Cody-Waite reduction by ln2,
+// degree-13 Taylor polynomial in Horner form).  Built this way so the host and
+// the device produce identical bits for the replica-exchange decision.
+__device__ __forceinline__ double det_exp(double x) {
+    if (x != x) return x;
+    if (x < -700.0) return 0.0;
+    if (x > 700.0) return __builtin_huge_val();
+    const double inv_ln2 = 1.4426950408889634074;
+    const double ln2_hi = 6.93147180369123816490e-01;
+    const double ln2_lo = 1.90821492927058770002e-10;
+    const double kd = __builtin_rint(x * inv_ln2);
+    double r = __builtin_fma(-kd, ln2_hi, x);
+    r = __builtin_fma(-kd, ln2_lo, r);
+    double p = 1.0 / 6227020800.0;
+    p = __builtin_fma(p, r, 1.0 / 479001600.0);
+    p = __builtin_fma(p, r, 1.0 / 39916800.0);
+    p = __builtin_fma(p, r, 1.0 / 3628800.0);
+    p = __builtin_fma(p, r, 1.0 / 362880.0);
+    p = __builtin_fma(p, r, 1.0 / 40320.0);
+    p = __builtin_fma(p, r, 1.0 / 5040.0);
+    p = __builtin_fma(p, r, 1.0 / 720.0);
+    p = __builtin_fma(p, r, 1.0 / 120.0);
+    p = __builtin_fma(p, r, 1.0 / 24.0);
+    p = __builtin_fma(p, r, 1.0 / 6.0);
+    p = __builtin_fma(p, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    return __builtin_ldexp(p, (int)kd);
+}
+
+// Wave-wide sums (64 lanes) by xor shuffles.
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble descriptors shared by the host API and the kernels.
+// ---------------------------------------------------------------------------
+
+// One lattice size of an ensemble: R replicas of an L^3 lattice, each stored
+// as two colour half-lattices (red/black) of L^3/2 sites, row-major in
+// (z, y, xh) with x = 2*xh + ((y + z + colour) & 1).
+struct DevGroup {
+    int32_t L, Lh, R, fast;
+    int32_t rep0;      // global index of replica 0 (tables are indexed by rep0 + i)
+    int32_t tiles;     // work tiles per replica half-sweep
+    int32_t blk0;      // first block of this group in a half-sweep launch
+    int32_t pad0;
+    int64_t n_half;    // L^3 / 2
+    int64_t stride;    // elements between consecutive half-lattices (padded)
+    int64_t off;       // element offset of replica 0 colour 0 in the pool
+};
+
+// Sweep control block, resident on the device.  Kernels read the sweep index
+// and cadence from here, so a captured launch sequence replays unchanged.
+struct DevCtl {
+    int64_t t;             // global index of the sweep in progress
+    int64_t sample_start;  // first sample sweep; INT64_MAX while warming up
+    int32_t measure_every;
+    int32_t exchange_every;
+    int32_t n_rec;         // measurement records written
+    int32_t rec_cap;
+    uint32_t done;         // blocks finished in the current closing launch
+    int32_t overflow;      // set when a record did not fit
+    int64_t exch_tries;
+    int64_t exch_accepts;
+};
+
+struct IsingRec {
+    int64_t bond;  // forward-bond sum B (E = -J B)
+    int64_t mag;   // sum of spins
+    int32_t tidx;  // temperature index of the replica when measured
+    int32_t pad;
+};
+
+struct ContRec {
+    double e, mx, my, mz;
+    int32_t tidx;
+    int32_t pad;
+};
+
+__device__ __forceinline__ void sweep_flags(const DevCtl* ctl, int64_t t, bool* meas, bool* exch, int64_t* s_out) {
+    const int64_t s = t - ctl->sample_start;
+    *s_out = s;
+    if (s < 0) {
+        *meas = false;
+        *exch = false;
+        return;
+    }
+    *meas = ((s + 1) % ctl->measure_every) == 0;
+    *exch = ((s + 1) % ctl->exchange_every) == 0;
+}
+
+}  // namespace ising

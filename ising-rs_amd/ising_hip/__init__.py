@@ -1,0 +1,334 @@
+"""Python mirror of the reference GPU API, over libising_hip.so (C ABI).
+
+Classes and functions follow faulknco/ising-rs's `ising::cuda` surface:
+
+* ``LatticeGpu``            -- src/cuda/lattice_gpu.rs:9-306
+* ``ContinuousGpuLattice``  -- src/cuda/gpu_lattice_continuous.rs:9-250
+* ``replica_exchange``, ``linspace_temperatures`` -- src/parallel_tempering.rs:5-46
+* ``jackknife_observables`` -- src/bin/gpu_fss.rs:327-412
+* ``PTEnsemble``            -- the device-resident PT loop of src/bin/gpu_fss.rs
+
+Every compute call goes through the HIP library; there is no CPU fallback.  If
+the library is not built, importing this module raises.
+"""
+import ctypes as C
+import os
+from collections import namedtuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libising_hip.so")
+
+MODEL_ISING, MODEL_XY, MODEL_HEISENBERG = 0, 1, 2
+_MODELS = {"ising": MODEL_ISING, "xy": MODEL_XY, "heisenberg": MODEL_HEISENBERG}
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("libising_hip.so is not built (%s); run `python -c 'import __graft_entry__ as g; g.build()'`"
+                      % LIB_PATH)
+lib = C.CDLL(LIB_PATH)
+
+
+class IsingError(RuntimeError):
+    """A libising_hip call failed; `code` is the ISING_E_* value."""
+
+    def __init__(self, code, msg):
+        super().__init__("libising_hip error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Opts(C.Structure):
+    _fields_ = [("j", C.c_double), ("h", C.c_double), ("exchange_every", C.c_int32), ("measure_every", C.c_int32),
+                ("delta", C.c_double), ("n_overrelax", C.c_int32), ("renorm_every", C.c_int32),
+                ("record_capacity", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Observables(C.Structure):
+    _fields_ = [("temperature", C.c_double), ("energy", C.c_double), ("magnetisation", C.c_double),
+                ("heat_capacity", C.c_double), ("susceptibility", C.c_double), ("m2", C.c_double),
+                ("m4", C.c_double)]
+
+
+_vp = C.c_void_p
+_i64p = C.POINTER(C.c_int64)
+_f64p = C.POINTER(C.c_double)
+_SIGS = {
+    "ising_last_error": (C.c_char_p, []),
+    "ising_version": (C.c_char_p, []),
+    "ising_device_count": (C.c_int, []),
+    "ising_lattice_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_uint64]),
+    "ising_lattice_destroy": (None, [_vp]),
+    "ising_lattice_size": (C.c_int, [_vp]),
+    "ising_lattice_step": (C.c_int, [_vp, C.c_float, C.c_float, C.c_float]),
+    "ising_lattice_warm_up": (C.c_int, [_vp, C.c_float, C.c_float, C.c_float, C.c_size_t]),
+    "ising_lattice_get_spins": (C.c_int, [_vp, _vp]),
+    "ising_lattice_set_spins": (C.c_int, [_vp, _vp]),
+    "ising_lattice_measure_gpu": (C.c_int, [_vp, C.c_float, _f64p, _f64p]),
+    "ising_lattice_energy_gpu": (C.c_int, [_vp, C.c_float, _f64p]),
+    "ising_lattice_measure_raw": (C.c_int, [_vp, _i64p, _i64p]),
+    "ising_lattice_measure": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_size_t,
+                                        C.POINTER(Observables)]),
+    "ising_lattice_domain_wall_density": (C.c_int, [_vp, _f64p]),
+    "ising_lattice_sweeps": (C.c_int, [_vp, _i64p]),
+    "ising_pt_default_opts": (None, [C.POINTER(Opts)]),
+    "ising_pt_create": (C.c_int, [C.POINTER(_vp), C.c_int, _vp, C.c_int32, C.c_int32, _vp, C.c_uint64,
+                                  C.POINTER(Opts)]),
+    "ising_pt_destroy": (None, [_vp]),
+    "ising_pt_warmup": (C.c_int, [_vp, C.c_int64]),
+    "ising_pt_sample": (C.c_int, [_vp, C.c_int64]),
+    "ising_pt_sync": (C.c_int, [_vp, _f64p]),
+    "ising_pt_pending": (C.c_int, [_vp, _i64p]),
+    "ising_pt_read_ising": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp, _i64p]),
+    "ising_pt_read_cont": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp, _vp, _vp, _i64p]),
+    "ising_pt_get_perm": (C.c_int, [_vp, C.c_int32, _vp, _vp]),
+    "ising_pt_exchange_stats": (C.c_int, [_vp, _i64p, _i64p]),
+    "ising_pt_get_spins_i8": (C.c_int, [_vp, C.c_int32, C.c_int32, _vp]),
+    "ising_pt_set_spins_i8": (C.c_int, [_vp, C.c_int32, C.c_int32, _vp]),
+    "ising_pt_get_spins_f32": (C.c_int, [_vp, C.c_int32, C.c_int32, _vp]),
+    "ising_pt_set_spins_f32": (C.c_int, [_vp, C.c_int32, C.c_int32, _vp]),
+    "ising_pt_sweeps": (C.c_int, [_vp, _i64p]),
+    "ising_pt_device_bytes": (C.c_int, [_vp, _i64p]),
+    "ising_linspace_temperatures": (C.c_int, [C.c_double, C.c_double, C.c_int32, _vp]),
+    "ising_replica_exchange": (C.c_int, [_vp, _vp, C.c_int32, _vp, _vp, _vp, C.c_int32, C.POINTER(C.c_int32)]),
+    "ising_jackknife": (C.c_int, [_vp, _vp, C.c_int64, C.c_double, C.c_double, C.c_int32, _vp]),
+}
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise IsingError(rc, lib.ising_last_error().decode())
+
+
+def version():
+    return lib.ising_version().decode()
+
+
+def device_count():
+    return lib.ising_device_count()
+
+
+ObservablesT = namedtuple("ObservablesT", "temperature energy magnetisation heat_capacity susceptibility m2 m4")
+
+
+class LatticeGpu:
+    """One 3D Ising lattice on the GPU (mirror of LatticeGpu, lattice_gpu.rs:9-306)."""
+
+    def __init__(self, n, seed):
+        self._h = _vp()
+        _check(lib.ising_lattice_create(C.byref(self._h), int(n), int(seed) & (2 ** 64 - 1)))
+        self.n = int(n)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.ising_lattice_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def step(self, beta, j=1.0, h=0.0):
+        _check(lib.ising_lattice_step(self._h, beta, j, h))
+
+    def warm_up(self, beta, j, h, sweeps):
+        _check(lib.ising_lattice_warm_up(self._h, beta, j, h, int(sweeps)))
+
+    def get_spins(self):
+        out = np.empty(self.n ** 3, np.int8)
+        _check(lib.ising_lattice_get_spins(self._h, _ptr(out)))
+        return out
+
+    def set_spins(self, spins):
+        a = np.ascontiguousarray(spins, np.int8).reshape(-1)
+        if a.size != self.n ** 3:
+            raise ValueError("need n^3 spins")
+        _check(lib.ising_lattice_set_spins(self._h, _ptr(a)))
+
+    def measure_gpu(self, j=1.0):
+        e, m = C.c_double(), C.c_double()
+        _check(lib.ising_lattice_measure_gpu(self._h, j, C.byref(e), C.byref(m)))
+        return e.value, m.value
+
+    def energy_gpu(self, j=1.0):
+        e = C.c_double()
+        _check(lib.ising_lattice_energy_gpu(self._h, j, C.byref(e)))
+        return e.value
+
+    def measure_raw(self):
+        b, m = C.c_int64(), C.c_int64()
+        _check(lib.ising_lattice_measure_raw(self._h, C.byref(b), C.byref(m)))
+        return b.value, m.value
+
+    def measure(self, temperature, j, h, samples):
+        o = Observables()
+        _check(lib.ising_lattice_measure(self._h, temperature, j, h, int(samples), C.byref(o)))
+        return ObservablesT(o.temperature, o.energy, o.magnetisation, o.heat_capacity, o.susceptibility, o.m2, o.m4)
+
+    def domain_wall_density(self):
+        d = C.c_double()
+        _check(lib.ising_lattice_domain_wall_density(self._h, C.byref(d)))
+        return d.value
+
+    def sweeps(self):
+        v = C.c_int64()
+        _check(lib.ising_lattice_sweeps(self._h, C.byref(v)))
+        return v.value
+
+
+def default_opts():
+    o = Opts()
+    lib.ising_pt_default_opts(C.byref(o))
+    return o
+
+
+class PTEnsemble:
+    """Every lattice size and replica of a PT run, resident on one GPU.
+
+    Records come back per measured sweep as arrays of shape (n_sweeps, n_sizes, R).
+    """
+
+    def __init__(self, model, sizes, replicas, temps, seed, **opts):
+        o = default_opts()
+        for k, v in opts.items():
+            if not hasattr(o, k):
+                raise TypeError("unknown option %r" % k)
+            setattr(o, k, v)
+        self.model = _MODELS[model] if isinstance(model, str) else int(model)
+        self.sizes = [int(s) for s in sizes]
+        self.R = int(replicas)
+        self.temps = np.ascontiguousarray(temps, np.float64)
+        if self.temps.size != self.R:
+            raise ValueError("need one temperature per replica")
+        self.opts = o
+        self._h = _vp()
+        sz = np.ascontiguousarray(self.sizes, np.int32)
+        _check(lib.ising_pt_create(C.byref(self._h), self.model, _ptr(sz), len(self.sizes), self.R,
+                                   _ptr(self.temps), int(seed) & (2 ** 64 - 1), C.byref(o)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.ising_pt_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    @property
+    def total_replicas(self):
+        return len(self.sizes) * self.R
+
+    def warmup(self, sweeps):
+        _check(lib.ising_pt_warmup(self._h, int(sweeps)))
+
+    def sample(self, sweeps):
+        _check(lib.ising_pt_sample(self._h, int(sweeps)))
+
+    def sync(self):
+        ms = C.c_double()
+        _check(lib.ising_pt_sync(self._h, C.byref(ms)))
+        return ms.value
+
+    def pending(self):
+        n = C.c_int64()
+        _check(lib.ising_pt_pending(self._h, C.byref(n)))
+        return n.value
+
+    def read_ising(self, max_sweeps=None):
+        if max_sweeps is None:
+            max_sweeps = self.pending()
+        n_tot = max(1, max_sweeps) * self.total_replicas
+        tidx = np.empty(n_tot, np.int32)
+        bond = np.empty(n_tot, np.int64)
+        mag = np.empty(n_tot, np.int64)
+        n = C.c_int64()
+        _check(lib.ising_pt_read_ising(self._h, int(max_sweeps), _ptr(tidx), _ptr(bond), _ptr(mag), C.byref(n)))
+        k = n.value * self.total_replicas
+        shape = (n.value, len(self.sizes), self.R)
+        return tidx[:k].reshape(shape), bond[:k].reshape(shape), mag[:k].reshape(shape)
+
+    def read_cont(self, max_sweeps=None):
+        if max_sweeps is None:
+            max_sweeps = self.pending()
+        n_tot = max(1, max_sweeps) * self.total_replicas
+        tidx = np.empty(n_tot, np.int32)
+        e, mx, my, mz = (np.empty(n_tot, np.float64) for _ in range(4))
+        n = C.c_int64()
+        _check(lib.ising_pt_read_cont(self._h, int(max_sweeps), _ptr(tidx), _ptr(e), _ptr(mx), _ptr(my), _ptr(mz),
+                                      C.byref(n)))
+        k = n.value * self.total_replicas
+        shape = (n.value, len(self.sizes), self.R)
+        return (tidx[:k].reshape(shape), e[:k].reshape(shape), mx[:k].reshape(shape), my[:k].reshape(shape),
+                mz[:k].reshape(shape))
+
+    def get_perm(self, size_index=0):
+        r2t = np.empty(self.R, np.int32)
+        t2r = np.empty(self.R, np.int32)
+        _check(lib.ising_pt_get_perm(self._h, size_index, _ptr(r2t), _ptr(t2r)))
+        return r2t, t2r
+
+    def exchange_stats(self):
+        a, b = C.c_int64(), C.c_int64()
+        _check(lib.ising_pt_exchange_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def get_spins(self, size_index, replica):
+        L = self.sizes[size_index]
+        if self.model == MODEL_ISING:
+            out = np.empty(L ** 3, np.int8)
+            _check(lib.ising_pt_get_spins_i8(self._h, size_index, replica, _ptr(out)))
+        else:
+            nc = 2 if self.model == MODEL_XY else 3
+            out = np.empty(L ** 3 * nc, np.float32)
+            _check(lib.ising_pt_get_spins_f32(self._h, size_index, replica, _ptr(out)))
+        return out
+
+    def set_spins(self, size_index, replica, spins):
+        if self.model == MODEL_ISING:
+            a = np.ascontiguousarray(spins, np.int8).reshape(-1)
+            _check(lib.ising_pt_set_spins_i8(self._h, size_index, replica, _ptr(a)))
+        else:
+            a = np.ascontiguousarray(spins, np.float32).reshape(-1)
+            _check(lib.ising_pt_set_spins_f32(self._h, size_index, replica, _ptr(a)))
+
+    def sweeps(self):
+        v = C.c_int64()
+        _check(lib.ising_pt_sweeps(self._h, C.byref(v)))
+        return v.value
+
+    def device_bytes(self):
+        v = C.c_int64()
+        _check(lib.ising_pt_device_bytes(self._h, C.byref(v)))
+        return v.value
+
+
+def linspace_temperatures(t_min, t_max, n):
+    out = np.empty(int(n), np.float64)
+    _check(lib.ising_linspace_temperatures(t_min, t_max, int(n), _ptr(out)))
+    return out
+
+
+def replica_exchange(temperatures, energies, replica_to_temp, temp_to_replica, uniforms, even_odd):
+    """In-place exchange on int32 permutation arrays; one uniform per attempted pair."""
+    t = np.ascontiguousarray(temperatures, np.float64)
+    e = np.ascontiguousarray(energies, np.float64)
+    u = np.ascontiguousarray(uniforms, np.float64)
+    acc = C.c_int32()
+    for a in (replica_to_temp, temp_to_replica):
+        if a.dtype != np.int32 or not a.flags.c_contiguous:
+            raise TypeError("permutation arrays must be contiguous int32")
+    _check(lib.ising_replica_exchange(_ptr(t), _ptr(e), len(t), _ptr(replica_to_temp), _ptr(temp_to_replica),
+                                      _ptr(u), int(even_odd), C.byref(acc)))
+    return acc.value
+
+
+def jackknife_observables(e, m, beta, n3, n_blocks):
+    e = np.ascontiguousarray(e, np.float64)
+    m = np.ascontiguousarray(m, np.float64)
+    out = np.empty(12, np.float64)
+    _check(lib.ising_jackknife(_ptr(e), _ptr(m), len(e), beta, n3, int(n_blocks), _ptr(out)))
+    return out

@@ -1,0 +1,154 @@
+/*
+ * CPU oracle for the Ising / XY / Heisenberg parallel-tempering path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (libising_hip.so, the
+ * gpu_fss command, the Python bindings) links, loads or calls this code.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
+ * and there only as the checker or the timed CPU baseline.
+ *
+ * Two families of functions live here:
+ *
+ *  1. Checkerboard replays ("o_*"): the same algorithm the GPU library runs,
+ *     restated on the full z-major lattice (index s = (z*L + y)*L + x) with the
+ *     same Philox4x32-10 counter assignment, so Ising lattices, int64
+ *     observables and the temperature permutation can be compared bit for bit,
+ *     and continuous-spin half-sweeps can be replayed in FP64.
+ *
+ *  2. Reference-faithful CPU algorithms ("o_ref_*"): restatements of the
+ *     reference's own CPU paths (random-site Metropolis, Wolff, Heisenberg cap
+ *     Metropolis + sequential overrelaxation, XY Wolff).  They use their own
+ *     xoshiro256++ stream, so they are algorithm-faithful, not stream-identical
+ *     to the reference's Rust `rand` draws.  They serve as the equilibrium
+ *     oracle and as the timed CPU baseline.
+ *
+ * Parity pins (what checks each function) are listed next to each group.
+ */
+#ifndef ISING_ORACLE_H
+#define ISING_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- Philox4x32-10 (Salmon et al., SC'11) ----------------------
+ * Pinned by: Random123 known-answer vectors and rocRAND's open-source
+ * philox4x32_10 engine header (tests/test_oracle_philox.py). */
+void o_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Counter word 3 = (purpose << 8) | model.  Counter word 2 = stream id. */
+enum { O_MODEL_ISING = 0, O_MODEL_XY = 1, O_MODEL_HEIS = 2 };
+enum { O_P_INIT = 1, O_P_ISING = 2, O_P_TIE = 3, O_P_EXCH = 4, O_P_CONT = 5 };
+uint32_t o_tag(int purpose, int model);
+uint32_t o_stream(int L, int replica);         /* (L << 16) | replica */
+
+/* FP64 exp built only from explicit fma, rint and ldexp, so the host and the
+ * device produce identical bits.  Pinned by comparison with libm exp. */
+double o_det_exp(double x);
+/* 53-bit uniform in [0,1) from two Philox words. */
+double o_u01(uint32_t a, uint32_t b);
+
+/* ---------------- Ising checkerboard replay --------------------------------
+ * Spins: int8 +-1, full z-major layout, even L.  Colour c = (x+y+z)&1.
+ * Half-sweep colour 0 then colour 1 (metropolis_sweep_kernel parity order,
+ * src/cuda/lattice_gpu.rs:88).  Acceptance follows src/cuda/kernels.cu:47-53:
+ * dE = 2 s (J sum_nb + h); accept iff dE <= 0 or u < exp(-beta dE), with u a
+ * 32-bit uniform compared against a host-built threshold table.
+ * Pinned by: brute-force stationary distribution on a 2x2x2 lattice,
+ * T->0 / T->inf limits (metropolis.rs:70-100), the 2D 4x4 exact-enumeration
+ * fixture and CPU Wolff equilibrium comparisons. */
+/* thr[k] with k = b*7 + n (b = spin up, n = up neighbours) as a 33-bit value
+ * split into hi (0..65536) and lo (0..65535). */
+void o_ising_thresholds(double beta, double J, double h, uint32_t thr_hi[14], uint32_t thr_lo[14]);
+void o_ising_init(int L, uint64_t seed, uint32_t stream, int8_t* spins);
+void o_ising_halfsweep(int L, int8_t* spins, int color, int64_t t, uint64_t seed, uint32_t stream,
+                       const uint32_t thr_hi[14], const uint32_t thr_lo[14]);
+/* Forward-bond sum B = sum_s s_s (s_{x+1} + s_{y+1} + s_{z+1}) and M = sum_s s_s
+ * (src/cuda/reduce_kernel.cu:7-66).  E_total = -J * B. */
+void o_ising_measure(int L, const int8_t* spins, int64_t* bond, int64_t* mag);
+/* 2D square-lattice variant (Lz = 1), used against the exact-enumeration and
+ * Onsager fixtures: same colour/RNG scheme with z removed. */
+void o_ising2d_halfsweep(int L, int8_t* spins, int color, int64_t t, uint64_t seed, uint32_t stream,
+                         const uint32_t thr_hi[10], const uint32_t thr_lo[10]);
+void o_ising2d_thresholds(double beta, double J, double h, uint32_t thr_hi[10], uint32_t thr_lo[10]);
+void o_ising2d_measure(int L, const int8_t* spins, int64_t* bond, int64_t* mag);
+
+/* ---------------- Replica exchange (parallel_tempering.rs:5-36) -------------
+ * Even/odd adjacent pairs on the temperature->replica permutation; one Philox
+ * uniform per pair is always drawn.  Returns the number of accepted swaps.
+ * Pinned by: bijectivity (parallel_tempering.rs:53-67), closed-form
+ * acceptance frequencies. */
+int o_exchange(int R, const double* betas, const double* energies, int* r2t, int* t2r, int64_t round,
+               uint64_t seed, uint32_t stream, int model);
+
+/* Full Ising PT run of one lattice size, exactly as the GPU ensemble runs it:
+ * W warm-up sweeps (no exchange), then S sample sweeps with measurement when
+ * (s+1) % measure_every == 0 and exchange when (s+1) % exchange_every == 0 at
+ * round s / exchange_every (src/bin/gpu_fss.rs:199-286).
+ * Records (written in replica order within a measured sweep): tidx, B, M.
+ * Returns the number of records written. */
+int64_t o_ising_pt_run(int L, int R, const double* temps, uint64_t seed, double J, double h,
+                       int64_t warmup, int64_t samples, int exchange_every, int measure_every,
+                       int32_t* rec_tidx, int64_t* rec_bond, int64_t* rec_mag, int8_t* final_spins);
+
+/* ---------------- Jackknife (src/bin/gpu_fss.rs:327-412) --------------------
+ * out = E, E_err, M, M_err, M2, M2_err, M4, M4_err, Cv, Cv_err, chi, chi_err.
+ * Pinned by closed-form series (constant, alternating, linear). */
+void o_jackknife(const double* e, const double* m, int64_t n, double beta, double n3, int n_blocks,
+                 double out[12]);
+void o_linspace(double tmin, double tmax, int n, double* out);
+
+/* ---------------- Continuous spins (XY n=2, Heisenberg n=3) ----------------
+ * Spins stored component-interleaved (AoS, as src/cuda/gpu_lattice_continuous.rs
+ * keeps them), full z-major layout, FP64.
+ * Metropolis half-sweep: one Philox call per site (counter = (s>>1, 2t+c,
+ * stream, tag(O_P_CONT, model))): words 0,1 drive the proposal, word 2 the
+ * acceptance.  Heisenberg: cap proposal around the current spin, cos(theta)
+ * uniform in [cos(delta), 1], phi uniform in [0, 2pi) (heisenberg/metropolis.rs:54-92).
+ * XY: rotation by alpha uniform in [-delta, delta).  Accept iff dE <= 0 or
+ * u < exp(-beta dE).
+ * Overrelaxation: S' = 2 (S.h) h / |h|^2 - S, skipped when |h|^2 < 1e-30
+ * (heisenberg/overrelax.rs:15-48), no renormalisation.
+ * When `accept_u`/`accept_p` are non-null they receive, per site of the colour,
+ * the uniform and the acceptance probability (for near-threshold masks).
+ * Pinned by: OR energy conservation, unit norm, single-site conditional
+ * histograms against the exact Boltzmann conditional, and equilibrium against
+ * the o_ref_* algorithms. */
+void o_cont_init(int L, int ncomp, uint64_t seed, uint32_t stream, double* spins);
+void o_cont_metropolis_half(int L, int ncomp, double* spins, int color, int64_t t, uint64_t seed,
+                            uint32_t stream, double beta, double J, double delta, int renorm,
+                            double* accept_u, double* accept_p);
+void o_cont_overrelax_half(int L, int ncomp, double* spins, int color, double J);
+void o_cont_measure(int L, int ncomp, const double* spins, double J, double* energy, double m[3]);
+/* Per-site local energy e_s = -J S_s . sum_nb S_nb (used by the 1e-5 parity check). */
+void o_cont_site_energy(int L, int ncomp, const double* spins, double J, double* out);
+
+/* ---------------- Reference-faithful CPU algorithms -------------------------
+ * RNG: xoshiro256++ seeded through splitmix64 (as rand_xoshiro's
+ * seed_from_u64 does), integer ranges by Lemire's method, f64 = (x>>11)*2^-53. */
+typedef struct { uint64_t s[4]; } o_rng;
+void o_rng_seed(o_rng* r, uint64_t seed);
+uint64_t o_rng_next(o_rng* r);
+double o_rng_f64(o_rng* r);
+uint64_t o_rng_below(o_rng* r, uint64_t n);
+
+/* Ising, 3D cubic, +-1 spins, full layout. */
+void o_ref_metropolis_sweep(int L, int8_t* spins, double beta, double J, double h, o_rng* rng); /* metropolis.rs:8-27 */
+int64_t o_ref_wolff_step(int L, int8_t* spins, double beta, double J, o_rng* rng, int32_t* stack, uint8_t* mark); /* wolff.rs:27-65 */
+/* Heisenberg, 3D cubic, AoS doubles. */
+void o_ref_heis_metropolis_sweep(int L, double* spins, double beta, double J, double delta, o_rng* rng, int64_t* accepted); /* heisenberg/metropolis.rs:10-36 */
+void o_ref_heis_overrelax_sweep(int L, double* spins, double J);  /* heisenberg/overrelax.rs:15-48 */
+/* XY Wolff, 3D cubic, AoS doubles. */
+int64_t o_ref_xy_wolff_sweep(int L, double* spins, double beta, double J, o_rng* rng, int32_t* stack, uint8_t* mark); /* xy/wolff.rs:16-73 */
+
+/* Timed baseline: random-site Metropolis sweeps until `seconds` elapse;
+ * returns updates performed and wall seconds. */
+int64_t o_bench_ref_metropolis(int L, double beta, uint64_t seed, double seconds, double* wall);
+int64_t o_bench_ref_heis(int L, double beta, double delta, int n_or, uint64_t seed, double seconds, double* wall);
+void o_ising_chain(int dim, int L, uint64_t seed, double beta, double J, double h, int64_t warmup,
+                   int64_t n_sweeps, int64_t* out_bond, int64_t* out_mag);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,140 @@
+"""GPU parity of the Ising path against the oracle (bit-exact), through the C ABI.
+
+Small sizes are replayed sweep for sweep by the oracle; the full config-2 size
+(L=128 x 32 replicas) is checked through size-independent properties: fused
+records equal a standalone reduction of the downloaded lattice, spins stay
++-1, the temperature permutation stays a bijection.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as o
+
+pytestmark = pytest.mark.gpu
+
+ising_hip = pytest.importorskip("ising_hip")
+
+
+def _lat(L, seed):
+    return ising_hip.LatticeGpu(L, seed)
+
+
+@pytest.mark.parametrize("L", [2, 4, 6, 8, 12, 16, 32, 48, 64])
+def test_init_matches_oracle(L):
+    lat = _lat(L, 42)
+    assert np.array_equal(lat.get_spins(), o.ising_init(L, 42, 0))
+
+
+@pytest.mark.parametrize("L,beta,h,sweeps", [(4, 0.2216, 0.0, 20), (6, 0.30, 0.0, 12), (8, 0.2216, 0.0, 10),
+                                             (12, 0.5, 0.3, 8), (16, 0.2216, 0.0, 8), (32, 0.2216, 0.0, 4),
+                                             (48, 0.18, -0.2, 3), (64, 0.2216, 0.0, 6)])
+def test_sweeps_are_bit_exact(L, beta, h, sweeps):
+    seed = 1000 + L
+    lat = _lat(L, seed)
+    ref = o.ising_init(L, seed, 0)
+    for t in range(sweeps):
+        lat.step(beta, 1.0, h)
+        o.ising_sweep(L, ref, t, seed, float(np.float32(beta)), 1.0, float(np.float32(h)))
+    got = lat.get_spins()
+    assert np.array_equal(got, ref), (L, int((got != ref).sum()))
+    assert lat.measure_raw() == o.ising_measure(L, ref)
+
+
+def test_measure_helpers_match_oracle():
+    L = 16
+    lat = _lat(L, 5)
+    for t in range(5):
+        lat.step(0.25, 1.0, 0.0)
+    sp = lat.get_spins()
+    B, M = o.ising_measure(L, sp)
+    e, m = lat.measure_gpu(1.0)
+    assert e == -B / L ** 3 and m == abs(M) / L ** 3
+    assert lat.energy_gpu(1.0) == -float(B)
+    walls = 0
+    s3 = sp.reshape(L, L, L)
+    for ax in range(3):
+        walls += int((s3 != np.roll(s3, -1, axis=ax)).sum())
+    assert abs(lat.domain_wall_density() - walls / (3 * L ** 3)) < 1e-15
+    lat.set_spins(np.ones(L ** 3, np.int8))
+    assert lat.measure_raw() == (3 * L ** 3, L ** 3)
+
+
+def test_ground_state_preserved_and_measure_observables():
+    L = 8
+    lat = _lat(L, 1)
+    lat.set_spins(np.ones(L ** 3, np.int8))
+    obs = lat.measure(0.01, 1.0, 0.0, 50)  # beta = 100
+    assert obs.energy == -3.0 and obs.magnetisation == 1.0 and obs.m4 == 1.0
+
+
+@pytest.mark.parametrize("sizes,R,warmup,samples,ee,me", [([4, 8], 4, 10, 60, 10, 5), ([8, 16, 6], 8, 20, 100, 10, 1),
+                                                          ([32], 6, 5, 40, 3, 2)])
+def test_pt_ensemble_records_are_bit_exact(sizes, R, warmup, samples, ee, me):
+    seed = 42
+    temps = o.linspace(4.40, 4.62, R)
+    pt = ising_hip.PTEnsemble("ising", sizes, R, temps, seed, exchange_every=ee, measure_every=me)
+    pt.warmup(warmup)
+    pt.sample(samples)
+    tidx, bond, mag = pt.read_ising()
+    assert tidx.shape == (samples // me, len(sizes), R)
+    for gi, L in enumerate(sizes):
+        rt, rb, rm, final = o.ising_pt_run(L, R, temps, seed, warmup, samples, ee, me)
+        assert np.array_equal(tidx[:, gi, :].reshape(-1), rt), L
+        assert np.array_equal(bond[:, gi, :].reshape(-1), rb), L
+        assert np.array_equal(mag[:, gi, :].reshape(-1), rm), L
+        for r in range(R):
+            assert np.array_equal(pt.get_spins(gi, r), final[r]), (L, r)
+        r2t, t2r = pt.get_perm(gi)
+        assert np.array_equal(t2r[r2t], np.arange(R))
+    tries, acc = pt.exchange_stats()
+    assert tries > 0 and 0 <= acc <= tries
+
+
+def test_batched_equals_unbatched():
+    temps = o.linspace(4.4, 4.62, 4)
+    both = ising_hip.PTEnsemble("ising", [8, 16], 4, temps, 9, exchange_every=5, measure_every=5)
+    one = ising_hip.PTEnsemble("ising", [16], 4, temps, 9, exchange_every=5, measure_every=5)
+    for p in (both, one):
+        p.warmup(7)
+        p.sample(30)
+    a = both.read_ising()
+    b = one.read_ising()
+    for x, y in zip(a, b):
+        assert np.array_equal(x[:, 1, :], y[:, 0, :])
+
+
+def test_record_buffer_drain_and_overflow_guard():
+    temps = o.linspace(4.4, 4.6, 2)
+    pt = ising_hip.PTEnsemble("ising", [4], 2, temps, 3, measure_every=1, record_capacity=8)
+    pt.sample(6)
+    with pytest.raises(ising_hip.IsingError):
+        pt.sample(5)  # 11 > 8
+    t1, b1, m1 = pt.read_ising(4)
+    t2, b2, m2 = pt.read_ising()
+    assert t1.shape[0] == 4 and t2.shape[0] == 2
+    pt.sample(8)
+    assert pt.pending() == 8
+    rt, rb, rm, _ = o.ising_pt_run(4, 2, temps, 3, 0, 14, 10, 1)
+    got_b = np.concatenate([b1.reshape(-1), b2.reshape(-1), pt.read_ising()[1].reshape(-1)])
+    assert np.array_equal(got_b, rb)
+
+
+def test_full_size_config_properties():
+    """L=128 x 32 replicas (config 2's largest size): fused records vs standalone reduction."""
+    R, L = 32, 128
+    temps = o.linspace(4.40, 4.62, R)
+    pt = ising_hip.PTEnsemble("ising", [L], R, temps, 42, exchange_every=2, measure_every=1)
+    pt.warmup(3)
+    pt.sample(4)
+    tidx, bond, mag = pt.read_ising()
+    assert tidx.shape == (4, 1, R)
+    for k in range(4):
+        assert sorted(tidx[k, 0]) == list(range(R))
+    for r in (0, 17, 31):
+        sp = pt.get_spins(0, r)
+        assert set(np.unique(sp)) <= {-1, 1}
+        s3 = sp.reshape(L, L, L).astype(np.int64)
+        B = int((s3 * (np.roll(s3, -1, 0) + np.roll(s3, -1, 1) + np.roll(s3, -1, 2))).sum())
+        assert (B, int(s3.sum())) == (int(bond[-1, 0, r]), int(mag[-1, 0, r]))
+    r2t, t2r = pt.get_perm(0)
+    assert np.array_equal(t2r[r2t], np.arange(R))
