@@ -95,6 +95,34 @@ int ising_lattice_domain_wall_density(ising_lattice_t* lat, double* out);
 int ising_lattice_sweeps(const ising_lattice_t* lat, int64_t* out);
 
 /* ------------------------------------------------------------------------
+ * Single continuous-spin lattice: mirror of ContinuousGpuLattice
+ * (src/cuda/gpu_lattice_continuous.rs).  n_comp = 2 (XY) or 3 (Heisenberg).
+ * ------------------------------------------------------------------------ */
+typedef struct ising_cont ising_cont_t;
+
+/* ContinuousGpuLattice::new(n, n_comp, seed, device) (gpu_lattice_continuous.rs:24):
+ * uniform random start from the library's counter-based stream. */
+int ising_cont_create(ising_cont_t** out, int n, int n_comp, uint64_t seed);
+void ising_cont_destroy(ising_cont_t* lat);
+/* ContinuousGpuLattice::sweep(beta, j, delta, n_overrelax) (:112): n_overrelax
+ * overrelaxation sweeps, then one Metropolis sweep (colour 0 then 1). Async.
+ * delta: Heisenberg cap half-angle / XY maximum rotation (radians). */
+int ising_cont_sweep(ising_cont_t* lat, float beta, float j, float delta, size_t n_overrelax);
+/* ContinuousGpuLattice::measure_gpu(j) (:177): totals E, Mx, My, Mz (FP64 sums). */
+int ising_cont_measure_gpu(ising_cont_t* lat, float j, double* e, double* mx, double* my, double* mz);
+/* ContinuousGpuLattice::measure_raw() (:243) = measure_gpu(1.0). */
+int ising_cont_measure_raw(ising_cont_t* lat, double* e, double* mx, double* my, double* mz);
+/* Spins as n^3 * n_comp floats, component-interleaved (the `spins` field, :13). */
+int ising_cont_get_spins(ising_cont_t* lat, float* out);
+int ising_cont_set_spins(ising_cont_t* lat, const float* in);
+/* Single colour passes (not in the reference; used by parity tests): one
+ * Metropolis or one overrelaxation pass over `color` at the current sweep
+ * index, without advancing it. */
+int ising_cont_metropolis_pass(ising_cont_t* lat, int color, float beta, float j, float delta);
+int ising_cont_overrelax_pass(ising_cont_t* lat, int color, float j);
+int ising_cont_sweeps(const ising_cont_t* lat, int64_t* out);
+
+/* ------------------------------------------------------------------------
  * Parallel-tempering ensemble: every lattice size and replica of a run in one
  * set of launches; exchange, measurement and the sweep counter on the device.
  * Replaces the host loop of gpu_fss (gpu_fss.rs:186-286 / 441-545).
@@ -106,15 +134,19 @@ typedef struct {
     double h;                /* field, Ising only (gpu_fss uses 0.0) */
     int32_t exchange_every;  /* --exchange-every, >= 1 */
     int32_t measure_every;   /* --measure-every, >= 1 */
-    double delta;            /* --delta: continuous proposal size (radians) */
+    double delta;            /* --delta: continuous proposal size (radians): Heisenberg cap
+                                half-angle, XY maximum rotation */
     int32_t n_overrelax;     /* --overrelax: OR passes per sweep (continuous) */
     int32_t renorm_every;    /* continuous: renormalise every k sweeps (0 = never) */
     int32_t record_capacity; /* measured sweeps buffered on the device before a drain */
-    int32_t reserved;
+    int32_t adapt_every;     /* continuous: during warm-up, rescale each temperature's
+                                delta every k sweeps towards adapt_target (0 = fixed) */
+    double adapt_target;     /* target Metropolis acceptance for the adaptation */
 } ising_pt_opts_t;
 
 /* Defaults: j=1, h=0, exchange_every=10, measure_every=1, delta=0.5,
- * n_overrelax=5, renorm_every=100, record_capacity=4096. */
+ * n_overrelax=5, renorm_every=100, record_capacity=4096, adapt_every=0,
+ * adapt_target=0.5. */
 void ising_pt_default_opts(ising_pt_opts_t* opts);
 
 /* model: ISING_MODEL_*.  sizes: n_sizes even L values.  temps: `replicas`
@@ -150,6 +182,8 @@ int ising_pt_get_spins_i8(ising_pt_t* pt, int32_t size_index, int32_t replica, i
 int ising_pt_set_spins_i8(ising_pt_t* pt, int32_t size_index, int32_t replica, const int8_t* in);
 int ising_pt_get_spins_f32(ising_pt_t* pt, int32_t size_index, int32_t replica, float* out);
 int ising_pt_set_spins_f32(ising_pt_t* pt, int32_t size_index, int32_t replica, const float* in);
+/* Current proposal size per temperature of one size (continuous models). */
+int ising_pt_get_deltas(ising_pt_t* pt, int32_t size_index, float* deltas);
 /* Sweeps applied so far (warm-up + sample). */
 int ising_pt_sweeps(const ising_pt_t* pt, int64_t* out);
 /* Bytes of spin storage on the device. */

@@ -88,6 +88,16 @@ struct ising_pt {
     DevCtl* d_ctl = nullptr;
     int64_t* d_acc = nullptr;
     void* d_rec = nullptr;
+    // continuous models
+    float* d_fpool = nullptr;
+    float* d_deltas = nullptr;
+    float* d_betas_f = nullptr;
+    double* d_part0 = nullptr;
+    double* d_part1 = nullptr;
+    double* d_dacc = nullptr;
+    int64_t* d_cnt = nullptr;
+    bool inline_cont = false;  // single-lattice continuous mode
+    float beta_inline = 0.f, delta_inline = 0.f, j_inline = 1.f;
     int64_t t_host = 0;        // sweeps enqueued
     int64_t s_host = -1;       // sample sweeps enqueued (-1 before sampling starts)
     int64_t pending_host = 0;  // measured sweeps the device will have written
@@ -105,6 +115,13 @@ struct ising_pt {
         (void)hipFree(d_ctl);
         (void)hipFree(d_acc);
         (void)hipFree(d_rec);
+        (void)hipFree(d_fpool);
+        (void)hipFree(d_deltas);
+        (void)hipFree(d_betas_f);
+        (void)hipFree(d_part0);
+        (void)hipFree(d_part1);
+        (void)hipFree(d_dacc);
+        (void)hipFree(d_cnt);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (st) (void)hipStreamDestroy(st);
@@ -131,15 +148,60 @@ struct ising_pt {
         return a;
     }
 
-    int enqueue_sweeps(int64_t n) {
+    ContArgs cont_args() const {
+        ContArgs a{};
+        a.groups = d_groups;
+        a.ngroups = (int32_t)groups.size();
+        a.total_reps = total_reps;
+        a.ncomp = ncomp;
+        a.model = model == ISING_MODEL_XY ? MODEL_XY : MODEL_HEIS;
+        a.pool = d_fpool;
+        a.deltas = d_deltas;
+        a.betas_f = d_betas_f;
+        a.betas = d_betas;
+        a.r2t = d_r2t;
+        a.t2r = d_t2r;
+        a.ctl = d_ctl;
+        a.part0 = d_part0;
+        a.part1 = d_part1;
+        a.dacc = d_dacc;
+        a.acc_cnt = d_cnt;
+        a.rec = (ContRec*)d_rec;
+        a.k0 = (uint32_t)seed;
+        a.k1 = (uint32_t)(seed >> 32);
+        a.J = inline_cont ? (double)j_inline : opts.j;
+        a.Jf = inline_cont ? j_inline : (float)opts.j;
+        a.renorm_every = opts.renorm_every;
+        a.adapt_every = opts.adapt_every;
+        a.adapt_target = (float)opts.adapt_target;
+        a.inline_params = inline_cont ? 1 : 0;
+        a.beta_inline = beta_inline;
+        a.delta_inline = delta_inline;
+        return a;
+    }
+
+    int enqueue_sweeps(int64_t n, int n_or = -1) {
         if (model == ISING_MODEL_ISING) {
             const IsingArgs a = ising_args();
             for (int64_t i = 0; i < n; ++i) {
                 HIP_TRY(launch_ising_half(a, nblocks, 0, st));
                 HIP_TRY(launch_ising_half(a, nblocks, 1, st));
+                HIP_TRY(launch_ising_close(a, st));
             }
         } else {
-            return fail(ISING_E_ARG, "continuous models are not available in this build");
+            // overrelaxation passes first, then one Metropolis sweep
+            // (the order of ContinuousGpuLattice::sweep, gpu_lattice_continuous.rs:124-168)
+            const ContArgs a = cont_args();
+            const int nor = n_or >= 0 ? n_or : opts.n_overrelax;
+            for (int64_t i = 0; i < n; ++i) {
+                for (int k = 0; k < nor; ++k) {
+                    HIP_TRY(launch_cont_or(a, nblocks, 0, st));
+                    HIP_TRY(launch_cont_or(a, nblocks, 1, st));
+                }
+                HIP_TRY(launch_cont_metro(a, nblocks, 0, 1, st));
+                HIP_TRY(launch_cont_metro(a, nblocks, 1, 1, st));
+                HIP_TRY(launch_cont_close(a, st));
+            }
         }
         t_host += n;
         return ISING_OK;
@@ -181,7 +243,13 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     else ising_pt_default_opts(&o);
     if (o.exchange_every < 1 || o.measure_every < 1) return fail(ISING_E_ARG, "exchange_every and measure_every must be >= 1");
     if (o.record_capacity < 1) return fail(ISING_E_ARG, "record_capacity must be >= 1");
-    if (model != ISING_MODEL_ISING) return fail(ISING_E_ARG, "continuous models are not available in this build");
+    if (model != ISING_MODEL_ISING) {
+        if (!(o.delta > 0.0) || !std::isfinite(o.delta)) return fail(ISING_E_ARG, "delta must be > 0");
+        if (o.n_overrelax < 0 || o.renorm_every < 0 || o.adapt_every < 0)
+            return fail(ISING_E_ARG, "n_overrelax, renorm_every and adapt_every must be >= 0");
+        if (o.adapt_every > 0 && !(o.adapt_target > 0.0 && o.adapt_target < 1.0))
+            return fail(ISING_E_ARG, "adapt_target must be in (0, 1)");
+    }
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(ISING_E_HIP, "no HIP device visible");
@@ -194,7 +262,10 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     p->temps.assign(temps, temps + R);
     p->total_reps = n_sizes * R;
 
-    const int64_t elem = 1;  // bytes per site (Ising)
+    const bool ising = model == ISING_MODEL_ISING;
+    const int64_t ncomp = ising ? 1 : p->ncomp;   // arrays per colour half-lattice
+    const int64_t elem = ising ? 1 : 4;           // bytes per element
+    const int64_t tile = ising ? kIsingTile : kContTile;
     int64_t off = 0;
     int blk = 0;
     for (int i = 0; i < n_sizes; ++i) {
@@ -207,10 +278,11 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
         G.n_half = (int64_t)sizes[i] * sizes[i] * sizes[i] / 2;
         G.stride = ((G.n_half * elem + 255) / 256) * 256 / elem;
         G.off = off;
-        G.tiles = (int32_t)((G.n_half + kIsingTile - 1) / kIsingTile);
+        G.tiles = (int32_t)((G.n_half + tile - 1) / tile);
         G.blk0 = blk;
+        if ((int64_t)blk + (int64_t)G.tiles * R > INT32_MAX / 2) return fail(ISING_E_ARG, "ensemble too large");
         blk += G.tiles * R;
-        off += 2 * (int64_t)R * G.stride;
+        off += 2 * (int64_t)R * ncomp * G.stride;
         p->groups.push_back(G);
     }
     p->nblocks = blk;
@@ -220,22 +292,35 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     HIP_TRY(hipEventCreate(&p->ev0));
     HIP_TRY(hipEventCreate(&p->ev1));
     HIP_TRY(hipMalloc(&p->d_groups, sizeof(DevGroup) * p->groups.size()));
-    HIP_TRY(hipMalloc(&p->d_pool, p->pool_bytes));
+    if (ising) HIP_TRY(hipMalloc(&p->d_pool, p->pool_bytes));
+    else HIP_TRY(hipMalloc(&p->d_fpool, p->pool_bytes));
     HIP_TRY(hipMalloc(&p->d_thr, sizeof(uint32_t) * 28 * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_r2t, sizeof(int32_t) * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_t2r, sizeof(int32_t) * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_betas, sizeof(double) * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_ctl, sizeof(DevCtl)));
     HIP_TRY(hipMalloc(&p->d_acc, sizeof(int64_t) * 8 * p->total_reps));
-    HIP_TRY(hipMalloc(&p->d_rec, sizeof(IsingRec) * (size_t)o.record_capacity * p->total_reps));
+    HIP_TRY(hipMalloc(&p->d_rec, (ising ? sizeof(IsingRec) : sizeof(ContRec)) * (size_t)o.record_capacity *
+                                     p->total_reps));
+    if (!ising) {
+        HIP_TRY(hipMalloc(&p->d_deltas, sizeof(float) * p->total_reps));
+        HIP_TRY(hipMalloc(&p->d_betas_f, sizeof(float) * p->total_reps));
+        HIP_TRY(hipMalloc(&p->d_part0, sizeof(double) * 4 * (size_t)p->nblocks));
+        HIP_TRY(hipMalloc(&p->d_part1, sizeof(double) * 4 * (size_t)p->nblocks));
+        HIP_TRY(hipMalloc(&p->d_dacc, sizeof(double) * 4 * (size_t)p->total_reps));
+        HIP_TRY(hipMalloc(&p->d_cnt, sizeof(int64_t) * (size_t)p->total_reps));
+    }
 
     std::vector<uint32_t> thr(28 * (size_t)p->total_reps);
     std::vector<int32_t> ident(p->total_reps);
     std::vector<double> betas(p->total_reps);
+    std::vector<float> betas_f(p->total_reps), deltas(p->total_reps);
     for (int g = 0; g < n_sizes; ++g)
         for (int i = 0; i < R; ++i) {
             const double beta = 1.0 / temps[i];
             betas[g * R + i] = beta;
+            betas_f[g * R + i] = (float)beta;
+            deltas[g * R + i] = (float)o.delta;
             ident[g * R + i] = i;
             ising_table(beta, o.j, o.h, &thr[28 * (size_t)(g * R + i)]);
         }
@@ -252,9 +337,22 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     HIP_TRY(hipMemcpy(p->d_betas, betas.data(), sizeof(double) * betas.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_ctl, &ctl, sizeof(DevCtl), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(p->d_acc, 0, sizeof(int64_t) * 8 * p->total_reps));
-    HIP_TRY(hipMemset(p->d_pool, 0, p->pool_bytes));
-    HIP_TRY(launch_ising_init(p->d_groups, (int)p->groups.size(), p->d_pool, p->nblocks, (uint32_t)seed,
-                              (uint32_t)(seed >> 32), p->st));
+    if (ising) {
+        HIP_TRY(hipMemset(p->d_pool, 0, p->pool_bytes));
+        HIP_TRY(launch_ising_init(p->d_groups, (int)p->groups.size(), p->d_pool, p->nblocks, (uint32_t)seed,
+                                  (uint32_t)(seed >> 32), p->st));
+    } else {
+        HIP_TRY(hipMemcpy(p->d_deltas, deltas.data(), sizeof(float) * deltas.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(p->d_betas_f, betas_f.data(), sizeof(float) * betas_f.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemset(p->d_fpool, 0, p->pool_bytes));
+        HIP_TRY(hipMemset(p->d_part0, 0, sizeof(double) * 4 * (size_t)p->nblocks));
+        HIP_TRY(hipMemset(p->d_part1, 0, sizeof(double) * 4 * (size_t)p->nblocks));
+        HIP_TRY(hipMemset(p->d_dacc, 0, sizeof(double) * 4 * (size_t)p->total_reps));
+        HIP_TRY(hipMemset(p->d_cnt, 0, sizeof(int64_t) * (size_t)p->total_reps));
+        HIP_TRY(launch_cont_init(p->d_groups, (int)p->groups.size(), p->d_fpool, p->ncomp,
+                                 model == ISING_MODEL_XY ? MODEL_XY : MODEL_HEIS, p->nblocks, (uint32_t)seed,
+                                 (uint32_t)(seed >> 32), p->st));
+    }
     HIP_TRY(hipStreamSynchronize(p->st));
     return ISING_OK;
 }
@@ -287,6 +385,8 @@ void ising_pt_default_opts(ising_pt_opts_t* o) {
     o->n_overrelax = 5;
     o->renorm_every = 100;
     o->record_capacity = 4096;
+    o->adapt_every = 0;
+    o->adapt_target = 0.5;
 }
 
 int ising_pt_create(ising_pt_t** out, int model, const int32_t* sizes, int32_t n_sizes, int32_t replicas,
@@ -399,9 +499,23 @@ int ising_pt_read_ising(ising_pt_t* p, int64_t max_sweeps, int32_t* tidx, int64_
     return ISING_OK;
 }
 
-int ising_pt_read_cont(ising_pt_t* p, int64_t, int32_t*, double*, double*, double*, double*, int64_t*) {
-    if (!p) return fail(ISING_E_ARG, "null handle");
-    return fail(ISING_E_STATE, "not a continuous-spin ensemble");
+int ising_pt_read_cont(ising_pt_t* p, int64_t max_sweeps, int32_t* tidx, double* e, double* mx, double* my,
+                       double* mz, int64_t* n_out) {
+    if (!p || !n_out) return fail(ISING_E_ARG, "null argument");
+    if (p->model == ISING_MODEL_ISING) return fail(ISING_E_STATE, "not a continuous-spin ensemble");
+    std::vector<ContRec> buf((size_t)std::max<int64_t>(0, max_sweeps) * p->total_reps);
+    int64_t n = 0;
+    const int rc = drain(p, max_sweeps, sizeof(ContRec), buf.data(), &n);
+    if (rc) return rc;
+    for (int64_t i = 0; i < n * p->total_reps; ++i) {
+        if (tidx) tidx[i] = buf[i].tidx;
+        if (e) e[i] = buf[i].e;
+        if (mx) mx[i] = buf[i].mx;
+        if (my) my[i] = buf[i].my;
+        if (mz) mz[i] = buf[i].mz;
+    }
+    *n_out = n;
+    return ISING_OK;
 }
 
 int ising_pt_get_perm(ising_pt_t* p, int32_t size_index, int32_t* r2t, int32_t* t2r) {
@@ -467,13 +581,68 @@ int ising_pt_set_spins_i8(ising_pt_t* p, int32_t size_index, int32_t replica, co
     return ISING_OK;
 }
 
-int ising_pt_get_spins_f32(ising_pt_t* p, int32_t, int32_t, float*) {
-    if (!p) return fail(ISING_E_ARG, "null handle");
-    return fail(ISING_E_STATE, "not a continuous-spin ensemble");
+int ising_pt_get_spins_f32(ising_pt_t* p, int32_t size_index, int32_t replica, float* out) {
+    if (!p || !out) return fail(ISING_E_ARG, "null argument");
+    if (p->model == ISING_MODEL_ISING) return fail(ISING_E_STATE, "not a continuous-spin ensemble");
+    const DevGroup* G = group_of(p, size_index);
+    if (!G || replica < 0 || replica >= G->R) return fail(ISING_E_ARG, "size_index or replica out of range");
+    const int nc = p->ncomp;
+    HIP_TRY(hipStreamSynchronize(p->st));
+    std::vector<float> half(2 * (size_t)nc * G->n_half);
+    for (int c = 0; c < 2; ++c)
+        for (int k = 0; k < nc; ++k)
+            HIP_TRY(hipMemcpy(half.data() + (size_t)(c * nc + k) * G->n_half,
+                              p->d_fpool + G->off + ((2 * (int64_t)replica + c) * nc + k) * G->stride,
+                              sizeof(float) * G->n_half, hipMemcpyDeviceToHost));
+    const int L = G->L, Lh = G->Lh;
+    for (int z = 0; z < L; ++z)
+        for (int y = 0; y < L; ++y)
+            for (int x = 0; x < L; ++x) {
+                const int c = (x + y + z) & 1;
+                const int64_t h = ((int64_t)z * L + y) * Lh + (x >> 1);
+                const int64_t s = ((int64_t)z * L + y) * L + x;
+                for (int k = 0; k < nc; ++k) out[s * nc + k] = half[(size_t)(c * nc + k) * G->n_half + h];
+            }
+    return ISING_OK;
 }
-int ising_pt_set_spins_f32(ising_pt_t* p, int32_t, int32_t, const float*) {
-    if (!p) return fail(ISING_E_ARG, "null handle");
-    return fail(ISING_E_STATE, "not a continuous-spin ensemble");
+
+int ising_pt_set_spins_f32(ising_pt_t* p, int32_t size_index, int32_t replica, const float* in) {
+    if (!p || !in) return fail(ISING_E_ARG, "null argument");
+    if (p->model == ISING_MODEL_ISING) return fail(ISING_E_STATE, "not a continuous-spin ensemble");
+    const DevGroup* G = group_of(p, size_index);
+    if (!G || replica < 0 || replica >= G->R) return fail(ISING_E_ARG, "size_index or replica out of range");
+    const int nc = p->ncomp;
+    const int L = G->L, Lh = G->Lh;
+    std::vector<float> half(2 * (size_t)nc * G->n_half);
+    for (int z = 0; z < L; ++z)
+        for (int y = 0; y < L; ++y)
+            for (int x = 0; x < L; ++x) {
+                const int c = (x + y + z) & 1;
+                const int64_t h = ((int64_t)z * L + y) * Lh + (x >> 1);
+                const int64_t s = ((int64_t)z * L + y) * L + x;
+                for (int k = 0; k < nc; ++k) {
+                    const float v = in[s * nc + k];
+                    if (!std::isfinite(v)) return fail(ISING_E_ARG, "spin components must be finite");
+                    half[(size_t)(c * nc + k) * G->n_half + h] = v;
+                }
+            }
+    HIP_TRY(hipStreamSynchronize(p->st));
+    for (int c = 0; c < 2; ++c)
+        for (int k = 0; k < nc; ++k)
+            HIP_TRY(hipMemcpy(p->d_fpool + G->off + ((2 * (int64_t)replica + c) * nc + k) * G->stride,
+                              half.data() + (size_t)(c * nc + k) * G->n_half, sizeof(float) * G->n_half,
+                              hipMemcpyHostToDevice));
+    return ISING_OK;
+}
+
+int ising_pt_get_deltas(ising_pt_t* p, int32_t size_index, float* deltas) {
+    if (!p || !deltas) return fail(ISING_E_ARG, "null argument");
+    if (p->model == ISING_MODEL_ISING) return fail(ISING_E_STATE, "not a continuous-spin ensemble");
+    const DevGroup* G = group_of(p, size_index);
+    if (!G) return fail(ISING_E_ARG, "size_index out of range");
+    HIP_TRY(hipStreamSynchronize(p->st));
+    HIP_TRY(hipMemcpy(deltas, p->d_deltas + G->rep0, sizeof(float) * G->R, hipMemcpyDeviceToHost));
+    return ISING_OK;
 }
 
 int ising_pt_sweeps(const ising_pt_t* p, int64_t* out) {
@@ -628,6 +797,120 @@ int ising_lattice_domain_wall_density(ising_lattice_t* l, double* out) {
 }
 
 int ising_lattice_sweeps(const ising_lattice_t* l, int64_t* out) {
+    if (!l || !out) return fail(ISING_E_ARG, "null argument");
+    *out = l->pt->t_host;
+    return ISING_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Single continuous lattice (ContinuousGpuLattice mirror)
+// ---------------------------------------------------------------------------
+
+}  // extern "C"
+
+struct ising_cont {
+    ising_pt* pt = nullptr;
+    int L = 0, ncomp = 0;
+    ~ising_cont() { delete pt; }
+};
+
+extern "C" {
+
+int ising_cont_create(ising_cont_t** out, int n, int n_comp, uint64_t seed) {
+    if (!out) return fail(ISING_E_ARG, "out is null");
+    *out = nullptr;
+    if (!valid_L(n)) return fail(ISING_E_ARG, "n must be even, 2..2046");
+    if (n_comp != 2 && n_comp != 3) return fail(ISING_E_ARG, "n_comp must be 2 (XY) or 3 (Heisenberg)");
+    ising_cont* l = new (std::nothrow) ising_cont();
+    if (!l) return fail(ISING_E_HIP, "host allocation failed");
+    l->L = n;
+    l->ncomp = n_comp;
+    const int32_t sizes[1] = {n};
+    const double temps[1] = {1.0};
+    ising_pt_opts_t o;
+    ising_pt_default_opts(&o);
+    o.record_capacity = 1;
+    o.renorm_every = 0;  // the reference lattice never renormalises (continuous_spin_kernel.cu:85-142)
+    const int rc = ising_pt_create(&l->pt, n_comp == 2 ? ISING_MODEL_XY : ISING_MODEL_HEISENBERG, sizes, 1, 1,
+                                   temps, seed, &o);
+    if (rc) {
+        const std::string keep = g_err;
+        delete l;
+        g_err = keep;
+        return rc;
+    }
+    l->pt->inline_cont = true;
+    *out = l;
+    return ISING_OK;
+}
+
+void ising_cont_destroy(ising_cont_t* lat) { delete lat; }
+
+static int cont_params(ising_cont* l, float beta, float j, float delta) {
+    if (!l) return fail(ISING_E_ARG, "null handle");
+    if (!(beta >= 0.0f) || !std::isfinite(beta) || !std::isfinite(j) || !(delta > 0.0f) || !std::isfinite(delta))
+        return fail(ISING_E_ARG, "beta >= 0, finite j and delta > 0 required");
+    l->pt->beta_inline = beta;
+    l->pt->j_inline = j;
+    l->pt->delta_inline = delta;
+    return ISING_OK;
+}
+
+int ising_cont_sweep(ising_cont_t* l, float beta, float j, float delta, size_t n_overrelax) {
+    const int rc = cont_params(l, beta, j, delta);
+    if (rc) return rc;
+    if (n_overrelax > 1000000) return fail(ISING_E_ARG, "n_overrelax too large");
+    return l->pt->enqueue_sweeps(1, (int)n_overrelax);
+}
+
+int ising_cont_metropolis_pass(ising_cont_t* l, int color, float beta, float j, float delta) {
+    const int rc = cont_params(l, beta, j, delta);
+    if (rc) return rc;
+    if (color != 0 && color != 1) return fail(ISING_E_ARG, "color must be 0 or 1");
+    HIP_TRY(launch_cont_metro(l->pt->cont_args(), l->pt->nblocks, color, 0, l->pt->st));
+    return ISING_OK;
+}
+
+int ising_cont_overrelax_pass(ising_cont_t* l, int color, float j) {
+    const int rc = cont_params(l, 1.0f, j, 1.0f);
+    if (rc) return rc;
+    if (color != 0 && color != 1) return fail(ISING_E_ARG, "color must be 0 or 1");
+    HIP_TRY(launch_cont_or(l->pt->cont_args(), l->pt->nblocks, color, l->pt->st));
+    return ISING_OK;
+}
+
+int ising_cont_measure_gpu(ising_cont_t* l, float j, double* e, double* mx, double* my, double* mz) {
+    if (!l) return fail(ISING_E_ARG, "null handle");
+    ising_pt* p = l->pt;
+    HIP_TRY(launch_cont_measure(p->d_groups, 1, p->d_fpool, p->ncomp, (double)j, p->d_part0, p->nblocks, p->st));
+    std::vector<double> part(4 * (size_t)p->nblocks);
+    HIP_TRY(hipMemcpyAsync(part.data(), p->d_part0, sizeof(double) * part.size(), hipMemcpyDeviceToHost, p->st));
+    HIP_TRY(hipStreamSynchronize(p->st));
+    double v[4] = {0, 0, 0, 0};
+    for (int b = 0; b < p->nblocks; ++b)
+        for (int k = 0; k < 4; ++k) v[k] += part[4 * (size_t)b + k];
+    if (e) *e = v[0];
+    if (mx) *mx = v[1];
+    if (my) *my = v[2];
+    if (mz) *mz = v[3];
+    return ISING_OK;
+}
+
+int ising_cont_measure_raw(ising_cont_t* l, double* e, double* mx, double* my, double* mz) {
+    return ising_cont_measure_gpu(l, 1.0f, e, mx, my, mz);
+}
+
+int ising_cont_get_spins(ising_cont_t* l, float* out) {
+    if (!l) return fail(ISING_E_ARG, "null handle");
+    return ising_pt_get_spins_f32(l->pt, 0, 0, out);
+}
+
+int ising_cont_set_spins(ising_cont_t* l, const float* in) {
+    if (!l) return fail(ISING_E_ARG, "null handle");
+    return ising_pt_set_spins_f32(l->pt, 0, 0, in);
+}
+
+int ising_cont_sweeps(const ising_cont_t* l, int64_t* out) {
     if (!l || !out) return fail(ISING_E_ARG, "null argument");
     *out = l->pt->t_host;
     return ISING_OK;

@@ -115,7 +115,7 @@ struct DevCtl {
     int32_t exchange_every;
     int32_t n_rec;         // measurement records written
     int32_t rec_cap;
-    uint32_t done;         // blocks finished in the current closing launch
+    uint32_t reserved;
     int32_t overflow;      // set when a record did not fit
     int64_t exch_tries;
     int64_t exch_accepts;

@@ -1,0 +1,431 @@
+// XY (n=2) and Heisenberg (n=3) checkerboard Metropolis + overrelaxation, gfx950.
+//
+// Replaces continuous_metropolis_kernel / continuous_overrelax_kernel /
+// init_continuous_rng_kernel (src/cuda/continuous_spin_kernel.cu:9-152) and
+// reduce_mag_continuous / reduce_energy_continuous (src/cuda/reduce_kernel.cu:70-151).
+//
+// Storage: FP32 unit vectors, structure of arrays per colour half-lattice:
+// component k of colour c of replica r lives at off + ((2r + c) * n + k) * stride.
+// A pass over colour c reads n floats of c and of 1-c and writes n floats of c:
+// 3 * 4n bytes per update (XY 24 B, Heisenberg 36 B).
+//
+// Proposal (symmetric, unlike the reference GPU's cube perturbation):
+//   Heisenberg: cap around the current spin, cos(theta) uniform in [cos d, 1],
+//               phi uniform in [0, 2 pi)   (heisenberg/metropolis.rs:54-92)
+//   XY:         rotation by alpha uniform in [-d, d)
+// One Philox call per site and pass: words 0,1 -> proposal, word 2 -> acceptance.
+// Overrelaxation: S' = 2 (S.h) h / |h|^2 - S, skipped when |h|^2 < 1e-30.
+//
+// Measurement is fused into the Metropolis passes and summed in FP64: block
+// partials are written to a buffer and a one-block closing kernel after the
+// colour-1 launch adds them in a fixed order, so every run with the same seed gives the same bits.
+#include "kernels.h"
+
+namespace ising {
+
+constexpr int kCT = 256;  // threads per block, one site per thread
+
+__device__ __forceinline__ int find_group_c(const DevGroup* groups, int ngroups, int b) {
+    int g = 0;
+    while (g + 1 < ngroups && b >= groups[g + 1].blk0) ++g;
+    return g;
+}
+
+__device__ __forceinline__ void neighbours(const DevGroup& G, int64_t h, int color, int64_t nb[6]) {
+    const int L = G.L, Lh = G.Lh;
+    const int64_t row = h / Lh;
+    const int xh = (int)(h - row * Lh);
+    const int z = (int)(row / L), y = (int)(row - (int64_t)z * L);
+    const int p = (y + z + color) & 1;
+    const int ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
+    const int zm = z == 0 ? L - 1 : z - 1, zp = z == L - 1 ? 0 : z + 1;
+    const int xo = p ? (xh + 1 == Lh ? 0 : xh + 1) : (xh == 0 ? Lh - 1 : xh - 1);
+    nb[0] = row * Lh + xh;
+    nb[1] = row * Lh + xo;
+    nb[2] = ((int64_t)z * L + ym) * Lh + xh;
+    nb[3] = ((int64_t)z * L + yp) * Lh + xh;
+    nb[4] = ((int64_t)zm * L + y) * Lh + xh;
+    nb[5] = ((int64_t)zp * L + y) * Lh + xh;
+}
+
+template <int N>
+__device__ __forceinline__ void load_field(const float* O, int64_t stride, const int64_t nb[6], float hf[3]) {
+    hf[0] = hf[1] = hf[2] = 0.f;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const float* Ok = O + k * stride;
+        hf[k] = ((Ok[nb[0]] + Ok[nb[1]]) + (Ok[nb[2]] + Ok[nb[3]])) + (Ok[nb[4]] + Ok[nb[5]]);
+    }
+}
+
+// Block sum of up to 4 doubles; result valid in thread 0.
+__device__ __forceinline__ void block_sum4(double v[4], double (*s_red)[kCT / 64]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < 4; ++k) s_red[k][wid] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) {
+            double s = 0.0;
+            for (int i = 0; i < kCT / 64; ++i) s += s_red[k][i];
+            v[k] = s;
+        }
+}
+
+// Closing step of a continuous sweep (one-block kernel after the colour-1 Metropolis pass):
+// ordered FP64 sums of the block partials, records, exchange, delta adaptation
+// during warm-up, sweep counter.
+__device__ void close_cont(const ContArgs& a, int64_t t, bool meas, bool exch, int64_t s) {
+    __shared__ double s_part[4][kCT];
+    DevCtl* ctl = a.ctl;
+    const int tid = threadIdx.x;
+    const bool need = meas || exch;
+    const int nrec = ctl->n_rec;
+    const bool room = nrec < ctl->rec_cap;
+    if (need) {
+        for (int g = 0; g < a.ngroups; ++g) {
+            const DevGroup G = a.groups[g];
+            for (int r = 0; r < G.R; ++r) {
+                const int64_t b0 = G.blk0 + (int64_t)r * G.tiles;
+                double v[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int64_t b = tid; b < G.tiles; b += kCT) {
+                    const double* p0 = a.part0 + 4 * (b0 + b);
+                    const double* p1 = a.part1 + 4 * (b0 + b);
+                    v[0] += p1[0];
+                    v[1] += p0[1] + p1[1];
+                    v[2] += p0[2] + p1[2];
+                    v[3] += p0[3] + p1[3];
+                }
+                for (int k = 0; k < 4; ++k) s_part[k][tid] = v[k];
+                __syncthreads();
+                for (int w = kCT / 2; w > 0; w >>= 1) {
+                    if (tid < w)
+                        for (int k = 0; k < 4; ++k) s_part[k][tid] += s_part[k][tid + w];
+                    __syncthreads();
+                }
+                if (tid == 0) {
+                    const int gr = G.rep0 + r;
+                    for (int k = 0; k < 4; ++k) a.dacc[4 * gr + k] = s_part[k][0];
+                    if (meas && room) {
+                        ContRec rc;
+                        rc.e = s_part[0][0];
+                        rc.mx = s_part[1][0];
+                        rc.my = s_part[2][0];
+                        rc.mz = s_part[3][0];
+                        rc.tidx = a.r2t[gr];
+                        rc.pad = 0;
+                        a.rec[(int64_t)nrec * a.total_reps + gr] = rc;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    if (exch) {
+        const int64_t round = s / ctl->exchange_every;
+        for (int g = 0; g < a.ngroups; ++g) {
+            const DevGroup G = a.groups[g];
+            const int npairs = (G.R - (int)(round & 1)) / 2;
+            for (int q = tid; q < npairs; q += kCT) {
+                const int i = (int)(round & 1) + 2 * q;
+                const int ri = a.t2r[G.rep0 + i], rj = a.t2r[G.rep0 + i + 1];
+                const double Ei = a.dacc[4 * (G.rep0 + ri)];
+                const double Ej = a.dacc[4 * (G.rep0 + rj)];
+                const double delta = (a.betas[G.rep0 + i] - a.betas[G.rep0 + i + 1]) * (Ej - Ei);
+                const u32x4 w = philox((uint32_t)i, (uint32_t)round, stream_id((uint32_t)G.L, 0u),
+                                       tag(P_EXCH, a.model), a.k0, a.k1);
+                const double u = u01(w.x, w.y);
+                if (delta <= 0.0 || u < det_exp(-delta)) {
+                    a.t2r[G.rep0 + i] = rj;
+                    a.t2r[G.rep0 + i + 1] = ri;
+                    a.r2t[G.rep0 + ri] = i + 1;
+                    a.r2t[G.rep0 + rj] = i;
+                    atomicAdd((unsigned long long*)&ctl->exch_accepts, 1ull);
+                }
+            }
+            if (tid == 0) atomicAdd((unsigned long long*)&ctl->exch_tries, (unsigned long long)npairs);
+        }
+    }
+    // Warm-up only: scale each temperature's proposal size towards the target
+    // acceptance.  Frozen during sampling, so the sampled chain is stationary.
+    if (a.adapt_every > 0 && s < 0 && ((t + 1) % a.adapt_every) == 0) {
+        for (int gr = tid; gr < a.total_reps; gr += kCT) {
+            int g = 0;
+            while (g + 1 < a.ngroups && gr >= a.groups[g + 1].rep0) ++g;
+            const DevGroup G = a.groups[g];
+            const double sites = (double)G.n_half * 2.0 * (double)a.adapt_every;
+            const double rate = (double)__hip_atomic_load(&a.acc_cnt[gr], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) / sites;
+            const int ti = G.rep0 + a.r2t[gr];
+            double f = rate / (double)a.adapt_target;
+            f = f < 0.5 ? 0.5 : (f > 2.0 ? 2.0 : f);
+            double d = (double)a.deltas[ti] * f;
+            d = d < 1e-3 ? 1e-3 : (d > 3.14159265358979 ? 3.14159265358979 : d);
+            a.deltas[ti] = (float)d;
+            a.acc_cnt[gr] = 0;
+        }
+    }
+    if (tid == 0) {
+        if (meas) {
+            if (room) ctl->n_rec = nrec + 1;
+            else ctl->overflow = 1;
+        }
+        ctl->t = t + 1;
+    }
+}
+
+// Metropolis pass over colour `color` of every replica.
+template <int N>
+__global__ __launch_bounds__(kCT) void cont_metro_kernel(ContArgs a, int color, int closes) {
+    __shared__ double s_red[4][kCT / 64];
+    const int g = find_group_c(a.groups, a.ngroups, blockIdx.x);
+    const DevGroup G = a.groups[g];
+    const int local = blockIdx.x - G.blk0;
+    const int r = local / G.tiles;
+    const int tile = local - r * G.tiles;
+    const int grep = G.rep0 + r;
+    const int ti = G.rep0 + a.r2t[grep];
+    const float beta = a.inline_params ? a.beta_inline : a.betas_f[ti];
+    const float dlt = a.inline_params ? a.delta_inline : a.deltas[ti];
+    const int64_t t = a.ctl->t;
+    bool meas, exch;
+    int64_t s;
+    sweep_flags(a.ctl, t, &meas, &exch, &s);
+    const bool need = (meas || exch) && closes;
+    const bool renorm = a.renorm_every > 0 && ((t + 1) % a.renorm_every) == 0;
+    const bool count = a.adapt_every > 0 && s < 0;
+    const uint32_t hs = (uint32_t)(2 * t + color);
+    const uint32_t stream = stream_id((uint32_t)G.L, (uint32_t)r);
+
+    float* C = a.pool + G.off + (int64_t)(2 * r + color) * N * G.stride;
+    const float* O = a.pool + G.off + (int64_t)(2 * r + (color ^ 1)) * N * G.stride;
+    const int64_t h = (int64_t)tile * kCT + threadIdx.x;
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+    int acc_n = 0;
+    if (h < G.n_half) {
+        int64_t nb[6];
+        neighbours(G, h, color, nb);
+        float hf[3];
+        load_field<N>(O, G.stride, nb, hf);
+        float S[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < N; ++k) S[k] = C[k * G.stride + h];
+        const u32x4 w = philox((uint32_t)h, hs, stream, tag(P_CONT, a.model), a.k0, a.k1);
+        float nw[3] = {0.f, 0.f, 0.f};
+        if (N == 2) {
+            const float al = dlt * (2.0f * u24f(w.x) - 1.0f);
+            float sa, ca;
+            sincosf(al, &sa, &ca);
+            nw[0] = S[0] * ca - S[1] * sa;
+            nw[1] = S[1] * ca + S[0] * sa;
+        } else {
+            const float cd = cosf(dlt);
+            const float ct = cd + (1.0f - cd) * u24f(w.x);
+            const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+            float sp, cp;
+            sincosf(6.28318530717958647692f * u24f(w.y), &sp, &cp);
+            float tv0 = 0.f, tv1 = 0.f;
+            if (fabsf(S[0]) < 0.9f) tv0 = 1.f; else tv1 = 1.f;
+            const float ts = tv0 * S[0] + tv1 * S[1];
+            float u0 = tv0 - ts * S[0], u1 = tv1 - ts * S[1], u2 = -ts * S[2];
+            const float ui = 1.0f / sqrtf(u0 * u0 + u1 * u1 + u2 * u2);
+            u0 *= ui; u1 *= ui; u2 *= ui;
+            const float v0 = S[1] * u2 - S[2] * u1, v1 = S[2] * u0 - S[0] * u2, v2 = S[0] * u1 - S[1] * u0;
+            const float a0 = st * cp, a1 = st * sp;
+            nw[0] = a0 * u0 + a1 * v0 + ct * S[0];
+            nw[1] = a0 * u1 + a1 * v1 + ct * S[1];
+            nw[2] = a0 * u2 + a1 * v2 + ct * S[2];
+            const float ni = 1.0f / sqrtf(nw[0] * nw[0] + nw[1] * nw[1] + nw[2] * nw[2]);
+            nw[0] *= ni; nw[1] *= ni; nw[2] *= ni;
+        }
+        float dE = 0.f;
+#pragma unroll
+        for (int k = 0; k < N; ++k) dE += (nw[k] - S[k]) * hf[k];
+        dE *= -a.Jf;
+        const bool accept = dE <= 0.0f || u24f(w.z) < expf(-beta * dE);
+        if (accept) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) S[k] = nw[k];
+            acc_n = 1;
+        }
+        if (renorm) {
+            float q = 0.f;
+#pragma unroll
+            for (int k = 0; k < N; ++k) q += S[k] * S[k];
+            const float qi = 1.0f / sqrtf(q);
+#pragma unroll
+            for (int k = 0; k < N; ++k) S[k] *= qi;
+        }
+        if (accept || renorm) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) C[k * G.stride + h] = S[k];
+        }
+        if (need) {
+            double e = 0.0;
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                part[1 + k] = (double)S[k];
+                e += (double)S[k] * (double)hf[k];
+            }
+            part[0] = color ? -a.J * e : 0.0;
+        }
+    }
+    if (count) {
+        const int wc = wave_sum(acc_n);
+        if ((threadIdx.x & 63) == 0 && wc) atomicAdd((unsigned long long*)&a.acc_cnt[grep], (unsigned long long)wc);
+    }
+    if (need) {
+        block_sum4(part, s_red);
+        if (threadIdx.x == 0) {
+            double* dst = (color ? a.part1 : a.part0) + 4 * (int64_t)blockIdx.x;
+            for (int k = 0; k < 4; ++k) dst[k] = part[k];
+        }
+    }
+}
+
+// Closes a continuous sweep (one block, after the colour-1 Metropolis pass).
+__global__ __launch_bounds__(kCT) void cont_close_kernel(ContArgs a) {
+    const int64_t t = a.ctl->t;
+    bool meas, exch;
+    int64_t s;
+    sweep_flags(a.ctl, t, &meas, &exch, &s);
+    close_cont(a, t, meas, exch, s);
+}
+
+// Overrelaxation pass over colour `color` (no random numbers, no renormalisation).
+template <int N>
+__global__ __launch_bounds__(kCT) void cont_or_kernel(ContArgs a, int color) {
+    const int g = find_group_c(a.groups, a.ngroups, blockIdx.x);
+    const DevGroup G = a.groups[g];
+    const int local = blockIdx.x - G.blk0;
+    const int r = local / G.tiles;
+    const int tile = local - r * G.tiles;
+    float* C = a.pool + G.off + (int64_t)(2 * r + color) * N * G.stride;
+    const float* O = a.pool + G.off + (int64_t)(2 * r + (color ^ 1)) * N * G.stride;
+    const int64_t h = (int64_t)tile * kCT + threadIdx.x;
+    if (h >= G.n_half) return;
+    int64_t nb[6];
+    neighbours(G, h, color, nb);
+    float hf[3];
+    load_field<N>(O, G.stride, nb, hf);
+    float h2 = 0.f, sh = 0.f, S[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        hf[k] *= a.Jf;
+        S[k] = C[k * G.stride + h];
+        h2 += hf[k] * hf[k];
+        sh += S[k] * hf[k];
+    }
+    if (h2 < 1e-30f) return;
+    const float sc = 2.0f * sh / h2;
+#pragma unroll
+    for (int k = 0; k < N; ++k) C[k * G.stride + h] = sc * hf[k] - S[k];
+}
+
+// Random start: Heisenberg z = 2u-1, phi = 2 pi u' (uniform on S^2); XY theta = 2 pi u.
+template <int N>
+__global__ __launch_bounds__(kCT) void cont_init_kernel(const DevGroup* groups, int ngroups, float* pool, int model,
+                                                      uint32_t k0, uint32_t k1) {
+    const int g = find_group_c(groups, ngroups, blockIdx.x);
+    const DevGroup G = groups[g];
+    const int local = blockIdx.x - G.blk0;
+    const int r = local / G.tiles;
+    const int tile = local - r * G.tiles;
+    const int64_t h = (int64_t)tile * kCT + threadIdx.x;
+    if (h >= G.n_half) return;
+    const uint32_t stream = stream_id((uint32_t)G.L, (uint32_t)r);
+    for (int c = 0; c < 2; ++c) {
+        float* C = pool + G.off + (int64_t)(2 * r + c) * N * G.stride;
+        const int64_t row = h / G.Lh;
+        const int xh = (int)(h - row * G.Lh);
+        const int z = (int)(row / G.L), y = (int)(row - (int64_t)z * G.L);
+        const int x = 2 * xh + ((y + z + c) & 1);
+        const uint32_t site = (uint32_t)(((int64_t)z * G.L + y) * G.L + x);
+        const u32x4 w = philox(site, 0u, stream, tag(P_INIT, (uint32_t)model), k0, k1);
+        if (N == 2) {
+            float sn, cs;
+            sincosf(6.28318530717958647692f * u24f(w.x), &sn, &cs);
+            C[h] = cs;
+            C[G.stride + h] = sn;
+        } else {
+            const float zz = 2.0f * u24f(w.x) - 1.0f;
+            float sn, cs;
+            sincosf(6.28318530717958647692f * u24f(w.y), &sn, &cs);
+            const float rr = sqrtf(fmaxf(0.0f, 1.0f - zz * zz));
+            C[h] = rr * cs;
+            C[G.stride + h] = rr * sn;
+            C[2 * G.stride + h] = zz;
+        }
+    }
+}
+
+// Standalone FP64 E/M of every replica: part[4*blk + k] partials (E, Mx, My, Mz).
+template <int N>
+__global__ __launch_bounds__(kCT) void cont_measure_kernel(const DevGroup* groups, int ngroups, const float* pool,
+                                                         double J, double* part) {
+    __shared__ double s_red[4][kCT / 64];
+    const int g = find_group_c(groups, ngroups, blockIdx.x);
+    const DevGroup G = groups[g];
+    const int local = blockIdx.x - G.blk0;
+    const int r = local / G.tiles;
+    const int tile = local - r * G.tiles;
+    const float* C1 = pool + G.off + (int64_t)(2 * r + 1) * N * G.stride;
+    const float* C0 = pool + G.off + (int64_t)(2 * r) * N * G.stride;
+    const int64_t h = (int64_t)tile * kCT + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (h < G.n_half) {
+        int64_t nb[6];
+        neighbours(G, h, 1, nb);
+        double e = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            double f = 0.0;
+            for (int q = 0; q < 6; ++q) f += (double)C0[k * G.stride + nb[q]];
+            const double s1 = (double)C1[k * G.stride + h];
+            e += s1 * f;
+            v[1 + k] = s1 + (double)C0[k * G.stride + h];
+        }
+        v[0] = -J * e;
+    }
+    block_sum4(v, s_red);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) part[4 * (int64_t)blockIdx.x + k] = v[k];
+}
+
+hipError_t launch_cont_metro(const ContArgs& a, int nblocks, int color, int closes, hipStream_t st) {
+    if (a.ncomp == 2) hipLaunchKernelGGL(cont_metro_kernel<2>, dim3(nblocks), dim3(kCT), 0, st, a, color, closes);
+    else hipLaunchKernelGGL(cont_metro_kernel<3>, dim3(nblocks), dim3(kCT), 0, st, a, color, closes);
+    return hipGetLastError();
+}
+
+hipError_t launch_cont_close(const ContArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(cont_close_kernel, dim3(1), dim3(kCT), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_cont_or(const ContArgs& a, int nblocks, int color, hipStream_t st) {
+    if (a.ncomp == 2) hipLaunchKernelGGL(cont_or_kernel<2>, dim3(nblocks), dim3(kCT), 0, st, a, color);
+    else hipLaunchKernelGGL(cont_or_kernel<3>, dim3(nblocks), dim3(kCT), 0, st, a, color);
+    return hipGetLastError();
+}
+
+hipError_t launch_cont_init(const DevGroup* groups, int ngroups, float* pool, int ncomp, int model, int nblocks,
+                            uint32_t k0, uint32_t k1, hipStream_t st) {
+    if (ncomp == 2)
+        hipLaunchKernelGGL(cont_init_kernel<2>, dim3(nblocks), dim3(kCT), 0, st, groups, ngroups, pool, model, k0, k1);
+    else
+        hipLaunchKernelGGL(cont_init_kernel<3>, dim3(nblocks), dim3(kCT), 0, st, groups, ngroups, pool, model, k0, k1);
+    return hipGetLastError();
+}
+
+hipError_t launch_cont_measure(const DevGroup* groups, int ngroups, const float* pool, int ncomp, double J,
+                               double* part, int nblocks, hipStream_t st) {
+    if (ncomp == 2)
+        hipLaunchKernelGGL(cont_measure_kernel<2>, dim3(nblocks), dim3(kCT), 0, st, groups, ngroups, pool, J, part);
+    else
+        hipLaunchKernelGGL(cont_measure_kernel<3>, dim3(nblocks), dim3(kCT), 0, st, groups, ngroups, pool, J, part);
+    return hipGetLastError();
+}
+
+}  // namespace ising

@@ -16,8 +16,9 @@
 // Measurement is fused: the colour-0 pass sums its new spins, the colour-1 pass
 // sums its new spins and s_i * (sum of its 6 neighbours), which counts every
 // bond exactly once (each bond joins the two colours).  Block partials go to
-// int64 accumulators by atomics; the last block of the colour-1 launch writes
-// the records, performs the replica exchange and advances the sweep counter.
+// int64 accumulators by atomics; after the colour-1 launch a closing kernel writes
+// the records, performs the replica exchange and advances the sweep counter
+// in a one-block closing kernel.
 #include "kernels.h"
 
 namespace ising {
@@ -97,7 +98,6 @@ __device__ void close_sweep(const IsingArgs& a, int64_t t, bool meas, bool exch,
             if (room) ctl->n_rec = nrec + 1;
             else ctl->overflow = 1;
         }
-        ctl->done = 0;
         ctl->t = t + 1;
     }
 }
@@ -107,8 +107,6 @@ __device__ void close_sweep(const IsingArgs& a, int64_t t, bool meas, bool exch,
 __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int color) {
     __shared__ uint32_t s_thr[28];
     __shared__ int64_t s_red[2][kThreads / 64];
-    __shared__ int s_last;
-
     const int g = find_group(a.groups, a.ngroups, blockIdx.x);
     const DevGroup G = a.groups[g];
     const int local = blockIdx.x - G.blk0;
@@ -233,20 +231,17 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
         }
     }
 
-    if (color) {
-        // Last block to finish closes the sweep.  Every other block has read the
-        // sweep index before incrementing the counter, so advancing it is safe.
-        __threadfence();
-        if (threadIdx.x == 0) {
-            const uint32_t prev = atomicAdd(&a.ctl->done, 1u);
-            s_last = (prev == gridDim.x - 1);
-        }
-        __syncthreads();
-        if (s_last) {
-            __threadfence();
-            close_sweep(a, t, meas, exch, s);
-        }
-    }
+}
+
+// Closes a sweep: records, replica exchange, sweep counter.  One block,
+// launched after the colour-1 half-sweep (a kernel boundary orders it after
+// every block's accumulator update).
+__global__ __launch_bounds__(kThreads) void ising_close_kernel(IsingArgs a) {
+    const int64_t t = a.ctl->t;
+    bool meas, exch;
+    int64_t s;
+    sweep_flags(a.ctl, t, &meas, &exch, &s);
+    close_sweep(a, t, meas, exch, s);
 }
 
 // Hot start: i.i.d. +-1 with p = 1/2 from Philox(site, 0, stream, init tag).
@@ -330,6 +325,11 @@ namespace ising {
 
 hipError_t launch_ising_half(const IsingArgs& a, int nblocks, int color, hipStream_t st) {
     hipLaunchKernelGGL(ising_half_kernel, dim3(nblocks), dim3(kThreads), 0, st, a, color);
+    return hipGetLastError();
+}
+
+hipError_t launch_ising_close(const IsingArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(ising_close_kernel, dim3(1), dim3(kThreads), 0, st, a);
     return hipGetLastError();
 }
 

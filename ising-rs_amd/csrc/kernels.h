@@ -22,6 +22,44 @@ struct IsingArgs {
     uint32_t thr_inline[28];
 };
 
+struct ContArgs {
+    const DevGroup* groups;
+    int32_t ngroups;
+    int32_t total_reps;
+    int32_t ncomp;         // 2 = XY, 3 = Heisenberg
+    uint32_t model;        // MODEL_XY or MODEL_HEIS (random-stream tag)
+    float* pool;
+    float* deltas;         // [rep0 + tidx] proposal size (radians), adapted during warm-up
+    const float* betas_f;  // [rep0 + tidx] beta for the Metropolis test
+    const double* betas;   // [rep0 + tidx] beta for the exchange test
+    int32_t* r2t;
+    int32_t* t2r;
+    DevCtl* ctl;
+    double* part0;         // [block][4] colour-0 partials (M)
+    double* part1;         // [block][4] colour-1 partials (E, M)
+    double* dacc;          // [global replica][4] E, Mx, My, Mz of the last measured sweep
+    int64_t* acc_cnt;      // [global replica] accepted Metropolis moves (warm-up adaptation)
+    ContRec* rec;
+    uint32_t k0, k1;
+    double J;
+    float Jf;
+    int32_t renorm_every;
+    int32_t adapt_every;   // 0: proposal size fixed
+    float adapt_target;    // target acceptance rate
+    int32_t inline_params; // 1: beta/delta from the two fields below (single-lattice passes)
+    float beta_inline, delta_inline;
+};
+
+hipError_t launch_ising_close(const IsingArgs& a, hipStream_t st);
+hipError_t launch_cont_metro(const ContArgs& a, int nblocks, int color, int closes, hipStream_t st);
+hipError_t launch_cont_close(const ContArgs& a, hipStream_t st);
+hipError_t launch_cont_or(const ContArgs& a, int nblocks, int color, hipStream_t st);
+hipError_t launch_cont_init(const DevGroup* groups, int ngroups, float* pool, int ncomp, int model, int nblocks,
+                            uint32_t k0, uint32_t k1, hipStream_t st);
+hipError_t launch_cont_measure(const DevGroup* groups, int ngroups, const float* pool, int ncomp, double J,
+                               double* part, int nblocks, hipStream_t st);
+constexpr int kContTile = 256;  // half-sites per block in continuous launches
+
 hipError_t launch_ising_half(const IsingArgs& a, int nblocks, int color, hipStream_t st);
 hipError_t launch_ising_init(const DevGroup* groups, int ngroups, uint8_t* pool, int nblocks, uint32_t k0,
                              uint32_t k1, hipStream_t st);

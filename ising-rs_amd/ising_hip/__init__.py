@@ -40,7 +40,7 @@ class IsingError(RuntimeError):
 class Opts(C.Structure):
     _fields_ = [("j", C.c_double), ("h", C.c_double), ("exchange_every", C.c_int32), ("measure_every", C.c_int32),
                 ("delta", C.c_double), ("n_overrelax", C.c_int32), ("renorm_every", C.c_int32),
-                ("record_capacity", C.c_int32), ("reserved", C.c_int32)]
+                ("record_capacity", C.c_int32), ("adapt_every", C.c_int32), ("adapt_target", C.c_double)]
 
 
 class Observables(C.Structure):
@@ -70,6 +70,16 @@ _SIGS = {
                                         C.POINTER(Observables)]),
     "ising_lattice_domain_wall_density": (C.c_int, [_vp, _f64p]),
     "ising_lattice_sweeps": (C.c_int, [_vp, _i64p]),
+    "ising_cont_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_uint64]),
+    "ising_cont_destroy": (None, [_vp]),
+    "ising_cont_sweep": (C.c_int, [_vp, C.c_float, C.c_float, C.c_float, C.c_size_t]),
+    "ising_cont_measure_gpu": (C.c_int, [_vp, C.c_float, _f64p, _f64p, _f64p, _f64p]),
+    "ising_cont_measure_raw": (C.c_int, [_vp, _f64p, _f64p, _f64p, _f64p]),
+    "ising_cont_get_spins": (C.c_int, [_vp, _vp]),
+    "ising_cont_set_spins": (C.c_int, [_vp, _vp]),
+    "ising_cont_metropolis_pass": (C.c_int, [_vp, C.c_int, C.c_float, C.c_float, C.c_float]),
+    "ising_cont_overrelax_pass": (C.c_int, [_vp, C.c_int, C.c_float]),
+    "ising_cont_sweeps": (C.c_int, [_vp, _i64p]),
     "ising_pt_default_opts": (None, [C.POINTER(Opts)]),
     "ising_pt_create": (C.c_int, [C.POINTER(_vp), C.c_int, _vp, C.c_int32, C.c_int32, _vp, C.c_uint64,
                                   C.POINTER(Opts)]),
@@ -86,6 +96,7 @@ _SIGS = {
     "ising_pt_set_spins_i8": (C.c_int, [_vp, C.c_int32, C.c_int32, _vp]),
     "ising_pt_get_spins_f32": (C.c_int, [_vp, C.c_int32, C.c_int32, _vp]),
     "ising_pt_set_spins_f32": (C.c_int, [_vp, C.c_int32, C.c_int32, _vp]),
+    "ising_pt_get_deltas": (C.c_int, [_vp, C.c_int32, _vp]),
     "ising_pt_sweeps": (C.c_int, [_vp, _i64p]),
     "ising_pt_device_bytes": (C.c_int, [_vp, _i64p]),
     "ising_linspace_temperatures": (C.c_int, [C.c_double, C.c_double, C.c_int32, _vp]),
@@ -178,6 +189,59 @@ class LatticeGpu:
     def sweeps(self):
         v = C.c_int64()
         _check(lib.ising_lattice_sweeps(self._h, C.byref(v)))
+        return v.value
+
+
+class ContinuousGpuLattice:
+    """One XY (n_comp=2) or Heisenberg (n_comp=3) lattice on the GPU
+    (mirror of ContinuousGpuLattice, gpu_lattice_continuous.rs:9-250)."""
+
+    def __init__(self, n, n_comp, seed):
+        self._h = _vp()
+        _check(lib.ising_cont_create(C.byref(self._h), int(n), int(n_comp), int(seed) & (2 ** 64 - 1)))
+        self.n = int(n)
+        self.n_comp = int(n_comp)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.ising_cont_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def sweep(self, beta, j, delta, n_overrelax):
+        _check(lib.ising_cont_sweep(self._h, beta, j, delta, int(n_overrelax)))
+
+    def measure_gpu(self, j=1.0):
+        v = [C.c_double() for _ in range(4)]
+        _check(lib.ising_cont_measure_gpu(self._h, j, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def measure_raw(self):
+        v = [C.c_double() for _ in range(4)]
+        _check(lib.ising_cont_measure_raw(self._h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def get_spins(self):
+        out = np.empty(self.n ** 3 * self.n_comp, np.float32)
+        _check(lib.ising_cont_get_spins(self._h, _ptr(out)))
+        return out
+
+    def set_spins(self, spins):
+        a = np.ascontiguousarray(spins, np.float32).reshape(-1)
+        if a.size != self.n ** 3 * self.n_comp:
+            raise ValueError("need n^3 * n_comp components")
+        _check(lib.ising_cont_set_spins(self._h, _ptr(a)))
+
+    def metropolis_pass(self, color, beta, j, delta):
+        _check(lib.ising_cont_metropolis_pass(self._h, int(color), beta, j, delta))
+
+    def overrelax_pass(self, color, j=1.0):
+        _check(lib.ising_cont_overrelax_pass(self._h, int(color), j))
+
+    def sweeps(self):
+        v = C.c_int64()
+        _check(lib.ising_cont_sweeps(self._h, C.byref(v)))
         return v.value
 
 
@@ -294,6 +358,11 @@ class PTEnsemble:
         else:
             a = np.ascontiguousarray(spins, np.float32).reshape(-1)
             _check(lib.ising_pt_set_spins_f32(self._h, size_index, replica, _ptr(a)))
+
+    def get_deltas(self, size_index=0):
+        out = np.empty(self.R, np.float32)
+        _check(lib.ising_pt_get_deltas(self._h, size_index, _ptr(out)))
+        return out
 
     def sweeps(self):
         v = C.c_int64()

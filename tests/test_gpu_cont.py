@@ -1,0 +1,163 @@
+"""GPU parity of the XY / Heisenberg path against the FP64 oracle replay.
+
+FP32 on the device vs FP64 on the host: one pass from an identical input state
+is compared per site (local energy to 1e-5 relative); sites whose Metropolis
+uniform lies within 1e-4 of the acceptance probability are masked as
+near-threshold and counted.  Reductions are FP64 and must match the oracle's
+FP64 sums over the same FP32 state to 1e-12.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as o
+
+pytestmark = pytest.mark.gpu
+ising_hip = pytest.importorskip("ising_hip")
+
+MODEL = {2: "xy", 3: "heisenberg"}
+
+
+def o_init(L, nc, seed, replica=0):
+    sp = np.zeros(L ** 3 * nc)
+    o.lib.o_cont_init(L, nc, seed, o.lib.o_stream(L, replica), sp)
+    return sp
+
+
+def o_measure(L, nc, sp, J=1.0):
+    e = C.c_double()
+    m = np.zeros(3)
+    o.lib.o_cont_measure(L, nc, np.ascontiguousarray(sp, np.float64), J, C.byref(e), m)
+    return e.value, m
+
+
+def site_energy(L, nc, sp):
+    out = np.zeros(L ** 3)
+    o.lib.o_cont_site_energy(L, nc, np.ascontiguousarray(sp, np.float64), 1.0, out)
+    return out
+
+
+def colour_sites(L, c):
+    s = np.arange(L ** 3)
+    x, y, z = s % L, (s // L) % L, s // (L * L)
+    return s[((x + y + z) & 1) == c]
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+@pytest.mark.parametrize("L", [4, 6, 16, 32])
+def test_init_matches_oracle(nc, L):
+    lat = ising_hip.ContinuousGpuLattice(L, nc, 42)
+    got = lat.get_spins().astype(np.float64)
+    assert np.max(np.abs(got - o_init(L, nc, 42))) < 2e-6
+
+
+@pytest.mark.parametrize("nc,beta,delta", [(3, 1.0 / 1.44, 0.9), (3, 0.3, 2.5), (2, 1.0 / 2.2, 1.2), (2, 1.5, 0.4)])
+@pytest.mark.parametrize("L", [8, 16])
+def test_one_metropolis_pass_matches_fp64_replay(nc, beta, delta, L):
+    seed = 77
+    lat = ising_hip.ContinuousGpuLattice(L, nc, seed)
+    # move away from the random start so the acceptance is non-trivial
+    for _ in range(3):
+        lat.sweep(beta, 1.0, delta, 1)
+    t = lat.sweeps()
+    for c in (0, 1):
+        before = lat.get_spins().astype(np.float64)
+        lat.metropolis_pass(c, beta, 1.0, delta)
+        got = lat.get_spins().astype(np.float64)
+        ref = before.copy()
+        u = np.zeros(L ** 3 // 2)
+        p = np.zeros(L ** 3 // 2)
+        o.lib.o_cont_metropolis_half(L, nc, ref, c, t, seed, o.lib.o_stream(L, 0), float(np.float32(beta)), 1.0,
+                                     float(np.float32(delta)), 0, u.ctypes.data, p.ctypes.data)
+        sites = colour_sites(L, c)
+        near = np.abs(u - p) < 1e-4
+        ok = sites[~near]
+        e_got = site_energy(L, nc, got)[ok]
+        e_ref = site_energy(L, nc, ref)[ok]
+        assert np.all(np.abs(e_got - e_ref) <= 1e-5 * np.maximum(np.abs(e_ref), 1.0)), \
+            np.max(np.abs(e_got - e_ref))
+        # the untouched colour is unchanged, and the moved colour agrees component-wise
+        other = colour_sites(L, 1 - c)
+        g3, r3 = got.reshape(-1, nc), ref.reshape(-1, nc)
+        assert np.array_equal(g3[other], before.reshape(-1, nc)[other])
+        assert np.max(np.abs(g3[ok] - r3[ok])) < 1e-5
+        assert near.sum() <= max(2, 0.01 * len(sites))
+        # lattice carried forward from the GPU state
+        lat.set_spins(got.astype(np.float32))
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_one_overrelaxation_pass_matches_fp64_replay(nc):
+    L = 16
+    lat = ising_hip.ContinuousGpuLattice(L, nc, 5)
+    for c in (0, 1):
+        before = lat.get_spins().astype(np.float64)
+        lat.overrelax_pass(c, 1.0)
+        got = lat.get_spins().astype(np.float64)
+        ref = before.copy()
+        o.lib.o_cont_overrelax_half(L, nc, ref, c, 1.0)
+        assert np.max(np.abs(got - ref)) < 1e-5
+        e0, _ = o_measure(L, nc, before)
+        e1, _ = o_measure(L, nc, got)
+        assert abs(e1 - e0) < 1e-5 * abs(e0)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_measure_gpu_matches_fp64_sums(nc):
+    L = 16
+    lat = ising_hip.ContinuousGpuLattice(L, nc, 9)
+    for _ in range(4):
+        lat.sweep(0.6, 1.0, 0.8, 2)
+    sp = lat.get_spins().astype(np.float64)
+    e, m = o_measure(L, nc, sp, 1.0)
+    got = lat.measure_gpu(1.0)
+    assert abs(got[0] - e) < 1e-12 * abs(e)
+    for k in range(nc):
+        assert abs(got[1 + k] - m[k]) < 1e-9
+    eh = lat.measure_gpu(0.5)
+    assert abs(eh[0] - 0.5 * e) < 1e-12 * abs(e)
+    assert lat.measure_raw() == got
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_pt_ensemble_records_and_renormalisation(nc):
+    sizes, R = [8, 16], 8
+    temps = np.linspace(1.40, 1.50, R) if nc == 3 else np.linspace(2.15, 2.25, R)
+    pt = ising_hip.PTEnsemble(MODEL[nc], sizes, R, temps, 42, exchange_every=2, measure_every=1, n_overrelax=2,
+                              renorm_every=3, delta=0.8)
+    pt.warmup(5)
+    pt.sample(9)  # last sample sweep is t=13; (13+1) % 7 ... renormalised at t+1 = 3,6,9,12
+    tidx, e, mx, my, mz = pt.read_cont()
+    assert tidx.shape == (9, 2, R)
+    for k in range(9):
+        for g in range(2):
+            assert sorted(tidx[k, g]) == list(range(R))
+    for g, L in enumerate(sizes):
+        for r in (0, R - 1):
+            sp = pt.get_spins(g, r).astype(np.float64)
+            ee, mm = o_measure(L, nc, sp)
+            assert abs(e[-1, g, r] - ee) < 1e-10 * abs(ee)
+            assert abs(mx[-1, g, r] - mm[0]) < 1e-9 and abs(my[-1, g, r] - mm[1]) < 1e-9
+            if nc == 3:
+                assert abs(mz[-1, g, r] - mm[2]) < 1e-9
+            norms = (sp.reshape(-1, nc) ** 2).sum(axis=1)
+            assert np.max(np.abs(norms - 1.0)) < 1e-5
+        r2t, t2r = pt.get_perm(g)
+        assert np.array_equal(t2r[r2t], np.arange(R))
+    tries, acc = pt.exchange_stats()
+    assert tries > 0
+
+
+def test_delta_adaptation_runs_in_warmup_only():
+    R = 4
+    temps = np.linspace(1.40, 1.50, R)
+    pt = ising_hip.PTEnsemble("heisenberg", [16], R, temps, 3, delta=3.0, adapt_every=10, adapt_target=0.5,
+                              n_overrelax=0, exchange_every=1000)
+    assert np.all(pt.get_deltas(0) == np.float32(3.0))
+    pt.warmup(200)
+    d = pt.get_deltas(0)
+    # near T_c a 3-radian cap accepts far less than half the moves, so the cap shrinks
+    assert np.all(d < 2.5) and np.all(d > 0.05)
+    pt.sample(50)
+    assert np.array_equal(pt.get_deltas(0), d)  # frozen while sampling
