@@ -52,6 +52,24 @@ def load_traffic():
         return None
 
 
+def copy_calibration(nbytes):
+    """Runs the HIP runtime's device-to-device copy kernel on `nbytes` (a plain
+    copy of known size) so PMC byte counters can be scaled: FETCH_SIZE should
+    read nbytes and WRITE_SIZE nbytes for one such dispatch."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "dispatches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,7 +77,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (PMC byte-counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
               file=sys.stderr)

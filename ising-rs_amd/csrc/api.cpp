@@ -132,6 +132,7 @@ struct ising_pt {
         a.groups = d_groups;
         a.ngroups = (int32_t)groups.size();
         a.total_reps = total_reps;
+        a.nblocks = nblocks;
         a.pool = d_pool;
         a.thr = d_thr;
         a.r2t = d_r2t;
@@ -274,6 +275,8 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
         G.Lh = sizes[i] / 2;
         G.R = R;
         G.fast = (sizes[i] % 16) == 0 ? 1 : 0;
+        if (G.Lh >= 2) fastdiv_magic((uint32_t)G.Lh, &G.mag_Lh, &G.sh_Lh);
+        fastdiv_magic((uint32_t)G.L, &G.mag_L, &G.sh_L);
         G.rep0 = i * R;
         G.n_half = (int64_t)sizes[i] * sizes[i] * sizes[i] / 2;
         G.stride = ((G.n_half * elem + 255) / 256) * 256 / elem;

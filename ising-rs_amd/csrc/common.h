@@ -104,7 +104,23 @@ struct DevGroup {
     int64_t n_half;    // L^3 / 2
     int64_t stride;    // elements between consecutive half-lattices (padded)
     int64_t off;       // element offset of replica 0 colour 0 in the pool
+    uint32_t mag_Lh, sh_Lh;  // n / Lh == fastdiv(n, mag_Lh, sh_Lh) for n < 2^32 (Lh >= 2)
+    uint32_t mag_L, sh_L;    // n / L  likewise
 };
+
+// Division by a run-time invariant d >= 2 of any 32-bit n: the round-up
+// method of Granlund & Montgomery (PLDI'94).  l = ceil(log2 d),
+// m = floor(2^32 (2^l - d) / d) + 1, q = (t + ((n - t) >> 1)) >> (l - 1), t = mulhi(m, n).
+__host__ __device__ inline void fastdiv_magic(uint32_t d, uint32_t* m, uint32_t* l) {
+    uint32_t k = 0;
+    while ((1ull << k) < d) ++k;
+    *l = k;
+    *m = (uint32_t)(((1ull << 32) * ((1ull << k) - d)) / d + 1);
+}
+__device__ __forceinline__ uint32_t fastdiv(uint32_t n, uint32_t m, uint32_t l) {
+    const uint32_t t = __umulhi(m, n);
+    return (t + ((n - t) >> 1)) >> (l - 1);
+}
 
 // Sweep control block, resident on the device.  Kernels read the sweep index
 // and cadence from here, so a captured launch sequence replays unchanged.

@@ -49,13 +49,27 @@ __device__ __forceinline__ void neighbours(const DevGroup& G, int64_t h, int col
 }
 
 template <int N>
-__device__ __forceinline__ void load_field(const float* O, int64_t stride, const int64_t nb[6], float hf[3]) {
-    hf[0] = hf[1] = hf[2] = 0.f;
+__device__ __forceinline__ void load_nbrs(const float* O, int64_t stride, const int64_t nb[6], float nv[3][6]) {
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         const float* Ok = O + k * stride;
-        hf[k] = ((Ok[nb[0]] + Ok[nb[1]]) + (Ok[nb[2]] + Ok[nb[3]])) + (Ok[nb[4]] + Ok[nb[5]]);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) nv[k][q] = Ok[nb[q]];
     }
+}
+
+template <int N>
+__device__ __forceinline__ void field_f32(const float nv[3][6], float hf[3]) {
+    hf[0] = hf[1] = hf[2] = 0.f;
+#pragma unroll
+    for (int k = 0; k < N; ++k) hf[k] = ((nv[k][0] + nv[k][1]) + (nv[k][2] + nv[k][3])) + (nv[k][4] + nv[k][5]);
+}
+
+template <int N>
+__device__ __forceinline__ void load_field(const float* O, int64_t stride, const int64_t nb[6], float hf[3]) {
+    float nv[3][6];
+    load_nbrs<N>(O, stride, nb, nv);
+    field_f32<N>(nv, hf);
 }
 
 // Block sum of up to 4 doubles; result valid in thread 0.
@@ -206,8 +220,10 @@ __global__ __launch_bounds__(kCT) void cont_metro_kernel(ContArgs a, int color, 
     if (h < G.n_half) {
         int64_t nb[6];
         neighbours(G, h, color, nb);
+        float nv[3][6];
+        load_nbrs<N>(O, G.stride, nb, nv);
         float hf[3];
-        load_field<N>(O, G.stride, nb, hf);
+        field_f32<N>(nv, hf);
         float S[3] = {0.f, 0.f, 0.f};
 #pragma unroll
         for (int k = 0; k < N; ++k) S[k] = C[k * G.stride + h];
@@ -262,11 +278,17 @@ __global__ __launch_bounds__(kCT) void cont_metro_kernel(ContArgs a, int color, 
             for (int k = 0; k < N; ++k) C[k * G.stride + h] = S[k];
         }
         if (need) {
+            // FP64 from the stored FP32 components (the field is re-summed in FP64)
             double e = 0.0;
 #pragma unroll
             for (int k = 0; k < N; ++k) {
                 part[1 + k] = (double)S[k];
-                e += (double)S[k] * (double)hf[k];
+                if (color) {
+                    double f = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) f += (double)nv[k][q];
+                    e += (double)S[k] * f;
+                }
             }
             part[0] = color ? -a.J * e : 0.0;
         }

@@ -102,14 +102,25 @@ __device__ void close_sweep(const IsingArgs& a, int64_t t, bool meas, bool exch,
     }
 }
 
+// Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// Remap so each XCD walks one contiguous range of tiles: the z-neighbour
+// planes a tile reads are then mostly in the same XCD's L2.
+__device__ __forceinline__ int xcd_block(int nblocks) {
+    const int per = (int)(gridDim.x >> 3);
+    return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+}
+
 // One colour half-sweep of every replica of every group in the ensemble.
-// Grid: one block per (group, replica, tile of 2048 half-sites).
+// Grid: one block per (group, replica, tile of 2048 half-sites), padded to a
+// multiple of 8 blocks.
 __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int color) {
     __shared__ uint32_t s_thr[28];
     __shared__ int64_t s_red[2][kThreads / 64];
-    const int g = find_group(a.groups, a.ngroups, blockIdx.x);
+    const int b = xcd_block(a.nblocks);
+    if (b >= a.nblocks) return;
+    const int g = find_group(a.groups, a.ngroups, b);
     const DevGroup G = a.groups[g];
-    const int local = blockIdx.x - G.blk0;
+    const int local = b - G.blk0;
     const int r = local / G.tiles;
     const int tile = local - r * G.tiles;
     const int grep = G.rep0 + r;
@@ -135,49 +146,69 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
     if (h0 < G.n_half) {
         const u32x4 w = philox((uint32_t)(h0 >> 3), hs, stream, tag(P_ISING, MODEL_ISING), a.k0, a.k1);
         if (G.fast) {
-            // 8 sites in one row: vector loads of the four off-row neighbours,
-            // the row itself and one edge byte for the shifted x-neighbour.
-            const int64_t row = h0 / Lh;
-            const int xh0 = (int)(h0 - row * Lh);
-            const int z = (int)(row / L), y = (int)(row - (int64_t)z * L);
-            const int p = (y + z + color) & 1;
-            const int ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
-            const int zm = z == 0 ? L - 1 : z - 1, zp = z == L - 1 ? 0 : z + 1;
-            uint64_t c8 = *(const uint64_t*)(C + h0);
-            const uint64_t xr = *(const uint64_t*)(O + h0);
-            const int64_t rb = row * Lh;
-            uint64_t xs;
-            if (p) {
-                const uint64_t e = O[rb + (xh0 + 8 == Lh ? 0 : xh0 + 8)];
-                xs = (xr >> 8) | (e << 56);
-            } else {
-                const uint64_t e = O[rb + (xh0 == 0 ? Lh - 1 : xh0 - 1)];
-                xs = (xr << 8) | e;
-            }
-            const uint64_t nym = *(const uint64_t*)(O + ((int64_t)z * L + ym) * Lh + xh0);
-            const uint64_t nyp = *(const uint64_t*)(O + ((int64_t)z * L + yp) * Lh + xh0);
-            const uint64_t nzm = *(const uint64_t*)(O + ((int64_t)zm * L + y) * Lh + xh0);
-            const uint64_t nzp = *(const uint64_t*)(O + ((int64_t)zp * L + y) * Lh + xh0);
+            // 8 sites in one row (Lh % 8 == 0): six 8-byte neighbour loads and one
+            // edge byte for the shifted x-neighbour.  All index math in 32 bits
+            // (L^3/2 < 2^32 for every L the library accepts).
+            const uint32_t hh = (uint32_t)h0;
+            const uint32_t row = fastdiv(hh, G.mag_Lh, G.sh_Lh);
+            const uint32_t xh0 = hh - row * (uint32_t)Lh;
+            const uint32_t z = fastdiv(row, G.mag_L, G.sh_L), y = row - z * (uint32_t)L;
+            const uint32_t p = (y + z + (uint32_t)color) & 1u;
+            const uint32_t ym = y == 0 ? L - 1 : y - 1, yp = y == (uint32_t)L - 1 ? 0 : y + 1;
+            const uint32_t zm = z == 0 ? L - 1 : z - 1, zp = z == (uint32_t)L - 1 ? 0 : z + 1;
+            const uint32_t rb = row * (uint32_t)Lh;
+            const uint32_t xe = p ? (xh0 + 8 == (uint32_t)Lh ? 0u : xh0 + 8) : (xh0 == 0 ? (uint32_t)Lh - 1 : xh0 - 1);
+            const uint64_t c8 = *(const uint64_t*)(C + hh);
+            const uint64_t xr = *(const uint64_t*)(O + hh);
+            const uint64_t e = O[rb + xe];
+            const uint64_t nym = *(const uint64_t*)(O + (z * (uint32_t)L + ym) * (uint32_t)Lh + xh0);
+            const uint64_t nyp = *(const uint64_t*)(O + (z * (uint32_t)L + yp) * (uint32_t)Lh + xh0);
+            const uint64_t nzm = *(const uint64_t*)(O + (zm * (uint32_t)L + y) * (uint32_t)Lh + xh0);
+            const uint64_t nzp = *(const uint64_t*)(O + (zp * (uint32_t)L + y) * (uint32_t)Lh + xh0);
+            const uint64_t xs = p ? ((xr >> 8) | (e << 56)) : ((xr << 8) | e);
             const uint64_t ns = xr + xs + nym + nyp + nzm + nzp;  // per-byte up-neighbour counts 0..6
-            uint64_t flip = 0;
+            // per-byte byte offset of the table entry k = 7 b + n (at most 52: no carries)
+            const uint64_t K4 = (c8 * 7ull + ns) << 2;
+            uint32_t flo = 0, fhi = 0;
+            bool anytie = false;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                const uint32_t b = (uint32_t)(c8 >> (8 * j)) & 1u;
-                const uint32_t n = (uint32_t)(ns >> (8 * j)) & 0xffu;
-                const uint32_t k = b * 7u + n;
-                const uint32_t thi = s_thr[k];
+                const uint32_t kk = (uint32_t)(K4 >> (8 * j)) & 0xffu;
+                const uint32_t thr = *(const uint32_t*)((const char*)s_thr + kk);
                 const uint32_t r16 = (word_of(w, j) >> (16 * (j & 1))) & 0xffffu;
-                bool acc = r16 < thi;
-                if (r16 == thi) acc = tie_bits((uint32_t)(h0 + j), hs, stream, a.k0, a.k1) < s_thr[14 + k];
-                flip |= (uint64_t)acc << (8 * j);
-                if (need && color) {
-                    const int sn = acc ? -(2 * (int)b - 1) : (2 * (int)b - 1);
-                    bsum += sn * (2 * (int)n - 6);
+                const uint32_t acc = (r16 - thr) >> 31;  // r16 < thr (both below 2^17)
+                if (j < 4) flo |= acc << (8 * j);
+                else fhi |= acc << (8 * (j - 4));
+                anytie |= (r16 == thr);
+            }
+            if (anytie) {
+                // rare (p = 2^-16 per site): decide tied sites with 16 more bits
+#pragma unroll 1
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t kk = (uint32_t)(K4 >> (8 * j)) & 0xffu;
+                    const uint32_t thr = *(const uint32_t*)((const char*)s_thr + kk);
+                    const uint32_t r16 = (word_of(w, j) >> (16 * (j & 1))) & 0xffffu;
+                    if (r16 == thr) {
+                        const uint32_t lo = *(const uint32_t*)((const char*)s_thr + 56 + kk);
+                        const uint32_t acc = tie_bits(hh + (uint32_t)j, hs, stream, a.k0, a.k1) < lo ? 1u : 0u;
+                        if (j < 4) flo = (flo & ~(1u << (8 * j))) | (acc << (8 * j));
+                        else fhi = (fhi & ~(1u << (8 * (j - 4)))) | (acc << (8 * (j - 4)));
+                    }
                 }
             }
-            c8 ^= flip;
-            *(uint64_t*)(C + h0) = c8;
-            if (need) msum = 2 * __builtin_popcountll(c8) - 8;
+            const uint64_t nc8 = c8 ^ (flo | ((uint64_t)fhi << 32));
+            *(uint64_t*)(C + hh) = nc8;
+            if (need) {
+                const int sb = __builtin_popcountll(nc8);
+                msum = 2 * sb - 8;
+                if (color) {
+                    // sum_j (2 b'_j - 1)(2 n_j - 6) = 4 sum b'n - 12 sum b' - 2 sum n + 48
+                    const uint64_t bn = (nc8 * 0xffull) & ns;
+                    const int sbn = (int)((bn * 0x0101010101010101ull) >> 56);
+                    const int sn = (int)((ns * 0x0101010101010101ull) >> 56);
+                    bsum = 4 * sbn - 12 * sb - 2 * sn + 48;
+                }
+            }
         } else {
             // Generic even L: per-site addressing, same random-number assignment.
 #pragma unroll 1
@@ -324,7 +355,8 @@ __global__ __launch_bounds__(kThreads) void ising_measure_kernel(const DevGroup*
 namespace ising {
 
 hipError_t launch_ising_half(const IsingArgs& a, int nblocks, int color, hipStream_t st) {
-    hipLaunchKernelGGL(ising_half_kernel, dim3(nblocks), dim3(kThreads), 0, st, a, color);
+    const int grid = (nblocks + 7) / 8 * 8;
+    hipLaunchKernelGGL(ising_half_kernel, dim3(grid), dim3(kThreads), 0, st, a, color);
     return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@ struct IsingArgs {
     const DevGroup* groups;
     int32_t ngroups;
     int32_t total_reps;
+    int32_t nblocks;      // work blocks of a half-sweep (the grid is padded to a multiple of 8)
     uint8_t* pool;
     const uint32_t* thr;  // [rep0 + tidx][28]: 14 high halves, then 14 low halves
     int32_t* r2t;         // replica -> temperature index (per group, global replica index)
