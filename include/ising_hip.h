@@ -142,11 +142,14 @@ typedef struct {
     int32_t adapt_every;     /* continuous: during warm-up, rescale each temperature's
                                 delta every k sweeps towards adapt_target (0 = fixed) */
     double adapt_target;     /* target Metropolis acceptance for the adaptation */
+    int32_t graph_sweeps;    /* sweeps per captured HIP graph replayed by warmup/sample
+                                (0 = launch every kernel directly) */
+    int32_t reserved;
 } ising_pt_opts_t;
 
 /* Defaults: j=1, h=0, exchange_every=10, measure_every=1, delta=0.5,
  * n_overrelax=5, renorm_every=100, record_capacity=4096, adapt_every=0,
- * adapt_target=0.5. */
+ * adapt_target=0.5, graph_sweeps=32. */
 void ising_pt_default_opts(ising_pt_opts_t* opts);
 
 /* model: ISING_MODEL_*.  sizes: n_sizes even L values.  temps: `replicas`
@@ -203,6 +206,12 @@ int ising_replica_exchange(const double* temps, const double* energies, int32_t 
  * M2_err, M4, M4_err, Cv, Cv_err, chi, chi_err. */
 int ising_jackknife(const double* e, const double* m, int64_t n, double beta, double n3, int32_t n_blocks,
                     double* out);
+
+/* Integrated autocorrelation time of a series by Sokal's automatic windowing:
+ * tau(W) = 1/2 + sum_{t=1..W} rho(t), W = smallest window with W >= c * tau(W)
+ * (c = 6 is customary).  Not in the reference (its plans name it); used to
+ * report independent samples per second.  *window receives W. */
+int ising_tau_int(const double* x, int64_t n, double c, double* tau, int64_t* window);
 
 #ifdef __cplusplus
 }

@@ -103,6 +103,7 @@ struct ising_pt {
     int64_t pending_host = 0;  // measured sweeps the device will have written
     bool inline_thr = false;   // single-lattice mode
     uint32_t thr_inline[28];
+    hipGraphExec_t graph_exec = nullptr;  // `opts.graph_sweeps` sweeps, captured once
 
     ~ising_pt() {
         if (st) (void)hipStreamSynchronize(st);
@@ -122,6 +123,7 @@ struct ising_pt {
         (void)hipFree(d_part1);
         (void)hipFree(d_dacc);
         (void)hipFree(d_cnt);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (st) (void)hipStreamDestroy(st);
@@ -179,6 +181,39 @@ struct ising_pt {
         a.beta_inline = beta_inline;
         a.delta_inline = delta_inline;
         return a;
+    }
+
+    // Sweeps through a captured graph where possible.  Kernel arguments of an
+    // ensemble never change after creation and every per-sweep decision is read
+    // from the device control block, so one captured block of sweeps replays
+    // unchanged.  Single-lattice handles pass per-call parameters by value and
+    // always launch directly.
+    int run_sweeps(int64_t n) {
+        const int G = opts.graph_sweeps;
+        if (G > 0 && !inline_thr && !inline_cont && n >= G) {
+            if (!graph_exec) {
+                hipGraph_t graph = nullptr;
+                HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                const int rc = enqueue_sweeps(G);
+                hipGraph_t captured = nullptr;
+                const hipError_t ec = hipStreamEndCapture(st, &captured);
+                t_host -= G;  // capture enqueued nothing
+                if (rc) {
+                    if (captured) (void)hipGraphDestroy(captured);
+                    return rc;
+                }
+                HIP_TRY(ec);
+                graph = captured;
+                const hipError_t ei = hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                HIP_TRY(ei);
+            }
+            const int64_t reps = n / G;
+            for (int64_t i = 0; i < reps; ++i) HIP_TRY(hipGraphLaunch(graph_exec, st));
+            t_host += reps * G;
+            n -= reps * G;
+        }
+        return n > 0 ? enqueue_sweeps(n) : ISING_OK;
     }
 
     int enqueue_sweeps(int64_t n, int n_or = -1) {
@@ -244,6 +279,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     else ising_pt_default_opts(&o);
     if (o.exchange_every < 1 || o.measure_every < 1) return fail(ISING_E_ARG, "exchange_every and measure_every must be >= 1");
     if (o.record_capacity < 1) return fail(ISING_E_ARG, "record_capacity must be >= 1");
+    if (o.graph_sweeps < 0 || o.graph_sweeps > 100000) return fail(ISING_E_ARG, "graph_sweeps must be in 0..100000");
     if (model != ISING_MODEL_ISING) {
         if (!(o.delta > 0.0) || !std::isfinite(o.delta)) return fail(ISING_E_ARG, "delta must be > 0");
         if (o.n_overrelax < 0 || o.renorm_every < 0 || o.adapt_every < 0)
@@ -390,6 +426,8 @@ void ising_pt_default_opts(ising_pt_opts_t* o) {
     o->record_capacity = 4096;
     o->adapt_every = 0;
     o->adapt_target = 0.5;
+    o->graph_sweeps = 32;
+    o->reserved = 0;
 }
 
 int ising_pt_create(ising_pt_t** out, int model, const int32_t* sizes, int32_t n_sizes, int32_t replicas,
@@ -416,7 +454,7 @@ int ising_pt_warmup(ising_pt_t* p, int64_t sweeps) {
     if (sweeps < 0) return fail(ISING_E_ARG, "sweeps must be >= 0");
     if (p->s_host >= 0) return fail(ISING_E_STATE, "warm-up after sampling started");
     HIP_TRY(hipEventRecord(p->ev0, p->st));
-    const int rc = p->enqueue_sweeps(sweeps);
+    const int rc = p->run_sweeps(sweeps);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(p->ev1, p->st));
     p->timed = true;
@@ -437,7 +475,7 @@ int ising_pt_sample(ising_pt_t* p, int64_t sweeps) {
     if (p->pending_host + add > p->opts.record_capacity)
         return fail(ISING_E_FULL, "measurement buffer would overflow: drain records before sampling more");
     HIP_TRY(hipEventRecord(p->ev0, p->st));
-    const int rc = p->enqueue_sweeps(sweeps);
+    const int rc = p->run_sweeps(sweeps);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(p->ev1, p->st));
     p->timed = true;
@@ -996,6 +1034,34 @@ int ising_jackknife(const double* e, const double* m, int64_t n, double beta, do
         out[2 * k] = full[k];
         out[2 * k + 1] = std::sqrt(var[k] * (nb - 1.0) / nb);
     }
+    return ISING_OK;
+}
+
+int ising_tau_int(const double* x, int64_t n, double c, double* tau, int64_t* window) {
+    if (!x || !tau || n < 2 || !(c > 0.0)) return fail(ISING_E_ARG, "need n >= 2 samples and c > 0");
+    double mean = 0.0;
+    for (int64_t i = 0; i < n; ++i) mean += x[i];
+    mean /= (double)n;
+    double c0 = 0.0;
+    for (int64_t i = 0; i < n; ++i) c0 += (x[i] - mean) * (x[i] - mean);
+    c0 /= (double)n;
+    if (!(c0 > 0.0)) {
+        *tau = 0.5;
+        if (window) *window = 0;
+        return ISING_OK;
+    }
+    double t = 0.5;
+    int64_t W = 0;
+    for (int64_t lag = 1; lag < n; ++lag) {
+        double ct = 0.0;
+        for (int64_t i = 0; i + lag < n; ++i) ct += (x[i] - mean) * (x[i + lag] - mean);
+        ct /= (double)n;
+        t += ct / c0;
+        W = lag;
+        if ((double)lag >= c * t) break;
+    }
+    *tau = t;
+    if (window) *window = W;
     return ISING_OK;
 }
 

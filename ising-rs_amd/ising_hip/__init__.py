@@ -40,7 +40,8 @@ class IsingError(RuntimeError):
 class Opts(C.Structure):
     _fields_ = [("j", C.c_double), ("h", C.c_double), ("exchange_every", C.c_int32), ("measure_every", C.c_int32),
                 ("delta", C.c_double), ("n_overrelax", C.c_int32), ("renorm_every", C.c_int32),
-                ("record_capacity", C.c_int32), ("adapt_every", C.c_int32), ("adapt_target", C.c_double)]
+                ("record_capacity", C.c_int32), ("adapt_every", C.c_int32), ("adapt_target", C.c_double),
+                ("graph_sweeps", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Observables(C.Structure):
@@ -102,6 +103,7 @@ _SIGS = {
     "ising_linspace_temperatures": (C.c_int, [C.c_double, C.c_double, C.c_int32, _vp]),
     "ising_replica_exchange": (C.c_int, [_vp, _vp, C.c_int32, _vp, _vp, _vp, C.c_int32, C.POINTER(C.c_int32)]),
     "ising_jackknife": (C.c_int, [_vp, _vp, C.c_int64, C.c_double, C.c_double, C.c_int32, _vp]),
+    "ising_tau_int": (C.c_int, [_vp, C.c_int64, C.c_double, _f64p, _i64p]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _f = getattr(lib, _name)
@@ -401,3 +403,11 @@ def jackknife_observables(e, m, beta, n3, n_blocks):
     out = np.empty(12, np.float64)
     _check(lib.ising_jackknife(_ptr(e), _ptr(m), len(e), beta, n3, int(n_blocks), _ptr(out)))
     return out
+
+
+def tau_int(x, c=6.0):
+    """Integrated autocorrelation time (Sokal automatic windowing) -> (tau, window)."""
+    a = np.ascontiguousarray(x, np.float64)
+    t, w = C.c_double(), C.c_int64()
+    _check(lib.ising_tau_int(_ptr(a), len(a), c, C.byref(t), C.byref(w)))
+    return t.value, w.value

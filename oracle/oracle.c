@@ -750,3 +750,27 @@ void o_ising_chain(int dim, int L, uint64_t seed, double beta, double J, double 
     }
     free(sp);
 }
+
+/* Sokal automatic windowing (test restatement): tau = 1/2 + sum_{t<=W} rho(t),
+ * W the first lag with W >= c tau(W). */
+double o_tau_int(const double* x, int64_t n, double c, int64_t* window) {
+    double mean = 0.0, c0 = 0.0, t = 0.5;
+    int64_t W = 0;
+    for (int64_t i = 0; i < n; ++i) mean += x[i];
+    mean /= (double)n;
+    for (int64_t i = 0; i < n; ++i) c0 += (x[i] - mean) * (x[i] - mean);
+    c0 /= (double)n;
+    if (!(c0 > 0.0)) {
+        if (window) *window = 0;
+        return 0.5;
+    }
+    for (int64_t lag = 1; lag < n; ++lag) {
+        double ct = 0.0;
+        for (int64_t i = 0; i + lag < n; ++i) ct += (x[i] - mean) * (x[i + lag] - mean);
+        t += ct / (double)n / c0;
+        W = lag;
+        if ((double)lag >= c * t) break;
+    }
+    if (window) *window = W;
+    return t;
+}

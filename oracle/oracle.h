@@ -145,6 +145,7 @@ int64_t o_ref_xy_wolff_sweep(int L, double* spins, double beta, double J, o_rng*
  * returns updates performed and wall seconds. */
 int64_t o_bench_ref_metropolis(int L, double beta, uint64_t seed, double seconds, double* wall);
 int64_t o_bench_ref_heis(int L, double beta, double delta, int n_or, uint64_t seed, double seconds, double* wall);
+double o_tau_int(const double* x, int64_t n, double c, int64_t* window);
 void o_ising_chain(int dim, int L, uint64_t seed, double beta, double J, double h, int64_t warmup,
                    int64_t n_sweeps, int64_t* out_bond, int64_t* out_mag);
 

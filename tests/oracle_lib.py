@@ -178,3 +178,12 @@ def ising_chain(dim, L, seed, beta, warmup, n, J=1.0, h=0.0):
     m = np.zeros(n, np.int64)
     lib.o_ising_chain(dim, L, seed, beta, J, h, warmup, n, b, m)
     return b, m
+
+lib.o_tau_int.restype = C.c_double
+lib.o_tau_int.argtypes = [_f64p, C.c_int64, C.c_double, C.POINTER(C.c_int64)]
+
+
+def tau_int(x, c=6.0):
+    w = C.c_int64()
+    t = lib.o_tau_int(np.ascontiguousarray(x, np.float64), len(x), c, C.byref(w))
+    return t, w.value

@@ -138,3 +138,24 @@ def test_full_size_config_properties():
         assert (B, int(s3.sum())) == (int(bond[-1, 0, r]), int(mag[-1, 0, r]))
     r2t, t2r = pt.get_perm(0)
     assert np.array_equal(t2r[r2t], np.arange(R))
+
+
+def test_graph_replays_match_direct_launches():
+    """Captured blocks of sweeps, replayed several times with eager sweeps and
+    host synchronisations in between, give the same records as direct launches."""
+    temps = o.linspace(4.4, 4.62, 6)
+    kw = dict(exchange_every=3, measure_every=2)
+    g = ising_hip.PTEnsemble("ising", [8, 16], 6, temps, 21, graph_sweeps=4, **kw)
+    d = ising_hip.PTEnsemble("ising", [8, 16], 6, temps, 21, graph_sweeps=0, **kw)
+    out_g, out_d = [], []
+    for p, out in ((g, out_g), (d, out_d)):
+        p.warmup(9)   # 2 replays + 1 direct
+        for n in (8, 3, 12, 1, 4):
+            p.sample(n)
+            p.sync()
+            out.append(p.read_ising())
+    for a, b in zip(out_g, out_d):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+    rt, rb, rm, _ = o.ising_pt_run(16, 6, temps, 21, 9, 28, 3, 2)
+    assert np.array_equal(np.concatenate([x[1][:, 1, :].reshape(-1) for x in out_g]), rb)

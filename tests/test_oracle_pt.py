@@ -116,3 +116,28 @@ def test_pt_run_records_layout():
     assert np.all(np.abs(mag) <= 64) and np.all(np.abs(bond) <= 192)
     for r in range(4):
         assert o.ising_measure(4, final[r].copy())[0] % 2 == 0
+
+
+@pytest.mark.parametrize("rho", [0.0, 0.5, 0.9])
+def test_tau_int_of_ar1_matches_closed_form(rho):
+    # AR(1): x_t = rho x_{t-1} + e_t has rho(t) = rho^t and tau_int = (1 + rho) / (2 (1 - rho))
+    rng = np.random.default_rng(1)
+    n = 200000
+    e = rng.normal(size=n)
+    x = np.empty(n)
+    x[0] = e[0] / math.sqrt(1 - rho * rho)
+    for i in range(1, n):
+        x[i] = rho * x[i - 1] + e[i]
+    tau, w = o.tau_int(x)
+    exact = (1 + rho) / (2 * (1 - rho))
+    # Sokal: relative error ~ sqrt(2 (2W + 1) / n)
+    assert abs(tau - exact) < 4 * exact * math.sqrt(2 * (2 * w + 1) / n) + 0.02
+    assert w >= 6 * tau - 1
+
+
+def test_tau_int_library_matches_oracle():
+    import ising_hip
+    rng = np.random.default_rng(2)
+    x = np.cumsum(rng.normal(size=5000)) * 0.01 + rng.normal(size=5000)
+    assert ising_hip.tau_int(x) == o.tau_int(x)
+    assert ising_hip.tau_int(np.ones(10)) == (0.5, 0)
