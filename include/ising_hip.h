@@ -40,6 +40,10 @@ extern "C" {
 #define ISING_MODEL_ISING 0
 #define ISING_MODEL_XY 1
 #define ISING_MODEL_HEISENBERG 2
+/* 3D Ising with the replicas of a size packed across the bits of a 32-bit
+ * word per site (multispin coding, 1 bit per spin).  Requires h = 0, J > 0.
+ * Same records and spin API as ISING_MODEL_ISING; its own random stream. */
+#define ISING_MODEL_ISING_MSC 3
 
 /* Message for the last failing call on this thread ("" if none). */
 const char* ising_last_error(void);

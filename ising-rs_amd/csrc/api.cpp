@@ -82,6 +82,7 @@ struct ising_pt {
     uint8_t* d_pool = nullptr;
     int64_t pool_bytes = 0;
     uint32_t* d_thr = nullptr;
+    uint32_t* d_mthr = nullptr;  // multispin coding thresholds
     int32_t* d_r2t = nullptr;
     int32_t* d_t2r = nullptr;
     double* d_betas = nullptr;
@@ -110,6 +111,7 @@ struct ising_pt {
         (void)hipFree(d_groups);
         (void)hipFree(d_pool);
         (void)hipFree(d_thr);
+        (void)hipFree(d_mthr);
         (void)hipFree(d_r2t);
         (void)hipFree(d_t2r);
         (void)hipFree(d_betas);
@@ -148,6 +150,8 @@ struct ising_pt {
         a.J = opts.j;
         a.inline_thr = inline_thr ? 1 : 0;
         if (inline_thr) std::memcpy(a.thr_inline, thr_inline, sizeof(thr_inline));
+        a.mthr = d_mthr;
+        a.acc_copies = model == ISING_MODEL_ISING_MSC ? kMscAccCopies : 1;
         return a;
     }
 
@@ -224,6 +228,13 @@ struct ising_pt {
                 HIP_TRY(launch_ising_half(a, nblocks, 1, st));
                 HIP_TRY(launch_ising_close(a, st));
             }
+        } else if (model == ISING_MODEL_ISING_MSC) {
+            const IsingArgs a = ising_args();
+            for (int64_t i = 0; i < n; ++i) {
+                HIP_TRY(launch_msc_half(a, nblocks, 0, st));
+                HIP_TRY(launch_msc_half(a, nblocks, 1, st));
+                HIP_TRY(launch_ising_close(a, st));
+            }
         } else {
             // overrelaxation passes first, then one Metropolis sweep
             // (the order of ContinuousGpuLattice::sweep, gpu_lattice_continuous.rs:124-168)
@@ -266,8 +277,9 @@ int64_t measured_in(int64_t s0, int64_t n, int every) {
 
 int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, int R, const double* temps,
                    uint64_t seed, const ising_pt_opts_t* opts) {
-    if (model != ISING_MODEL_ISING && model != ISING_MODEL_XY && model != ISING_MODEL_HEISENBERG)
-        return fail(ISING_E_ARG, "model must be ISING_MODEL_ISING, _XY or _HEISENBERG");
+    if (model != ISING_MODEL_ISING && model != ISING_MODEL_XY && model != ISING_MODEL_HEISENBERG &&
+        model != ISING_MODEL_ISING_MSC)
+        return fail(ISING_E_ARG, "model must be ISING_MODEL_ISING, _ISING_MSC, _XY or _HEISENBERG");
     if (n_sizes < 1 || n_sizes > 64 || !sizes) return fail(ISING_E_ARG, "need 1..64 lattice sizes");
     if (R < 1 || R > 65535 || !temps) return fail(ISING_E_ARG, "need 1..65535 replicas and their temperatures");
     for (int i = 0; i < n_sizes; ++i)
@@ -280,7 +292,9 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     if (o.exchange_every < 1 || o.measure_every < 1) return fail(ISING_E_ARG, "exchange_every and measure_every must be >= 1");
     if (o.record_capacity < 1) return fail(ISING_E_ARG, "record_capacity must be >= 1");
     if (o.graph_sweeps < 0 || o.graph_sweeps > 100000) return fail(ISING_E_ARG, "graph_sweeps must be in 0..100000");
-    if (model != ISING_MODEL_ISING) {
+    if (model == ISING_MODEL_ISING_MSC && !(o.h == 0.0 && o.j > 0.0))
+        return fail(ISING_E_ARG, "multispin coding needs h = 0 and J > 0");
+    if (model == ISING_MODEL_XY || model == ISING_MODEL_HEISENBERG) {
         if (!(o.delta > 0.0) || !std::isfinite(o.delta)) return fail(ISING_E_ARG, "delta must be > 0");
         if (o.n_overrelax < 0 || o.renorm_every < 0 || o.adapt_every < 0)
             return fail(ISING_E_ARG, "n_overrelax, renorm_every and adapt_every must be >= 0");
@@ -292,7 +306,8 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(ISING_E_HIP, "no HIP device visible");
 
     p->model = model;
-    p->ncomp = model == ISING_MODEL_ISING ? 0 : (model == ISING_MODEL_XY ? 2 : 3);
+    const bool msc = model == ISING_MODEL_ISING_MSC;
+    p->ncomp = (model == ISING_MODEL_ISING || msc) ? 0 : (model == ISING_MODEL_XY ? 2 : 3);
     p->R = R;
     p->seed = seed;
     p->opts = o;
@@ -300,9 +315,11 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     p->total_reps = n_sizes * R;
 
     const bool ising = model == ISING_MODEL_ISING;
-    const int64_t ncomp = ising ? 1 : p->ncomp;   // arrays per colour half-lattice
+    const int64_t words = (R + 31) / 32;          // multispin coding: words per site
+    const int64_t ncomp = ising ? 1 : (msc ? words : p->ncomp);  // arrays per colour half-lattice
     const int64_t elem = ising ? 1 : 4;           // bytes per element
-    const int64_t tile = ising ? kIsingTile : kContTile;
+    const int64_t tile = ising ? kIsingTile : (msc ? kMscTile : kContTile);
+    const int64_t units = msc ? words : R;        // work units per group: replicas or replica words
     int64_t off = 0;
     int blk = 0;
     for (int i = 0; i < n_sizes; ++i) {
@@ -319,9 +336,9 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
         G.off = off;
         G.tiles = (int32_t)((G.n_half + tile - 1) / tile);
         G.blk0 = blk;
-        if ((int64_t)blk + (int64_t)G.tiles * R > INT32_MAX / 2) return fail(ISING_E_ARG, "ensemble too large");
-        blk += G.tiles * R;
-        off += 2 * (int64_t)R * ncomp * G.stride;
+        if ((int64_t)blk + (int64_t)G.tiles * units > INT32_MAX / 2) return fail(ISING_E_ARG, "ensemble too large");
+        blk += G.tiles * (int32_t)units;
+        off += 2 * (msc ? words : (int64_t)R * ncomp) * G.stride;
         p->groups.push_back(G);
     }
     p->nblocks = blk;
@@ -331,17 +348,18 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     HIP_TRY(hipEventCreate(&p->ev0));
     HIP_TRY(hipEventCreate(&p->ev1));
     HIP_TRY(hipMalloc(&p->d_groups, sizeof(DevGroup) * p->groups.size()));
-    if (ising) HIP_TRY(hipMalloc(&p->d_pool, p->pool_bytes));
+    if (ising || msc) HIP_TRY(hipMalloc(&p->d_pool, p->pool_bytes));
     else HIP_TRY(hipMalloc(&p->d_fpool, p->pool_bytes));
+    if (msc) HIP_TRY(hipMalloc(&p->d_mthr, sizeof(uint32_t) * 3 * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_thr, sizeof(uint32_t) * 28 * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_r2t, sizeof(int32_t) * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_t2r, sizeof(int32_t) * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_betas, sizeof(double) * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_ctl, sizeof(DevCtl)));
-    HIP_TRY(hipMalloc(&p->d_acc, sizeof(int64_t) * 8 * p->total_reps));
-    HIP_TRY(hipMalloc(&p->d_rec, (ising ? sizeof(IsingRec) : sizeof(ContRec)) * (size_t)o.record_capacity *
-                                     p->total_reps));
-    if (!ising) {
+    HIP_TRY(hipMalloc(&p->d_acc, sizeof(int64_t) * 2 * kMscAccCopies * p->total_reps + 8 * sizeof(int64_t)));
+    HIP_TRY(hipMalloc(&p->d_rec, ((ising || msc) ? sizeof(IsingRec) : sizeof(ContRec)) *
+                                     (size_t)o.record_capacity * p->total_reps));
+    if (!ising && !msc) {
         HIP_TRY(hipMalloc(&p->d_deltas, sizeof(float) * p->total_reps));
         HIP_TRY(hipMalloc(&p->d_betas_f, sizeof(float) * p->total_reps));
         HIP_TRY(hipMalloc(&p->d_part0, sizeof(double) * 4 * (size_t)p->nblocks));
@@ -354,6 +372,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     std::vector<int32_t> ident(p->total_reps);
     std::vector<double> betas(p->total_reps);
     std::vector<float> betas_f(p->total_reps), deltas(p->total_reps);
+    std::vector<uint32_t> mthr(3 * (size_t)p->total_reps);
     for (int g = 0; g < n_sizes; ++g)
         for (int i = 0; i < R; ++i) {
             const double beta = 1.0 / temps[i];
@@ -362,6 +381,8 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
             deltas[g * R + i] = (float)o.delta;
             ident[g * R + i] = i;
             ising_table(beta, o.j, o.h, &thr[28 * (size_t)(g * R + i)]);
+            for (int k = 0; k < 3; ++k)  // multispin coding: T(a) for a = 4, 5, 6 aligned neighbours
+                mthr[3 * (size_t)(g * R + i) + k] = (uint32_t)std::ldexp(std::exp(-beta * o.j * (4.0 * (4 + k) - 12.0)), 32);
         }
     DevCtl ctl{};
     ctl.t = 0;
@@ -375,8 +396,13 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     HIP_TRY(hipMemcpy(p->d_t2r, ident.data(), sizeof(int32_t) * ident.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_betas, betas.data(), sizeof(double) * betas.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_ctl, &ctl, sizeof(DevCtl), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(p->d_acc, 0, sizeof(int64_t) * 8 * p->total_reps));
-    if (ising) {
+    HIP_TRY(hipMemset(p->d_acc, 0, sizeof(int64_t) * 2 * kMscAccCopies * p->total_reps + 8 * sizeof(int64_t)));
+    if (msc) {
+        HIP_TRY(hipMemcpy(p->d_mthr, mthr.data(), sizeof(uint32_t) * mthr.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemset(p->d_pool, 0, p->pool_bytes));
+        HIP_TRY(launch_msc_init(p->d_groups, (int)p->groups.size(), (uint32_t*)p->d_pool, p->nblocks, (uint32_t)seed,
+                                (uint32_t)(seed >> 32), p->st));
+    } else if (ising) {
         HIP_TRY(hipMemset(p->d_pool, 0, p->pool_bytes));
         HIP_TRY(launch_ising_init(p->d_groups, (int)p->groups.size(), p->d_pool, p->nblocks, (uint32_t)seed,
                                   (uint32_t)(seed >> 32), p->st));
@@ -526,7 +552,8 @@ static int drain(ising_pt* p, int64_t max_sweeps, size_t rec_size, void* host, i
 int ising_pt_read_ising(ising_pt_t* p, int64_t max_sweeps, int32_t* tidx, int64_t* bond, int64_t* mag,
                         int64_t* n_out) {
     if (!p || !n_out) return fail(ISING_E_ARG, "null argument");
-    if (p->model != ISING_MODEL_ISING) return fail(ISING_E_STATE, "not an Ising ensemble");
+    if (p->model != ISING_MODEL_ISING && p->model != ISING_MODEL_ISING_MSC)
+        return fail(ISING_E_STATE, "not an Ising ensemble");
     std::vector<IsingRec> buf((size_t)std::max<int64_t>(0, max_sweeps) * p->total_reps);
     int64_t n = 0;
     const int rc = drain(p, max_sweeps, sizeof(IsingRec), buf.data(), &n);
@@ -543,7 +570,8 @@ int ising_pt_read_ising(ising_pt_t* p, int64_t max_sweeps, int32_t* tidx, int64_
 int ising_pt_read_cont(ising_pt_t* p, int64_t max_sweeps, int32_t* tidx, double* e, double* mx, double* my,
                        double* mz, int64_t* n_out) {
     if (!p || !n_out) return fail(ISING_E_ARG, "null argument");
-    if (p->model == ISING_MODEL_ISING) return fail(ISING_E_STATE, "not a continuous-spin ensemble");
+    if (p->model == ISING_MODEL_ISING || p->model == ISING_MODEL_ISING_MSC)
+        return fail(ISING_E_STATE, "not a continuous-spin ensemble");
     std::vector<ContRec> buf((size_t)std::max<int64_t>(0, max_sweeps) * p->total_reps);
     int64_t n = 0;
     const int rc = drain(p, max_sweeps, sizeof(ContRec), buf.data(), &n);
@@ -579,16 +607,37 @@ int ising_pt_exchange_stats(ising_pt_t* p, int64_t* tries, int64_t* accepts) {
     return ISING_OK;
 }
 
+// Host copy of both colour halves of one Ising replica as 0/1 bytes (either storage).
+static int ising_halves(ising_pt* p, const DevGroup* G, int replica, std::vector<uint8_t>& half,
+                        std::vector<uint32_t>* words) {
+    half.assign(2 * (size_t)G->n_half, 0);
+    HIP_TRY(hipStreamSynchronize(p->st));
+    if (p->model == ISING_MODEL_ISING) {
+        for (int c = 0; c < 2; ++c)
+            HIP_TRY(hipMemcpy(half.data() + c * G->n_half,
+                              p->d_pool + G->off + (2 * (int64_t)replica + c) * G->stride, G->n_half,
+                              hipMemcpyDeviceToHost));
+        return ISING_OK;
+    }
+    const int wi = replica / 32, bit = replica % 32;
+    words->assign(2 * (size_t)G->n_half, 0);
+    const uint32_t* base = (const uint32_t*)p->d_pool + G->off;
+    for (int c = 0; c < 2; ++c)
+        HIP_TRY(hipMemcpy(words->data() + c * G->n_half, base + (2 * (int64_t)wi + c) * G->stride,
+                          sizeof(uint32_t) * G->n_half, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < half.size(); ++i) half[i] = ((*words)[i] >> bit) & 1u;
+    return ISING_OK;
+}
+
 int ising_pt_get_spins_i8(ising_pt_t* p, int32_t size_index, int32_t replica, int8_t* out) {
     if (!p || !out) return fail(ISING_E_ARG, "null argument");
-    if (p->model != ISING_MODEL_ISING) return fail(ISING_E_STATE, "not an Ising ensemble");
+    if (p->ncomp != 0) return fail(ISING_E_STATE, "not an Ising ensemble");
     const DevGroup* G = group_of(p, size_index);
     if (!G || replica < 0 || replica >= G->R) return fail(ISING_E_ARG, "size_index or replica out of range");
-    HIP_TRY(hipStreamSynchronize(p->st));
-    std::vector<uint8_t> half(2 * (size_t)G->n_half);
-    for (int c = 0; c < 2; ++c)
-        HIP_TRY(hipMemcpy(half.data() + c * G->n_half, p->d_pool + G->off + (2 * (int64_t)replica + c) * G->stride,
-                          G->n_half, hipMemcpyDeviceToHost));
+    std::vector<uint8_t> half;
+    std::vector<uint32_t> words;
+    const int rc = ising_halves(p, G, replica, half, &words);
+    if (rc) return rc;
     const int L = G->L, Lh = G->Lh;
     for (int z = 0; z < L; ++z)
         for (int y = 0; y < L; ++y)
@@ -602,7 +651,7 @@ int ising_pt_get_spins_i8(ising_pt_t* p, int32_t size_index, int32_t replica, in
 
 int ising_pt_set_spins_i8(ising_pt_t* p, int32_t size_index, int32_t replica, const int8_t* in) {
     if (!p || !in) return fail(ISING_E_ARG, "null argument");
-    if (p->model != ISING_MODEL_ISING) return fail(ISING_E_STATE, "not an Ising ensemble");
+    if (p->ncomp != 0) return fail(ISING_E_STATE, "not an Ising ensemble");
     const DevGroup* G = group_of(p, size_index);
     if (!G || replica < 0 || replica >= G->R) return fail(ISING_E_ARG, "size_index or replica out of range");
     const int L = G->L, Lh = G->Lh;
@@ -616,15 +665,28 @@ int ising_pt_set_spins_i8(ising_pt_t* p, int32_t size_index, int32_t replica, co
                 half[c * G->n_half + ((int64_t)z * L + y) * Lh + (x >> 1)] = v > 0 ? 1 : 0;
             }
     HIP_TRY(hipStreamSynchronize(p->st));
+    if (p->model == ISING_MODEL_ISING) {
+        for (int c = 0; c < 2; ++c)
+            HIP_TRY(hipMemcpy(p->d_pool + G->off + (2 * (int64_t)replica + c) * G->stride,
+                              half.data() + c * G->n_half, G->n_half, hipMemcpyHostToDevice));
+        return ISING_OK;
+    }
+    std::vector<uint8_t> old;
+    std::vector<uint32_t> words;
+    const int rc = ising_halves(p, G, replica, old, &words);
+    if (rc) return rc;
+    const int wi = replica / 32, bit = replica % 32;
+    for (size_t i = 0; i < words.size(); ++i) words[i] = (words[i] & ~(1u << bit)) | ((uint32_t)half[i] << bit);
+    uint32_t* base = (uint32_t*)p->d_pool + G->off;
     for (int c = 0; c < 2; ++c)
-        HIP_TRY(hipMemcpy(p->d_pool + G->off + (2 * (int64_t)replica + c) * G->stride, half.data() + c * G->n_half,
-                          G->n_half, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(base + (2 * (int64_t)wi + c) * G->stride, words.data() + c * G->n_half,
+                          sizeof(uint32_t) * G->n_half, hipMemcpyHostToDevice));
     return ISING_OK;
 }
 
 int ising_pt_get_spins_f32(ising_pt_t* p, int32_t size_index, int32_t replica, float* out) {
     if (!p || !out) return fail(ISING_E_ARG, "null argument");
-    if (p->model == ISING_MODEL_ISING) return fail(ISING_E_STATE, "not a continuous-spin ensemble");
+    if (p->ncomp == 0) return fail(ISING_E_STATE, "not a continuous-spin ensemble");
     const DevGroup* G = group_of(p, size_index);
     if (!G || replica < 0 || replica >= G->R) return fail(ISING_E_ARG, "size_index or replica out of range");
     const int nc = p->ncomp;
@@ -649,7 +711,7 @@ int ising_pt_get_spins_f32(ising_pt_t* p, int32_t size_index, int32_t replica, f
 
 int ising_pt_set_spins_f32(ising_pt_t* p, int32_t size_index, int32_t replica, const float* in) {
     if (!p || !in) return fail(ISING_E_ARG, "null argument");
-    if (p->model == ISING_MODEL_ISING) return fail(ISING_E_STATE, "not a continuous-spin ensemble");
+    if (p->ncomp == 0) return fail(ISING_E_STATE, "not a continuous-spin ensemble");
     const DevGroup* G = group_of(p, size_index);
     if (!G || replica < 0 || replica >= G->R) return fail(ISING_E_ARG, "size_index or replica out of range");
     const int nc = p->ncomp;
@@ -678,7 +740,7 @@ int ising_pt_set_spins_f32(ising_pt_t* p, int32_t size_index, int32_t replica, c
 
 int ising_pt_get_deltas(ising_pt_t* p, int32_t size_index, float* deltas) {
     if (!p || !deltas) return fail(ISING_E_ARG, "null argument");
-    if (p->model == ISING_MODEL_ISING) return fail(ISING_E_STATE, "not a continuous-spin ensemble");
+    if (p->ncomp == 0) return fail(ISING_E_STATE, "not a continuous-spin ensemble");
     const DevGroup* G = group_of(p, size_index);
     if (!G) return fail(ISING_E_ARG, "size_index out of range");
     HIP_TRY(hipStreamSynchronize(p->st));

@@ -41,7 +41,17 @@ __device__ __forceinline__ uint32_t word_of(const u32x4& w, int j) {
     return j < 2 ? w.x : (j < 4 ? w.y : (j < 6 ? w.z : w.w));
 }
 
-// Closing step of a sweep, run by the last block of the colour-1 launch.
+// Accumulated quantity q (0: bond sum, 1: magnetisation) of global replica gr,
+// summed over the accumulator copies.
+__device__ __forceinline__ int64_t acc_total(const IsingArgs& a, int gr, int q) {
+    int64_t v = 0;
+    for (int c = 0; c < a.acc_copies; ++c)
+        v += __hip_atomic_load(&a.acc[2 * ((int64_t)c * a.total_reps + gr) + q], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    return v;
+}
+
+// Closing step of a sweep (one-block kernel after the colour-1 launch).
 __device__ void close_sweep(const IsingArgs& a, int64_t t, bool meas, bool exch, int64_t s) {
     DevCtl* ctl = a.ctl;
     const int tid = threadIdx.x;
@@ -50,8 +60,8 @@ __device__ void close_sweep(const IsingArgs& a, int64_t t, bool meas, bool exch,
     const bool room = nrec < ctl->rec_cap;
     if (need) {
         for (int gr = tid; gr < a.total_reps; gr += blockDim.x) {
-            const int64_t B = __hip_atomic_load(&a.acc[2 * gr], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const int64_t M = __hip_atomic_load(&a.acc[2 * gr + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int64_t B = acc_total(a, gr, 0);
+            const int64_t M = acc_total(a, gr, 1);
             if (meas && room) {
                 IsingRec r;
                 r.bond = B;
@@ -71,8 +81,8 @@ __device__ void close_sweep(const IsingArgs& a, int64_t t, bool meas, bool exch,
             for (int q = tid; q < npairs; q += blockDim.x) {
                 const int i = (int)(round & 1) + 2 * q;
                 const int ri = a.t2r[G.rep0 + i], rj = a.t2r[G.rep0 + i + 1];
-                const int64_t Bi = __hip_atomic_load(&a.acc[2 * (G.rep0 + ri)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const int64_t Bj = __hip_atomic_load(&a.acc[2 * (G.rep0 + rj)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const int64_t Bi = acc_total(a, G.rep0 + ri, 0);
+                const int64_t Bj = acc_total(a, G.rep0 + rj, 0);
                 const double Ei = -a.J * (double)Bi;
                 const double Ej = -a.J * (double)Bj;
                 const double delta = (a.betas[G.rep0 + i] - a.betas[G.rep0 + i + 1]) * (Ej - Ei);
@@ -92,7 +102,7 @@ __device__ void close_sweep(const IsingArgs& a, int64_t t, bool meas, bool exch,
     }
     __syncthreads();
     if (need)
-        for (int gr = tid; gr < 2 * a.total_reps; gr += blockDim.x) a.acc[gr] = 0;
+        for (int64_t i = tid; i < 2 * (int64_t)a.total_reps * a.acc_copies; i += blockDim.x) a.acc[i] = 0;
     if (tid == 0) {
         if (meas) {
             if (room) ctl->n_rec = nrec + 1;

@@ -21,7 +21,17 @@ struct IsingArgs {
     double J;
     int32_t inline_thr;           // 1: thresholds come from thr_inline (single-lattice steps)
     uint32_t thr_inline[28];
+    const uint32_t* mthr;         // multispin coding: [rep0 + tidx][3] thresholds for a = 4, 5, 6
+    int32_t acc_copies;           // accumulator copies (spread atomics), summed by the closing kernel
 };
+
+constexpr int kMscAccCopies = 16;
+constexpr int kMscTile = 256;  // site words per block in multispin-coded launches
+hipError_t launch_msc_half(const IsingArgs& a, int nblocks, int color, hipStream_t st);
+hipError_t launch_msc_init(const DevGroup* groups, int ngroups, uint32_t* pool, int nblocks, uint32_t k0, uint32_t k1,
+                           hipStream_t st);
+hipError_t launch_msc_measure(const DevGroup* groups, int ngroups, const uint32_t* pool, int total_reps, int nblocks,
+                              int64_t* acc, hipStream_t st);
 
 struct ContArgs {
     const DevGroup* groups;

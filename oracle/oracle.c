@@ -774,3 +774,173 @@ double o_tau_int(const double* x, int64_t n, double c, int64_t* window) {
     if (window) *window = W;
     return t;
 }
+
+/* Reference-faithful Ising chain (kind 0: random-site Metropolis sweeps,
+ * metropolis.rs:8-27; kind 1: Wolff steps, wolff.rs:27-65) from a hot start,
+ * measured (forward-bond sum, magnetisation) after every step. */
+void o_ref_ising_chain(int kind, int L, uint64_t seed, double beta, int64_t warmup, int64_t n,
+                       int64_t* out_bond, int64_t* out_mag) {
+    int64_t N = (int64_t)L * L * L;
+    int8_t* sp = (int8_t*)malloc((size_t)N);
+    int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)N);
+    uint8_t* mark = (uint8_t*)malloc((size_t)N);
+    o_rng rng;
+    o_rng_seed(&rng, seed);
+    for (int64_t i = 0; i < N; ++i) sp[i] = (o_rng_next(&rng) >> 63) ? 1 : -1;
+    for (int64_t s = 0; s < warmup + n; ++s) {
+        if (kind == 0) o_ref_metropolis_sweep(L, sp, beta, 1.0, 0.0, &rng);
+        else o_ref_wolff_step(L, sp, beta, 1.0, &rng, stack, mark);
+        if (s >= warmup) o_ising_measure(L, sp, &out_bond[s - warmup], &out_mag[s - warmup]);
+    }
+    free(sp); free(stack); free(mark);
+}
+
+/* Reference-faithful Heisenberg chain: cap Metropolis + n_or sequential OR
+ * sweeps (heisenberg/sweep.rs:50-62), measured (E, |M|) after every sweep. */
+void o_ref_heis_chain(int L, uint64_t seed, double beta, double delta, int n_or, int64_t warmup, int64_t n,
+                      double* out_e, double* out_m) {
+    int64_t N = (int64_t)L * L * L;
+    double* sp = (double*)malloc(sizeof(double) * 3 * (size_t)N);
+    o_rng rng;
+    o_rng_seed(&rng, seed);
+    for (int64_t i = 0; i < N; ++i) {
+        double z = 2.0 * o_rng_f64(&rng) - 1.0, phi = TWO_PI * o_rng_f64(&rng), r = sqrt(1.0 - z * z);
+        sp[3 * i] = r * cos(phi); sp[3 * i + 1] = r * sin(phi); sp[3 * i + 2] = z;
+    }
+    for (int64_t s = 0; s < warmup + n; ++s) {
+        o_ref_heis_metropolis_sweep(L, sp, beta, 1.0, delta, &rng, NULL);
+        for (int k = 0; k < n_or; ++k) o_ref_heis_overrelax_sweep(L, sp, 1.0);
+        if (s >= warmup) {
+            double e, m[3];
+            o_cont_measure(L, 3, sp, 1.0, &e, m);
+            out_e[s - warmup] = e;
+            out_m[s - warmup] = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        }
+    }
+    free(sp);
+}
+
+/* Reference-faithful XY chain: Wolff sweeps (xy/wolff.rs:16-73), measured (E, |M|). */
+void o_ref_xy_chain(int L, uint64_t seed, double beta, int64_t warmup, int64_t n, double* out_e, double* out_m) {
+    int64_t N = (int64_t)L * L * L;
+    double* sp = (double*)malloc(sizeof(double) * 2 * (size_t)N);
+    int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)N);
+    uint8_t* mark = (uint8_t*)malloc((size_t)N);
+    o_rng rng;
+    o_rng_seed(&rng, seed);
+    for (int64_t i = 0; i < N; ++i) {
+        double phi = TWO_PI * o_rng_f64(&rng);
+        sp[2 * i] = cos(phi); sp[2 * i + 1] = sin(phi);
+    }
+    for (int64_t s = 0; s < warmup + n; ++s) {
+        o_ref_xy_wolff_sweep(L, sp, beta, 1.0, &rng, stack, mark);
+        if (s >= warmup) {
+            double e, m[3];
+            o_cont_measure(L, 2, sp, 1.0, &e, m);
+            out_e[s - warmup] = e;
+            out_m[s - warmup] = sqrt(m[0] * m[0] + m[1] * m[1]);
+        }
+    }
+    free(sp); free(stack); free(mark);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multispin-coded Ising (replicas packed across the bits of a site word)   */
+/* ------------------------------------------------------------------------ */
+/* Replica r of a size-L group lives in bit r % 32 of word w = r / 32 of each
+ * site.  A colour-c half-sweep of word w: aligned-neighbour count a_r per
+ * replica; a_r <= 3 (dE <= 0) flips; otherwise accept iff u_r < T(a_r, beta_r)
+ * with u_r the 32-bit uniform whose bit (31 - j) is bit (r % 32) of random
+ * plane j, plane j = word (j & 3) of Philox(h, 2t + c, (L << 16) | w,
+ * (O_P_MSC << 8) | (j >> 2)).  T(a, beta) = floor(2^32 exp(-beta J (4a - 12))).
+ * Spins: R arrays of L^3 int8 +-1 (full layout).  thr[r][3] = T(4), T(5), T(6)
+ * for replica r's current temperature. */
+void o_msc_halfsweep(int L, int R, int8_t* spins, int color, int64_t t, uint64_t seed, const uint32_t* thr) {
+    const uint32_t hs = (uint32_t)(2 * t + color);
+    int64_t n3 = (int64_t)L * L * L;
+    for (int z = 0; z < L; ++z)
+        for (int y = 0; y < L; ++y)
+            for (int x = 0; x < L; ++x) {
+                if (((x + y + z) & 1) != color) continue;
+                int64_t s = ((int64_t)z * L + y) * L + x;
+                int64_t nb[6] = {((int64_t)z * L + y) * L + (x + 1) % L, ((int64_t)z * L + y) * L + (x + L - 1) % L,
+                                 ((int64_t)z * L + (y + 1) % L) * L + x, ((int64_t)z * L + (y + L - 1) % L) * L + x,
+                                 ((int64_t)((z + 1) % L) * L + y) * L + x,
+                                 ((int64_t)((z + L - 1) % L) * L + y) * L + x};
+                uint32_t h = (uint32_t)(s >> 1);
+                for (int w = 0; w * 32 < R; ++w) {
+                    uint32_t planes[32];
+                    for (int q = 0; q < 8; ++q) {
+                        uint32_t o[4];
+                        ph(h, hs, ((uint32_t)L << 16) | (uint32_t)w, (O_P_MSC << 8) | (uint32_t)q, seed, o);
+                        for (int k = 0; k < 4; ++k) planes[4 * q + k] = o[k];
+                    }
+                    for (int b = 0; b < 32 && w * 32 + b < R; ++b) {
+                        int r = w * 32 + b;
+                        int8_t* sp = spins + n3 * r;
+                        int a = 0;
+                        for (int k = 0; k < 6; ++k) a += sp[nb[k]] == sp[s];
+                        int flip = 1;
+                        if (a >= 4) {
+                            uint32_t u = 0;
+                            for (int j = 0; j < 32; ++j) u |= ((planes[j] >> b) & 1u) << (31 - j);
+                            flip = u < thr[3 * r + (a - 4)];
+                        }
+                        if (flip) sp[s] = (int8_t)-sp[s];
+                    }
+                }
+            }
+}
+
+void o_msc_thresholds(double beta, double J, uint32_t out[3]) {
+    for (int a = 4; a <= 6; ++a) {
+        double p = exp(-beta * J * (4.0 * a - 12.0));
+        out[a - 4] = (uint32_t)ldexp(p, 32); /* p < 1 for a >= 4, J > 0 */
+    }
+}
+
+int64_t o_msc_pt_run(int L, int R, const double* temps, uint64_t seed, double J, int64_t warmup, int64_t samples,
+                     int exchange_every, int measure_every, int32_t* rec_tidx, int64_t* rec_bond, int64_t* rec_mag,
+                     int8_t* final_spins) {
+    int64_t n3 = (int64_t)L * L * L;
+    int8_t* sp = (int8_t*)malloc((size_t)(n3 * R));
+    int* r2t = (int*)malloc(sizeof(int) * R);
+    int* t2r = (int*)malloc(sizeof(int) * R);
+    double* betas = (double*)malloc(sizeof(double) * R);
+    double* en = (double*)malloc(sizeof(double) * R);
+    uint32_t* tt = (uint32_t*)malloc(sizeof(uint32_t) * 3 * R);   /* per temperature */
+    uint32_t* tr = (uint32_t*)malloc(sizeof(uint32_t) * 3 * R);   /* per replica */
+    for (int i = 0; i < R; ++i) {
+        r2t[i] = t2r[i] = i;
+        betas[i] = 1.0 / temps[i];
+        o_msc_thresholds(betas[i], J, tt + 3 * i);
+        o_ising_init(L, seed, o_stream(L, i), sp + n3 * i);
+    }
+    int64_t t = 0, nrec = 0;
+    for (int64_t s = -warmup; s < samples; ++s, ++t) {
+        for (int r = 0; r < R; ++r)
+            for (int k = 0; k < 3; ++k) tr[3 * r + k] = tt[3 * r2t[r] + k];
+        for (int c = 0; c < 2; ++c) o_msc_halfsweep(L, R, sp, c, t, seed, tr);
+        if (s < 0) continue;
+        int do_m = ((s + 1) % measure_every) == 0;
+        int do_x = ((s + 1) % exchange_every) == 0;
+        if (do_m || do_x) {
+            for (int r = 0; r < R; ++r) {
+                int64_t B, M;
+                o_ising_measure(L, sp + n3 * r, &B, &M);
+                en[r] = -J * (double)B;
+                if (do_m) {
+                    if (rec_tidx) rec_tidx[nrec] = r2t[r];
+                    if (rec_bond) rec_bond[nrec] = B;
+                    if (rec_mag) rec_mag[nrec] = M;
+                    ++nrec;
+                }
+            }
+        }
+        if (do_x)
+            o_exchange(R, betas, en, r2t, t2r, s / exchange_every, seed, o_stream(L, 0), O_MODEL_ISING);
+    }
+    if (final_spins) memcpy(final_spins, sp, (size_t)(n3 * R));
+    free(sp); free(r2t); free(t2r); free(betas); free(en); free(tt); free(tr);
+    return nrec;
+}

@@ -38,7 +38,7 @@ void o_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[
 
 /* Counter word 3 = (purpose << 8) | model.  Counter word 2 = stream id. */
 enum { O_MODEL_ISING = 0, O_MODEL_XY = 1, O_MODEL_HEIS = 2 };
-enum { O_P_INIT = 1, O_P_ISING = 2, O_P_TIE = 3, O_P_EXCH = 4, O_P_CONT = 5 };
+enum { O_P_INIT = 1, O_P_ISING = 2, O_P_TIE = 3, O_P_EXCH = 4, O_P_CONT = 5, O_P_MSC = 6 };
 uint32_t o_tag(int purpose, int model);
 uint32_t o_stream(int L, int replica);         /* (L << 16) | replica */
 
@@ -145,6 +145,20 @@ int64_t o_ref_xy_wolff_sweep(int L, double* spins, double beta, double J, o_rng*
  * returns updates performed and wall seconds. */
 int64_t o_bench_ref_metropolis(int L, double beta, uint64_t seed, double seconds, double* wall);
 int64_t o_bench_ref_heis(int L, double beta, double delta, int n_or, uint64_t seed, double seconds, double* wall);
+/* ---------------- Multispin-coded Ising (replicas across word bits) --------
+ * Pinned by: the 2x2x2 brute-force Boltzmann distribution per replica, the
+ * replica-independence of the bit lanes, and equality of the lazy bit-plane
+ * comparison with the full 32-bit one (tests/test_oracle_msc.py). */
+void o_msc_thresholds(double beta, double J, uint32_t out[3]);
+void o_msc_halfsweep(int L, int R, int8_t* spins, int color, int64_t t, uint64_t seed, const uint32_t* thr);
+int64_t o_msc_pt_run(int L, int R, const double* temps, uint64_t seed, double J, int64_t warmup, int64_t samples,
+                     int exchange_every, int measure_every, int32_t* rec_tidx, int64_t* rec_bond, int64_t* rec_mag,
+                     int8_t* final_spins);
+void o_ref_ising_chain(int kind, int L, uint64_t seed, double beta, int64_t warmup, int64_t n,
+                       int64_t* out_bond, int64_t* out_mag);
+void o_ref_heis_chain(int L, uint64_t seed, double beta, double delta, int n_or, int64_t warmup, int64_t n,
+                      double* out_e, double* out_m);
+void o_ref_xy_chain(int L, uint64_t seed, double beta, int64_t warmup, int64_t n, double* out_e, double* out_m);
 double o_tau_int(const double* x, int64_t n, double c, int64_t* window);
 void o_ising_chain(int dim, int L, uint64_t seed, double beta, double J, double h, int64_t warmup,
                    int64_t n_sweeps, int64_t* out_bond, int64_t* out_mag);

@@ -13,7 +13,7 @@ _LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 ROCRAND_CHECK = os.path.join(ROOT, "oracle", "_build", "philox_rocrand_check")
 
 MODEL_ISING, MODEL_XY, MODEL_HEIS = 0, 1, 2
-P_INIT, P_ISING, P_TIE, P_EXCH, P_CONT = 1, 2, 3, 4, 5
+P_INIT, P_ISING, P_TIE, P_EXCH, P_CONT, P_MSC = 1, 2, 3, 4, 5, 6
 
 
 def _load():
@@ -187,3 +187,53 @@ def tau_int(x, c=6.0):
     w = C.c_int64()
     t = lib.o_tau_int(np.ascontiguousarray(x, np.float64), len(x), c, C.byref(w))
     return t, w.value
+
+lib.o_ref_ising_chain.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_double, C.c_int64, C.c_int64, _i64p, _i64p]
+lib.o_ref_heis_chain.argtypes = [C.c_int, C.c_uint64, C.c_double, C.c_double, C.c_int, C.c_int64, C.c_int64, _f64p,
+                                 _f64p]
+lib.o_ref_xy_chain.argtypes = [C.c_int, C.c_uint64, C.c_double, C.c_int64, C.c_int64, _f64p, _f64p]
+
+
+def ref_ising_chain(kind, L, seed, beta, warmup, n):
+    """kind 'metropolis' (random-site sweeps) or 'wolff' (cluster steps): (bond, mag) per step."""
+    b = np.zeros(n, np.int64)
+    m = np.zeros(n, np.int64)
+    lib.o_ref_ising_chain(0 if kind == "metropolis" else 1, L, seed, beta, warmup, n, b, m)
+    return b, m
+
+
+def ref_heis_chain(L, seed, beta, delta, n_or, warmup, n):
+    e = np.zeros(n)
+    m = np.zeros(n)
+    lib.o_ref_heis_chain(L, seed, beta, delta, n_or, warmup, n, e, m)
+    return e, m
+
+
+def ref_xy_chain(L, seed, beta, warmup, n):
+    e = np.zeros(n)
+    m = np.zeros(n)
+    lib.o_ref_xy_chain(L, seed, beta, warmup, n, e, m)
+    return e, m
+
+lib.o_msc_thresholds.argtypes = [C.c_double, C.c_double, _u32p]
+lib.o_msc_halfsweep.argtypes = [C.c_int, C.c_int, _i8p, C.c_int, C.c_int64, C.c_uint64, _u32p]
+lib.o_msc_pt_run.restype = C.c_int64
+lib.o_msc_pt_run.argtypes = [C.c_int, C.c_int, _f64p, C.c_uint64, C.c_double, C.c_int64, C.c_int64, C.c_int, C.c_int,
+                             _i32p, _i64p, _i64p, _i8p]
+
+
+def msc_thresholds(beta, J=1.0):
+    out = np.zeros(3, np.uint32)
+    lib.o_msc_thresholds(beta, J, out)
+    return out
+
+
+def msc_pt_run(L, R, temps, seed, warmup, samples, exchange_every, measure_every, J=1.0):
+    n_meas = samples // measure_every
+    tidx = np.zeros(max(1, n_meas * R), np.int32)
+    bond = np.zeros(max(1, n_meas * R), np.int64)
+    mag = np.zeros(max(1, n_meas * R), np.int64)
+    final = np.zeros(L ** 3 * R, np.int8)
+    n = lib.o_msc_pt_run(L, R, np.asarray(temps, np.float64), seed, J, warmup, samples, exchange_every,
+                         measure_every, tidx, bond, mag, final)
+    return tidx[:n], bond[:n], mag[:n], final.reshape(R, L ** 3)
