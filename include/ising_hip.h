@@ -44,6 +44,10 @@ extern "C" {
  * word per site (multispin coding, 1 bit per spin).  Requires h = 0, J > 0.
  * Same records and spin API as ISING_MODEL_ISING; its own random stream. */
 #define ISING_MODEL_ISING_MSC 3
+/* 2D square-lattice Ising (L x L, 4 neighbours): the same checkerboard,
+ * random-number and record scheme with z removed; for the exact 2D
+ * (Onsager / enumeration) checks.  Spin arrays hold L^2 int8. */
+#define ISING_MODEL_ISING2D 4
 
 /* Message for the last failing call on this thread ("" if none). */
 const char* ising_last_error(void);

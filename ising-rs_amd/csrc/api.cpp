@@ -49,14 +49,17 @@ void threshold_pair(double p, uint32_t* hi, uint32_t* lo) {
     *lo = (uint32_t)(t & 0xffffu);
 }
 
-// Acceptance table for k = b*7 + n_up: dE = 2 s (J (2 n_up - 6) + h).
-void ising_table(double beta, double J, double h, uint32_t out[28]) {
+// Acceptance table for k = b*(z+1) + n_up, z = 2 * dims neighbours:
+// dE = 2 s (J (2 n_up - z) + h).  High halves in out[0..13], low in out[14..27].
+void ising_table(double beta, double J, double h, uint32_t out[28], int dims = 3) {
+    const int z = 2 * dims;
+    for (int i = 0; i < 28; ++i) out[i] = 0;
     for (int b = 0; b <= 1; ++b) {
         const double s = b ? 1.0 : -1.0;
-        for (int n = 0; n <= 6; ++n) {
-            const double dE = 2.0 * s * (J * (double)(2 * n - 6) + h);
+        for (int n = 0; n <= z; ++n) {
+            const double dE = 2.0 * s * (J * (double)(2 * n - z) + h);
             const double p = dE <= 0.0 ? 1.0 : std::exp(-beta * dE);
-            threshold_pair(p, &out[b * 7 + n], &out[14 + b * 7 + n]);
+            threshold_pair(p, &out[b * (z + 1) + n], &out[14 + b * (z + 1) + n]);
         }
     }
 }
@@ -68,6 +71,7 @@ bool valid_L(int L) { return L >= 2 && L <= 2046 && (L % 2) == 0; }
 struct ising_pt {
     int model = ISING_MODEL_ISING;
     int ncomp = 0;
+    int dims = 3;
     int R = 0;
     int total_reps = 0;
     int nblocks = 0;
@@ -278,8 +282,8 @@ int64_t measured_in(int64_t s0, int64_t n, int every) {
 int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, int R, const double* temps,
                    uint64_t seed, const ising_pt_opts_t* opts) {
     if (model != ISING_MODEL_ISING && model != ISING_MODEL_XY && model != ISING_MODEL_HEISENBERG &&
-        model != ISING_MODEL_ISING_MSC)
-        return fail(ISING_E_ARG, "model must be ISING_MODEL_ISING, _ISING_MSC, _XY or _HEISENBERG");
+        model != ISING_MODEL_ISING_MSC && model != ISING_MODEL_ISING2D)
+        return fail(ISING_E_ARG, "model must be ISING_MODEL_ISING, _ISING_MSC, _ISING2D, _XY or _HEISENBERG");
     if (n_sizes < 1 || n_sizes > 64 || !sizes) return fail(ISING_E_ARG, "need 1..64 lattice sizes");
     if (R < 1 || R > 65535 || !temps) return fail(ISING_E_ARG, "need 1..65535 replicas and their temperatures");
     for (int i = 0; i < n_sizes; ++i)
@@ -305,16 +309,18 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(ISING_E_HIP, "no HIP device visible");
 
-    p->model = model;
+    const bool two_d = model == ISING_MODEL_ISING2D;
+    p->model = two_d ? ISING_MODEL_ISING : model;  // the 2D lattice runs the Ising kernels
+    p->dims = two_d ? 2 : 3;
     const bool msc = model == ISING_MODEL_ISING_MSC;
-    p->ncomp = (model == ISING_MODEL_ISING || msc) ? 0 : (model == ISING_MODEL_XY ? 2 : 3);
+    p->ncomp = (model == ISING_MODEL_ISING || two_d || msc) ? 0 : (model == ISING_MODEL_XY ? 2 : 3);
     p->R = R;
     p->seed = seed;
     p->opts = o;
     p->temps.assign(temps, temps + R);
     p->total_reps = n_sizes * R;
 
-    const bool ising = model == ISING_MODEL_ISING;
+    const bool ising = model == ISING_MODEL_ISING || two_d;
     const int64_t words = (R + 31) / 32;          // multispin coding: words per site
     const int64_t ncomp = ising ? 1 : (msc ? words : p->ncomp);  // arrays per colour half-lattice
     const int64_t elem = ising ? 1 : 4;           // bytes per element
@@ -327,11 +333,12 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
         G.L = sizes[i];
         G.Lh = sizes[i] / 2;
         G.R = R;
-        G.fast = (sizes[i] % 16) == 0 ? 1 : 0;
+        G.dims = p->dims;
+        G.fast = (!two_d && (sizes[i] % 16) == 0) ? 1 : 0;
         if (G.Lh >= 2) fastdiv_magic((uint32_t)G.Lh, &G.mag_Lh, &G.sh_Lh);
         fastdiv_magic((uint32_t)G.L, &G.mag_L, &G.sh_L);
         G.rep0 = i * R;
-        G.n_half = (int64_t)sizes[i] * sizes[i] * sizes[i] / 2;
+        G.n_half = (int64_t)sizes[i] * sizes[i] * (two_d ? 1 : sizes[i]) / 2;
         G.stride = ((G.n_half * elem + 255) / 256) * 256 / elem;
         G.off = off;
         G.tiles = (int32_t)((G.n_half + tile - 1) / tile);
@@ -380,7 +387,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
             betas_f[g * R + i] = (float)beta;
             deltas[g * R + i] = (float)o.delta;
             ident[g * R + i] = i;
-            ising_table(beta, o.j, o.h, &thr[28 * (size_t)(g * R + i)]);
+            ising_table(beta, o.j, o.h, &thr[28 * (size_t)(g * R + i)], p->dims);
             for (int k = 0; k < 3; ++k)  // multispin coding: T(a) for a = 4, 5, 6 aligned neighbours
                 mthr[3 * (size_t)(g * R + i) + k] = (uint32_t)std::ldexp(std::exp(-beta * o.j * (4.0 * (4 + k) - 12.0)), 32);
         }
@@ -638,8 +645,8 @@ int ising_pt_get_spins_i8(ising_pt_t* p, int32_t size_index, int32_t replica, in
     std::vector<uint32_t> words;
     const int rc = ising_halves(p, G, replica, half, &words);
     if (rc) return rc;
-    const int L = G->L, Lh = G->Lh;
-    for (int z = 0; z < L; ++z)
+    const int L = G->L, Lh = G->Lh, Lz = G->dims == 3 ? L : 1;
+    for (int z = 0; z < Lz; ++z)
         for (int y = 0; y < L; ++y)
             for (int x = 0; x < L; ++x) {
                 const int c = (x + y + z) & 1;
@@ -654,9 +661,9 @@ int ising_pt_set_spins_i8(ising_pt_t* p, int32_t size_index, int32_t replica, co
     if (p->ncomp != 0) return fail(ISING_E_STATE, "not an Ising ensemble");
     const DevGroup* G = group_of(p, size_index);
     if (!G || replica < 0 || replica >= G->R) return fail(ISING_E_ARG, "size_index or replica out of range");
-    const int L = G->L, Lh = G->Lh;
+    const int L = G->L, Lh = G->Lh, Lz = G->dims == 3 ? L : 1;
     std::vector<uint8_t> half(2 * (size_t)G->n_half);
-    for (int z = 0; z < L; ++z)
+    for (int z = 0; z < Lz; ++z)
         for (int y = 0; y < L; ++y)
             for (int x = 0; x < L; ++x) {
                 const int8_t v = in[((int64_t)z * L + y) * L + x];

@@ -100,7 +100,7 @@ struct DevGroup {
     int32_t rep0;      // global index of replica 0 (tables are indexed by rep0 + i)
     int32_t tiles;     // work tiles per replica half-sweep
     int32_t blk0;      // first block of this group in a half-sweep launch
-    int32_t pad0;
+    int32_t dims;      // 3: L^3 cube; 2: L^2 square lattice (Ising only; z dropped)
     int64_t n_half;    // L^3 / 2
     int64_t stride;    // elements between consecutive half-lattices (padded)
     int64_t off;       // element offset of replica 0 colour 0 in the pool

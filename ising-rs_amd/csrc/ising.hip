@@ -232,11 +232,11 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
                 const int ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
                 const int zm = z == 0 ? L - 1 : z - 1, zp = z == L - 1 ? 0 : z + 1;
                 const int xo = p ? (xh + 1 == Lh ? 0 : xh + 1) : (xh == 0 ? Lh - 1 : xh - 1);
-                const uint32_t n = (uint32_t)O[row * Lh + xh] + O[row * Lh + xo] +
-                                   O[((int64_t)z * L + ym) * Lh + xh] + O[((int64_t)z * L + yp) * Lh + xh] +
-                                   O[((int64_t)zm * L + y) * Lh + xh] + O[((int64_t)zp * L + y) * Lh + xh];
+                uint32_t n = (uint32_t)O[row * Lh + xh] + O[row * Lh + xo] + O[((int64_t)z * L + ym) * Lh + xh] +
+                             O[((int64_t)z * L + yp) * Lh + xh];
+                if (G.dims == 3) n += (uint32_t)O[((int64_t)zm * L + y) * Lh + xh] + O[((int64_t)zp * L + y) * Lh + xh];
                 const uint32_t b = C[h] & 1u;
-                const uint32_t k = b * 7u + n;
+                const uint32_t k = b * (uint32_t)(2 * G.dims + 1) + n;  // table row: b * (z + 1) + n_up
                 const uint32_t thi = s_thr[k];
                 const uint32_t r16 = (word_of(w, j) >> (16 * (j & 1))) & 0xffffu;
                 bool acc = r16 < thi;
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
                 if (need) {
                     const int sn = 2 * (int)nb - 1;
                     msum += sn;
-                    if (color) bsum += sn * (2 * (int)n - 6);
+                    if (color) bsum += sn * (2 * (int)n - 2 * G.dims);
                 }
             }
         }
@@ -333,11 +333,11 @@ __global__ __launch_bounds__(kThreads) void ising_measure_kernel(const DevGroup*
         const int ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
         const int zm = z == 0 ? L - 1 : z - 1, zp = z == L - 1 ? 0 : z + 1;
         const int xo = p ? (xh + 1 == Lh ? 0 : xh + 1) : (xh == 0 ? Lh - 1 : xh - 1);
-        const int n = C0[row * Lh + xh] + C0[row * Lh + xo] + C0[((int64_t)z * L + ym) * Lh + xh] +
-                      C0[((int64_t)z * L + yp) * Lh + xh] + C0[((int64_t)zm * L + y) * Lh + xh] +
-                      C0[((int64_t)zp * L + y) * Lh + xh];
+        int n = C0[row * Lh + xh] + C0[row * Lh + xo] + C0[((int64_t)z * L + ym) * Lh + xh] +
+                C0[((int64_t)z * L + yp) * Lh + xh];
+        if (G.dims == 3) n += C0[((int64_t)zm * L + y) * Lh + xh] + C0[((int64_t)zp * L + y) * Lh + xh];
         const int s1 = 2 * (int)C1[h] - 1;
-        bsum += s1 * (2 * n - 6);
+        bsum += s1 * (2 * n - 2 * G.dims);
         msum += s1 + 2 * (int)C0[h] - 1;
     }
     const int wb = wave_sum(bsum), wm = wave_sum(msum);

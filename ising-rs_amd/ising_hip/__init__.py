@@ -20,8 +20,9 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libising_hip.so")
 
-MODEL_ISING, MODEL_XY, MODEL_HEISENBERG, MODEL_ISING_MSC = 0, 1, 2, 3
-_MODELS = {"ising": MODEL_ISING, "xy": MODEL_XY, "heisenberg": MODEL_HEISENBERG, "ising_msc": MODEL_ISING_MSC}
+MODEL_ISING, MODEL_XY, MODEL_HEISENBERG, MODEL_ISING_MSC, MODEL_ISING2D = 0, 1, 2, 3, 4
+_MODELS = {"ising": MODEL_ISING, "xy": MODEL_XY, "heisenberg": MODEL_HEISENBERG, "ising_msc": MODEL_ISING_MSC,
+           "ising2d": MODEL_ISING2D}
 
 if not os.path.exists(LIB_PATH):
     raise ImportError("libising_hip.so is not built (%s); run `python -c 'import __graft_entry__ as g; g.build()'`"
@@ -344,7 +345,10 @@ class PTEnsemble:
 
     def get_spins(self, size_index, replica):
         L = self.sizes[size_index]
-        if self.model in (MODEL_ISING, MODEL_ISING_MSC):
+        if self.model == MODEL_ISING2D:
+            out = np.empty(L ** 2, np.int8)
+            _check(lib.ising_pt_get_spins_i8(self._h, size_index, replica, _ptr(out)))
+        elif self.model in (MODEL_ISING, MODEL_ISING_MSC):
             out = np.empty(L ** 3, np.int8)
             _check(lib.ising_pt_get_spins_i8(self._h, size_index, replica, _ptr(out)))
         else:
@@ -354,7 +358,7 @@ class PTEnsemble:
         return out
 
     def set_spins(self, size_index, replica, spins):
-        if self.model in (MODEL_ISING, MODEL_ISING_MSC):
+        if self.model in (MODEL_ISING, MODEL_ISING_MSC, MODEL_ISING2D):
             a = np.ascontiguousarray(spins, np.int8).reshape(-1)
             _check(lib.ising_pt_set_spins_i8(self._h, size_index, replica, _ptr(a)))
         else:

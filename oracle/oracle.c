@@ -251,10 +251,16 @@ void o_linspace(double tmin, double tmax, int n, double* out) {
     for (int i = 0; i < n; ++i) out[i] = tmin + (tmax - tmin) * (double)i / (double)(n - 1);
 }
 
-int64_t o_ising_pt_run(int L, int R, const double* temps, uint64_t seed, double J, double h,
+static void half(int dim, int L, int8_t* sp, int c, int64_t t, uint64_t seed, uint32_t st, const uint32_t* hi,
+                 const uint32_t* lo) {
+    if (dim == 2) o_ising2d_halfsweep(L, sp, c, t, seed, st, hi, lo);
+    else o_ising_halfsweep(L, sp, c, t, seed, st, hi, lo);
+}
+
+int64_t o_ising_pt_run_dim(int dim, int L, int R, const double* temps, uint64_t seed, double J, double h,
                        int64_t warmup, int64_t samples, int exchange_every, int measure_every,
                        int32_t* rec_tidx, int64_t* rec_bond, int64_t* rec_mag, int8_t* final_spins) {
-    int64_t n3 = (int64_t)L * L * L;
+    int64_t n3 = dim == 2 ? (int64_t)L * L : (int64_t)L * L * L;
     int8_t* sp = (int8_t*)malloc((size_t)(n3 * R));
     int* r2t = (int*)malloc(sizeof(int) * R);
     int* t2r = (int*)malloc(sizeof(int) * R);
@@ -265,24 +271,34 @@ int64_t o_ising_pt_run(int L, int R, const double* temps, uint64_t seed, double 
     for (int i = 0; i < R; ++i) {
         r2t[i] = t2r[i] = i;
         betas[i] = 1.0 / temps[i];
-        o_ising_thresholds(betas[i], J, h, thi[i], tlo[i]);
-        o_ising_init(L, seed, o_stream(L, i), sp + n3 * i);
+        if (dim == 2) {
+            o_ising2d_thresholds(betas[i], J, h, thi[i], tlo[i]);
+            for (int64_t s = 0; s < n3; ++s) {
+                uint32_t w[4];
+                ph((uint32_t)s, 0u, o_stream(L, i), o_tag(O_P_INIT, O_MODEL_ISING), seed, w);
+                sp[n3 * i + s] = (w[0] >> 31) ? 1 : -1;
+            }
+        } else {
+            o_ising_thresholds(betas[i], J, h, thi[i], tlo[i]);
+            o_ising_init(L, seed, o_stream(L, i), sp + n3 * i);
+        }
     }
     int64_t t = 0, nrec = 0;
     for (int64_t w = 0; w < warmup; ++w, ++t)
         for (int r = 0; r < R; ++r)
             for (int c = 0; c < 2; ++c)
-                o_ising_halfsweep(L, sp + n3 * r, c, t, seed, o_stream(L, r), thi[r2t[r]], tlo[r2t[r]]);
+                half(dim, L, sp + n3 * r, c, t, seed, o_stream(L, r), thi[r2t[r]], tlo[r2t[r]]);
     for (int64_t s = 0; s < samples; ++s, ++t) {
         for (int r = 0; r < R; ++r)
             for (int c = 0; c < 2; ++c)
-                o_ising_halfsweep(L, sp + n3 * r, c, t, seed, o_stream(L, r), thi[r2t[r]], tlo[r2t[r]]);
+                half(dim, L, sp + n3 * r, c, t, seed, o_stream(L, r), thi[r2t[r]], tlo[r2t[r]]);
         int do_m = ((s + 1) % measure_every) == 0;
         int do_x = ((s + 1) % exchange_every) == 0;
         if (do_m || do_x) {
             for (int r = 0; r < R; ++r) {
                 int64_t B, M;
-                o_ising_measure(L, sp + n3 * r, &B, &M);
+                if (dim == 2) o_ising2d_measure(L, sp + n3 * r, &B, &M);
+                else o_ising_measure(L, sp + n3 * r, &B, &M);
                 en[r] = -J * (double)B;
                 if (do_m) {
                     if (rec_tidx) rec_tidx[nrec] = r2t[r];
@@ -298,6 +314,13 @@ int64_t o_ising_pt_run(int L, int R, const double* temps, uint64_t seed, double 
     if (final_spins) memcpy(final_spins, sp, (size_t)(n3 * R));
     free(sp); free(r2t); free(t2r); free(betas); free(en); free(thi); free(tlo);
     return nrec;
+}
+
+int64_t o_ising_pt_run(int L, int R, const double* temps, uint64_t seed, double J, double h,
+                       int64_t warmup, int64_t samples, int exchange_every, int measure_every,
+                       int32_t* rec_tidx, int64_t* rec_bond, int64_t* rec_mag, int8_t* final_spins) {
+    return o_ising_pt_run_dim(3, L, R, temps, seed, J, h, warmup, samples, exchange_every, measure_every, rec_tidx,
+                              rec_bond, rec_mag, final_spins);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -91,6 +91,11 @@ int64_t o_ising_pt_run(int L, int R, const double* temps, uint64_t seed, double 
                        int64_t warmup, int64_t samples, int exchange_every, int measure_every,
                        int32_t* rec_tidx, int64_t* rec_bond, int64_t* rec_mag, int8_t* final_spins);
 
+/* Same for the 2D square lattice (dim = 2) or the cube (dim = 3). */
+int64_t o_ising_pt_run_dim(int dim, int L, int R, const double* temps, uint64_t seed, double J, double h,
+                           int64_t warmup, int64_t samples, int exchange_every, int measure_every,
+                           int32_t* rec_tidx, int64_t* rec_bond, int64_t* rec_mag, int8_t* final_spins);
+
 /* ---------------- Jackknife (src/bin/gpu_fss.rs:327-412) --------------------
  * out = E, E_err, M, M_err, M2, M2_err, M4, M4_err, Cv, Cv_err, chi, chi_err.
  * Pinned by closed-form series (constant, alternating, linear). */

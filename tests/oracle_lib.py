@@ -237,3 +237,18 @@ def msc_pt_run(L, R, temps, seed, warmup, samples, exchange_every, measure_every
     n = lib.o_msc_pt_run(L, R, np.asarray(temps, np.float64), seed, J, warmup, samples, exchange_every,
                          measure_every, tidx, bond, mag, final)
     return tidx[:n], bond[:n], mag[:n], final.reshape(R, L ** 3)
+
+lib.o_ising_pt_run_dim.restype = C.c_int64
+lib.o_ising_pt_run_dim.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, C.c_uint64, C.c_double, C.c_double, C.c_int64,
+                                   C.c_int64, C.c_int, C.c_int, _i32p, _i64p, _i64p, _i8p]
+
+
+def ising2d_pt_run(L, R, temps, seed, warmup, samples, exchange_every, measure_every, J=1.0, h=0.0):
+    n_meas = samples // measure_every
+    tidx = np.zeros(max(1, n_meas * R), np.int32)
+    bond = np.zeros(max(1, n_meas * R), np.int64)
+    mag = np.zeros(max(1, n_meas * R), np.int64)
+    final = np.zeros(L ** 2 * R, np.int8)
+    n = lib.o_ising_pt_run_dim(2, L, R, np.asarray(temps, np.float64), seed, J, h, warmup, samples, exchange_every,
+                               measure_every, tidx, bond, mag, final)
+    return tidx[:n], bond[:n], mag[:n], final.reshape(R, L ** 2)
