@@ -195,6 +195,30 @@ int ising_pt_get_spins_f32(ising_pt_t* pt, int32_t size_index, int32_t replica, 
 int ising_pt_set_spins_f32(ising_pt_t* pt, int32_t size_index, int32_t replica, const float* in);
 /* Current proposal size per temperature of one size (continuous models). */
 int ising_pt_get_deltas(ising_pt_t* pt, int32_t size_index, float* deltas);
+/* ---- Learned proposal (continuous models; no counterpart in the reference) ----
+ * An MLP (6 rotation-invariant neighbour features -> H -> H -> H -> 3, tanh,
+ * BF16 weights, FP32 accumulation, H in {8, 16, 32, 64}) gives a two-component
+ * von Mises (XY) / Fisher (Heisenberg) mixture around the local field; moves
+ * are accepted by exact Metropolis-Hastings including the proposal ratio.
+ * Parameters: flat FP32 vector W1[H][6], b1[H], W2[H][H], b2[H], W3[H][H],
+ * b3[H], W4[3][H], b4[3] (row-major). */
+int ising_pt_learn_init(ising_pt_t* pt, int32_t hidden, uint64_t seed);
+/* Number of network parameters (after ising_pt_learn_init). */
+int ising_pt_learn_size(ising_pt_t* pt, int64_t* n_params);
+int ising_pt_learn_get_params(ising_pt_t* pt, float* out, int64_t n);
+int ising_pt_learn_set_params(ising_pt_t* pt, const float* in, int64_t n);
+/* Adam steps on batches of sites drawn from the ensemble's current
+ * configurations; *mean_loss = mean -log q of the last batch. */
+int ising_pt_learn_train(ising_pt_t* pt, int32_t steps, int32_t batch, double lr, double* mean_loss);
+/* Loss and mean gradient on explicit samples (x: n * 6 features, c: n values of s.h^). */
+int ising_pt_learn_eval(ising_pt_t* pt, int64_t n, const float* x, const float* c, double* mean_loss, float* grad);
+/* 1: sweeps use the learned proposal for the Metropolis passes; 0: the cap/rotation proposal. */
+int ising_pt_learn_enable(ising_pt_t* pt, int32_t on);
+/* Learned Metropolis-Hastings moves attempted / accepted since creation. */
+int ising_pt_mh_stats(ising_pt_t* pt, int64_t* tries, int64_t* accepts);
+/* One Metropolis pass over one colour at sweep index t, without measurement,
+ * exchange or advancing the sweep counter (used by exactness tests). */
+int ising_pt_metropolis_pass(ising_pt_t* pt, int32_t color, int64_t t);
 /* Sweeps applied so far (warm-up + sample). */
 int ising_pt_sweeps(const ising_pt_t* pt, int64_t* out);
 /* Bytes of spin storage on the device. */

@@ -102,6 +102,15 @@ struct ising_pt {
     double* d_dacc = nullptr;
     int64_t* d_cnt = nullptr;
     bool inline_cont = false;  // single-lattice continuous mode
+    // learned proposal
+    int learn_H = 0, learn_P = 0;
+    bool learned_on = false;
+    int64_t learn_step = 0;
+    float* d_lmaster = nullptr;
+    uint16_t* d_lnet = nullptr;
+    float *d_lm1 = nullptr, *d_lm2 = nullptr, *d_lgpart = nullptr, *d_lgout = nullptr;
+    double* d_llpart = nullptr;
+    int learn_cap_blocks = 0;
     float beta_inline = 0.f, delta_inline = 0.f, j_inline = 1.f;
     int64_t t_host = 0;        // sweeps enqueued
     int64_t s_host = -1;       // sample sweeps enqueued (-1 before sampling starts)
@@ -129,6 +138,13 @@ struct ising_pt {
         (void)hipFree(d_part1);
         (void)hipFree(d_dacc);
         (void)hipFree(d_cnt);
+        (void)hipFree(d_lmaster);
+        (void)hipFree(d_lnet);
+        (void)hipFree(d_lm1);
+        (void)hipFree(d_lm2);
+        (void)hipFree(d_lgpart);
+        (void)hipFree(d_lgout);
+        (void)hipFree(d_llpart);
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -188,6 +204,8 @@ struct ising_pt {
         a.inline_params = inline_cont ? 1 : 0;
         a.beta_inline = beta_inline;
         a.delta_inline = delta_inline;
+        a.net = d_lnet;
+        a.learn_H = learn_H;
         return a;
     }
 
@@ -249,8 +267,13 @@ struct ising_pt {
                     HIP_TRY(launch_cont_or(a, nblocks, 0, st));
                     HIP_TRY(launch_cont_or(a, nblocks, 1, st));
                 }
-                HIP_TRY(launch_cont_metro(a, nblocks, 0, 1, st));
-                HIP_TRY(launch_cont_metro(a, nblocks, 1, 1, st));
+                if (learned_on) {
+                    HIP_TRY(launch_cont_learned(a, nblocks, 0, 1, -1, st));
+                    HIP_TRY(launch_cont_learned(a, nblocks, 1, 1, -1, st));
+                } else {
+                    HIP_TRY(launch_cont_metro(a, nblocks, 0, 1, st));
+                    HIP_TRY(launch_cont_metro(a, nblocks, 1, 1, st));
+                }
                 HIP_TRY(launch_cont_close(a, st));
             }
         }
@@ -752,6 +775,192 @@ int ising_pt_get_deltas(ising_pt_t* p, int32_t size_index, float* deltas) {
     if (!G) return fail(ISING_E_ARG, "size_index out of range");
     HIP_TRY(hipStreamSynchronize(p->st));
     HIP_TRY(hipMemcpy(deltas, p->d_deltas + G->rep0, sizeof(float) * G->R, hipMemcpyDeviceToHost));
+    return ISING_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Learned proposal
+// ---------------------------------------------------------------------------
+
+static int need_learn(ising_pt* p) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    if (p->ncomp == 0) return fail(ISING_E_STATE, "the learned proposal needs an XY or Heisenberg ensemble");
+    if (!p->d_lnet) return fail(ISING_E_STATE, "call ising_pt_learn_init first");
+    return ISING_OK;
+}
+
+int ising_pt_learn_init(ising_pt_t* p, int32_t hidden, uint64_t seed) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    if (p->ncomp == 0) return fail(ISING_E_STATE, "the learned proposal needs an XY or Heisenberg ensemble");
+    const int P = learn_params(hidden);
+    if (P < 0) return fail(ISING_E_ARG, "hidden width must be 8, 16, 32 or 64");
+    HIP_TRY(hipStreamSynchronize(p->st));
+    const int cap = 4096;  // training blocks per step (64 samples each)
+    (void)hipFree(p->d_lmaster); (void)hipFree(p->d_lnet); (void)hipFree(p->d_lm1); (void)hipFree(p->d_lm2);
+    (void)hipFree(p->d_lgpart); (void)hipFree(p->d_lgout); (void)hipFree(p->d_llpart);
+    p->d_lmaster = nullptr; p->d_lnet = nullptr;
+    HIP_TRY(hipMalloc(&p->d_lmaster, sizeof(float) * P));
+    HIP_TRY(hipMalloc(&p->d_lnet, sizeof(uint16_t) * P));
+    HIP_TRY(hipMalloc(&p->d_lm1, sizeof(float) * P));
+    HIP_TRY(hipMalloc(&p->d_lm2, sizeof(float) * P));
+    HIP_TRY(hipMalloc(&p->d_lgpart, sizeof(float) * (size_t)P * cap));
+    HIP_TRY(hipMalloc(&p->d_lgout, sizeof(float) * P));
+    HIP_TRY(hipMalloc(&p->d_llpart, sizeof(double) * cap));
+    p->learn_H = hidden;
+    p->learn_P = P;
+    p->learn_cap_blocks = cap;
+    p->learn_step = 0;
+    // W ~ N(0, 1/fan_in) (output layer 0.1 of that), biases 0; Box-Muller on splitmix64
+    std::vector<float> w(P, 0.0f);
+    uint64_t state = seed ^ 0x5DEECE66DULL;
+    auto next = [&]() {
+        uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return ((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto gauss = [&]() {
+        const double u1 = next() + 1e-300, u2 = next();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    const int H = hidden;
+    int o = 0;
+    for (int i = 0; i < H * 6; ++i) w[o++] = (float)(gauss() / std::sqrt(6.0));
+    o += H;
+    for (int i = 0; i < H * H; ++i) w[o++] = (float)(gauss() / std::sqrt((double)H));
+    o += H;
+    for (int i = 0; i < H * H; ++i) w[o++] = (float)(gauss() / std::sqrt((double)H));
+    o += H;
+    for (int i = 0; i < 3 * H; ++i) w[o++] = (float)(0.1 * gauss() / std::sqrt((double)H));
+    HIP_TRY(hipMemcpy(p->d_lmaster, w.data(), sizeof(float) * P, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(p->d_lm1, 0, sizeof(float) * P));
+    HIP_TRY(hipMemset(p->d_lm2, 0, sizeof(float) * P));
+    HIP_TRY(launch_learn_pack(p->d_lmaster, p->d_lnet, P, p->st));
+    HIP_TRY(hipStreamSynchronize(p->st));
+    if (p->graph_exec) {  // captured launches carry the old network pointer
+        (void)hipGraphExecDestroy(p->graph_exec);
+        p->graph_exec = nullptr;
+    }
+    return ISING_OK;
+}
+
+int ising_pt_learn_size(ising_pt_t* p, int64_t* n) {
+    const int rc = need_learn(p);
+    if (rc) return rc;
+    if (!n) return fail(ISING_E_ARG, "null argument");
+    *n = p->learn_P;
+    return ISING_OK;
+}
+
+int ising_pt_learn_get_params(ising_pt_t* p, float* out, int64_t n) {
+    const int rc = need_learn(p);
+    if (rc) return rc;
+    if (!out || n != p->learn_P) return fail(ISING_E_ARG, "need exactly ising_pt_learn_size() floats");
+    HIP_TRY(hipStreamSynchronize(p->st));
+    HIP_TRY(hipMemcpy(out, p->d_lmaster, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return ISING_OK;
+}
+
+int ising_pt_learn_set_params(ising_pt_t* p, const float* in, int64_t n) {
+    const int rc = need_learn(p);
+    if (rc) return rc;
+    if (!in || n != p->learn_P) return fail(ISING_E_ARG, "need exactly ising_pt_learn_size() floats");
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(in[i])) return fail(ISING_E_ARG, "parameters must be finite");
+    HIP_TRY(hipStreamSynchronize(p->st));
+    HIP_TRY(hipMemcpy(p->d_lmaster, in, sizeof(float) * n, hipMemcpyHostToDevice));
+    HIP_TRY(launch_learn_pack(p->d_lmaster, p->d_lnet, (int)n, p->st));
+    HIP_TRY(hipStreamSynchronize(p->st));
+    return ISING_OK;
+}
+
+int ising_pt_learn_train(ising_pt_t* p, int32_t steps, int32_t batch, double lr, double* mean_loss) {
+    int rc = need_learn(p);
+    if (rc) return rc;
+    if (steps < 0 || batch < kLearnBlock || !(lr > 0.0)) return fail(ISING_E_ARG, "steps >= 0, batch >= 64, lr > 0");
+    const int nblk = std::min(p->learn_cap_blocks, batch / kLearnBlock);
+    const ContArgs a = p->cont_args();
+    for (int s = 0; s < steps; ++s) {
+        ++p->learn_step;
+        HIP_TRY(launch_learn_grad(a, nblk, p->learn_step, 0, nullptr, nullptr, 0, p->d_lgpart, p->d_llpart, p->st));
+        HIP_TRY(launch_learn_adam(p->d_lmaster, p->d_lnet, p->d_lm1, p->d_lm2, p->d_lgpart, nblk, p->learn_P,
+                                  1.0f / (float)(nblk * kLearnBlock), (float)lr, p->learn_step, nullptr, p->st));
+    }
+    if (mean_loss) {
+        std::vector<double> lp(nblk);
+        HIP_TRY(hipMemcpyAsync(lp.data(), p->d_llpart, sizeof(double) * nblk, hipMemcpyDeviceToHost, p->st));
+        HIP_TRY(hipStreamSynchronize(p->st));
+        double sum = 0.0;
+        for (double v : lp) sum += v;
+        *mean_loss = steps > 0 ? sum / (double)(nblk * kLearnBlock) : NAN;
+    }
+    return ISING_OK;
+}
+
+int ising_pt_learn_eval(ising_pt_t* p, int64_t n, const float* x, const float* c, double* mean_loss, float* grad) {
+    int rc = need_learn(p);
+    if (rc) return rc;
+    if (n < 1 || !x || !c) return fail(ISING_E_ARG, "need n >= 1 samples");
+    const int nblk = (int)((n + kLearnBlock - 1) / kLearnBlock);
+    if (nblk > p->learn_cap_blocks) return fail(ISING_E_ARG, "too many samples");
+    float *dx = nullptr, *dc = nullptr;
+    HIP_TRY(hipMalloc(&dx, sizeof(float) * 6 * n));
+    HIP_TRY(hipMalloc(&dc, sizeof(float) * n));
+    HIP_TRY(hipMemcpy(dx, x, sizeof(float) * 6 * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dc, c, sizeof(float) * n, hipMemcpyHostToDevice));
+    const ContArgs a = p->cont_args();
+    hipError_t e = launch_learn_grad(a, nblk, 0, 1, dx, dc, n, p->d_lgpart, p->d_llpart, p->st);
+    if (e == hipSuccess)
+        e = launch_learn_adam(p->d_lmaster, p->d_lnet, p->d_lm1, p->d_lm2, p->d_lgpart, nblk, p->learn_P,
+                              1.0f / (float)n, 0.0f, 1, p->d_lgout, p->st);
+    std::vector<double> lp(nblk);
+    if (e == hipSuccess) e = hipMemcpyAsync(lp.data(), p->d_llpart, sizeof(double) * nblk, hipMemcpyDeviceToHost, p->st);
+    if (e == hipSuccess && grad)
+        e = hipMemcpyAsync(grad, p->d_lgout, sizeof(float) * p->learn_P, hipMemcpyDeviceToHost, p->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->st);
+    (void)hipFree(dx);
+    (void)hipFree(dc);
+    HIP_TRY(e);
+    if (mean_loss) {
+        double sum = 0.0;
+        for (double v : lp) sum += v;
+        *mean_loss = sum / (double)n;
+    }
+    return ISING_OK;
+}
+
+int ising_pt_learn_enable(ising_pt_t* p, int32_t on) {
+    if (on) {
+        const int rc = need_learn(p);
+        if (rc) return rc;
+    } else if (!p) {
+        return fail(ISING_E_ARG, "null handle");
+    }
+    if ((bool)on != p->learned_on && p->graph_exec) {
+        HIP_TRY(hipStreamSynchronize(p->st));
+        (void)hipGraphExecDestroy(p->graph_exec);
+        p->graph_exec = nullptr;
+    }
+    p->learned_on = on != 0;
+    return ISING_OK;
+}
+
+int ising_pt_mh_stats(ising_pt_t* p, int64_t* tries, int64_t* accepts) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    DevCtl c;
+    const int rc = p->read_ctl(&c);
+    if (rc) return rc;
+    if (tries) *tries = c.mh_tries;
+    if (accepts) *accepts = c.mh_accepts;
+    return ISING_OK;
+}
+
+int ising_pt_metropolis_pass(ising_pt_t* p, int32_t color, int64_t t) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    if (p->ncomp == 0) return fail(ISING_E_STATE, "continuous ensembles only");
+    if ((color != 0 && color != 1) || t < 0) return fail(ISING_E_ARG, "color 0/1 and t >= 0");
+    if (p->learned_on) HIP_TRY(launch_cont_learned(p->cont_args(), p->nblocks, color, 0, t, p->st));
+    else return fail(ISING_E_STATE, "enable the learned proposal first");
     return ISING_OK;
 }
 

@@ -135,6 +135,8 @@ struct DevCtl {
     int32_t overflow;      // set when a record did not fit
     int64_t exch_tries;
     int64_t exch_accepts;
+    int64_t mh_tries;      // learned-proposal Metropolis-Hastings moves attempted
+    int64_t mh_accepts;
 };
 
 struct IsingRec {

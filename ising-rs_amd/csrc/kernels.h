@@ -59,7 +59,19 @@ struct ContArgs {
     float adapt_target;    // target acceptance rate
     int32_t inline_params; // 1: beta/delta from the two fields below (single-lattice passes)
     float beta_inline, delta_inline;
+    const uint16_t* net;   // learned proposal: BF16 parameters (null when off)
+    int32_t learn_H;       // hidden width of the learned proposal's MLP
 };
+
+int learn_params(int H);  // parameter count of the MLP with hidden width H (8, 16, 32, 64), -1 otherwise
+constexpr int kLearnBlock = 64;  // samples per training block
+hipError_t launch_cont_learned(const ContArgs& a, int nblocks, int color, int closes, int64_t t_override,
+                               hipStream_t st);
+hipError_t launch_learn_grad(const ContArgs& a, int nblk, int64_t step, int mode, const float* xin, const float* cin,
+                             int64_t n_in, float* gpart, double* lpart, hipStream_t st);
+hipError_t launch_learn_adam(float* master, uint16_t* net, float* m1, float* m2, const float* gpart, int nblk, int P,
+                             float inv_n, float lr, int64_t step, float* gout, hipStream_t st);
+hipError_t launch_learn_pack(const float* master, uint16_t* net, int P, hipStream_t st);
 
 hipError_t launch_ising_close(const IsingArgs& a, hipStream_t st);
 hipError_t launch_cont_metro(const ContArgs& a, int nblocks, int color, int closes, hipStream_t st);

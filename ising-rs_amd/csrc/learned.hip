@@ -1,0 +1,607 @@
+// Learned proposal for XY / Heisenberg Metropolis-Hastings, gfx950.
+//
+// A small MLP maps rotation-invariant features of a site's six neighbours to a
+// two-component mixture around the local-field direction h^ = h/|h|:
+//   x = [beta, beta|h|, |h|/6, <q^2>, <q^3>, <q^4>],  q_j = S_j . h^
+//   6 -> H (tanh) -> H (tanh) -> H (tanh) -> 3:   kappa_k = softplus(o_k) + 1e-3,
+//   w_1 = sigmoid(o_2), w_2 = 1 - w_1
+// Components, in c = s . h^:  XY von Mises  f = e^{kappa c} / (2 pi I0(kappa)),
+//                             Heisenberg Fisher f = kappa e^{kappa c} / (4 pi sinh kappa).
+// Weights are stored in BF16 and every product is accumulated in FP32.
+//
+// The proposal depends only on the neighbours, which a checkerboard pass does
+// not change, so the move s -> s' is accepted with the exact Metropolis-Hastings
+// ratio  r = exp(beta J |h| (c' - c)) q(c) / q(c').  Proposals are drawn exactly:
+// Heisenberg by inverting the Fisher CDF; XY by the Best-Fisher rejection
+// sampler (Appl. Stat. 28:152, 1979) with at most kVmAttempts attempts, after
+// which the site keeps its spin -- that failure probability depends only on
+// the neighbours, so it cancels between forward and reverse moves.
+//
+// Training: maximum likelihood of the single-site conditional on sites of the
+// current PT configurations (loss = -log q(s | neighbours)); the minimiser is
+// the exact conditional (heat bath).  Per-block gradient partials are reduced
+// in a fixed order by the Adam kernel, so training is deterministic.
+#include "kernels.h"
+
+namespace ising {
+
+constexpr int kLearnThreads = 64;  // training: one wave per block, one sample per lane
+constexpr int kVmAttempts = 8;
+constexpr int kNIn = 6;
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);  // round to nearest even
+}
+
+// exp(-x) I0(x) and exp(-x) I1(x), x >= 0: Abramowitz & Stegun 9.8.1-9.8.4.
+__device__ __forceinline__ float i0e_f(float x) {
+    if (x <= 3.75f) {
+        const float t = (x / 3.75f) * (x / 3.75f);
+        const float p = 1.0f + t * (3.5156229f + t * (3.0899424f + t * (1.2067492f + t * (0.2659732f + t * (0.0360768f + t * 0.0045813f)))));
+        return p * __expf(-x);
+    }
+    const float y = 3.75f / x;
+    const float p = 0.39894228f + y * (0.01328592f + y * (0.00225319f + y * (-0.00157565f + y * (0.00916281f + y * (-0.02057706f + y * (0.02635537f + y * (-0.01647633f + y * 0.00392377f)))))));
+    return p * rsqrtf(x);
+}
+__device__ __forceinline__ float i1e_f(float x) {
+    if (x <= 3.75f) {
+        const float t = (x / 3.75f) * (x / 3.75f);
+        const float p = 0.5f + t * (0.87890594f + t * (0.51498869f + t * (0.15084934f + t * (0.02658733f + t * (0.00301532f + t * 0.00032411f)))));
+        return x * p * __expf(-x);
+    }
+    const float y = 3.75f / x;
+    const float p = 0.39894228f + y * (-0.03988024f + y * (-0.00362018f + y * (0.00163801f + y * (-0.01031555f + y * (0.02282967f + y * (-0.02895312f + y * (0.01787654f + y * -0.00420059f)))))));
+    return p * rsqrtf(x);
+}
+
+__device__ __forceinline__ float tanh_f(float x) {
+    const float e = __expf(-2.0f * fabsf(x));
+    const float r = (1.0f - e) / (1.0f + e);
+    return copysignf(r, x);
+}
+__device__ __forceinline__ float softplus_f(float x) { return x > 15.0f ? x : log1pf(__expf(x)); }
+__device__ __forceinline__ float sigmoid_f(float x) { return 1.0f / (1.0f + __expf(-x)); }
+
+// log f_k(c) per component (see header).
+template <int N>
+__device__ __forceinline__ float log_comp(float kap, float c) {
+    if (N == 2) return kap * (c - 1.0f) - 1.8378770664093453f - logf(i0e_f(kap));
+    return logf(kap) + kap * (c - 1.0f) - 1.8378770664093453f - log1pf(-expf(-2.0f * kap));
+}
+// E[c] under one component: I1/I0 (XY), coth k - 1/k (Heisenberg); derivative of log f is c - mean_c.
+template <int N>
+__device__ __forceinline__ float mean_c(float kap) {
+    if (N == 2) return i1e_f(kap) / i0e_f(kap);
+    if (kap < 1e-2f) return kap * (1.0f / 3.0f - kap * kap / 45.0f);
+    const float e = __expf(-2.0f * kap);
+    return (1.0f + e) / (1.0f - e) - 1.0f / kap;
+}
+
+struct Mix {
+    float k0, k1, w0;
+};
+
+__device__ __forceinline__ Mix mixture(const float o[3]) {
+    Mix m;
+    m.k0 = softplus_f(o[0]) + 1e-3f;
+    m.k1 = softplus_f(o[1]) + 1e-3f;
+    m.w0 = sigmoid_f(o[2]);
+    return m;
+}
+
+template <int N>
+__device__ __forceinline__ float log_q(const Mix& m, float c) {
+    const float l0 = logf(m.w0) + log_comp<N>(m.k0, c);
+    const float l1 = logf(1.0f - m.w0) + log_comp<N>(m.k1, c);
+    const float mx = fmaxf(l0, l1);
+    return mx + logf(expf(l0 - mx) + expf(l1 - mx));
+}
+
+// Parameter offsets of the flat vector W1,b1,W2,b2,W3,b3,W4,b4 (row-major).
+template <int H>
+struct Off {
+    static constexpr int W1 = 0, b1 = W1 + H * kNIn, W2 = b1 + H, b2 = W2 + H * H, W3 = b2 + H, b3 = W3 + H * H,
+                         W4 = b3 + H, b4 = W4 + 3 * H, P = b4 + 3;
+};
+
+// Forward pass for one sample; activations written to act[3][H] when non-null.
+template <int H>
+__device__ __forceinline__ void mlp_forward(const uint16_t* __restrict__ w, const float x[kNIn], float o[3],
+                                            float* act, int act_stride) {
+    using F = Off<H>;
+    float a[H], b[H];
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+        float s = bf2f(w[F::b1 + j]);
+#pragma unroll
+        for (int i = 0; i < kNIn; ++i) s += bf2f(w[F::W1 + j * kNIn + i]) * x[i];
+        a[j] = tanh_f(s);
+        if (act) act[(0 * H + j) * act_stride] = a[j];
+    }
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+        float s = bf2f(w[F::b2 + j]);
+#pragma unroll
+        for (int i = 0; i < H; ++i) s += bf2f(w[F::W2 + j * H + i]) * a[i];
+        b[j] = tanh_f(s);
+        if (act) act[(1 * H + j) * act_stride] = b[j];
+    }
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+        float s = bf2f(w[F::b3 + j]);
+#pragma unroll
+        for (int i = 0; i < H; ++i) s += bf2f(w[F::W3 + j * H + i]) * b[i];
+        a[j] = tanh_f(s);
+        if (act) act[(2 * H + j) * act_stride] = a[j];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float s = bf2f(w[F::b4 + k]);
+#pragma unroll
+        for (int i = 0; i < H; ++i) s += bf2f(w[F::W4 + k * H + i]) * a[i];
+        o[k] = s;
+    }
+}
+
+// Features of a site from its neighbour vectors (N components each).
+template <int N>
+__device__ __forceinline__ float site_features(float beta, const float nv[3][6], float x[kNIn], float hhat[3]) {
+    float h[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < N; ++k) h[k] = ((nv[k][0] + nv[k][1]) + (nv[k][2] + nv[k][3])) + (nv[k][4] + nv[k][5]);
+    float hn2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < N; ++k) hn2 += h[k] * h[k];
+    const float hn = sqrtf(hn2);
+    const float inv = hn > 1e-20f ? 1.0f / hn : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) hhat[k] = k < N ? h[k] * inv : 0.0f;
+    float m2 = 0.f, m3 = 0.f, m4 = 0.f;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        float d = 0.f;
+#pragma unroll
+        for (int k = 0; k < N; ++k) d += nv[k][q] * hhat[k];
+        const float d2 = d * d;
+        m2 += d2;
+        m3 += d2 * d;
+        m4 += d2 * d2;
+    }
+    x[0] = beta;
+    x[1] = beta * hn;
+    x[2] = hn * (1.0f / 6.0f);
+    x[3] = m2 * (1.0f / 6.0f);
+    x[4] = m3 * (1.0f / 6.0f);
+    x[5] = m4 * (1.0f / 6.0f);
+    return hn;
+}
+
+// ---------------------------------------------------------------------------
+// Learned Metropolis-Hastings pass over one colour (replaces the cap/rotation
+// proposal when the ensemble's learned mode is on).  Same measurement fusion
+// as the plain pass.
+// ---------------------------------------------------------------------------
+template <int N, int H>
+__global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int color, int closes, int64_t t_override) {
+    __shared__ double s_red[4][4];
+    __shared__ uint16_t s_w[Off<H>::P];
+    int g = 0;
+    while (g + 1 < a.ngroups && (int)blockIdx.x >= a.groups[g + 1].blk0) ++g;
+    const DevGroup G = a.groups[g];
+    const int local = blockIdx.x - G.blk0;
+    const int r = local / G.tiles;
+    const int tile = local - r * G.tiles;
+    const int grep = G.rep0 + r;
+    const int ti = G.rep0 + a.r2t[grep];
+    for (int i = threadIdx.x; i < Off<H>::P; i += blockDim.x) s_w[i] = a.net[i];
+    const float beta = a.betas_f[ti];
+    const int64_t t = t_override >= 0 ? t_override : a.ctl->t;
+    bool meas, exch;
+    int64_t s;
+    sweep_flags(a.ctl, t, &meas, &exch, &s);
+    const bool need = (meas || exch) && closes;
+    const bool renorm = a.renorm_every > 0 && ((t + 1) % a.renorm_every) == 0;
+    const uint32_t hs = (uint32_t)(2 * t + color);
+    const uint32_t stream = stream_id((uint32_t)G.L, (uint32_t)r);
+    __syncthreads();
+
+    float* C = a.pool + G.off + (int64_t)(2 * r + color) * N * G.stride;
+    const float* O = a.pool + G.off + (int64_t)(2 * r + (color ^ 1)) * N * G.stride;
+    const int64_t h = (int64_t)tile * 256 + threadIdx.x;
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+    int acc_n = 0, tried = 0;
+    if (h < G.n_half) {
+        // neighbours (same addressing as the plain pass)
+        const int L = G.L, Lh = G.Lh;
+        const int64_t row = h / Lh;
+        const int xh = (int)(h - row * Lh);
+        const int z = (int)(row / L), y = (int)(row - (int64_t)z * L);
+        const int p = (y + z + color) & 1;
+        const int ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
+        const int zm = z == 0 ? L - 1 : z - 1, zp = z == L - 1 ? 0 : z + 1;
+        const int xo = p ? (xh + 1 == Lh ? 0 : xh + 1) : (xh == 0 ? Lh - 1 : xh - 1);
+        const int64_t nb[6] = {row * Lh + xh, row * Lh + xo, ((int64_t)z * L + ym) * Lh + xh,
+                               ((int64_t)z * L + yp) * Lh + xh, ((int64_t)zm * L + y) * Lh + xh,
+                               ((int64_t)zp * L + y) * Lh + xh};
+        float nv[3][6];
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) nv[k][q] = O[k * G.stride + nb[q]];
+        float S[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < N; ++k) S[k] = C[k * G.stride + h];
+        float x[kNIn], hh[3];
+        const float hn = site_features<N>(beta, nv, x, hh);
+        bool accept = false;
+        float nw[3] = {S[0], S[1], S[2]};
+        if (hn > 1e-6f) {
+            float o[3];
+            mlp_forward<H>(s_w, x, o, nullptr, 0);
+            const Mix m = mixture(o);
+            const u32x4 w = philox((uint32_t)h, hs, stream, (uint32_t)((7u << 8) | a.model), a.k0, a.k1);
+            const float kap = u24f(w.x) < m.w0 ? m.k0 : m.k1;
+            float cn = 0.f;
+            bool have = false;
+            if (N == 3) {
+                // Fisher: c = 1 + log(u + (1 - u) e^{-2 kappa}) / kappa, azimuth uniform
+                const float u = u24f(w.y);
+                cn = 1.0f + __logf(u + (1.0f - u) * __expf(-2.0f * kap)) / kap;
+                cn = fminf(1.0f, fmaxf(-1.0f, cn));
+                float sp, cp;
+                sincosf(6.28318530717958647692f * u24f(w.z), &sp, &cp);
+                // orthonormal e1, e2 perpendicular to hh
+                float t0 = 0.f, t1 = 0.f;
+                if (fabsf(hh[0]) < 0.9f) t0 = 1.f; else t1 = 1.f;
+                const float th = t0 * hh[0] + t1 * hh[1];
+                float e0 = t0 - th * hh[0], e1 = t1 - th * hh[1], e2 = -th * hh[2];
+                const float ei = rsqrtf(e0 * e0 + e1 * e1 + e2 * e2);
+                e0 *= ei; e1 *= ei; e2 *= ei;
+                const float f0 = hh[1] * e2 - hh[2] * e1, f1 = hh[2] * e0 - hh[0] * e2, f2 = hh[0] * e1 - hh[1] * e0;
+                const float sn = sqrtf(fmaxf(0.0f, 1.0f - cn * cn));
+                nw[0] = cn * hh[0] + sn * (cp * e0 + sp * f0);
+                nw[1] = cn * hh[1] + sn * (cp * e1 + sp * f1);
+                nw[2] = cn * hh[2] + sn * (cp * e2 + sp * f2);
+                have = true;
+            } else {
+                // von Mises by Best-Fisher rejection
+                const float sr = 0.5f / kap;
+                const float rr = sr + sqrtf(1.0f + sr * sr);
+#pragma unroll 1
+                for (int at = 1; at <= kVmAttempts && !have; ++at) {
+                    const u32x4 v = philox((uint32_t)h, hs, stream, (uint32_t)((7u << 8) | a.model | (at << 16)),
+                                           a.k0, a.k1);
+                    const float zc = __cosf(3.14159265358979f * u24f(v.x));
+                    const float W = (1.0f + rr * zc) / (rr + zc);
+                    const float Y = kap * (rr - W);
+                    const float V = fmaxf(u24f(v.y), 1e-30f);
+                    if (Y * (2.0f - Y) - V >= 0.0f || __logf(Y / V) + 1.0f - Y >= 0.0f) {
+                        cn = fminf(1.0f, fmaxf(-1.0f, W));
+                        const float sn = sqrtf(fmaxf(0.0f, 1.0f - cn * cn)) * (u24f(v.z) < 0.5f ? -1.0f : 1.0f);
+                        nw[0] = cn * hh[0] - sn * hh[1];
+                        nw[1] = sn * hh[0] + cn * hh[1];
+                        have = true;
+                    }
+                }
+            }
+            if (have) {
+                float co = 0.f;
+#pragma unroll
+                for (int k = 0; k < N; ++k) co += S[k] * hh[k];
+                const float lr = beta * a.Jf * hn * (cn - co) + log_q<N>(m, co) - log_q<N>(m, cn);
+                accept = lr >= 0.0f || __logf(fmaxf(u24f(w.w), 1e-30f)) < lr;
+                tried = 1;
+            }
+        }
+        if (accept) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) S[k] = nw[k];
+            acc_n = 1;
+        }
+        if (renorm) {
+            float q2 = 0.f;
+#pragma unroll
+            for (int k = 0; k < N; ++k) q2 += S[k] * S[k];
+            const float qi = rsqrtf(q2);
+#pragma unroll
+            for (int k = 0; k < N; ++k) S[k] *= qi;
+        }
+        if (accept || renorm) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) C[k * G.stride + h] = S[k];
+        }
+        if (need) {
+            double e = 0.0;
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                part[1 + k] = (double)S[k];
+                if (color) {
+                    double f = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) f += (double)nv[k][q];
+                    e += (double)S[k] * f;
+                }
+            }
+            part[0] = color ? -a.J * e : 0.0;
+        }
+    }
+    // acceptance statistics (every pass)
+    {
+        const int wa = wave_sum(acc_n), wt = wave_sum(tried);
+        if ((threadIdx.x & 63) == 0 && wt) {
+            atomicAdd((unsigned long long*)&a.ctl->mh_tries, (unsigned long long)wt);
+            atomicAdd((unsigned long long*)&a.ctl->mh_accepts, (unsigned long long)wa);
+        }
+    }
+    if (need) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) part[k] = wave_sum(part[k]);
+        if ((threadIdx.x & 63) == 0)
+            for (int k = 0; k < 4; ++k) s_red[k][threadIdx.x >> 6] = part[k];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double* dst = (color ? a.part1 : a.part0) + 4 * (int64_t)blockIdx.x;
+            for (int k = 0; k < 4; ++k) dst[k] = s_red[k][0] + s_red[k][1] + s_red[k][2] + s_red[k][3];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Training: one sample per lane, 64 samples per block.  Samples are sites of
+// the current configurations (mode 0) or explicit feature rows (mode 1, for
+// gradient checks).  Block partial gradients go to gpart[block][P].
+// ---------------------------------------------------------------------------
+template <int N, int H>
+__global__ __launch_bounds__(kLearnThreads) void learn_grad_kernel(ContArgs a, int64_t step, int mode,
+                                                                    const float* __restrict__ xin,
+                                                                    const float* __restrict__ cin, int64_t n_in,
+                                                                    float* __restrict__ gpart,
+                                                                    double* __restrict__ lpart) {
+    using F = Off<H>;
+    __shared__ float s_x[kNIn][kLearnThreads];
+    __shared__ float s_a[3][H][kLearnThreads];   // activations of layers 1..3
+    __shared__ float s_d[H][kLearnThreads];      // back-propagated pre-activation gradient
+    __shared__ float s_do[3][kLearnThreads];
+    __shared__ uint16_t s_w[F::P];
+    const int t = threadIdx.x;
+    for (int i = t; i < F::P; i += kLearnThreads) s_w[i] = a.net[i];
+    __syncthreads();
+    float x[kNIn], c = 0.f;
+    bool live = true;
+    if (mode == 1) {
+        const int64_t idx = (int64_t)blockIdx.x * kLearnThreads + t;
+        live = idx < n_in;
+        for (int i = 0; i < kNIn; ++i) x[i] = live ? xin[idx * kNIn + i] : 0.0f;
+        c = live ? cin[idx] : 0.0f;
+    } else {
+        // a uniformly random site of a uniformly random replica of the ensemble
+        const u32x4 w = philox((uint32_t)(blockIdx.x * kLearnThreads + t), (uint32_t)step, 0xFFFF0000u,
+                               (uint32_t)((8u << 8) | a.model), a.k0, a.k1);
+        const int gr = (int)(((uint64_t)w.x * (uint32_t)a.total_reps) >> 32);
+        int g = 0;
+        while (g + 1 < a.ngroups && gr >= a.groups[g + 1].rep0) ++g;
+        const DevGroup G = a.groups[g];
+        const int r = gr - G.rep0;
+        const int color = (int)(w.z & 1u);
+        const int64_t h = (int64_t)(((uint64_t)w.y * (uint64_t)G.n_half) >> 32);
+        const int L = G.L, Lh = G.Lh;
+        const int64_t row = h / Lh;
+        const int xh = (int)(h - row * Lh);
+        const int z = (int)(row / L), y = (int)(row - (int64_t)z * L);
+        const int p = (y + z + color) & 1;
+        const int ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
+        const int zm = z == 0 ? L - 1 : z - 1, zp = z == L - 1 ? 0 : z + 1;
+        const int xo = p ? (xh + 1 == Lh ? 0 : xh + 1) : (xh == 0 ? Lh - 1 : xh - 1);
+        const int64_t nb[6] = {row * Lh + xh, row * Lh + xo, ((int64_t)z * L + ym) * Lh + xh,
+                               ((int64_t)z * L + yp) * Lh + xh, ((int64_t)zm * L + y) * Lh + xh,
+                               ((int64_t)zp * L + y) * Lh + xh};
+        const float* Cp = a.pool + G.off + (int64_t)(2 * r + color) * N * G.stride;
+        const float* Op = a.pool + G.off + (int64_t)(2 * r + (color ^ 1)) * N * G.stride;
+        float nv[3][6];
+        for (int k = 0; k < N; ++k)
+            for (int q = 0; q < 6; ++q) nv[k][q] = Op[k * G.stride + nb[q]];
+        float hh[3];
+        const float beta = a.betas_f[G.rep0 + a.r2t[gr]];
+        const float hn = site_features<N>(beta, nv, x, hh);
+        live = hn > 1e-6f;
+        for (int k = 0; k < N; ++k) c += Cp[k * G.stride + h] * hh[k];
+    }
+    for (int i = 0; i < kNIn; ++i) s_x[i][t] = x[i];
+    float o[3];
+    mlp_forward<H>(s_w, x, o, &s_a[0][0][t], kLearnThreads);
+    // loss and d loss / d o for this sample
+    const Mix m = mixture(o);
+    const float l0 = __logf(m.w0) + log_comp<N>(m.k0, c);
+    const float l1 = __logf(1.0f - m.w0) + log_comp<N>(m.k1, c);
+    const float mx = fmaxf(l0, l1);
+    const float e0 = __expf(l0 - mx), e1 = __expf(l1 - mx);
+    const float lq = mx + __logf(e0 + e1);
+    const float r0 = e0 / (e0 + e1), r1 = e1 / (e0 + e1);  // responsibilities
+    const float f0 = __expf(log_comp<N>(m.k0, c) - lq), f1 = __expf(log_comp<N>(m.k1, c) - lq);
+    const float sc = live ? 1.0f : 0.0f;
+    s_do[0][t] = sc * -r0 * (c - mean_c<N>(m.k0)) * sigmoid_f(o[0]);
+    s_do[1][t] = sc * -r1 * (c - mean_c<N>(m.k1)) * sigmoid_f(o[1]);
+    s_do[2][t] = sc * -(f0 - f1) * m.w0 * (1.0f - m.w0);
+    const double lsum = wave_sum(live ? (double)(-lq) : 0.0);
+    if (t == 0) lpart[blockIdx.x] = lsum;
+    __syncthreads();
+
+    float* gb = gpart + (int64_t)blockIdx.x * F::P;
+    // output layer: dW4[k][i] = sum_t do[k] a3[i]; db4
+    for (int e = t; e < 3 * H; e += kLearnThreads) {
+        const int k = e / H, i = e - k * H;
+        float sacc = 0.f;
+        for (int u = 0; u < kLearnThreads; ++u) sacc += s_do[k][u] * s_a[2][i][u];
+        gb[F::W4 + e] = sacc;
+    }
+    if (t < 3) {
+        float sacc = 0.f;
+        for (int u = 0; u < kLearnThreads; ++u) sacc += s_do[t][u];
+        gb[F::b4 + t] = sacc;
+    }
+    // d z3 = (W4^T do) (1 - a3^2)
+    for (int i = 0; i < H; ++i) {
+        float v = 0.f;
+        for (int k = 0; k < 3; ++k) v += bf2f(s_w[F::W4 + k * H + i]) * s_do[k][t];
+        const float ai = s_a[2][i][t];
+        s_d[i][t] = v * (1.0f - ai * ai);
+    }
+    __syncthreads();
+    // layer 3 weights from d z3 and a2, then d z2 into s_a[2] (a3 no longer needed)
+    for (int e = t; e < H * H; e += kLearnThreads) {
+        const int j = e / H, i = e - j * H;
+        float sacc = 0.f;
+        for (int u = 0; u < kLearnThreads; ++u) sacc += s_d[j][u] * s_a[1][i][u];
+        gb[F::W3 + e] = sacc;
+    }
+    for (int j = t; j < H; j += kLearnThreads) {
+        float sacc = 0.f;
+        for (int u = 0; u < kLearnThreads; ++u) sacc += s_d[j][u];
+        gb[F::b3 + j] = sacc;
+    }
+    __syncthreads();
+    for (int i = 0; i < H; ++i) {
+        float v = 0.f;
+        for (int j = 0; j < H; ++j) v += bf2f(s_w[F::W3 + j * H + i]) * s_d[j][t];
+        const float ai = s_a[1][i][t];
+        s_a[2][i][t] = v * (1.0f - ai * ai);
+    }
+    __syncthreads();
+    // layer 2 weights from d z2 (in s_a[2]) and a1; then d z1 into s_d
+    for (int e = t; e < H * H; e += kLearnThreads) {
+        const int j = e / H, i = e - j * H;
+        float sacc = 0.f;
+        for (int u = 0; u < kLearnThreads; ++u) sacc += s_a[2][j][u] * s_a[0][i][u];
+        gb[F::W2 + e] = sacc;
+    }
+    for (int j = t; j < H; j += kLearnThreads) {
+        float sacc = 0.f;
+        for (int u = 0; u < kLearnThreads; ++u) sacc += s_a[2][j][u];
+        gb[F::b2 + j] = sacc;
+    }
+    __syncthreads();
+    for (int i = 0; i < H; ++i) {
+        float v = 0.f;
+        for (int j = 0; j < H; ++j) v += bf2f(s_w[F::W2 + j * H + i]) * s_a[2][j][t];
+        const float ai = s_a[0][i][t];
+        s_d[i][t] = v * (1.0f - ai * ai);
+    }
+    __syncthreads();
+    for (int e = t; e < H * kNIn; e += kLearnThreads) {
+        const int j = e / kNIn, i = e - j * kNIn;
+        float sacc = 0.f;
+        for (int u = 0; u < kLearnThreads; ++u) sacc += s_d[j][u] * s_x[i][u];
+        gb[F::W1 + e] = sacc;
+    }
+    for (int j = t; j < H; j += kLearnThreads) {
+        float sacc = 0.f;
+        for (int u = 0; u < kLearnThreads; ++u) sacc += s_d[j][u];
+        gb[F::b1 + j] = sacc;
+    }
+}
+
+// Sum block partials in block order (deterministic), scale by 1/n, then Adam
+// (or, with lr == 0, just write the mean gradient to `gout`).
+__global__ void learn_adam_kernel(float* master, uint16_t* net, float* m1, float* m2, const float* gpart, int nblk,
+                                  int P, float inv_n, float lr, float b1, float b2, float eps, int64_t step,
+                                  float* gout) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float g = 0.f;
+    for (int b = 0; b < nblk; ++b) g += gpart[(int64_t)b * P + p];
+    g *= inv_n;
+    if (gout) gout[p] = g;
+    if (lr == 0.0f) return;
+    const float mm = b1 * m1[p] + (1.0f - b1) * g;
+    const float vv = b2 * m2[p] + (1.0f - b2) * g * g;
+    m1[p] = mm;
+    m2[p] = vv;
+    const float bc1 = 1.0f - powf(b1, (float)step), bc2 = 1.0f - powf(b2, (float)step);
+    const float th = master[p] - lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+    master[p] = th;
+    net[p] = f2bf(th);
+}
+
+__global__ void learn_pack_kernel(const float* master, uint16_t* net, int P) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < P) net[p] = f2bf(master[p]);
+}
+
+template <int H>
+int learn_param_count() {
+    return Off<H>::P;
+}
+
+int learn_params(int H) {
+    switch (H) {
+        case 8: return Off<8>::P;
+        case 16: return Off<16>::P;
+        case 32: return Off<32>::P;
+        case 64: return Off<64>::P;
+    }
+    return -1;
+}
+
+template <int N, int H>
+static hipError_t launch_learned_nh(const ContArgs& a, int nblocks, int color, int closes, int64_t t_override,
+                                    hipStream_t st) {
+    hipLaunchKernelGGL((cont_learned_kernel<N, H>), dim3(nblocks), dim3(256), 0, st, a, color, closes, t_override);
+    return hipGetLastError();
+}
+
+hipError_t launch_cont_learned(const ContArgs& a, int nblocks, int color, int closes, int64_t t_override,
+                               hipStream_t st) {
+#define LEARN_CASE(HH)                                                                              \
+    case HH:                                                                                        \
+        return a.ncomp == 2 ? launch_learned_nh<2, HH>(a, nblocks, color, closes, t_override, st)   \
+                            : launch_learned_nh<3, HH>(a, nblocks, color, closes, t_override, st);
+    switch (a.learn_H) {
+        LEARN_CASE(8)
+        LEARN_CASE(16)
+        LEARN_CASE(32)
+        LEARN_CASE(64)
+    }
+#undef LEARN_CASE
+    return hipErrorInvalidValue;
+}
+
+template <int N, int H>
+static hipError_t launch_grad_nh(const ContArgs& a, int nblk, int64_t step, int mode, const float* xin,
+                                 const float* cin, int64_t n_in, float* gpart, double* lpart, hipStream_t st) {
+    hipLaunchKernelGGL((learn_grad_kernel<N, H>), dim3(nblk), dim3(kLearnThreads), 0, st, a, step, mode, xin, cin,
+                       n_in, gpart, lpart);
+    return hipGetLastError();
+}
+
+hipError_t launch_learn_grad(const ContArgs& a, int nblk, int64_t step, int mode, const float* xin, const float* cin,
+                             int64_t n_in, float* gpart, double* lpart, hipStream_t st) {
+#define GRAD_CASE(HH)                                                                                            \
+    case HH:                                                                                                     \
+        return a.ncomp == 2 ? launch_grad_nh<2, HH>(a, nblk, step, mode, xin, cin, n_in, gpart, lpart, st)       \
+                            : launch_grad_nh<3, HH>(a, nblk, step, mode, xin, cin, n_in, gpart, lpart, st);
+    switch (a.learn_H) {
+        GRAD_CASE(8)
+        GRAD_CASE(16)
+        GRAD_CASE(32)
+        GRAD_CASE(64)
+    }
+#undef GRAD_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_learn_adam(float* master, uint16_t* net, float* m1, float* m2, const float* gpart, int nblk, int P,
+                             float inv_n, float lr, int64_t step, float* gout, hipStream_t st) {
+    hipLaunchKernelGGL(learn_adam_kernel, dim3((P + 255) / 256), dim3(256), 0, st, master, net, m1, m2, gpart, nblk, P,
+                       inv_n, lr, 0.9f, 0.999f, 1e-8f, step, gout);
+    return hipGetLastError();
+}
+
+hipError_t launch_learn_pack(const float* master, uint16_t* net, int P, hipStream_t st) {
+    hipLaunchKernelGGL(learn_pack_kernel, dim3((P + 255) / 256), dim3(256), 0, st, master, net, P);
+    return hipGetLastError();
+}
+
+}  // namespace ising

@@ -99,6 +99,15 @@ _SIGS = {
     "ising_pt_get_spins_f32": (C.c_int, [_vp, C.c_int32, C.c_int32, _vp]),
     "ising_pt_set_spins_f32": (C.c_int, [_vp, C.c_int32, C.c_int32, _vp]),
     "ising_pt_get_deltas": (C.c_int, [_vp, C.c_int32, _vp]),
+    "ising_pt_learn_init": (C.c_int, [_vp, C.c_int32, C.c_uint64]),
+    "ising_pt_learn_size": (C.c_int, [_vp, _i64p]),
+    "ising_pt_learn_get_params": (C.c_int, [_vp, _vp, C.c_int64]),
+    "ising_pt_learn_set_params": (C.c_int, [_vp, _vp, C.c_int64]),
+    "ising_pt_learn_train": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_double, _f64p]),
+    "ising_pt_learn_eval": (C.c_int, [_vp, C.c_int64, _vp, _vp, _f64p, _vp]),
+    "ising_pt_learn_enable": (C.c_int, [_vp, C.c_int32]),
+    "ising_pt_mh_stats": (C.c_int, [_vp, _i64p, _i64p]),
+    "ising_pt_metropolis_pass": (C.c_int, [_vp, C.c_int32, C.c_int64]),
     "ising_pt_sweeps": (C.c_int, [_vp, _i64p]),
     "ising_pt_device_bytes": (C.c_int, [_vp, _i64p]),
     "ising_linspace_temperatures": (C.c_int, [C.c_double, C.c_double, C.c_int32, _vp]),
@@ -369,6 +378,47 @@ class PTEnsemble:
         out = np.empty(self.R, np.float32)
         _check(lib.ising_pt_get_deltas(self._h, size_index, _ptr(out)))
         return out
+
+    # ---- learned proposal ----
+    def learn_init(self, hidden=64, seed=0):
+        _check(lib.ising_pt_learn_init(self._h, int(hidden), int(seed)))
+
+    def learn_params(self):
+        n = C.c_int64()
+        _check(lib.ising_pt_learn_size(self._h, C.byref(n)))
+        out = np.empty(n.value, np.float32)
+        _check(lib.ising_pt_learn_get_params(self._h, _ptr(out), n.value))
+        return out
+
+    def learn_set_params(self, params):
+        a = np.ascontiguousarray(params, np.float32)
+        _check(lib.ising_pt_learn_set_params(self._h, _ptr(a), a.size))
+
+    def learn_train(self, steps, batch=4096, lr=1e-3):
+        loss = C.c_double()
+        _check(lib.ising_pt_learn_train(self._h, int(steps), int(batch), float(lr), C.byref(loss)))
+        return loss.value
+
+    def learn_eval(self, x, c):
+        x = np.ascontiguousarray(x, np.float32)
+        c = np.ascontiguousarray(c, np.float32)
+        n = C.c_int64()
+        _check(lib.ising_pt_learn_size(self._h, C.byref(n)))
+        g = np.empty(n.value, np.float32)
+        loss = C.c_double()
+        _check(lib.ising_pt_learn_eval(self._h, len(c), _ptr(x), _ptr(c), C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def learn_enable(self, on=True):
+        _check(lib.ising_pt_learn_enable(self._h, 1 if on else 0))
+
+    def mh_stats(self):
+        a, b = C.c_int64(), C.c_int64()
+        _check(lib.ising_pt_mh_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def metropolis_pass(self, color, t):
+        _check(lib.ising_pt_metropolis_pass(self._h, int(color), int(t)))
 
     def sweeps(self):
         v = C.c_int64()
