@@ -30,16 +30,24 @@ def linspace(a, b, n):
     return [a + (b - a) * i / (n - 1) for i in range(n)]
 
 
-def cpu_baseline(seconds):
+def cpu_baseline(seconds, model="ising"):
     import ctypes as C
 
     import oracle_lib as o
     wall = C.c_double()
-    L = 128
-    upd = o.lib.o_bench_ref_metropolis(L, 1.0 / 4.5115, 42, seconds, C.byref(wall))
-    return {"value": upd / (wall.value * 1e9), "unit": "updates/ns", "cores": 1, "kind": "port",
-            "sample": "oracle restatement of metropolis.rs:8-27 (random-site Metropolis), one L=128 replica at "
-                      "T=4.5115, %d updates in %.1f s on 1 core" % (upd, wall.value)}
+    if model.startswith("ising"):
+        L = 128
+        upd = o.lib.o_bench_ref_metropolis(L, 1.0 / 4.5115, 42, seconds, C.byref(wall))
+        what = ("oracle restatement of metropolis.rs:8-27 (random-site Metropolis), one L=128 replica at T=4.5115, "
+                "%d updates in %.1f s on 1 core" % (upd, wall.value))
+    else:
+        L = 64
+        upd = o.lib.o_bench_ref_heis(L, 1.0 / 1.443, 1.0, 2, 42, seconds, C.byref(wall))
+        upd //= 3  # report Metropolis updates (each sweep = 1 Metropolis + 2 overrelaxation passes)
+        what = ("oracle restatement of heisenberg/metropolis.rs:10-36 + overrelax.rs:15-48 (combined sweep with 2 "
+                "OR passes), one L=64 replica at T=1.443, %d Metropolis updates in %.1f s on 1 core"
+                % (upd, wall.value))
+    return {"value": upd / (wall.value * 1e9), "unit": "updates/ns", "cores": 1, "kind": "port", "sample": what}
 
 
 def load_traffic():
@@ -70,6 +78,25 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_calibration_bytes": nbytes, "dispatches": 3}), flush=True)
 
 
+WORKLOADS = {
+    # name: (model, sizes, temps, bytes/update, dtype, options, description)
+    "ising": ("ising", SIZES, (4.40, 4.62), 3, "u8", dict(exchange_every=10, measure_every=5),
+              "3D Ising PT (config 2): L=8,16,32,64,128 x 32 replicas, T=linspace(4.40,4.62,32), exchange every 10, "
+              "measure every 5, int8 spins, Philox4x32-10"),
+    "ising_msc": ("ising_msc", SIZES, (4.40, 4.62), 0.375, "u32 (1 bit/spin)", dict(exchange_every=10, measure_every=5),
+                  "3D Ising PT (config 2), multispin coded (32 replicas per 32-bit word), exchange every 10, "
+                  "measure every 5"),
+    "heisenberg": ("heisenberg", [16, 32, 64], (1.40, 1.50), 36, "f32",
+                   dict(exchange_every=10, measure_every=5, n_overrelax=2, delta=1.0, renorm_every=100),
+                   "3D Heisenberg PT (config 3): L=16,32,64 x 32 replicas, T=linspace(1.40,1.50,32), cap Metropolis "
+                   "+ 2 overrelaxation passes per sweep, FP64 sums"),
+    "xy": ("xy", [16, 32, 64], (2.15, 2.25), 24, "f32",
+           dict(exchange_every=10, measure_every=5, n_overrelax=2, delta=1.5, renorm_every=100),
+           "3D XY PT: L=16,32,64 x 32 replicas, T=linspace(2.15,2.25,32), rotation Metropolis + 2 overrelaxation "
+           "passes per sweep, FP64 sums"),
+}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +104,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--workload", default="ising", choices=sorted(WORKLOADS))
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -93,10 +121,10 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
 
-    temps = linspace(4.40, 4.62, R)
-    me, ee = 5, 10
-    pt = ising_hip.PTEnsemble("ising", SIZES, R, temps, args.seed, exchange_every=ee, measure_every=me,
-                              record_capacity=args.steps // me + 8)
+    model, sizes, (tlo, thi), bpu, dtype, opts, desc = WORKLOADS[args.workload]
+    temps = linspace(tlo, thi, R)
+    me = opts["measure_every"]
+    pt = ising_hip.PTEnsemble(model, sizes, R, temps, args.seed, record_capacity=args.steps // me + 8, **opts)
     pt.warmup(args.warmup)
     pt.sync()
 
@@ -105,16 +133,18 @@ def main():
     ev_ms = pt.sync()
     wall = time.perf_counter() - t0
 
-    tidx, bond, mag = pt.read_ising()
-    assert tidx.shape[0] == args.steps // me
+    recs = pt.read_ising() if model.startswith("ising") else pt.read_cont()
+    assert recs[0].shape[0] == args.steps // me
 
-    upd_per_sweep = sum(L ** 3 for L in SIZES) * R
-    updates = upd_per_sweep * args.steps
+    sites = sum(L ** 3 for L in sizes) * R
+    passes = 1 + opts.get("n_overrelax", 0) if not model.startswith("ising") else 1
+    updates = sites * args.steps                   # Metropolis updates (the reported metric)
     value = updates / (wall * 1e9)
-    launches = 2 * args.steps
+    launches = 2 * passes * args.steps              # colour passes (Metropolis + overrelaxation)
     avg_launch_s = ev_ms * 1e-3 / launches
-    alg_bytes_per_launch = 3 * upd_per_sweep // 2  # 3 B per update (read c, write c, read other colour)
+    alg_bytes_per_launch = int(bpu * sites / 2)     # bytes per update x updates per colour pass
     achieved = alg_bytes_per_launch / avg_launch_s / 1e9
+    kernel = {"ising": "ising_half_kernel", "ising_msc": "msc_half_kernel"}.get(model, "cont_metro_kernel+cont_or_kernel")
     line = {
         "metric": "spin updates/ns",
         "value": round(value, 3),
@@ -126,19 +156,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u8",
+        "dtype": dtype,
         "data": "synthetic",
-        "config": {"workload": "3D Ising PT (config 2): L=8,16,32,64,128 x 32 replicas, T=linspace(4.40,4.62,32), "
-                               "exchange every 10, measure every 5, int8 spins, Philox4x32-10",
-                   "sizes": SIZES, "replicas": R, "updates_per_step": upd_per_sweep},
+        "config": {"workload": desc, "sizes": sizes, "replicas": R, "updates_per_step": sites,
+                   "site_passes_per_step": passes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(),
-                     "kernel": "ising_half_kernel", "alg_bytes_per_launch": alg_bytes_per_launch,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic() if args.workload == "ising" else None,
+                     "kernel": kernel, "alg_bytes_per_launch": alg_bytes_per_launch,
                      "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "device_updates_per_ns": round(updates / (ev_ms * 1e6), 3)},
     }
+    if passes > 1:
+        line["config"]["or_updates_per_ns"] = round(updates * (passes - 1) / (wall * 1e9), 3)
+        line["config"]["combined_updates_per_ns"] = round(updates * passes / (wall * 1e9), 3)
     if args.cpu_seconds > 0:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds, model)
     print(json.dumps(line), flush=True)
 
 
