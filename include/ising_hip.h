@@ -239,6 +239,18 @@ int ising_replica_exchange(const double* temps, const double* energies, int32_t 
 int ising_jackknife(const double* e, const double* m, int64_t n, double beta, double n3, int32_t n_blocks,
                     double* out);
 
+/* Multiple-histogram (WHAM / Ferrenberg-Swendsen) reweighting on the device in
+ * FP64, restating analysis/scripts/reweighting.py:45-108 (wham_reweight):
+ * K runs at betas[k] with n_k[k] samples each; `energies` holds the total
+ * energies of all runs concatenated (N = sum n_k), `obs` holds n_obs
+ * observables, each an array of N values (observable-major).  Iterates the
+ * free energies until max |df| < tol or max_iter (f_out[K], f_out[0] = 0,
+ * *iters = iterations used), then writes the reweighted means
+ * out[t * n_obs + j] = <obs_j> at beta_targets[t].  K <= 1024, n_obs <= 8. */
+int ising_wham(int32_t K, const int64_t* n_k, const double* betas, const double* energies, int32_t n_obs,
+               const double* obs, int32_t max_iter, double tol, double* f_out, int32_t* iters, int32_t n_targets,
+               const double* beta_targets, double* out);
+
 /* Integrated autocorrelation time of a series by Sokal's automatic windowing:
  * tau(W) = 1/2 + sum_{t=1..W} rho(t), W = smallest window with W >= c * tau(W)
  * (c = 6 is customary).  Not in the reference (its plans name it); used to

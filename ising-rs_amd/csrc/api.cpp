@@ -1343,4 +1343,87 @@ int ising_tau_int(const double* x, int64_t n, double c, double* tau, int64_t* wi
     return ISING_OK;
 }
 
+int ising_wham(int32_t K, const int64_t* n_k, const double* betas, const double* energies, int32_t n_obs,
+               const double* obs, int32_t max_iter, double tol, double* f_out, int32_t* iters, int32_t n_targets,
+               const double* beta_targets, double* out) {
+    if (K < 1 || K > 1024 || !n_k || !betas || !energies || n_obs < 0 || n_obs > 8 || (n_obs > 0 && !obs) ||
+        max_iter < 1 || n_targets < 0 || (n_targets > 0 && (!beta_targets || !out)))
+        return fail(ISING_E_ARG, "ising_wham: invalid arguments");
+    int64_t N = 0;
+    std::vector<double> logN(K);
+    for (int k = 0; k < K; ++k) {
+        if (n_k[k] < 1) return fail(ISING_E_ARG, "every run needs at least one sample");
+        N += n_k[k];
+        logN[k] = std::log((double)n_k[k]);
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(ISING_E_HIP, "no HIP device visible");
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    const int nchunk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (N + 256 * 16 - 1) / (256 * 16)));
+    const int maxT = std::max<int>(K, n_targets);
+    double *dE = nullptr, *dO = nullptr, *dlogN = nullptr, *dF = nullptr, *dB = nullptr, *dD = nullptr,
+           *dT = nullptr, *dP = nullptr, *dOut = nullptr;
+    hipError_t e = hipSuccess;
+    auto M = [&](double** p, size_t n) {
+        if (e == hipSuccess) e = hipMalloc(p, sizeof(double) * std::max<size_t>(n, 1));
+    };
+    M(&dE, N);
+    M(&dO, (size_t)N * n_obs);
+    M(&dlogN, K);
+    M(&dF, K);
+    M(&dB, K);
+    M(&dD, N);
+    M(&dT, maxT);
+    M(&dP, (size_t)maxT * nchunk * 10);
+    M(&dOut, (size_t)maxT * (1 + n_obs));
+    std::vector<double> f(K, 0.0), res((size_t)maxT * (1 + n_obs));
+    int it = 0;
+    if (e == hipSuccess) e = hipMemcpy(dE, energies, sizeof(double) * N, hipMemcpyHostToDevice);
+    if (e == hipSuccess && n_obs) e = hipMemcpy(dO, obs, sizeof(double) * N * n_obs, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dlogN, logN.data(), sizeof(double) * K, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dB, betas, sizeof(double) * K, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dT, betas, sizeof(double) * K, hipMemcpyHostToDevice);
+    // free-energy iteration (reweighting.py:77-99)
+    while (e == hipSuccess && it < max_iter) {
+        ++it;
+        e = hipMemcpyAsync(dF, f.data(), sizeof(double) * K, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = launch_wham_denom(dE, N, K, dlogN, dF, dB, dD, st);
+        if (e == hipSuccess) e = launch_wham_reduce(dE, N, dD, dT, K, 0, dO, dP, nchunk, dOut, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(res.data(), dOut, sizeof(double) * K, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) break;
+        std::vector<double> fn(K);
+        for (int k = 0; k < K; ++k) fn[k] = -res[k];
+        const double f0 = fn[0];
+        double diff = 0.0;
+        for (int k = 0; k < K; ++k) {
+            fn[k] -= f0;
+            diff = std::max(diff, std::fabs(fn[k] - f[k]));
+        }
+        f = fn;
+        if (diff < tol) break;
+    }
+    // log D_n with the final f (reweighting.py:101 uses the denominator of the last iteration)
+    if (e == hipSuccess) e = hipMemcpyAsync(dF, f.data(), sizeof(double) * K, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_wham_denom(dE, N, K, dlogN, dF, dB, dD, st);
+    if (e == hipSuccess && n_targets > 0) {
+        e = hipMemcpyAsync(dT, beta_targets, sizeof(double) * n_targets, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = launch_wham_reduce(dE, N, dD, dT, n_targets, n_obs, dO, dP, nchunk, dOut, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(res.data(), dOut, sizeof(double) * n_targets * (1 + n_obs), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e == hipSuccess)
+            for (int t = 0; t < n_targets; ++t)
+                for (int j = 0; j < n_obs; ++j) out[(size_t)t * n_obs + j] = res[(size_t)t * (1 + n_obs) + 1 + j];
+    }
+    for (double* p : {dE, dO, dlogN, dF, dB, dD, dT, dP, dOut}) (void)hipFree(p);
+    (void)hipStreamDestroy(st);
+    HIP_TRY(e);
+    if (f_out)
+        for (int k = 0; k < K; ++k) f_out[k] = f[k];
+    if (iters) *iters = it;
+    return ISING_OK;
+}
+
 }  // extern "C"

@@ -83,6 +83,11 @@ hipError_t launch_cont_measure(const DevGroup* groups, int ngroups, const float*
                                double* part, int nblocks, hipStream_t st);
 constexpr int kContTile = 256;  // half-sites per block in continuous launches
 
+hipError_t launch_wham_denom(const double* E, int64_t n, int K, const double* logN, const double* f,
+                             const double* beta, double* logD, hipStream_t st);
+hipError_t launch_wham_reduce(const double* E, int64_t n, const double* logD, const double* tb, int n_targets,
+                              int n_obs, const double* obs, double* part, int nchunk, double* out, hipStream_t st);
+
 hipError_t launch_ising_half(const IsingArgs& a, int nblocks, int color, hipStream_t st);
 hipError_t launch_ising_init(const DevGroup* groups, int ngroups, uint8_t* pool, int nblocks, uint32_t k0,
                              uint32_t k1, hipStream_t st);

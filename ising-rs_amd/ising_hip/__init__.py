@@ -114,6 +114,8 @@ _SIGS = {
     "ising_replica_exchange": (C.c_int, [_vp, _vp, C.c_int32, _vp, _vp, _vp, C.c_int32, C.POINTER(C.c_int32)]),
     "ising_jackknife": (C.c_int, [_vp, _vp, C.c_int64, C.c_double, C.c_double, C.c_int32, _vp]),
     "ising_tau_int": (C.c_int, [_vp, C.c_int64, C.c_double, _f64p, _i64p]),
+    "ising_wham": (C.c_int, [C.c_int32, _vp, _vp, _vp, C.c_int32, _vp, C.c_int32, C.c_double, _vp,
+                             C.POINTER(C.c_int32), C.c_int32, _vp, _vp]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _f = getattr(lib, _name)
@@ -465,3 +467,24 @@ def tau_int(x, c=6.0):
     t, w = C.c_double(), C.c_int64()
     _check(lib.ising_tau_int(_ptr(a), len(a), c, C.byref(t), C.byref(w)))
     return t.value, w.value
+
+
+def wham(energy_lists, betas, observables=(), beta_targets=(), max_iter=100, tol=1e-8):
+    """Multiple-histogram reweighting on the GPU (reweighting.py:45-108).
+
+    energy_lists[k]: total energies of run k at betas[k]; observables: list of
+    lists shaped like energy_lists.  Returns (f, iterations, means[n_targets, n_obs]).
+    """
+    K = len(energy_lists)
+    n_k = np.array([len(e) for e in energy_lists], np.int64)
+    E = np.ascontiguousarray(np.concatenate(energy_lists), np.float64)
+    obs = (np.ascontiguousarray(np.concatenate([np.concatenate(o) for o in observables]), np.float64)
+           if len(observables) else np.zeros(1))
+    b = np.ascontiguousarray(betas, np.float64)
+    bt = np.ascontiguousarray(beta_targets, np.float64) if len(beta_targets) else np.zeros(1)
+    f = np.empty(K)
+    out = np.empty(max(1, len(beta_targets) * len(observables)))
+    it = C.c_int32()
+    _check(lib.ising_wham(K, _ptr(n_k), _ptr(b), _ptr(E), len(observables), _ptr(obs), max_iter, tol, _ptr(f),
+                          C.byref(it), len(beta_targets), _ptr(bt), _ptr(out)))
+    return f, it.value, out[: len(beta_targets) * len(observables)].reshape(len(beta_targets), len(observables))
