@@ -152,12 +152,13 @@ typedef struct {
     double adapt_target;     /* target Metropolis acceptance for the adaptation */
     int32_t graph_sweeps;    /* sweeps per captured HIP graph replayed by warmup/sample
                                 (0 = launch every kernel directly) */
-    int32_t reserved;
+    int32_t ising_spt;       /* Ising: 8-site groups per thread, 1 or 2 (0 = automatic:
+                                2 for L >= 64); results do not depend on it */
 } ising_pt_opts_t;
 
 /* Defaults: j=1, h=0, exchange_every=10, measure_every=1, delta=0.5,
  * n_overrelax=5, renorm_every=100, record_capacity=4096, adapt_every=0,
- * adapt_target=0.5, graph_sweeps=32. */
+ * adapt_target=0.5, graph_sweeps=32, ising_spt=0. */
 void ising_pt_default_opts(ising_pt_opts_t* opts);
 
 /* model: ISING_MODEL_*.  sizes: n_sizes even L values.  temps: `replicas`

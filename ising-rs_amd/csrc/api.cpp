@@ -319,6 +319,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     if (o.exchange_every < 1 || o.measure_every < 1) return fail(ISING_E_ARG, "exchange_every and measure_every must be >= 1");
     if (o.record_capacity < 1) return fail(ISING_E_ARG, "record_capacity must be >= 1");
     if (o.graph_sweeps < 0 || o.graph_sweeps > 100000) return fail(ISING_E_ARG, "graph_sweeps must be in 0..100000");
+    if (o.ising_spt < 0 || o.ising_spt > 2) return fail(ISING_E_ARG, "ising_spt must be 0, 1 or 2");
     if (model == ISING_MODEL_ISING_MSC && !(o.h == 0.0 && o.j > 0.0))
         return fail(ISING_E_ARG, "multispin coding needs h = 0 and J > 0");
     if (model == ISING_MODEL_XY || model == ISING_MODEL_HEISENBERG) {
@@ -358,13 +359,17 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
         G.R = R;
         G.dims = p->dims;
         G.fast = (!two_d && (sizes[i] % 16) == 0) ? 1 : 0;
+        G.spt = 1;
         if (G.Lh >= 2) fastdiv_magic((uint32_t)G.Lh, &G.mag_Lh, &G.sh_Lh);
         fastdiv_magic((uint32_t)G.L, &G.mag_L, &G.sh_L);
         G.rep0 = i * R;
         G.n_half = (int64_t)sizes[i] * sizes[i] * (two_d ? 1 : sizes[i]) / 2;
         G.stride = ((G.n_half * elem + 255) / 256) * 256 / elem;
         G.off = off;
-        G.tiles = (int32_t)((G.n_half + tile - 1) / tile);
+        if (ising && G.fast && o.ising_spt == 0 && sizes[i] >= 64) G.spt = 2;  // 16 sites per thread
+        if (ising && G.fast && o.ising_spt > 0) G.spt = o.ising_spt;
+        G.tiles = (int32_t)((G.n_half + tile * G.spt - 1) / (tile * G.spt));
+        if (G.tiles >= 2) fastdiv_magic((uint32_t)G.tiles, &G.mag_T, &G.sh_T);
         G.blk0 = blk;
         if ((int64_t)blk + (int64_t)G.tiles * units > INT32_MAX / 2) return fail(ISING_E_ARG, "ensemble too large");
         blk += G.tiles * (int32_t)units;
@@ -420,6 +425,9 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     ctl.measure_every = o.measure_every;
     ctl.exchange_every = o.exchange_every;
     ctl.rec_cap = o.record_capacity;
+    ctl.renorm_every = (model == ISING_MODEL_XY || model == ISING_MODEL_HEISENBERG) ? o.renorm_every : 0;
+    ctl.adapt_every = (model == ISING_MODEL_XY || model == ISING_MODEL_HEISENBERG) ? o.adapt_every : 0;
+    set_flags(&ctl, 0);
     HIP_TRY(hipMemcpy(p->d_groups, p->groups.data(), sizeof(DevGroup) * p->groups.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_thr, thr.data(), sizeof(uint32_t) * thr.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_r2t, ident.data(), sizeof(int32_t) * ident.size(), hipMemcpyHostToDevice));
@@ -483,7 +491,7 @@ void ising_pt_default_opts(ising_pt_opts_t* o) {
     o->adapt_every = 0;
     o->adapt_target = 0.5;
     o->graph_sweeps = 32;
-    o->reserved = 0;
+    o->ising_spt = 0;
 }
 
 int ising_pt_create(ising_pt_t** out, int model, const int32_t* sizes, int32_t n_sizes, int32_t replicas,
@@ -522,9 +530,12 @@ int ising_pt_sample(ising_pt_t* p, int64_t sweeps) {
     if (sweeps < 0) return fail(ISING_E_ARG, "sweeps must be >= 0");
     if (p->s_host < 0) {
         // sampling starts at the next sweep; the device reads sample_start from its control block
-        const int64_t start = p->t_host;
-        HIP_TRY(hipStreamSynchronize(p->st));
-        HIP_TRY(hipMemcpy(&p->d_ctl->sample_start, &start, sizeof(int64_t), hipMemcpyHostToDevice));
+        DevCtl c;
+        int rc0 = p->read_ctl(&c);
+        if (rc0) return rc0;
+        c.sample_start = p->t_host;
+        set_flags(&c, p->t_host);
+        HIP_TRY(hipMemcpy(p->d_ctl, &c, sizeof(DevCtl), hipMemcpyHostToDevice));
         p->s_host = 0;
     }
     const int64_t add = measured_in(p->s_host, sweeps, p->opts.measure_every);

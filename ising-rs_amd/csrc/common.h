@@ -92,22 +92,6 @@ __device__ __forceinline__ T wave_sum(T v) {
 // Ensemble descriptors shared by the host API and the kernels.
 // ---------------------------------------------------------------------------
 
-// One lattice size of an ensemble: R replicas of an L^3 lattice, each stored
-// as two colour half-lattices (red/black) of L^3/2 sites, row-major in
-// (z, y, xh) with x = 2*xh + ((y + z + colour) & 1).
-struct DevGroup {
-    int32_t L, Lh, R, fast;
-    int32_t rep0;      // global index of replica 0 (tables are indexed by rep0 + i)
-    int32_t tiles;     // work tiles per replica half-sweep
-    int32_t blk0;      // first block of this group in a half-sweep launch
-    int32_t dims;      // 3: L^3 cube; 2: L^2 square lattice (Ising only; z dropped)
-    int64_t n_half;    // L^3 / 2
-    int64_t stride;    // elements between consecutive half-lattices (padded)
-    int64_t off;       // element offset of replica 0 colour 0 in the pool
-    uint32_t mag_Lh, sh_Lh;  // n / Lh == fastdiv(n, mag_Lh, sh_Lh) for n < 2^32 (Lh >= 2)
-    uint32_t mag_L, sh_L;    // n / L  likewise
-};
-
 // Division by a run-time invariant d >= 2 of any 32-bit n: the round-up
 // method of Granlund & Montgomery (PLDI'94).  l = ceil(log2 d),
 // m = floor(2^32 (2^l - d) / d) + 1, q = (t + ((n - t) >> 1)) >> (l - 1), t = mulhi(m, n).
@@ -122,6 +106,33 @@ __device__ __forceinline__ uint32_t fastdiv(uint32_t n, uint32_t m, uint32_t l) 
     return (t + ((n - t) >> 1)) >> (l - 1);
 }
 
+// One lattice size of an ensemble: R replicas of an L^3 lattice, each stored
+// as two colour half-lattices (red/black) of L^3/2 sites, row-major in
+// (z, y, xh) with x = 2*xh + ((y + z + colour) & 1).
+struct DevGroup {
+    int32_t L, Lh, R, fast;
+    int32_t rep0;      // global index of replica 0 (tables are indexed by rep0 + i)
+    int32_t tiles;     // work tiles per replica half-sweep
+    int32_t blk0;      // first block of this group in a half-sweep launch
+    int32_t dims;      // 3: L^3 cube; 2: L^2 square lattice (Ising only; z dropped)
+    int64_t n_half;    // L^3 / 2
+    int64_t stride;    // elements between consecutive half-lattices (padded)
+    int64_t off;       // element offset of replica 0 colour 0 in the pool
+    uint32_t mag_Lh, sh_Lh;  // n / Lh == fastdiv(n, mag_Lh, sh_Lh) for n < 2^32 (Lh >= 2)
+    uint32_t mag_L, sh_L;    // n / L  likewise
+    uint32_t mag_T, sh_T;    // n / tiles likewise (tiles >= 2)
+    int32_t spt;             // Ising fast path: 8-site groups per thread (1 or 2)
+    int32_t pad1;
+};
+
+// Replica (or replica word) and tile of a block's local index.
+__device__ __forceinline__ void split_tiles(const DevGroup& G, int local, int* unit, int* tile) {
+    const uint32_t u = G.tiles >= 2 ? fastdiv((uint32_t)local, G.mag_T, G.sh_T) : (uint32_t)local;
+    *unit = (int)u;
+    *tile = local - (int)u * G.tiles;
+}
+
+
 // Sweep control block, resident on the device.  Kernels read the sweep index
 // and cadence from here, so a captured launch sequence replays unchanged.
 struct DevCtl {
@@ -131,13 +142,38 @@ struct DevCtl {
     int32_t exchange_every;
     int32_t n_rec;         // measurement records written
     int32_t rec_cap;
-    uint32_t reserved;
+    int32_t flags;         // F_* decisions for sweep t (set by the host and the closing kernels)
     int32_t overflow;      // set when a record did not fit
     int64_t exch_tries;
     int64_t exch_accepts;
     int64_t mh_tries;      // learned-proposal Metropolis-Hastings moves attempted
     int64_t mh_accepts;
+    int64_t round;         // exchange round of sweep t
+    int32_t renorm_every;
+    int32_t adapt_every;
 };
+
+enum : int32_t { F_MEAS = 1, F_EXCH = 2, F_RENORM = 4, F_ADAPT = 8, F_WARM = 16 };
+
+// Per-sweep decisions for sweep index t, computed once per sweep (by the
+// closing kernel for the next sweep, by the host when a phase starts), so the
+// sweep kernels never divide 64-bit integers.
+__host__ __device__ inline void set_flags(DevCtl* c, int64_t t) {
+    const int64_t s = t - c->sample_start;
+    int32_t f = 0;
+    int64_t round = 0;
+    if (s >= 0) {
+        if ((s + 1) % c->measure_every == 0) f |= F_MEAS;
+        if ((s + 1) % c->exchange_every == 0) f |= F_EXCH;
+        round = s / c->exchange_every;
+    } else {
+        f |= F_WARM;
+        if (c->adapt_every > 0 && (t + 1) % c->adapt_every == 0) f |= F_ADAPT;
+    }
+    if (c->renorm_every > 0 && (t + 1) % c->renorm_every == 0) f |= F_RENORM;
+    c->flags = f;
+    c->round = round;
+}
 
 struct IsingRec {
     int64_t bond;  // forward-bond sum B (E = -J B)
@@ -152,16 +188,5 @@ struct ContRec {
     int32_t pad;
 };
 
-__device__ __forceinline__ void sweep_flags(const DevCtl* ctl, int64_t t, bool* meas, bool* exch, int64_t* s_out) {
-    const int64_t s = t - ctl->sample_start;
-    *s_out = s;
-    if (s < 0) {
-        *meas = false;
-        *exch = false;
-        return;
-    }
-    *meas = ((s + 1) % ctl->measure_every) == 0;
-    *exch = ((s + 1) % ctl->exchange_every) == 0;
-}
 
 }  // namespace ising

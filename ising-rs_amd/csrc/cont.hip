@@ -91,7 +91,8 @@ __device__ __forceinline__ void block_sum4(double v[4], double (*s_red)[kCT / 64
 // Closing step of a continuous sweep (one-block kernel after the colour-1 Metropolis pass):
 // ordered FP64 sums of the block partials, records, exchange, delta adaptation
 // during warm-up, sweep counter.
-__device__ void close_cont(const ContArgs& a, int64_t t, bool meas, bool exch, int64_t s) {
+__device__ void close_cont(const ContArgs& a, int64_t t, int32_t fl) {
+    const bool meas = (fl & F_MEAS) != 0, exch = (fl & F_EXCH) != 0;
     __shared__ double s_part[4][kCT];
     DevCtl* ctl = a.ctl;
     const int tid = threadIdx.x;
@@ -138,7 +139,7 @@ __device__ void close_cont(const ContArgs& a, int64_t t, bool meas, bool exch, i
         }
     }
     if (exch) {
-        const int64_t round = s / ctl->exchange_every;
+        const int64_t round = ctl->round;
         for (int g = 0; g < a.ngroups; ++g) {
             const DevGroup G = a.groups[g];
             const int npairs = (G.R - (int)(round & 1)) / 2;
@@ -164,7 +165,7 @@ __device__ void close_cont(const ContArgs& a, int64_t t, bool meas, bool exch, i
     }
     // Warm-up only: scale each temperature's proposal size towards the target
     // acceptance.  Frozen during sampling, so the sampled chain is stationary.
-    if (a.adapt_every > 0 && s < 0 && ((t + 1) % a.adapt_every) == 0) {
+    if (a.adapt_every > 0 && (fl & F_ADAPT)) {
         for (int gr = tid; gr < a.total_reps; gr += kCT) {
             int g = 0;
             while (g + 1 < a.ngroups && gr >= a.groups[g + 1].rep0) ++g;
@@ -186,6 +187,7 @@ __device__ void close_cont(const ContArgs& a, int64_t t, bool meas, bool exch, i
             else ctl->overflow = 1;
         }
         ctl->t = t + 1;
+        set_flags(ctl, t + 1);
     }
 }
 
@@ -195,20 +197,17 @@ __global__ __launch_bounds__(kCT) void cont_metro_kernel(ContArgs a, int color, 
     __shared__ double s_red[4][kCT / 64];
     const int g = find_group_c(a.groups, a.ngroups, blockIdx.x);
     const DevGroup G = a.groups[g];
-    const int local = blockIdx.x - G.blk0;
-    const int r = local / G.tiles;
-    const int tile = local - r * G.tiles;
+    int r, tile;
+    split_tiles(G, (int)blockIdx.x - G.blk0, &r, &tile);
     const int grep = G.rep0 + r;
     const int ti = G.rep0 + a.r2t[grep];
     const float beta = a.inline_params ? a.beta_inline : a.betas_f[ti];
     const float dlt = a.inline_params ? a.delta_inline : a.deltas[ti];
     const int64_t t = a.ctl->t;
-    bool meas, exch;
-    int64_t s;
-    sweep_flags(a.ctl, t, &meas, &exch, &s);
-    const bool need = (meas || exch) && closes;
-    const bool renorm = a.renorm_every > 0 && ((t + 1) % a.renorm_every) == 0;
-    const bool count = a.adapt_every > 0 && s < 0;
+    const int32_t fl = a.ctl->flags;
+    const bool need = (fl & (F_MEAS | F_EXCH)) != 0 && closes;
+    const bool renorm = (fl & F_RENORM) != 0;
+    const bool count = a.adapt_every > 0 && (fl & F_WARM) != 0;
     const uint32_t hs = (uint32_t)(2 * t + color);
     const uint32_t stream = stream_id((uint32_t)G.L, (uint32_t)r);
 
@@ -308,11 +307,7 @@ __global__ __launch_bounds__(kCT) void cont_metro_kernel(ContArgs a, int color, 
 
 // Closes a continuous sweep (one block, after the colour-1 Metropolis pass).
 __global__ __launch_bounds__(kCT) void cont_close_kernel(ContArgs a) {
-    const int64_t t = a.ctl->t;
-    bool meas, exch;
-    int64_t s;
-    sweep_flags(a.ctl, t, &meas, &exch, &s);
-    close_cont(a, t, meas, exch, s);
+    close_cont(a, a.ctl->t, a.ctl->flags);
 }
 
 // Overrelaxation pass over colour `color` (no random numbers, no renormalisation).
@@ -320,9 +315,8 @@ template <int N>
 __global__ __launch_bounds__(kCT) void cont_or_kernel(ContArgs a, int color) {
     const int g = find_group_c(a.groups, a.ngroups, blockIdx.x);
     const DevGroup G = a.groups[g];
-    const int local = blockIdx.x - G.blk0;
-    const int r = local / G.tiles;
-    const int tile = local - r * G.tiles;
+    int r, tile;
+    split_tiles(G, (int)blockIdx.x - G.blk0, &r, &tile);
     float* C = a.pool + G.off + (int64_t)(2 * r + color) * N * G.stride;
     const float* O = a.pool + G.off + (int64_t)(2 * r + (color ^ 1)) * N * G.stride;
     const int64_t h = (int64_t)tile * kCT + threadIdx.x;
@@ -351,9 +345,8 @@ __global__ __launch_bounds__(kCT) void cont_init_kernel(const DevGroup* groups, 
                                                       uint32_t k0, uint32_t k1) {
     const int g = find_group_c(groups, ngroups, blockIdx.x);
     const DevGroup G = groups[g];
-    const int local = blockIdx.x - G.blk0;
-    const int r = local / G.tiles;
-    const int tile = local - r * G.tiles;
+    int r, tile;
+    split_tiles(G, (int)blockIdx.x - G.blk0, &r, &tile);
     const int64_t h = (int64_t)tile * kCT + threadIdx.x;
     if (h >= G.n_half) return;
     const uint32_t stream = stream_id((uint32_t)G.L, (uint32_t)r);
@@ -389,9 +382,8 @@ __global__ __launch_bounds__(kCT) void cont_measure_kernel(const DevGroup* group
     __shared__ double s_red[4][kCT / 64];
     const int g = find_group_c(groups, ngroups, blockIdx.x);
     const DevGroup G = groups[g];
-    const int local = blockIdx.x - G.blk0;
-    const int r = local / G.tiles;
-    const int tile = local - r * G.tiles;
+    int r, tile;
+    split_tiles(G, (int)blockIdx.x - G.blk0, &r, &tile);
     const float* C1 = pool + G.off + (int64_t)(2 * r + 1) * N * G.stride;
     const float* C0 = pool + G.off + (int64_t)(2 * r) * N * G.stride;
     const int64_t h = (int64_t)tile * kCT + threadIdx.x;

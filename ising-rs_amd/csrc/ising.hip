@@ -52,7 +52,7 @@ __device__ __forceinline__ int64_t acc_total(const IsingArgs& a, int gr, int q) 
 }
 
 // Closing step of a sweep (one-block kernel after the colour-1 launch).
-__device__ void close_sweep(const IsingArgs& a, int64_t t, bool meas, bool exch, int64_t s) {
+__device__ void close_sweep(const IsingArgs& a, int64_t t, bool meas, bool exch, int64_t round) {
     DevCtl* ctl = a.ctl;
     const int tid = threadIdx.x;
     const bool need = meas || exch;
@@ -74,7 +74,6 @@ __device__ void close_sweep(const IsingArgs& a, int64_t t, bool meas, bool exch,
     }
     __syncthreads();
     if (exch) {
-        const int64_t round = s / ctl->exchange_every;
         for (int g = 0; g < a.ngroups; ++g) {
             const DevGroup G = a.groups[g];
             const int npairs = (G.R - (int)(round & 1)) / 2;
@@ -109,6 +108,7 @@ __device__ void close_sweep(const IsingArgs& a, int64_t t, bool meas, bool exch,
             else ctl->overflow = 1;
         }
         ctl->t = t + 1;
+        set_flags(ctl, t + 1);
     }
 }
 
@@ -130,19 +130,14 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
     if (b >= a.nblocks) return;
     const int g = find_group(a.groups, a.ngroups, b);
     const DevGroup G = a.groups[g];
-    const int local = b - G.blk0;
-    const int r = local / G.tiles;
-    const int tile = local - r * G.tiles;
+    int r, tile;
+    split_tiles(G, b - G.blk0, &r, &tile);
     const int grep = G.rep0 + r;
     const int tidx = a.r2t[grep];
     if (threadIdx.x < 28)
         s_thr[threadIdx.x] = a.inline_thr ? a.thr_inline[threadIdx.x] : a.thr[(int64_t)(G.rep0 + tidx) * 28 + threadIdx.x];
-
     const int64_t t = a.ctl->t;
-    bool meas, exch;
-    int64_t s;
-    sweep_flags(a.ctl, t, &meas, &exch, &s);
-    const bool need = meas || exch;
+    const bool need = (a.ctl->flags & (F_MEAS | F_EXCH)) != 0;
     __syncthreads();
 
     const uint32_t hs = (uint32_t)(2 * t + color);
@@ -152,7 +147,8 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
     const uint8_t* O = a.pool + G.off + (int64_t)(2 * r + (color ^ 1)) * G.stride;
 
     int bsum = 0, msum = 0;
-    const int64_t h0 = (int64_t)tile * kTile + (int64_t)threadIdx.x * kSitesPerThread;
+    for (int part = 0; part < G.spt; ++part) {
+    const int64_t h0 = ((int64_t)tile * G.spt + part) * kTile + (int64_t)threadIdx.x * kSitesPerThread;
     if (h0 < G.n_half) {
         const u32x4 w = philox((uint32_t)(h0 >> 3), hs, stream, tag(P_ISING, MODEL_ISING), a.k0, a.k1);
         if (G.fast) {
@@ -210,13 +206,13 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
             *(uint64_t*)(C + hh) = nc8;
             if (need) {
                 const int sb = __builtin_popcountll(nc8);
-                msum = 2 * sb - 8;
+                msum += 2 * sb - 8;
                 if (color) {
                     // sum_j (2 b'_j - 1)(2 n_j - 6) = 4 sum b'n - 12 sum b' - 2 sum n + 48
                     const uint64_t bn = (nc8 * 0xffull) & ns;
                     const int sbn = (int)((bn * 0x0101010101010101ull) >> 56);
                     const int sn = (int)((ns * 0x0101010101010101ull) >> 56);
-                    bsum = 4 * sbn - 12 * sb - 2 * sn + 48;
+                    bsum += 4 * sbn - 12 * sb - 2 * sn + 48;
                 }
             }
         } else {
@@ -251,6 +247,7 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
             }
         }
     }
+    }
 
     // Fused measurement: wave -> block -> one int64 atomic per block and quantity.
     if (need) {
@@ -279,10 +276,8 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
 // every block's accumulator update).
 __global__ __launch_bounds__(kThreads) void ising_close_kernel(IsingArgs a) {
     const int64_t t = a.ctl->t;
-    bool meas, exch;
-    int64_t s;
-    sweep_flags(a.ctl, t, &meas, &exch, &s);
-    close_sweep(a, t, meas, exch, s);
+    const int32_t fl = a.ctl->flags;
+    close_sweep(a, t, (fl & F_MEAS) != 0, (fl & F_EXCH) != 0, a.ctl->round);
 }
 
 // Hot start: i.i.d. +-1 with p = 1/2 from Philox(site, 0, stream, init tag).
@@ -290,14 +285,13 @@ __global__ __launch_bounds__(kThreads) void ising_init_kernel(const DevGroup* gr
                                                             uint32_t k0, uint32_t k1) {
     const int g = find_group(groups, ngroups, blockIdx.x);
     const DevGroup G = groups[g];
-    const int local = blockIdx.x - G.blk0;
-    const int r = local / G.tiles;
-    const int tile = local - r * G.tiles;
+    int r, tile;
+    split_tiles(G, blockIdx.x - G.blk0, &r, &tile);
     const uint32_t stream = stream_id((uint32_t)G.L, (uint32_t)r);
     for (int c = 0; c < 2; ++c) {
         uint8_t* C = pool + G.off + (int64_t)(2 * r + c) * G.stride;
-        for (int j = 0; j < kSitesPerThread; ++j) {
-            const int64_t h = (int64_t)tile * kTile + (int64_t)j * kThreads + threadIdx.x;
+        for (int j = 0; j < kSitesPerThread * G.spt; ++j) {
+            const int64_t h = (int64_t)tile * kTile * G.spt + (int64_t)j * kThreads + threadIdx.x;
             if (h >= G.n_half) break;
             const int64_t row = h / G.Lh;
             const int xh = (int)(h - row * G.Lh);
@@ -316,15 +310,14 @@ __global__ __launch_bounds__(kThreads) void ising_measure_kernel(const DevGroup*
     __shared__ int64_t s_red[2][kThreads / 64];
     const int g = find_group(groups, ngroups, blockIdx.x);
     const DevGroup G = groups[g];
-    const int local = blockIdx.x - G.blk0;
-    const int r = local / G.tiles;
-    const int tile = local - r * G.tiles;
+    int r, tile;
+    split_tiles(G, blockIdx.x - G.blk0, &r, &tile);
     const int L = G.L, Lh = G.Lh;
     const uint8_t* C1 = pool + G.off + (int64_t)(2 * r + 1) * G.stride;
     const uint8_t* C0 = pool + G.off + (int64_t)(2 * r) * G.stride;
     int bsum = 0, msum = 0;
-    for (int j = 0; j < kSitesPerThread; ++j) {
-        const int64_t h = (int64_t)tile * kTile + (int64_t)j * kThreads + threadIdx.x;
+    for (int j = 0; j < kSitesPerThread * G.spt; ++j) {
+        const int64_t h = (int64_t)tile * kTile * G.spt + (int64_t)j * kThreads + threadIdx.x;
         if (h >= G.n_half) break;
         const int64_t row = h / Lh;
         const int xh = (int)(h - row * Lh);

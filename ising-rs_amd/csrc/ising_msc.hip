@@ -75,9 +75,8 @@ __global__ __launch_bounds__(kMT) void msc_half_kernel(IsingArgs a, int color) {
     if (b >= a.nblocks) return;
     const int g = find_group_m(a.groups, a.ngroups, b);
     const DevGroup G = a.groups[g];
-    const int local = b - G.blk0;
-    const int wi = local / G.tiles;
-    const int tile = local - wi * G.tiles;
+    int wi, tile;
+    split_tiles(G, b - G.blk0, &wi, &tile);
     const int nrep = min(32, G.R - 32 * wi);
     const uint32_t valid = nrep >= 32 ? 0xffffffffu : ((1u << nrep) - 1u);
     const int tid = threadIdx.x;
@@ -93,10 +92,7 @@ __global__ __launch_bounds__(kMT) void msc_half_kernel(IsingArgs a, int color) {
     }
     if (tid < 64) s_acc[tid >> 1][tid & 1] = 0;
     const int64_t t = a.ctl->t;
-    bool meas, exch;
-    int64_t s;
-    sweep_flags(a.ctl, t, &meas, &exch, &s);
-    const bool need = meas || exch;
+    const bool need = (a.ctl->flags & (F_MEAS | F_EXCH)) != 0;
     __syncthreads();
 
     const uint32_t hs = (uint32_t)(2 * t + color);
@@ -178,9 +174,8 @@ __global__ __launch_bounds__(kMT) void msc_init_kernel(const DevGroup* groups, i
     if (b >= nblocks) return;
     const int g = find_group_m(groups, ngroups, b);
     const DevGroup G = groups[g];
-    const int local = b - G.blk0;
-    const int wi = local / G.tiles;
-    const int tile = local - wi * G.tiles;
+    int wi, tile;
+    split_tiles(G, b - G.blk0, &wi, &tile);
     const int nrep = min(32, G.R - 32 * wi);
     const int64_t h = (int64_t)tile * kMT + threadIdx.x;
     if (h >= G.n_half) return;
@@ -207,9 +202,8 @@ __global__ __launch_bounds__(kMT) void msc_measure_kernel(const DevGroup* groups
     if (b >= nblocks) return;
     const int g = find_group_m(groups, ngroups, b);
     const DevGroup G = groups[g];
-    const int local = b - G.blk0;
-    const int wi = local / G.tiles;
-    const int tile = local - wi * G.tiles;
+    int wi, tile;
+    split_tiles(G, b - G.blk0, &wi, &tile);
     const int nrep = min(32, G.R - 32 * wi);
     const int tid = threadIdx.x;
     if (tid < 64) s_acc[tid >> 1][tid & 1] = 0;

@@ -191,19 +191,16 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int color
     int g = 0;
     while (g + 1 < a.ngroups && (int)blockIdx.x >= a.groups[g + 1].blk0) ++g;
     const DevGroup G = a.groups[g];
-    const int local = blockIdx.x - G.blk0;
-    const int r = local / G.tiles;
-    const int tile = local - r * G.tiles;
+    int r, tile;
+    split_tiles(G, (int)blockIdx.x - G.blk0, &r, &tile);
     const int grep = G.rep0 + r;
     const int ti = G.rep0 + a.r2t[grep];
     for (int i = threadIdx.x; i < Off<H>::P; i += blockDim.x) s_w[i] = a.net[i];
     const float beta = a.betas_f[ti];
     const int64_t t = t_override >= 0 ? t_override : a.ctl->t;
-    bool meas, exch;
-    int64_t s;
-    sweep_flags(a.ctl, t, &meas, &exch, &s);
-    const bool need = (meas || exch) && closes;
-    const bool renorm = a.renorm_every > 0 && ((t + 1) % a.renorm_every) == 0;
+    const int32_t fl = t_override >= 0 ? 0 : a.ctl->flags;
+    const bool need = (fl & (F_MEAS | F_EXCH)) != 0 && closes;
+    const bool renorm = (fl & F_RENORM) != 0;
     const uint32_t hs = (uint32_t)(2 * t + color);
     const uint32_t stream = stream_id((uint32_t)G.L, (uint32_t)r);
     __syncthreads();

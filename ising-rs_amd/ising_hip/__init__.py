@@ -42,7 +42,7 @@ class Opts(C.Structure):
     _fields_ = [("j", C.c_double), ("h", C.c_double), ("exchange_every", C.c_int32), ("measure_every", C.c_int32),
                 ("delta", C.c_double), ("n_overrelax", C.c_int32), ("renorm_every", C.c_int32),
                 ("record_capacity", C.c_int32), ("adapt_every", C.c_int32), ("adapt_target", C.c_double),
-                ("graph_sweeps", C.c_int32), ("reserved", C.c_int32)]
+                ("graph_sweeps", C.c_int32), ("ising_spt", C.c_int32)]
 
 
 class Observables(C.Structure):
