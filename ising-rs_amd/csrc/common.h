@@ -151,6 +151,10 @@ struct DevCtl {
     int64_t round;         // exchange round of sweep t
     int32_t renorm_every;
     int32_t adapt_every;
+    // phase counters of sweep t, advanced without division by the closing kernel
+    int64_t s;             // t - sample_start (negative while warming up)
+    int32_t ph_me, ph_ee;  // (s + 1) % measure_every, (s + 1) % exchange_every
+    int32_t ph_rn, ph_ad;  // (t + 1) % renorm_every, (t + 1) % adapt_every
 };
 
 enum : int32_t { F_MEAS = 1, F_EXCH = 2, F_RENORM = 4, F_ADAPT = 8, F_WARM = 16 };
@@ -158,21 +162,48 @@ enum : int32_t { F_MEAS = 1, F_EXCH = 2, F_RENORM = 4, F_ADAPT = 8, F_WARM = 16 
 // Per-sweep decisions for sweep index t, computed once per sweep (by the
 // closing kernel for the next sweep, by the host when a phase starts), so the
 // sweep kernels never divide 64-bit integers.
-__host__ __device__ inline void set_flags(DevCtl* c, int64_t t) {
-    const int64_t s = t - c->sample_start;
+__host__ __device__ inline void flags_from_phases(DevCtl* c) {
     int32_t f = 0;
-    int64_t round = 0;
-    if (s >= 0) {
-        if ((s + 1) % c->measure_every == 0) f |= F_MEAS;
-        if ((s + 1) % c->exchange_every == 0) f |= F_EXCH;
-        round = s / c->exchange_every;
+    if (c->s >= 0) {
+        if (c->ph_me == 0) f |= F_MEAS;
+        if (c->ph_ee == 0) f |= F_EXCH;
     } else {
         f |= F_WARM;
-        if (c->adapt_every > 0 && (t + 1) % c->adapt_every == 0) f |= F_ADAPT;
+        if (c->adapt_every > 0 && c->ph_ad == 0) f |= F_ADAPT;
     }
-    if (c->renorm_every > 0 && (t + 1) % c->renorm_every == 0) f |= F_RENORM;
+    if (c->renorm_every > 0 && c->ph_rn == 0) f |= F_RENORM;
     c->flags = f;
-    c->round = round;
+}
+
+// Exact phases for sweep t (host side, where dividing is free).
+inline void set_flags(DevCtl* c, int64_t t) {
+    const bool sampling = c->sample_start <= t;
+    c->s = sampling ? t - c->sample_start : -1;
+    c->ph_me = sampling ? (int32_t)((c->s + 1) % c->measure_every) : 1;
+    c->ph_ee = sampling ? (int32_t)((c->s + 1) % c->exchange_every) : 1;
+    c->round = sampling ? c->s / c->exchange_every : 0;
+    c->ph_rn = c->renorm_every > 0 ? (int32_t)((t + 1) % c->renorm_every) : 1;
+    c->ph_ad = c->adapt_every > 0 ? (int32_t)((t + 1) % c->adapt_every) : 1;
+    flags_from_phases(c);
+}
+
+__device__ __forceinline__ int32_t phase_step(int32_t ph, int32_t period) {
+    return ph + 1 == period ? 0 : ph + 1;
+}
+
+// Advance the phases from sweep t to t + 1 (device side, no division).  Once
+// sampling has started, s counts up; during warm-up it stays negative (the
+// host re-derives every phase exactly when sampling starts).
+__device__ __forceinline__ void advance_flags(DevCtl* c) {
+    if (c->s >= 0) {
+        if (c->ph_ee == 0) c->round += 1;
+        c->s += 1;
+        c->ph_me = phase_step(c->ph_me, c->measure_every);
+        c->ph_ee = phase_step(c->ph_ee, c->exchange_every);
+    }
+    if (c->renorm_every > 0) c->ph_rn = phase_step(c->ph_rn, c->renorm_every);
+    if (c->adapt_every > 0) c->ph_ad = phase_step(c->ph_ad, c->adapt_every);
+    flags_from_phases(c);
 }
 
 struct IsingRec {

@@ -187,7 +187,7 @@ __device__ void close_cont(const ContArgs& a, int64_t t, int32_t fl) {
             else ctl->overflow = 1;
         }
         ctl->t = t + 1;
-        set_flags(ctl, t + 1);
+        advance_flags(ctl);
     }
 }
 

@@ -108,7 +108,7 @@ __device__ void close_sweep(const IsingArgs& a, int64_t t, bool meas, bool exch,
             else ctl->overflow = 1;
         }
         ctl->t = t + 1;
-        set_flags(ctl, t + 1);
+        advance_flags(ctl);
     }
 }
 
