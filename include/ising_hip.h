@@ -226,6 +226,21 @@ int ising_pt_sweeps(const ising_pt_t* pt, int64_t* out);
 int ising_pt_device_bytes(const ising_pt_t* pt, int64_t* out);
 
 /* ------------------------------------------------------------------------
+ * Batch: several ensembles (e.g. Ising, XY and Heisenberg of config 4)
+ * advanced together.  One captured HIP graph per `graph_sweeps` sweeps holds
+ * a branch per member on the member's own stream, so one graph launch runs
+ * every model concurrently.  Members keep their own records, permutations
+ * and read functions; results are identical to running them separately.
+ * Not in the reference (it runs one model and one size at a time).
+ * ------------------------------------------------------------------------ */
+typedef struct ising_batch ising_batch_t;
+int ising_batch_create(ising_batch_t** out, ising_pt_t* const* members, int32_t n, int32_t graph_sweeps);
+void ising_batch_destroy(ising_batch_t* b);
+int ising_batch_warmup(ising_batch_t* b, int64_t sweeps);
+int ising_batch_sample(ising_batch_t* b, int64_t sweeps);
+int ising_batch_sync(ising_batch_t* b, double* last_ms);
+
+/* ------------------------------------------------------------------------
  * Host-side statistics and PT helpers (no GPU needed).
  * ------------------------------------------------------------------------ */
 /* linspace_temperatures (parallel_tempering.rs:39-46). */

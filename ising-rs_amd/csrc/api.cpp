@@ -39,6 +39,18 @@ int fail(int code, const std::string& msg) {
             return fail(ISING_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
     } while (0)
 
+#define SYNC_ALL(p)                        \
+    do {                                   \
+        const int rs_ = (p)->sync_all();   \
+        if (rs_ != ISING_OK) return rs_;   \
+    } while (0)
+
+#define ORDER(p)                           \
+    do {                                   \
+        const int ro_ = (p)->order();      \
+        if (ro_ != ISING_OK) return ro_;   \
+    } while (0)
+
 // 33-bit threshold T = floor(p * 2^32) (T = 2^32 when p >= 1), split in halves.
 void threshold_pair(double p, uint32_t* hi, uint32_t* lo) {
     uint64_t t;
@@ -146,6 +158,7 @@ struct ising_pt {
         (void)hipFree(d_lgout);
         (void)hipFree(d_llpart);
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (ext_ev) (void)hipEventDestroy(ext_ev);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (st) (void)hipStreamDestroy(st);
@@ -281,9 +294,48 @@ struct ising_pt {
         return ISING_OK;
     }
 
-    int read_ctl(DevCtl* out) {
+    hipStream_t ext_st = nullptr;  // stream of a batch that also drives this ensemble
+    hipEvent_t ext_ev = nullptr;
+    int graph_gen = 0;             // bumped whenever captured launches go stale
+
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        graph_exec = nullptr;
+        ++graph_gen;
+    }
+
+    // Work enqueued on this ensemble's stream follows what a batch queued before.
+    int order() {
+        if (!ext_st) return ISING_OK;
+        if (!ext_ev) HIP_TRY(hipEventCreateWithFlags(&ext_ev, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(ext_ev, ext_st));
+        HIP_TRY(hipStreamWaitEvent(st, ext_ev, 0));
+        return ISING_OK;
+    }
+
+    int sync_all() {
+        if (ext_st) HIP_TRY(hipStreamSynchronize(ext_st));
         HIP_TRY(hipStreamSynchronize(st));
+        return ISING_OK;
+    }
+
+    int read_ctl(DevCtl* out) {
+        const int rc = sync_all();
+        if (rc) return rc;
         HIP_TRY(hipMemcpy(out, d_ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
+        return ISING_OK;
+    }
+
+    // First sample sweep: fix sample_start and the phase counters on the device.
+    int begin_sampling() {
+        if (s_host >= 0) return ISING_OK;
+        DevCtl c;
+        const int rc = read_ctl(&c);
+        if (rc) return rc;
+        c.sample_start = t_host;
+        set_flags(&c, t_host);
+        HIP_TRY(hipMemcpy(d_ctl, &c, sizeof(DevCtl), hipMemcpyHostToDevice));
+        s_host = 0;
         return ISING_OK;
     }
 };
@@ -456,7 +508,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
                                  model == ISING_MODEL_XY ? MODEL_XY : MODEL_HEIS, p->nblocks, (uint32_t)seed,
                                  (uint32_t)(seed >> 32), p->st));
     }
-    HIP_TRY(hipStreamSynchronize(p->st));
+    SYNC_ALL(p);
     return ISING_OK;
 }
 
@@ -517,6 +569,7 @@ int ising_pt_warmup(ising_pt_t* p, int64_t sweeps) {
     if (!p) return fail(ISING_E_ARG, "null handle");
     if (sweeps < 0) return fail(ISING_E_ARG, "sweeps must be >= 0");
     if (p->s_host >= 0) return fail(ISING_E_STATE, "warm-up after sampling started");
+    ORDER(p);
     HIP_TRY(hipEventRecord(p->ev0, p->st));
     const int rc = p->run_sweeps(sweeps);
     if (rc) return rc;
@@ -528,19 +581,12 @@ int ising_pt_warmup(ising_pt_t* p, int64_t sweeps) {
 int ising_pt_sample(ising_pt_t* p, int64_t sweeps) {
     if (!p) return fail(ISING_E_ARG, "null handle");
     if (sweeps < 0) return fail(ISING_E_ARG, "sweeps must be >= 0");
-    if (p->s_host < 0) {
-        // sampling starts at the next sweep; the device reads sample_start from its control block
-        DevCtl c;
-        int rc0 = p->read_ctl(&c);
-        if (rc0) return rc0;
-        c.sample_start = p->t_host;
-        set_flags(&c, p->t_host);
-        HIP_TRY(hipMemcpy(p->d_ctl, &c, sizeof(DevCtl), hipMemcpyHostToDevice));
-        p->s_host = 0;
-    }
+    const int rc0 = p->begin_sampling();
+    if (rc0) return rc0;
     const int64_t add = measured_in(p->s_host, sweeps, p->opts.measure_every);
     if (p->pending_host + add > p->opts.record_capacity)
         return fail(ISING_E_FULL, "measurement buffer would overflow: drain records before sampling more");
+    ORDER(p);
     HIP_TRY(hipEventRecord(p->ev0, p->st));
     const int rc = p->run_sweeps(sweeps);
     if (rc) return rc;
@@ -553,7 +599,8 @@ int ising_pt_sample(ising_pt_t* p, int64_t sweeps) {
 
 int ising_pt_sync(ising_pt_t* p, double* last_ms) {
     if (!p) return fail(ISING_E_ARG, "null handle");
-    HIP_TRY(hipStreamSynchronize(p->st));
+    const int rc = p->sync_all();
+    if (rc) return rc;
     if (last_ms) {
         float ms = 0.f;
         if (p->timed) HIP_TRY(hipEventElapsedTime(&ms, p->ev0, p->ev1));
@@ -632,7 +679,7 @@ int ising_pt_get_perm(ising_pt_t* p, int32_t size_index, int32_t* r2t, int32_t* 
     if (!p) return fail(ISING_E_ARG, "null handle");
     const DevGroup* G = group_of(p, size_index);
     if (!G) return fail(ISING_E_ARG, "size_index out of range");
-    HIP_TRY(hipStreamSynchronize(p->st));
+    SYNC_ALL(p);
     if (r2t) HIP_TRY(hipMemcpy(r2t, p->d_r2t + G->rep0, sizeof(int32_t) * G->R, hipMemcpyDeviceToHost));
     if (t2r) HIP_TRY(hipMemcpy(t2r, p->d_t2r + G->rep0, sizeof(int32_t) * G->R, hipMemcpyDeviceToHost));
     return ISING_OK;
@@ -652,7 +699,7 @@ int ising_pt_exchange_stats(ising_pt_t* p, int64_t* tries, int64_t* accepts) {
 static int ising_halves(ising_pt* p, const DevGroup* G, int replica, std::vector<uint8_t>& half,
                         std::vector<uint32_t>* words) {
     half.assign(2 * (size_t)G->n_half, 0);
-    HIP_TRY(hipStreamSynchronize(p->st));
+    SYNC_ALL(p);
     if (p->model == ISING_MODEL_ISING) {
         for (int c = 0; c < 2; ++c)
             HIP_TRY(hipMemcpy(half.data() + c * G->n_half,
@@ -705,7 +752,7 @@ int ising_pt_set_spins_i8(ising_pt_t* p, int32_t size_index, int32_t replica, co
                 const int c = (x + y + z) & 1;
                 half[c * G->n_half + ((int64_t)z * L + y) * Lh + (x >> 1)] = v > 0 ? 1 : 0;
             }
-    HIP_TRY(hipStreamSynchronize(p->st));
+    SYNC_ALL(p);
     if (p->model == ISING_MODEL_ISING) {
         for (int c = 0; c < 2; ++c)
             HIP_TRY(hipMemcpy(p->d_pool + G->off + (2 * (int64_t)replica + c) * G->stride,
@@ -731,7 +778,7 @@ int ising_pt_get_spins_f32(ising_pt_t* p, int32_t size_index, int32_t replica, f
     const DevGroup* G = group_of(p, size_index);
     if (!G || replica < 0 || replica >= G->R) return fail(ISING_E_ARG, "size_index or replica out of range");
     const int nc = p->ncomp;
-    HIP_TRY(hipStreamSynchronize(p->st));
+    SYNC_ALL(p);
     std::vector<float> half(2 * (size_t)nc * G->n_half);
     for (int c = 0; c < 2; ++c)
         for (int k = 0; k < nc; ++k)
@@ -770,7 +817,7 @@ int ising_pt_set_spins_f32(ising_pt_t* p, int32_t size_index, int32_t replica, c
                     half[(size_t)(c * nc + k) * G->n_half + h] = v;
                 }
             }
-    HIP_TRY(hipStreamSynchronize(p->st));
+    SYNC_ALL(p);
     for (int c = 0; c < 2; ++c)
         for (int k = 0; k < nc; ++k)
             HIP_TRY(hipMemcpy(p->d_fpool + G->off + ((2 * (int64_t)replica + c) * nc + k) * G->stride,
@@ -784,7 +831,7 @@ int ising_pt_get_deltas(ising_pt_t* p, int32_t size_index, float* deltas) {
     if (p->ncomp == 0) return fail(ISING_E_STATE, "not a continuous-spin ensemble");
     const DevGroup* G = group_of(p, size_index);
     if (!G) return fail(ISING_E_ARG, "size_index out of range");
-    HIP_TRY(hipStreamSynchronize(p->st));
+    SYNC_ALL(p);
     HIP_TRY(hipMemcpy(deltas, p->d_deltas + G->rep0, sizeof(float) * G->R, hipMemcpyDeviceToHost));
     return ISING_OK;
 }
@@ -805,7 +852,7 @@ int ising_pt_learn_init(ising_pt_t* p, int32_t hidden, uint64_t seed) {
     if (p->ncomp == 0) return fail(ISING_E_STATE, "the learned proposal needs an XY or Heisenberg ensemble");
     const int P = learn_params(hidden);
     if (P < 0) return fail(ISING_E_ARG, "hidden width must be 8, 16, 32 or 64");
-    HIP_TRY(hipStreamSynchronize(p->st));
+    SYNC_ALL(p);
     const int cap = 4096;  // training blocks per step (64 samples each)
     (void)hipFree(p->d_lmaster); (void)hipFree(p->d_lnet); (void)hipFree(p->d_lm1); (void)hipFree(p->d_lm2);
     (void)hipFree(p->d_lgpart); (void)hipFree(p->d_lgout); (void)hipFree(p->d_llpart);
@@ -847,11 +894,8 @@ int ising_pt_learn_init(ising_pt_t* p, int32_t hidden, uint64_t seed) {
     HIP_TRY(hipMemset(p->d_lm1, 0, sizeof(float) * P));
     HIP_TRY(hipMemset(p->d_lm2, 0, sizeof(float) * P));
     HIP_TRY(launch_learn_pack(p->d_lmaster, p->d_lnet, P, p->st));
-    HIP_TRY(hipStreamSynchronize(p->st));
-    if (p->graph_exec) {  // captured launches carry the old network pointer
-        (void)hipGraphExecDestroy(p->graph_exec);
-        p->graph_exec = nullptr;
-    }
+    SYNC_ALL(p);
+    p->drop_graph();  // captured launches carry the old network pointer
     return ISING_OK;
 }
 
@@ -867,7 +911,7 @@ int ising_pt_learn_get_params(ising_pt_t* p, float* out, int64_t n) {
     const int rc = need_learn(p);
     if (rc) return rc;
     if (!out || n != p->learn_P) return fail(ISING_E_ARG, "need exactly ising_pt_learn_size() floats");
-    HIP_TRY(hipStreamSynchronize(p->st));
+    SYNC_ALL(p);
     HIP_TRY(hipMemcpy(out, p->d_lmaster, sizeof(float) * n, hipMemcpyDeviceToHost));
     return ISING_OK;
 }
@@ -878,10 +922,10 @@ int ising_pt_learn_set_params(ising_pt_t* p, const float* in, int64_t n) {
     if (!in || n != p->learn_P) return fail(ISING_E_ARG, "need exactly ising_pt_learn_size() floats");
     for (int64_t i = 0; i < n; ++i)
         if (!std::isfinite(in[i])) return fail(ISING_E_ARG, "parameters must be finite");
-    HIP_TRY(hipStreamSynchronize(p->st));
+    SYNC_ALL(p);
     HIP_TRY(hipMemcpy(p->d_lmaster, in, sizeof(float) * n, hipMemcpyHostToDevice));
     HIP_TRY(launch_learn_pack(p->d_lmaster, p->d_lnet, (int)n, p->st));
-    HIP_TRY(hipStreamSynchronize(p->st));
+    SYNC_ALL(p);
     return ISING_OK;
 }
 
@@ -890,6 +934,7 @@ int ising_pt_learn_train(ising_pt_t* p, int32_t steps, int32_t batch, double lr,
     if (rc) return rc;
     if (steps < 0 || batch < kLearnBlock || !(lr > 0.0)) return fail(ISING_E_ARG, "steps >= 0, batch >= 64, lr > 0");
     const int nblk = std::min(p->learn_cap_blocks, batch / kLearnBlock);
+    ORDER(p);
     const ContArgs a = p->cont_args();
     for (int s = 0; s < steps; ++s) {
         ++p->learn_step;
@@ -900,7 +945,7 @@ int ising_pt_learn_train(ising_pt_t* p, int32_t steps, int32_t batch, double lr,
     if (mean_loss) {
         std::vector<double> lp(nblk);
         HIP_TRY(hipMemcpyAsync(lp.data(), p->d_llpart, sizeof(double) * nblk, hipMemcpyDeviceToHost, p->st));
-        HIP_TRY(hipStreamSynchronize(p->st));
+        SYNC_ALL(p);
         double sum = 0.0;
         for (double v : lp) sum += v;
         *mean_loss = steps > 0 ? sum / (double)(nblk * kLearnBlock) : NAN;
@@ -914,6 +959,7 @@ int ising_pt_learn_eval(ising_pt_t* p, int64_t n, const float* x, const float* c
     if (n < 1 || !x || !c) return fail(ISING_E_ARG, "need n >= 1 samples");
     const int nblk = (int)((n + kLearnBlock - 1) / kLearnBlock);
     if (nblk > p->learn_cap_blocks) return fail(ISING_E_ARG, "too many samples");
+    ORDER(p);
     float *dx = nullptr, *dc = nullptr;
     HIP_TRY(hipMalloc(&dx, sizeof(float) * 6 * n));
     HIP_TRY(hipMalloc(&dc, sizeof(float) * n));
@@ -947,10 +993,9 @@ int ising_pt_learn_enable(ising_pt_t* p, int32_t on) {
     } else if (!p) {
         return fail(ISING_E_ARG, "null handle");
     }
-    if ((bool)on != p->learned_on && p->graph_exec) {
-        HIP_TRY(hipStreamSynchronize(p->st));
-        (void)hipGraphExecDestroy(p->graph_exec);
-        p->graph_exec = nullptr;
+    if ((bool)on != p->learned_on) {
+        SYNC_ALL(p);
+        p->drop_graph();
     }
     p->learned_on = on != 0;
     return ISING_OK;
@@ -970,6 +1015,7 @@ int ising_pt_metropolis_pass(ising_pt_t* p, int32_t color, int64_t t) {
     if (!p) return fail(ISING_E_ARG, "null handle");
     if (p->ncomp == 0) return fail(ISING_E_STATE, "continuous ensembles only");
     if ((color != 0 && color != 1) || t < 0) return fail(ISING_E_ARG, "color 0/1 and t >= 0");
+    ORDER(p);
     if (p->learned_on) HIP_TRY(launch_cont_learned(p->cont_args(), p->nblocks, color, 0, t, p->st));
     else return fail(ISING_E_STATE, "enable the learned proposal first");
     return ISING_OK;
@@ -1057,7 +1103,7 @@ int ising_lattice_measure_raw(ising_lattice_t* l, int64_t* bond, int64_t* mag) {
     HIP_TRY(launch_ising_measure(p->d_groups, 1, p->d_pool, acc - 0, p->nblocks, p->st));
     int64_t v[2];
     HIP_TRY(hipMemcpyAsync(v, acc, sizeof(v), hipMemcpyDeviceToHost, p->st));
-    HIP_TRY(hipStreamSynchronize(p->st));
+    SYNC_ALL(p);
     if (bond) *bond = v[0];
     if (mag) *mag = v[1];
     return ISING_OK;
@@ -1215,7 +1261,7 @@ int ising_cont_measure_gpu(ising_cont_t* l, float j, double* e, double* mx, doub
     HIP_TRY(launch_cont_measure(p->d_groups, 1, p->d_fpool, p->ncomp, (double)j, p->d_part0, p->nblocks, p->st));
     std::vector<double> part(4 * (size_t)p->nblocks);
     HIP_TRY(hipMemcpyAsync(part.data(), p->d_part0, sizeof(double) * part.size(), hipMemcpyDeviceToHost, p->st));
-    HIP_TRY(hipStreamSynchronize(p->st));
+    SYNC_ALL(p);
     double v[4] = {0, 0, 0, 0};
     for (int b = 0; b < p->nblocks; ++b)
         for (int k = 0; k < 4; ++k) v[k] += part[4 * (size_t)b + k];
@@ -1243,6 +1289,180 @@ int ising_cont_set_spins(ising_cont_t* l, const float* in) {
 int ising_cont_sweeps(const ising_cont_t* l, int64_t* out) {
     if (!l || !out) return fail(ISING_E_ARG, "null argument");
     *out = l->pt->t_host;
+    return ISING_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batch of ensembles advanced together (one graph with a branch per member)
+// ---------------------------------------------------------------------------
+
+}  // extern "C"
+
+struct ising_batch {
+    std::vector<ising_pt*> m;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, fork = nullptr;
+    std::vector<hipEvent_t> join;
+    std::vector<int> gens;  // members' graph generations the captured graph was built for
+    hipGraphExec_t exec = nullptr;
+    int G = 32;
+    bool timed = false;
+    ~ising_batch() {
+        if (st) (void)hipStreamSynchronize(st);
+        for (ising_pt* p : m) p->ext_st = nullptr;
+        if (exec) (void)hipGraphExecDestroy(exec);
+        for (hipEvent_t e : join) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    // order: members' own streams wait for everything queued on the batch stream
+    int members_after_batch() {
+        HIP_TRY(hipEventRecord(fork, st));
+        for (ising_pt* p : m) HIP_TRY(hipStreamWaitEvent(p->st, fork, 0));
+        return ISING_OK;
+    }
+    int batch_after_members() {
+        for (size_t i = 0; i < m.size(); ++i) {
+            HIP_TRY(hipEventRecord(join[i], m[i]->st));
+            HIP_TRY(hipStreamWaitEvent(st, join[i], 0));
+        }
+        return ISING_OK;
+    }
+
+    int run(int64_t n) {
+        int rc = members_after_batch();
+        if (rc) return rc;
+        rc = batch_after_members();  // the batch stream now follows any eager member work
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(ev0, st));
+        if (n >= G) {
+            bool stale = gens.size() != m.size();
+            for (size_t i = 0; !stale && i < m.size(); ++i) stale = gens[i] != m[i]->graph_gen;
+            if (exec && stale) {
+                (void)hipGraphExecDestroy(exec);
+                exec = nullptr;
+            }
+            if (!exec) {
+                HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                hipError_t e = hipEventRecord(fork, st);
+                int rc2 = ISING_OK;
+                for (size_t i = 0; i < m.size() && e == hipSuccess && rc2 == ISING_OK; ++i) {
+                    e = hipStreamWaitEvent(m[i]->st, fork, 0);
+                    if (e != hipSuccess) break;
+                    rc2 = m[i]->enqueue_sweeps(G);
+                    if (rc2 == ISING_OK) m[i]->t_host -= G;  // capture enqueued nothing
+                    if (rc2 == ISING_OK) e = hipEventRecord(join[i], m[i]->st);
+                    if (e == hipSuccess && rc2 == ISING_OK) e = hipStreamWaitEvent(st, join[i], 0);
+                }
+                hipGraph_t graph = nullptr;
+                const hipError_t ec = hipStreamEndCapture(st, &graph);
+                if (rc2 != ISING_OK) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return rc2;
+                }
+                HIP_TRY(e);
+                HIP_TRY(ec);
+                const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                HIP_TRY(ei);
+                gens.clear();
+                for (ising_pt* p : m) gens.push_back(p->graph_gen);
+            }
+            const int64_t reps = n / G;
+            for (int64_t i = 0; i < reps; ++i) HIP_TRY(hipGraphLaunch(exec, st));
+            for (ising_pt* p : m) p->t_host += reps * G;
+            n -= reps * G;
+        }
+        if (n > 0) {
+            rc = members_after_batch();
+            if (rc) return rc;
+            for (ising_pt* p : m) {
+                rc = p->enqueue_sweeps(n);
+                if (rc) return rc;
+            }
+            rc = batch_after_members();
+            if (rc) return rc;
+        }
+        HIP_TRY(hipEventRecord(ev1, st));
+        timed = true;
+        return ISING_OK;
+    }
+};
+
+extern "C" {
+
+int ising_batch_create(ising_batch_t** out, ising_pt_t* const* members, int32_t n, int32_t graph_sweeps) {
+    if (!out || !members || n < 1 || n > 64 || graph_sweeps < 1 || graph_sweeps > 100000)
+        return fail(ISING_E_ARG, "need 1..64 ensembles and graph_sweeps in 1..100000");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!members[i]) return fail(ISING_E_ARG, "null member");
+        if (members[i]->ext_st) return fail(ISING_E_STATE, "an ensemble can belong to one batch only");
+        if (members[i]->inline_thr || members[i]->inline_cont) return fail(ISING_E_ARG, "single-lattice handles cannot be batched");
+        for (int j = 0; j < i; ++j)
+            if (members[j] == members[i]) return fail(ISING_E_ARG, "duplicate member");
+    }
+    ising_batch* b = new (std::nothrow) ising_batch();
+    if (!b) return fail(ISING_E_HIP, "host allocation failed");
+    auto bad = [&](hipError_t e, const char* what) {
+        delete b;
+        return fail(ISING_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking)) != hipSuccess) return bad(e, "stream");
+    if ((e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess) return bad(e, "event");
+    if ((e = hipEventCreate(&b->ev0)) != hipSuccess) return bad(e, "event");
+    if ((e = hipEventCreate(&b->ev1)) != hipSuccess) return bad(e, "event");
+    b->join.resize(n, nullptr);
+    for (int i = 0; i < n; ++i)
+        if ((e = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming)) != hipSuccess) return bad(e, "event");
+    b->G = graph_sweeps;
+    for (int i = 0; i < n; ++i) {
+        b->m.push_back(members[i]);
+        members[i]->ext_st = b->st;
+    }
+    *out = b;
+    return ISING_OK;
+}
+
+void ising_batch_destroy(ising_batch_t* b) { delete b; }
+
+int ising_batch_warmup(ising_batch_t* b, int64_t sweeps) {
+    if (!b || sweeps < 0) return fail(ISING_E_ARG, "null handle or negative sweeps");
+    for (ising_pt* p : b->m)
+        if (p->s_host >= 0) return fail(ISING_E_STATE, "warm-up after sampling started");
+    return b->run(sweeps);
+}
+
+int ising_batch_sample(ising_batch_t* b, int64_t sweeps) {
+    if (!b || sweeps < 0) return fail(ISING_E_ARG, "null handle or negative sweeps");
+    for (ising_pt* p : b->m) {
+        const int rc = p->begin_sampling();
+        if (rc) return rc;
+        if (p->pending_host + measured_in(p->s_host, sweeps, p->opts.measure_every) > p->opts.record_capacity)
+            return fail(ISING_E_FULL, "a member's measurement buffer would overflow: drain records first");
+    }
+    const int rc = b->run(sweeps);
+    if (rc) return rc;
+    for (ising_pt* p : b->m) {
+        p->pending_host += measured_in(p->s_host, sweeps, p->opts.measure_every);
+        p->s_host += sweeps;
+    }
+    return ISING_OK;
+}
+
+int ising_batch_sync(ising_batch_t* b, double* last_ms) {
+    if (!b) return fail(ISING_E_ARG, "null handle");
+    for (ising_pt* p : b->m) SYNC_ALL(p);
+    HIP_TRY(hipStreamSynchronize(b->st));
+    if (last_ms) {
+        float ms = 0.f;
+        if (b->timed) HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        *last_ms = ms;
+    }
     return ISING_OK;
 }
 

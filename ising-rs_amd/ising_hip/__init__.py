@@ -110,6 +110,11 @@ _SIGS = {
     "ising_pt_metropolis_pass": (C.c_int, [_vp, C.c_int32, C.c_int64]),
     "ising_pt_sweeps": (C.c_int, [_vp, _i64p]),
     "ising_pt_device_bytes": (C.c_int, [_vp, _i64p]),
+    "ising_batch_create": (C.c_int, [C.POINTER(_vp), _vp, C.c_int32, C.c_int32]),
+    "ising_batch_destroy": (None, [_vp]),
+    "ising_batch_warmup": (C.c_int, [_vp, C.c_int64]),
+    "ising_batch_sample": (C.c_int, [_vp, C.c_int64]),
+    "ising_batch_sync": (C.c_int, [_vp, _f64p]),
     "ising_linspace_temperatures": (C.c_int, [C.c_double, C.c_double, C.c_int32, _vp]),
     "ising_replica_exchange": (C.c_int, [_vp, _vp, C.c_int32, _vp, _vp, _vp, C.c_int32, C.POINTER(C.c_int32)]),
     "ising_jackknife": (C.c_int, [_vp, _vp, C.c_int64, C.c_double, C.c_double, C.c_int32, _vp]),
@@ -431,6 +436,35 @@ class PTEnsemble:
         v = C.c_int64()
         _check(lib.ising_pt_device_bytes(self._h, C.byref(v)))
         return v.value
+
+
+class Batch:
+    """Several PTEnsembles advanced together by one captured graph per block of sweeps."""
+
+    def __init__(self, members, graph_sweeps=32):
+        self.members = list(members)
+        arr = (_vp * len(self.members))(*[m._h for m in self.members])
+        self._arr = arr
+        self._h = _vp()
+        _check(lib.ising_batch_create(C.byref(self._h), arr, len(self.members), int(graph_sweeps)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.ising_batch_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def warmup(self, sweeps):
+        _check(lib.ising_batch_warmup(self._h, int(sweeps)))
+
+    def sample(self, sweeps):
+        _check(lib.ising_batch_sample(self._h, int(sweeps)))
+
+    def sync(self):
+        ms = C.c_double()
+        _check(lib.ising_batch_sync(self._h, C.byref(ms)))
+        return ms.value
 
 
 def linspace_temperatures(t_min, t_max, n):
