@@ -97,6 +97,72 @@ WORKLOADS = {
 }
 
 
+BATCH_SIZES = [8, 12, 16, 24, 32, 48, 64, 96, 128]
+BATCH_MODELS = [
+    # model, temperatures, bytes/update, options
+    ("ising", (4.40, 4.62), 3, dict(exchange_every=10, measure_every=5)),
+    ("xy", (2.15, 2.25), 24, dict(exchange_every=10, measure_every=5, n_overrelax=2, delta=1.5, renorm_every=100)),
+    ("heisenberg", (1.40, 1.50), 36,
+     dict(exchange_every=10, measure_every=5, n_overrelax=2, delta=1.0, renorm_every=100)),
+]
+
+
+def run_batched(args, ising_hip):
+    """Ising + XY + Heisenberg, L = 8..128 (9 sizes) x 32 replicas each, advanced
+    together by one captured graph per block of sweeps (a branch per model).
+    One step = one PT sweep of all 3 x 109,041,664 sites (one Metropolis pass
+    per site; the continuous models add 2 overrelaxation passes)."""
+    sites = sum(L ** 3 for L in BATCH_SIZES) * R
+    members = []
+    for model, (tlo, thi), _, opts in BATCH_MODELS:
+        cap = args.steps // opts["measure_every"] + 8
+        members.append(ising_hip.PTEnsemble(model, BATCH_SIZES, R, linspace(tlo, thi, R), args.seed,
+                                            record_capacity=cap, graph_sweeps=0, **opts))
+    batch = ising_hip.Batch(members, graph_sweeps=16)
+    batch.warmup(args.warmup)
+    batch.sync()
+    t0 = time.perf_counter()
+    batch.sample(args.steps)
+    ev_ms = batch.sync()
+    wall = time.perf_counter() - t0
+    for p, (model, _, _, opts) in zip(members, BATCH_MODELS):
+        recs = p.read_ising() if model == "ising" else p.read_cont()
+        assert recs[0].shape[0] == args.steps // opts["measure_every"]
+    updates = 3 * sites * args.steps
+    passes = sum(1 + o.get("n_overrelax", 0) for _, _, _, o in BATCH_MODELS)
+    alg_bytes_per_step = sum(bpu * sites * (1 + o.get("n_overrelax", 0)) for _, _, bpu, o in BATCH_MODELS)
+    step_s = ev_ms * 1e-3 / args.steps
+    achieved = alg_bytes_per_step / step_s / 1e9
+    line = {
+        "metric": "spin updates/ns",
+        "value": round(updates / (wall * 1e9), 3),
+        "unit": "updates/ns",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8 + f32",
+        "data": "synthetic",
+        "config": {"workload": "batched PT (config 4): Ising int8 T=4.40-4.62 + XY T=2.15-2.25 + Heisenberg "
+                               "T=1.40-1.50, L=8,12,16,24,32,48,64,96,128 x 32 replicas each, exchange every 10, "
+                               "measure every 5, continuous models 1 Metropolis + 2 overrelaxation passes",
+                   "sizes": BATCH_SIZES, "replicas": R, "models": [m for m, _, _, _ in BATCH_MODELS],
+                   "updates_per_step": 3 * sites, "site_passes_per_step": passes * sites,
+                   "combined_updates_per_ns": round(passes * sites * args.steps / (wall * 1e9), 3)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "whole sweep graph (all kernels of the three models, concurrent branches)",
+                     "alg_bytes_per_step": int(alg_bytes_per_step), "avg_step_us": round(step_s * 1e6, 3),
+                     "device_updates_per_ns": round(updates / (ev_ms * 1e6), 3)},
+    }
+    if args.cpu_seconds > 0:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds, "ising")
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,7 +170,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--workload", default="ising", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="ising", choices=sorted(WORKLOADS) + ["batched"])
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -121,6 +187,9 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.workload == "batched":
+        run_batched(args, ising_hip)
+        return
     model, sizes, (tlo, thi), bpu, dtype, opts, desc = WORKLOADS[args.workload]
     temps = linspace(tlo, thi, R)
     me = opts["measure_every"]

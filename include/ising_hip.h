@@ -152,8 +152,8 @@ typedef struct {
     double adapt_target;     /* target Metropolis acceptance for the adaptation */
     int32_t graph_sweeps;    /* sweeps per captured HIP graph replayed by warmup/sample
                                 (0 = launch every kernel directly) */
-    int32_t ising_spt;       /* Ising: 8-site groups per thread, 1 or 2 (0 = automatic:
-                                2 for L >= 64); results do not depend on it */
+    int32_t ising_spt;       /* Ising: 16-site chunks per thread, 1..4 (0 = automatic);
+                                results do not depend on it */
 } ising_pt_opts_t;
 
 /* Defaults: j=1, h=0, exchange_every=10, measure_every=1, delta=0.5,

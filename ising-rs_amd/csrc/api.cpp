@@ -51,21 +51,39 @@ int fail(int code, const std::string& msg) {
         if (ro_ != ISING_OK) return ro_;   \
     } while (0)
 
+// Transfers to and from an ensemble's buffers are ordered on the ensemble's
+// own stream and waited for: that stream is non-blocking, so the legacy null
+// stream (plain hipMemcpy / hipMemset) would not be ordered against its kernels.
+#define ST_COPY(p, dst, src, n, kind)                                        \
+    do {                                                                     \
+        HIP_TRY(hipMemcpyAsync((dst), (src), (n), (kind), (p)->st));         \
+        HIP_TRY(hipStreamSynchronize((p)->st));                              \
+    } while (0)
+
+#define ST_FILL(p, dst, v, n)                                                \
+    do {                                                                     \
+        HIP_TRY(hipMemsetAsync((dst), (v), (n), (p)->st));                   \
+        HIP_TRY(hipStreamSynchronize((p)->st));                              \
+    } while (0)
+
 // 33-bit threshold T = floor(p * 2^32) (T = 2^32 when p >= 1), split in halves.
 void threshold_pair(double p, uint32_t* hi, uint32_t* lo) {
     uint64_t t;
     if (!(p < 1.0)) t = 1ull << 32;
     else if (p <= 0.0) t = 0;
     else t = (uint64_t)std::ldexp(p, 32);
-    *hi = (uint32_t)(t >> 16);
-    *lo = (uint32_t)(t & 0xffffu);
+    *hi = (uint32_t)(t >> 25);         // 0..128, compared with a 7-bit draw
+    *lo = (uint32_t)(t & 0x1ffffffu);  // compared with 25 more bits on a tie
 }
 
 // Acceptance table for k = b*(z+1) + n_up, z = 2 * dims neighbours:
-// dE = 2 s (J (2 n_up - z) + h).  High halves in out[0..13], low in out[14..27].
-void ising_table(double beta, double J, double h, uint32_t out[28], int dims = 3) {
+// dE = 2 s (J (2 n_up - z) + h), T(k) = floor(2^32 min(1, exp(-beta dE))) =
+// hi * 2^25 + lo.  out[0..13] = hi, out[14..27] = lo, and out[28..31] the hi
+// bytes packed for byte-permute lookups: out[28] = hi[0..3], out[29] =
+// hi[4..6] (b = 0), out[30] = hi[7..10], out[31] = hi[11..13] (b = 1).
+void ising_table(double beta, double J, double h, uint32_t out[kIsingTab], int dims = 3) {
     const int z = 2 * dims;
-    for (int i = 0; i < 28; ++i) out[i] = 0;
+    for (int i = 0; i < kIsingTab; ++i) out[i] = 0;
     for (int b = 0; b <= 1; ++b) {
         const double s = b ? 1.0 : -1.0;
         for (int n = 0; n <= z; ++n) {
@@ -74,6 +92,9 @@ void ising_table(double beta, double J, double h, uint32_t out[28], int dims = 3
             threshold_pair(p, &out[b * (z + 1) + n], &out[14 + b * (z + 1) + n]);
         }
     }
+    if (dims == 3)
+        for (int b = 0; b <= 1; ++b)
+            for (int n = 0; n <= 6; ++n) out[28 + 2 * b + (n >> 2)] |= out[7 * b + n] << (8 * (n & 3));
 }
 
 bool valid_L(int L) { return L >= 2 && L <= 2046 && (L % 2) == 0; }
@@ -99,6 +120,7 @@ struct ising_pt {
     int64_t pool_bytes = 0;
     uint32_t* d_thr = nullptr;
     uint32_t* d_mthr = nullptr;  // multispin coding thresholds
+    uint32_t* d_mplanes = nullptr;  // multispin coding threshold bit planes per word
     int32_t* d_r2t = nullptr;
     int32_t* d_t2r = nullptr;
     double* d_betas = nullptr;
@@ -128,7 +150,7 @@ struct ising_pt {
     int64_t s_host = -1;       // sample sweeps enqueued (-1 before sampling starts)
     int64_t pending_host = 0;  // measured sweeps the device will have written
     bool inline_thr = false;   // single-lattice mode
-    uint32_t thr_inline[28];
+    uint32_t thr_inline[kIsingTab];
     hipGraphExec_t graph_exec = nullptr;  // `opts.graph_sweeps` sweeps, captured once
 
     ~ising_pt() {
@@ -137,6 +159,7 @@ struct ising_pt {
         (void)hipFree(d_pool);
         (void)hipFree(d_thr);
         (void)hipFree(d_mthr);
+        (void)hipFree(d_mplanes);
         (void)hipFree(d_r2t);
         (void)hipFree(d_t2r);
         (void)hipFree(d_betas);
@@ -184,6 +207,7 @@ struct ising_pt {
         a.inline_thr = inline_thr ? 1 : 0;
         if (inline_thr) std::memcpy(a.thr_inline, thr_inline, sizeof(thr_inline));
         a.mthr = d_mthr;
+        a.mplanes = d_mplanes;
         a.acc_copies = model == ISING_MODEL_ISING_MSC ? kMscAccCopies : 1;
         return a;
     }
@@ -322,7 +346,7 @@ struct ising_pt {
     int read_ctl(DevCtl* out) {
         const int rc = sync_all();
         if (rc) return rc;
-        HIP_TRY(hipMemcpy(out, d_ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
+        ST_COPY(this, out, d_ctl, sizeof(DevCtl), hipMemcpyDeviceToHost);
         return ISING_OK;
     }
 
@@ -334,7 +358,7 @@ struct ising_pt {
         if (rc) return rc;
         c.sample_start = t_host;
         set_flags(&c, t_host);
-        HIP_TRY(hipMemcpy(d_ctl, &c, sizeof(DevCtl), hipMemcpyHostToDevice));
+        ST_COPY(this, d_ctl, &c, sizeof(DevCtl), hipMemcpyHostToDevice);
         s_host = 0;
         return ISING_OK;
     }
@@ -371,7 +395,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     if (o.exchange_every < 1 || o.measure_every < 1) return fail(ISING_E_ARG, "exchange_every and measure_every must be >= 1");
     if (o.record_capacity < 1) return fail(ISING_E_ARG, "record_capacity must be >= 1");
     if (o.graph_sweeps < 0 || o.graph_sweeps > 100000) return fail(ISING_E_ARG, "graph_sweeps must be in 0..100000");
-    if (o.ising_spt < 0 || o.ising_spt > 2) return fail(ISING_E_ARG, "ising_spt must be 0, 1 or 2");
+    if (o.ising_spt < 0 || o.ising_spt > 4) return fail(ISING_E_ARG, "ising_spt must be in 0..4");
     if (model == ISING_MODEL_ISING_MSC && !(o.h == 0.0 && o.j > 0.0))
         return fail(ISING_E_ARG, "multispin coding needs h = 0 and J > 0");
     if (model == ISING_MODEL_XY || model == ISING_MODEL_HEISENBERG) {
@@ -410,7 +434,10 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
         G.Lh = sizes[i] / 2;
         G.R = R;
         G.dims = p->dims;
-        G.fast = (!two_d && (sizes[i] % 16) == 0) ? 1 : 0;
+        // Ising thread-step layout: 3 = 32 sites in one row (L % 64 == 0),
+        // 2 = 16 sites in one row (L % 32 == 0), 1 = two 8-site runs (L % 16
+        // == 0), 0 = 16 sites addressed one by one (other L, and 2D).
+        G.fast = two_d ? 0 : ((G.Lh % 32) == 0 ? 3 : ((G.Lh % 16) == 0 ? 2 : ((G.Lh % 8) == 0 ? 1 : 0)));
         G.spt = 1;
         if (G.Lh >= 2) fastdiv_magic((uint32_t)G.Lh, &G.mag_Lh, &G.sh_Lh);
         fastdiv_magic((uint32_t)G.L, &G.mag_L, &G.sh_L);
@@ -418,9 +445,9 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
         G.n_half = (int64_t)sizes[i] * sizes[i] * (two_d ? 1 : sizes[i]) / 2;
         G.stride = ((G.n_half * elem + 255) / 256) * 256 / elem;
         G.off = off;
-        if (ising && G.fast && o.ising_spt == 0 && sizes[i] >= 64) G.spt = 2;  // 16 sites per thread
-        if (ising && G.fast && o.ising_spt > 0) G.spt = o.ising_spt;
-        G.tiles = (int32_t)((G.n_half + tile * G.spt - 1) / (tile * G.spt));
+        if (ising && G.fast && o.ising_spt > 0) G.spt = o.ising_spt;  // thread steps per block tile
+        G.tsz = (int32_t)((ising && G.fast == 3 ? 2 : 1) * tile * G.spt);
+        G.tiles = (int32_t)((G.n_half + G.tsz - 1) / G.tsz);
         if (G.tiles >= 2) fastdiv_magic((uint32_t)G.tiles, &G.mag_T, &G.sh_T);
         G.blk0 = blk;
         if ((int64_t)blk + (int64_t)G.tiles * units > INT32_MAX / 2) return fail(ISING_E_ARG, "ensemble too large");
@@ -438,7 +465,8 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     if (ising || msc) HIP_TRY(hipMalloc(&p->d_pool, p->pool_bytes));
     else HIP_TRY(hipMalloc(&p->d_fpool, p->pool_bytes));
     if (msc) HIP_TRY(hipMalloc(&p->d_mthr, sizeof(uint32_t) * 3 * p->total_reps));
-    HIP_TRY(hipMalloc(&p->d_thr, sizeof(uint32_t) * 28 * p->total_reps));
+    if (msc) HIP_TRY(hipMalloc(&p->d_mplanes, sizeof(uint32_t) * 96 * p->total_reps));
+    HIP_TRY(hipMalloc(&p->d_thr, sizeof(uint32_t) * kIsingTab * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_r2t, sizeof(int32_t) * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_t2r, sizeof(int32_t) * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_betas, sizeof(double) * p->total_reps));
@@ -455,7 +483,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
         HIP_TRY(hipMalloc(&p->d_cnt, sizeof(int64_t) * (size_t)p->total_reps));
     }
 
-    std::vector<uint32_t> thr(28 * (size_t)p->total_reps);
+    std::vector<uint32_t> thr(kIsingTab * (size_t)p->total_reps);
     std::vector<int32_t> ident(p->total_reps);
     std::vector<double> betas(p->total_reps);
     std::vector<float> betas_f(p->total_reps), deltas(p->total_reps);
@@ -467,7 +495,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
             betas_f[g * R + i] = (float)beta;
             deltas[g * R + i] = (float)o.delta;
             ident[g * R + i] = i;
-            ising_table(beta, o.j, o.h, &thr[28 * (size_t)(g * R + i)], p->dims);
+            ising_table(beta, o.j, o.h, &thr[kIsingTab * (size_t)(g * R + i)], p->dims);
             for (int k = 0; k < 3; ++k)  // multispin coding: T(a) for a = 4, 5, 6 aligned neighbours
                 mthr[3 * (size_t)(g * R + i) + k] = (uint32_t)std::ldexp(std::exp(-beta * o.j * (4.0 * (4 + k) - 12.0)), 32);
         }
@@ -480,30 +508,41 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     ctl.renorm_every = (model == ISING_MODEL_XY || model == ISING_MODEL_HEISENBERG) ? o.renorm_every : 0;
     ctl.adapt_every = (model == ISING_MODEL_XY || model == ISING_MODEL_HEISENBERG) ? o.adapt_every : 0;
     set_flags(&ctl, 0);
-    HIP_TRY(hipMemcpy(p->d_groups, p->groups.data(), sizeof(DevGroup) * p->groups.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(p->d_thr, thr.data(), sizeof(uint32_t) * thr.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(p->d_r2t, ident.data(), sizeof(int32_t) * ident.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(p->d_t2r, ident.data(), sizeof(int32_t) * ident.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(p->d_betas, betas.data(), sizeof(double) * betas.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(p->d_ctl, &ctl, sizeof(DevCtl), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(p->d_acc, 0, sizeof(int64_t) * 2 * kMscAccCopies * p->total_reps + 8 * sizeof(int64_t)));
+    ST_COPY(p, p->d_groups, p->groups.data(), sizeof(DevGroup) * p->groups.size(), hipMemcpyHostToDevice);
+    ST_COPY(p, p->d_thr, thr.data(), sizeof(uint32_t) * thr.size(), hipMemcpyHostToDevice);
+    ST_COPY(p, p->d_r2t, ident.data(), sizeof(int32_t) * ident.size(), hipMemcpyHostToDevice);
+    ST_COPY(p, p->d_t2r, ident.data(), sizeof(int32_t) * ident.size(), hipMemcpyHostToDevice);
+    ST_COPY(p, p->d_betas, betas.data(), sizeof(double) * betas.size(), hipMemcpyHostToDevice);
+    ST_COPY(p, p->d_ctl, &ctl, sizeof(DevCtl), hipMemcpyHostToDevice);
+    ST_FILL(p, p->d_acc, 0, sizeof(int64_t) * 2 * kMscAccCopies * p->total_reps + 8 * sizeof(int64_t));
     if (msc) {
-        HIP_TRY(hipMemcpy(p->d_mthr, mthr.data(), sizeof(uint32_t) * mthr.size(), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemset(p->d_pool, 0, p->pool_bytes));
+        ST_COPY(p, p->d_mthr, mthr.data(), sizeof(uint32_t) * mthr.size(), hipMemcpyHostToDevice);
+        // threshold bit planes per 32-replica word under the identity permutation
+        std::vector<uint32_t> planes(96 * (size_t)p->total_reps, 0u);
+        for (int g = 0; g < n_sizes; ++g)
+            for (int wi = 0; 32 * wi < R; ++wi)
+                for (int kj = 0; kj < 96; ++kj) {
+                    uint32_t v = 0;
+                    for (int bit = 0; bit < std::min(32, R - 32 * wi); ++bit)
+                        v |= ((mthr[3 * (size_t)(g * R + 32 * wi + bit) + (kj >> 5)] >> (31 - (kj & 31))) & 1u) << bit;
+                    planes[(size_t)(g * R + 32 * wi) * 96 + kj] = v;
+                }
+        ST_COPY(p, p->d_mplanes, planes.data(), sizeof(uint32_t) * planes.size(), hipMemcpyHostToDevice);
+        ST_FILL(p, p->d_pool, 0, p->pool_bytes);
         HIP_TRY(launch_msc_init(p->d_groups, (int)p->groups.size(), (uint32_t*)p->d_pool, p->nblocks, (uint32_t)seed,
                                 (uint32_t)(seed >> 32), p->st));
     } else if (ising) {
-        HIP_TRY(hipMemset(p->d_pool, 0, p->pool_bytes));
+        ST_FILL(p, p->d_pool, 0, p->pool_bytes);
         HIP_TRY(launch_ising_init(p->d_groups, (int)p->groups.size(), p->d_pool, p->nblocks, (uint32_t)seed,
                                   (uint32_t)(seed >> 32), p->st));
     } else {
-        HIP_TRY(hipMemcpy(p->d_deltas, deltas.data(), sizeof(float) * deltas.size(), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(p->d_betas_f, betas_f.data(), sizeof(float) * betas_f.size(), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemset(p->d_fpool, 0, p->pool_bytes));
-        HIP_TRY(hipMemset(p->d_part0, 0, sizeof(double) * 4 * (size_t)p->nblocks));
-        HIP_TRY(hipMemset(p->d_part1, 0, sizeof(double) * 4 * (size_t)p->nblocks));
-        HIP_TRY(hipMemset(p->d_dacc, 0, sizeof(double) * 4 * (size_t)p->total_reps));
-        HIP_TRY(hipMemset(p->d_cnt, 0, sizeof(int64_t) * (size_t)p->total_reps));
+        ST_COPY(p, p->d_deltas, deltas.data(), sizeof(float) * deltas.size(), hipMemcpyHostToDevice);
+        ST_COPY(p, p->d_betas_f, betas_f.data(), sizeof(float) * betas_f.size(), hipMemcpyHostToDevice);
+        ST_FILL(p, p->d_fpool, 0, p->pool_bytes);
+        ST_FILL(p, p->d_part0, 0, sizeof(double) * 4 * (size_t)p->nblocks);
+        ST_FILL(p, p->d_part1, 0, sizeof(double) * 4 * (size_t)p->nblocks);
+        ST_FILL(p, p->d_dacc, 0, sizeof(double) * 4 * (size_t)p->total_reps);
+        ST_FILL(p, p->d_cnt, 0, sizeof(int64_t) * (size_t)p->total_reps);
         HIP_TRY(launch_cont_init(p->d_groups, (int)p->groups.size(), p->d_fpool, p->ncomp,
                                  model == ISING_MODEL_XY ? MODEL_XY : MODEL_HEIS, p->nblocks, (uint32_t)seed,
                                  (uint32_t)(seed >> 32), p->st));
@@ -626,12 +665,12 @@ static int drain(ising_pt* p, int64_t max_sweeps, size_t rec_size, void* host, i
     if (c.overflow) return fail(ISING_E_FULL, "measurement records were dropped (device buffer overflow)");
     const int64_t n = std::min<int64_t>(c.n_rec, std::max<int64_t>(max_sweeps, 0));
     const size_t per = rec_size * (size_t)p->total_reps;
-    if (n > 0) HIP_TRY(hipMemcpy(host, p->d_rec, per * n, hipMemcpyDeviceToHost));
+    if (n > 0) ST_COPY(p, host, p->d_rec, per * n, hipMemcpyDeviceToHost);
     const int64_t rest = c.n_rec - n;
     if (rest > 0 && n > 0)
-        HIP_TRY(hipMemcpy(p->d_rec, (char*)p->d_rec + per * n, per * rest, hipMemcpyDeviceToDevice));
+        ST_COPY(p, p->d_rec, (char*)p->d_rec + per * n, per * rest, hipMemcpyDeviceToDevice);
     const int32_t nr = (int32_t)rest;
-    HIP_TRY(hipMemcpy(&p->d_ctl->n_rec, &nr, sizeof(int32_t), hipMemcpyHostToDevice));
+    ST_COPY(p, &p->d_ctl->n_rec, &nr, sizeof(int32_t), hipMemcpyHostToDevice);
     p->pending_host -= n;
     *n_out = n;
     return ISING_OK;
@@ -680,8 +719,8 @@ int ising_pt_get_perm(ising_pt_t* p, int32_t size_index, int32_t* r2t, int32_t* 
     const DevGroup* G = group_of(p, size_index);
     if (!G) return fail(ISING_E_ARG, "size_index out of range");
     SYNC_ALL(p);
-    if (r2t) HIP_TRY(hipMemcpy(r2t, p->d_r2t + G->rep0, sizeof(int32_t) * G->R, hipMemcpyDeviceToHost));
-    if (t2r) HIP_TRY(hipMemcpy(t2r, p->d_t2r + G->rep0, sizeof(int32_t) * G->R, hipMemcpyDeviceToHost));
+    if (r2t) ST_COPY(p, r2t, p->d_r2t + G->rep0, sizeof(int32_t) * G->R, hipMemcpyDeviceToHost);
+    if (t2r) ST_COPY(p, t2r, p->d_t2r + G->rep0, sizeof(int32_t) * G->R, hipMemcpyDeviceToHost);
     return ISING_OK;
 }
 
@@ -702,17 +741,17 @@ static int ising_halves(ising_pt* p, const DevGroup* G, int replica, std::vector
     SYNC_ALL(p);
     if (p->model == ISING_MODEL_ISING) {
         for (int c = 0; c < 2; ++c)
-            HIP_TRY(hipMemcpy(half.data() + c * G->n_half,
+            ST_COPY(p, half.data() + c * G->n_half,
                               p->d_pool + G->off + (2 * (int64_t)replica + c) * G->stride, G->n_half,
-                              hipMemcpyDeviceToHost));
+                              hipMemcpyDeviceToHost);
         return ISING_OK;
     }
     const int wi = replica / 32, bit = replica % 32;
     words->assign(2 * (size_t)G->n_half, 0);
     const uint32_t* base = (const uint32_t*)p->d_pool + G->off;
     for (int c = 0; c < 2; ++c)
-        HIP_TRY(hipMemcpy(words->data() + c * G->n_half, base + (2 * (int64_t)wi + c) * G->stride,
-                          sizeof(uint32_t) * G->n_half, hipMemcpyDeviceToHost));
+        ST_COPY(p, words->data() + c * G->n_half, base + (2 * (int64_t)wi + c) * G->stride,
+                          sizeof(uint32_t) * G->n_half, hipMemcpyDeviceToHost);
     for (size_t i = 0; i < half.size(); ++i) half[i] = ((*words)[i] >> bit) & 1u;
     return ISING_OK;
 }
@@ -755,8 +794,8 @@ int ising_pt_set_spins_i8(ising_pt_t* p, int32_t size_index, int32_t replica, co
     SYNC_ALL(p);
     if (p->model == ISING_MODEL_ISING) {
         for (int c = 0; c < 2; ++c)
-            HIP_TRY(hipMemcpy(p->d_pool + G->off + (2 * (int64_t)replica + c) * G->stride,
-                              half.data() + c * G->n_half, G->n_half, hipMemcpyHostToDevice));
+            ST_COPY(p, p->d_pool + G->off + (2 * (int64_t)replica + c) * G->stride,
+                              half.data() + c * G->n_half, G->n_half, hipMemcpyHostToDevice);
         return ISING_OK;
     }
     std::vector<uint8_t> old;
@@ -767,8 +806,8 @@ int ising_pt_set_spins_i8(ising_pt_t* p, int32_t size_index, int32_t replica, co
     for (size_t i = 0; i < words.size(); ++i) words[i] = (words[i] & ~(1u << bit)) | ((uint32_t)half[i] << bit);
     uint32_t* base = (uint32_t*)p->d_pool + G->off;
     for (int c = 0; c < 2; ++c)
-        HIP_TRY(hipMemcpy(base + (2 * (int64_t)wi + c) * G->stride, words.data() + c * G->n_half,
-                          sizeof(uint32_t) * G->n_half, hipMemcpyHostToDevice));
+        ST_COPY(p, base + (2 * (int64_t)wi + c) * G->stride, words.data() + c * G->n_half,
+                          sizeof(uint32_t) * G->n_half, hipMemcpyHostToDevice);
     return ISING_OK;
 }
 
@@ -782,9 +821,9 @@ int ising_pt_get_spins_f32(ising_pt_t* p, int32_t size_index, int32_t replica, f
     std::vector<float> half(2 * (size_t)nc * G->n_half);
     for (int c = 0; c < 2; ++c)
         for (int k = 0; k < nc; ++k)
-            HIP_TRY(hipMemcpy(half.data() + (size_t)(c * nc + k) * G->n_half,
+            ST_COPY(p, half.data() + (size_t)(c * nc + k) * G->n_half,
                               p->d_fpool + G->off + ((2 * (int64_t)replica + c) * nc + k) * G->stride,
-                              sizeof(float) * G->n_half, hipMemcpyDeviceToHost));
+                              sizeof(float) * G->n_half, hipMemcpyDeviceToHost);
     const int L = G->L, Lh = G->Lh;
     for (int z = 0; z < L; ++z)
         for (int y = 0; y < L; ++y)
@@ -820,9 +859,9 @@ int ising_pt_set_spins_f32(ising_pt_t* p, int32_t size_index, int32_t replica, c
     SYNC_ALL(p);
     for (int c = 0; c < 2; ++c)
         for (int k = 0; k < nc; ++k)
-            HIP_TRY(hipMemcpy(p->d_fpool + G->off + ((2 * (int64_t)replica + c) * nc + k) * G->stride,
+            ST_COPY(p, p->d_fpool + G->off + ((2 * (int64_t)replica + c) * nc + k) * G->stride,
                               half.data() + (size_t)(c * nc + k) * G->n_half, sizeof(float) * G->n_half,
-                              hipMemcpyHostToDevice));
+                              hipMemcpyHostToDevice);
     return ISING_OK;
 }
 
@@ -832,7 +871,7 @@ int ising_pt_get_deltas(ising_pt_t* p, int32_t size_index, float* deltas) {
     const DevGroup* G = group_of(p, size_index);
     if (!G) return fail(ISING_E_ARG, "size_index out of range");
     SYNC_ALL(p);
-    HIP_TRY(hipMemcpy(deltas, p->d_deltas + G->rep0, sizeof(float) * G->R, hipMemcpyDeviceToHost));
+    ST_COPY(p, deltas, p->d_deltas + G->rep0, sizeof(float) * G->R, hipMemcpyDeviceToHost);
     return ISING_OK;
 }
 
@@ -890,9 +929,9 @@ int ising_pt_learn_init(ising_pt_t* p, int32_t hidden, uint64_t seed) {
     for (int i = 0; i < H * H; ++i) w[o++] = (float)(gauss() / std::sqrt((double)H));
     o += H;
     for (int i = 0; i < 3 * H; ++i) w[o++] = (float)(0.1 * gauss() / std::sqrt((double)H));
-    HIP_TRY(hipMemcpy(p->d_lmaster, w.data(), sizeof(float) * P, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(p->d_lm1, 0, sizeof(float) * P));
-    HIP_TRY(hipMemset(p->d_lm2, 0, sizeof(float) * P));
+    ST_COPY(p, p->d_lmaster, w.data(), sizeof(float) * P, hipMemcpyHostToDevice);
+    ST_FILL(p, p->d_lm1, 0, sizeof(float) * P);
+    ST_FILL(p, p->d_lm2, 0, sizeof(float) * P);
     HIP_TRY(launch_learn_pack(p->d_lmaster, p->d_lnet, P, p->st));
     SYNC_ALL(p);
     p->drop_graph();  // captured launches carry the old network pointer
@@ -912,7 +951,7 @@ int ising_pt_learn_get_params(ising_pt_t* p, float* out, int64_t n) {
     if (rc) return rc;
     if (!out || n != p->learn_P) return fail(ISING_E_ARG, "need exactly ising_pt_learn_size() floats");
     SYNC_ALL(p);
-    HIP_TRY(hipMemcpy(out, p->d_lmaster, sizeof(float) * n, hipMemcpyDeviceToHost));
+    ST_COPY(p, out, p->d_lmaster, sizeof(float) * n, hipMemcpyDeviceToHost);
     return ISING_OK;
 }
 
@@ -923,7 +962,7 @@ int ising_pt_learn_set_params(ising_pt_t* p, const float* in, int64_t n) {
     for (int64_t i = 0; i < n; ++i)
         if (!std::isfinite(in[i])) return fail(ISING_E_ARG, "parameters must be finite");
     SYNC_ALL(p);
-    HIP_TRY(hipMemcpy(p->d_lmaster, in, sizeof(float) * n, hipMemcpyHostToDevice));
+    ST_COPY(p, p->d_lmaster, in, sizeof(float) * n, hipMemcpyHostToDevice);
     HIP_TRY(launch_learn_pack(p->d_lmaster, p->d_lnet, (int)n, p->st));
     SYNC_ALL(p);
     return ISING_OK;
@@ -963,8 +1002,8 @@ int ising_pt_learn_eval(ising_pt_t* p, int64_t n, const float* x, const float* c
     float *dx = nullptr, *dc = nullptr;
     HIP_TRY(hipMalloc(&dx, sizeof(float) * 6 * n));
     HIP_TRY(hipMalloc(&dc, sizeof(float) * n));
-    HIP_TRY(hipMemcpy(dx, x, sizeof(float) * 6 * n, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(dc, c, sizeof(float) * n, hipMemcpyHostToDevice));
+    ST_COPY(p, dx, x, sizeof(float) * 6 * n, hipMemcpyHostToDevice);
+    ST_COPY(p, dc, c, sizeof(float) * n, hipMemcpyHostToDevice);
     const ContArgs a = p->cont_args();
     hipError_t e = launch_learn_grad(a, nblk, 0, 1, dx, dc, n, p->d_lgpart, p->d_llpart, p->st);
     if (e == hipSuccess)
@@ -1610,11 +1649,11 @@ int ising_wham(int32_t K, const int64_t* n_k, const double* betas, const double*
     M(&dOut, (size_t)maxT * (1 + n_obs));
     std::vector<double> f(K, 0.0), res((size_t)maxT * (1 + n_obs));
     int it = 0;
-    if (e == hipSuccess) e = hipMemcpy(dE, energies, sizeof(double) * N, hipMemcpyHostToDevice);
-    if (e == hipSuccess && n_obs) e = hipMemcpy(dO, obs, sizeof(double) * N * n_obs, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dlogN, logN.data(), sizeof(double) * K, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dB, betas, sizeof(double) * K, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dT, betas, sizeof(double) * K, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpyAsync(dE, energies, sizeof(double) * N, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && n_obs) e = hipMemcpyAsync(dO, obs, sizeof(double) * N * n_obs, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dlogN, logN.data(), sizeof(double) * K, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dB, betas, sizeof(double) * K, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dT, betas, sizeof(double) * K, hipMemcpyHostToDevice, st);
     // free-energy iteration (reweighting.py:77-99)
     while (e == hipSuccess && it < max_iter) {
         ++it;

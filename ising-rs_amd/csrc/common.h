@@ -121,8 +121,8 @@ struct DevGroup {
     uint32_t mag_Lh, sh_Lh;  // n / Lh == fastdiv(n, mag_Lh, sh_Lh) for n < 2^32 (Lh >= 2)
     uint32_t mag_L, sh_L;    // n / L  likewise
     uint32_t mag_T, sh_T;    // n / tiles likewise (tiles >= 2)
-    int32_t spt;             // Ising fast path: 8-site groups per thread (1 or 2)
-    int32_t pad1;
+    int32_t spt;             // Ising: thread steps per block tile
+    int32_t tsz;             // Ising: half-sites per block tile (256 threads x step width x spt)
 };
 
 // Replica (or replica word) and tile of a block's local index.

@@ -6,12 +6,20 @@
 // Storage: one byte per site, 0 = down, 1 = up, in colour half-lattices.  A
 // half-sweep of colour c reads c and 1-c and writes c: 3 bytes per update.
 //
-// Acceptance: accept iff dE <= 0 or u < exp(-beta dE) with u a 32-bit uniform
-// compared against a host-built 33-bit threshold T(k), k = b*7 + n_up.  The
-// high 16 bits of u come from one Philox call shared by 8 consecutive sites
-// (counter c0 = h >> 3); only when they equal T's high half (p = 2^-16) is a
-// second Philox call (keyed by the site) drawn for the low 16 bits.  This is
-// exactly a 32-bit comparison, at 1/8 of a Philox call per update.
+// Acceptance: accept iff u < T(k), k = b*7 + n_up, with u a 32-bit uniform
+// and T = floor(2^32 min(1, exp(-beta dE))) a host-built 33-bit threshold
+// split as T = hi * 2^25 + lo (hi in 0..128).  The top 7 bits r7 of u are the
+// low 7 bits of the site's byte of one Philox call shared by 16 consecutive
+// sites (counter c0 = h >> 4, site h takes byte h & 15); only when r7 == hi
+// (p = 2^-7) is a second Philox call, keyed by the site, drawn for the other
+// 25 bits.  This is exactly the 32-bit comparison, at 1/16 of a Philox call
+// per update.  A wave queues its tied sites in LDS and decides them 64 at a
+// time, so the second call costs one lane per tie, not a whole wave.
+//
+// The 16 sites of a chunk are handled four per 32-bit word: neighbour counts
+// by byte-wise adds, the threshold byte of each site by two byte permutes of
+// the packed table (one per spin value), and r7 < hi by one subtraction of
+// (r | 0x80) - hi whose per-byte sign bit is the comparison.
 //
 // Measurement is fused: the colour-0 pass sums its new spins, the colour-1 pass
 // sums its new spins and s_i * (sum of its 6 neighbours), which counts every
@@ -24,8 +32,9 @@
 namespace ising {
 
 constexpr int kThreads = 256;
-constexpr int kSitesPerThread = 8;
-constexpr int kTile = kThreads * kSitesPerThread;  // half-sites per block
+constexpr int kSitesPerThread = 16;                // sites per Philox call (and per generic thread step)
+constexpr int kTieCap = 128;                       // tied thread steps queued per wave
+static_assert(kThreads * kSitesPerThread == kIsingTile, "host tiling must match the kernel");
 
 __device__ __forceinline__ int find_group(const DevGroup* groups, int ngroups, int b) {
     int g = 0;
@@ -33,12 +42,13 @@ __device__ __forceinline__ int find_group(const DevGroup* groups, int ngroups, i
     return g;
 }
 
+// The 25 low bits of a tied site's uniform: top 25 bits of its own Philox call.
 __device__ __forceinline__ uint32_t tie_bits(uint32_t h, uint32_t hs, uint32_t stream, uint32_t k0, uint32_t k1) {
-    return philox(h, hs, stream, tag(P_TIE, MODEL_ISING), k0, k1).x & 0xffffu;
+    return philox(h, hs, stream, tag(P_TIE, MODEL_ISING), k0, k1).x >> 7;
 }
 
-__device__ __forceinline__ uint32_t word_of(const u32x4& w, int j) {
-    return j < 2 ? w.x : (j < 4 ? w.y : (j < 6 ? w.z : w.w));
+__device__ __forceinline__ uint32_t word_of(const u32x4& w, int d) {
+    return d == 0 ? w.x : (d == 1 ? w.y : (d == 2 ? w.z : w.w));
 }
 
 // Accumulated quantity q (0: bond sum, 1: magnetisation) of global replica gr,
@@ -49,6 +59,26 @@ __device__ __forceinline__ int64_t acc_total(const IsingArgs& a, int gr, int q) 
         v += __hip_atomic_load(&a.acc[2 * ((int64_t)c * a.total_reps + gr) + q], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     return v;
+}
+
+// Multispin coding: bit plane j (MSB first) of the class-k threshold of every
+// replica of a 32-replica word, bit r of the plane belonging to replica r.
+// Rebuilt after each exchange, since the thresholds follow the temperatures.
+__device__ void msc_planes(const IsingArgs& a, int tid, int nthreads) {
+    for (int g = 0; g < a.ngroups; ++g) {
+        const DevGroup G = a.groups[g];
+        const int nwords = (G.R + 31) / 32;
+        for (int e = tid; e < nwords * 96; e += nthreads) {
+            const int wi = e / 96, kj = e - wi * 96, kk = kj >> 5, j = kj & 31;
+            const int nrep = min(32, G.R - 32 * wi);
+            uint32_t v = 0;
+            for (int bit = 0; bit < nrep; ++bit) {
+                const int ti = a.r2t[G.rep0 + 32 * wi + bit];
+                v |= ((a.mthr[3 * (G.rep0 + ti) + kk] >> (31 - j)) & 1u) << bit;
+            }
+            a.mplanes[(int64_t)(G.rep0 + 32 * wi) * 96 + kj] = v;
+        }
+    }
 }
 
 // Closing step of a sweep (one-block kernel after the colour-1 launch).
@@ -100,6 +130,7 @@ __device__ void close_sweep(const IsingArgs& a, int64_t t, bool meas, bool exch,
         }
     }
     __syncthreads();
+    if (exch && a.mplanes) msc_planes(a, tid, blockDim.x);
     if (need)
         for (int64_t i = tid; i < 2 * (int64_t)a.total_reps * a.acc_copies; i += blockDim.x) a.acc[i] = 0;
     if (tid == 0) {
@@ -120,12 +151,213 @@ __device__ __forceinline__ int xcd_block(int nblocks) {
     return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
 }
 
+// 24-bit multiply (one full-rate instruction); both operands are below 2^24
+// for every lattice the library accepts (L <= 2046, so L^2 < 2^22).
+__device__ __forceinline__ uint32_t mul24(uint32_t x, uint32_t y) { return (x & 0xffffffu) * (y & 0xffffffu); }
+
+// Spin bytes c[] and up-neighbour counts ns[] of the ND words (4 * ND sites)
+// starting at half index hh, all in one row: one ND-word load per neighbour
+// row plus one edge byte for the x-neighbour shifted across words.  Offsets
+// are 32-bit (L^3/2 < 2^32), added to the uniform base pointers.
+template <int ND>
+__device__ __forceinline__ void gather_row(const DevGroup& G, const uint8_t* __restrict__ C,
+                                           const uint8_t* __restrict__ O, uint32_t hh, int color, uint32_t* c,
+                                           uint32_t* ns) {
+    typedef uint32_t vec __attribute__((ext_vector_type(ND)));
+    const uint32_t L = (uint32_t)G.L, Lh = (uint32_t)G.Lh;
+    const uint32_t row = fastdiv(hh, G.mag_Lh, G.sh_Lh);
+    const uint32_t xh0 = hh - mul24(row, Lh);
+    const uint32_t z = fastdiv(row, G.mag_L, G.sh_L), y = row - mul24(z, L);
+    const uint32_t p = (y + z + (uint32_t)color) & 1u;
+    const uint32_t ym = (y == 0 ? L : y) - 1, yp = y + 1 == L ? 0 : y + 1;
+    const uint32_t zm = (z == 0 ? L : z) - 1, zp = z + 1 == L ? 0 : z + 1;
+    const uint32_t zL = mul24(z, L);
+    uint32_t xf = xh0 + 4 * ND;
+    xf = xf == Lh ? 0u : xf;
+    const uint32_t xb = (xh0 == 0 ? Lh : xh0) - 1;
+    const uint32_t xe = p ? xf : xb;
+    const vec vc = *(const vec*)(C + hh);
+    const vec vx = *(const vec*)(O + hh);
+    const vec v1 = *(const vec*)(O + (mul24(zL + ym, Lh) + xh0));
+    const vec v2 = *(const vec*)(O + (mul24(zL + yp, Lh) + xh0));
+    const vec v3 = *(const vec*)(O + (mul24(mul24(zm, L) + y, Lh) + xh0));
+    const vec v4 = *(const vec*)(O + (mul24(mul24(zp, L) + y, Lh) + xh0));
+    const uint32_t e = O[(hh - xh0) + xe];
+    // x-neighbour bytes: shifted down one byte (p = 1, neighbour at xh + 1) or
+    // up one byte (p = 0, neighbour at xh - 1), the edge byte filling the gap;
+    // both shifts are formed and the row parity picks one, without branches
+    const uint32_t pm = 0u - p;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        const uint32_t up = __builtin_amdgcn_alignbyte(d + 1 < ND ? vx[d + 1 < ND ? d + 1 : 0] : e, vx[d], 1u);
+        const uint32_t dn = __builtin_amdgcn_alignbyte(vx[d], d > 0 ? vx[d > 0 ? d - 1 : 0] : e << 24, 3u);
+        const uint32_t xs = (up & pm) | (dn & ~pm);
+        c[d] = vc[d];
+        ns[d] = vx[d] + xs + v1[d] + v2[d] + v3[d] + v4[d];  // per byte 0..6: no carries
+    }
+}
+
+// Decides one tied site of a 3D chunk layout (r7 == hi): recounts its up
+// neighbours, draws the 25 low bits and flips it when u < T.  The chunk's
+// provisional value (no flip) is what C[h] holds.
+__device__ __forceinline__ void resolve_tie(const IsingArgs& a, const DevGroup& G, uint8_t* C, const uint8_t* O,
+                                            const uint32_t* s_thr, uint32_t h, int color, uint32_t hs,
+                                            uint32_t stream, bool need, int* msum, int* bsum) {
+    const uint32_t L = (uint32_t)G.L, Lh = (uint32_t)G.Lh;
+    const uint32_t row = fastdiv(h, G.mag_Lh, G.sh_Lh);
+    const uint32_t xh = h - mul24(row, Lh);
+    const uint32_t z = fastdiv(row, G.mag_L, G.sh_L), y = row - mul24(z, L);
+    const uint32_t p = (y + z + (uint32_t)color) & 1u;
+    const uint32_t ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
+    const uint32_t zm = z == 0 ? L - 1 : z - 1, zp = z == L - 1 ? 0 : z + 1;
+    const uint32_t xo = p ? (xh + 1 == Lh ? 0 : xh + 1) : (xh == 0 ? Lh - 1 : xh - 1);
+    const uint32_t rb = h - xh;
+    const uint32_t n = (uint32_t)O[rb + xh] + O[rb + xo] + O[mul24(mul24(z, L) + ym, Lh) + xh] +
+                       O[mul24(mul24(z, L) + yp, Lh) + xh] + O[mul24(mul24(zm, L) + y, Lh) + xh] +
+                       O[mul24(mul24(zp, L) + y, Lh) + xh];
+    const uint32_t b = C[h];
+    const uint32_t k = b * 7u + n;
+    if (tie_bits(h, hs, stream, a.k0, a.k1) < s_thr[14 + k]) {
+        C[h] = (uint8_t)(b ^ 1u);
+        if (need) {
+            const int ds = b ? -2 : 2;
+            *msum += ds;
+            if (color) *bsum += ds * (2 * (int)n - 6);
+        }
+    }
+}
+
+// Per-wave queue of tied thread steps (LDS): step base and tie mask, bit
+// 8j + 7 - d marking site 4d + j of the step.
+struct TieQueue {
+    uint32_t* qh;
+    uint32_t* qm;
+    int n;  // wave-uniform length
+};
+
+// Decides every queued tie: lane i takes entries i, i + 64, ...
+__device__ __forceinline__ void flush_ties(const IsingArgs& a, const DevGroup& G, uint8_t* C, const uint8_t* O,
+                                           const uint32_t* s_thr, int color, uint32_t hs, uint32_t stream, bool need,
+                                           TieQueue& q, int lane, int* msum, int* bsum) {
+    __threadfence_block();  // the step stores are done before tied bytes are rewritten
+    __builtin_amdgcn_wave_barrier();
+    for (int base = 0; base < q.n; base += 64) {
+        const int e = base + lane;
+        if (e < q.n) {
+            const uint32_t h0 = q.qh[e];
+            uint32_t m = q.qm[e];
+            while (m) {
+                const uint32_t pb = (uint32_t)__builtin_ctz(m);
+                m &= m - 1u;
+                resolve_tie(a, G, C, O, s_thr, h0 + 4u * (7u - (pb & 7u)) + (pb >> 3), color, hs, stream, need, msum,
+                            bsum);
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    q.n = 0;
+}
+
+// Thread steps of one block tile for a 3D lattice with Lh % 8 == 0.
+// MODE 1: 16 sites as two 8-site runs (each in one row); MODE 2: 16 sites in
+// one row; MODE 3: 32 sites in one row (two Philox calls).  Per 4-site word:
+// thresholds by byte permutes of the packed table, r7 < hi by the sign bit of
+// (r | 0x80) - hi, ties (byte exactly 0x80) queued for the wave.
+template <int MODE>
+__device__ __forceinline__ void fast_tile(const IsingArgs& a, const DevGroup& G, int tile, uint8_t* C,
+                                          const uint8_t* O, const uint32_t* s_thr, int color, uint32_t hs,
+                                          uint32_t stream, bool need, TieQueue& q, int lane, int* msum, int* bsum) {
+    constexpr int NW = MODE == 3 ? 8 : 4;  // 32-bit words (4 sites each) per thread step
+    constexpr int W = 4 * NW;
+    const uint32_t T0l = s_thr[28], T0h = s_thr[29], T1l = s_thr[30], T1h = s_thr[31];
+    for (int part = 0; part < G.spt; ++part) {
+        const int64_t h0 = (int64_t)tile * G.tsz + (int64_t)part * (kThreads * W) + (int64_t)threadIdx.x * W;
+        uint32_t tie = 0;
+        if (h0 < G.n_half) {
+            const uint32_t hh = (uint32_t)h0;  // L^3/2 < 2^32 for every accepted L
+            uint32_t c[NW], ns[NW], rw[NW];
+            if (MODE == 1) {
+                gather_row<2>(G, C, O, hh, color, c, ns);
+                gather_row<2>(G, C, O, hh + 8u, color, c + 2, ns + 2);
+            } else {
+                gather_row<NW>(G, C, O, hh, color, c, ns);
+            }
+#pragma unroll
+            for (int k = 0; k < NW / 4; ++k) {
+                const u32x4 w = philox((hh >> 4) + (uint32_t)k, hs, stream, tag(P_ISING, MODEL_ISING), a.k0, a.k1);
+                rw[4 * k] = w.x;
+                rw[4 * k + 1] = w.y;
+                rw[4 * k + 2] = w.z;
+                rw[4 * k + 3] = w.w;
+            }
+            uint32_t nc[NW];
+#pragma unroll
+            for (int d = 0; d < NW; ++d) {
+                const uint32_t m = __builtin_amdgcn_perm(0u, 0x0000ff00u, c[d]);  // 0xff in up-spin bytes
+                const uint32_t t1 = __builtin_amdgcn_perm(T1h, T1l, ns[d]);
+                const uint32_t t0 = __builtin_amdgcn_perm(T0h, T0l, ns[d]);
+                const uint32_t thr = (m & t1) | (~m & t0);
+                // per byte (0x80 + r7) - hi >= 0: sign bit clear iff r7 < hi
+                const uint32_t zz = (rw[d] | 0x80808080u) - thr;
+                nc[d] = c[d] ^ (((zz >> 7) & 0x01010101u) ^ 0x01010101u);
+                // tie iff the byte is exactly 0x80 (r7 == hi)
+                const uint32_t u = (zz & 0x7f7f7f7fu) + 0x7f7f7f7fu;
+                tie |= (zz & ~u & 0x80808080u) >> (d & 7);
+            }
+            if (MODE == 1) {
+                typedef uint32_t vec2 __attribute__((ext_vector_type(2)));
+                vec2 o0 = {nc[0], nc[1]}, o1 = {nc[2], nc[3]};
+                *(vec2*)(C + hh) = o0;
+                *(vec2*)(C + hh + 8u) = o1;
+            } else {
+                typedef uint32_t vecw __attribute__((ext_vector_type(NW)));
+                vecw o;
+#pragma unroll
+                for (int d = 0; d < NW; ++d) o[d] = nc[d];
+                *(vecw*)(C + hh) = o;
+            }
+            if (need) {
+                int sb = 0;
+#pragma unroll
+                for (int d = 0; d < NW; ++d) sb += __builtin_popcount(nc[d]);
+                *msum += 2 * sb - W;
+                if (color) {
+                    // sum_j (2 b'_j - 1)(2 n_j - 6) = 4 sum b'n - 12 sum b' - 2 sum n + 6 W
+                    uint32_t bn = 0, sn = 0;  // byte sums stay below 256 (at most 8 words of 6)
+#pragma unroll
+                    for (int d = 0; d < NW; ++d) {
+                        bn += __builtin_amdgcn_perm(0u, 0x0000ff00u, nc[d]) & ns[d];
+                        sn += ns[d];
+                    }
+                    const int sbn = (int)__builtin_amdgcn_sad_u8(bn, 0u, 0u);
+                    const int snn = (int)__builtin_amdgcn_sad_u8(sn, 0u, 0u);
+                    *bsum += 4 * sbn - 12 * sb - 2 * snn + 6 * W;
+                }
+            }
+        }
+        // queue this lane's tied step
+        const uint64_t bal = __ballot(tie != 0u);
+        if (bal) {
+            if (tie) {
+                const uint32_t pos = (uint32_t)q.n + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                q.qh[pos] = (uint32_t)h0;
+                q.qm[pos] = tie;
+            }
+            q.n += __popcll(bal);
+            if (q.n >= 64) flush_ties(a, G, C, O, s_thr, color, hs, stream, need, q, lane, msum, bsum);
+        }
+    }
+    if (q.n) flush_ties(a, G, C, O, s_thr, color, hs, stream, need, q, lane, msum, bsum);
+}
+
 // One colour half-sweep of every replica of every group in the ensemble.
-// Grid: one block per (group, replica, tile of 2048 half-sites), padded to a
-// multiple of 8 blocks.
+// Grid: one block per (group, replica, tile of G.tsz half-sites), padded to
+// a multiple of 8 blocks.
 __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int color) {
-    __shared__ uint32_t s_thr[28];
+    __shared__ uint32_t s_thr[kIsingTab];
     __shared__ int64_t s_red[2][kThreads / 64];
+    __shared__ uint32_t s_qh[kThreads / 64][kTieCap], s_qm[kThreads / 64][kTieCap];
     const int b = xcd_block(a.nblocks);
     if (b >= a.nblocks) return;
     const int g = find_group(a.groups, a.ngroups, b);
@@ -134,91 +366,35 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
     split_tiles(G, b - G.blk0, &r, &tile);
     const int grep = G.rep0 + r;
     const int tidx = a.r2t[grep];
-    if (threadIdx.x < 28)
-        s_thr[threadIdx.x] = a.inline_thr ? a.thr_inline[threadIdx.x] : a.thr[(int64_t)(G.rep0 + tidx) * 28 + threadIdx.x];
+    if (threadIdx.x < kIsingTab)
+        s_thr[threadIdx.x] =
+            a.inline_thr ? a.thr_inline[threadIdx.x] : a.thr[(int64_t)(G.rep0 + tidx) * kIsingTab + threadIdx.x];
     const int64_t t = a.ctl->t;
     const bool need = (a.ctl->flags & (F_MEAS | F_EXCH)) != 0;
     __syncthreads();
 
     const uint32_t hs = (uint32_t)(2 * t + color);
     const uint32_t stream = stream_id((uint32_t)G.L, (uint32_t)r);
-    const int L = G.L, Lh = G.Lh;
     uint8_t* C = a.pool + G.off + (int64_t)(2 * r + color) * G.stride;
     const uint8_t* O = a.pool + G.off + (int64_t)(2 * r + (color ^ 1)) * G.stride;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    TieQueue q{s_qh[wid], s_qm[wid], 0};
 
     int bsum = 0, msum = 0;
-    for (int part = 0; part < G.spt; ++part) {
-    const int64_t h0 = ((int64_t)tile * G.spt + part) * kTile + (int64_t)threadIdx.x * kSitesPerThread;
-    if (h0 < G.n_half) {
-        const u32x4 w = philox((uint32_t)(h0 >> 3), hs, stream, tag(P_ISING, MODEL_ISING), a.k0, a.k1);
-        if (G.fast) {
-            // 8 sites in one row (Lh % 8 == 0): six 8-byte neighbour loads and one
-            // edge byte for the shifted x-neighbour.  All index math in 32 bits
-            // (L^3/2 < 2^32 for every L the library accepts).
-            const uint32_t hh = (uint32_t)h0;
-            const uint32_t row = fastdiv(hh, G.mag_Lh, G.sh_Lh);
-            const uint32_t xh0 = hh - row * (uint32_t)Lh;
-            const uint32_t z = fastdiv(row, G.mag_L, G.sh_L), y = row - z * (uint32_t)L;
-            const uint32_t p = (y + z + (uint32_t)color) & 1u;
-            const uint32_t ym = y == 0 ? L - 1 : y - 1, yp = y == (uint32_t)L - 1 ? 0 : y + 1;
-            const uint32_t zm = z == 0 ? L - 1 : z - 1, zp = z == (uint32_t)L - 1 ? 0 : z + 1;
-            const uint32_t rb = row * (uint32_t)Lh;
-            const uint32_t xe = p ? (xh0 + 8 == (uint32_t)Lh ? 0u : xh0 + 8) : (xh0 == 0 ? (uint32_t)Lh - 1 : xh0 - 1);
-            const uint64_t c8 = *(const uint64_t*)(C + hh);
-            const uint64_t xr = *(const uint64_t*)(O + hh);
-            const uint64_t e = O[rb + xe];
-            const uint64_t nym = *(const uint64_t*)(O + (z * (uint32_t)L + ym) * (uint32_t)Lh + xh0);
-            const uint64_t nyp = *(const uint64_t*)(O + (z * (uint32_t)L + yp) * (uint32_t)Lh + xh0);
-            const uint64_t nzm = *(const uint64_t*)(O + (zm * (uint32_t)L + y) * (uint32_t)Lh + xh0);
-            const uint64_t nzp = *(const uint64_t*)(O + (zp * (uint32_t)L + y) * (uint32_t)Lh + xh0);
-            const uint64_t xs = p ? ((xr >> 8) | (e << 56)) : ((xr << 8) | e);
-            const uint64_t ns = xr + xs + nym + nyp + nzm + nzp;  // per-byte up-neighbour counts 0..6
-            // per-byte byte offset of the table entry k = 7 b + n (at most 52: no carries)
-            const uint64_t K4 = (c8 * 7ull + ns) << 2;
-            uint32_t flo = 0, fhi = 0;
-            bool anytie = false;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uint32_t kk = (uint32_t)(K4 >> (8 * j)) & 0xffu;
-                const uint32_t thr = *(const uint32_t*)((const char*)s_thr + kk);
-                const uint32_t r16 = (word_of(w, j) >> (16 * (j & 1))) & 0xffffu;
-                const uint32_t acc = (r16 - thr) >> 31;  // r16 < thr (both below 2^17)
-                if (j < 4) flo |= acc << (8 * j);
-                else fhi |= acc << (8 * (j - 4));
-                anytie |= (r16 == thr);
-            }
-            if (anytie) {
-                // rare (p = 2^-16 per site): decide tied sites with 16 more bits
-#pragma unroll 1
-                for (int j = 0; j < 8; ++j) {
-                    const uint32_t kk = (uint32_t)(K4 >> (8 * j)) & 0xffu;
-                    const uint32_t thr = *(const uint32_t*)((const char*)s_thr + kk);
-                    const uint32_t r16 = (word_of(w, j) >> (16 * (j & 1))) & 0xffffu;
-                    if (r16 == thr) {
-                        const uint32_t lo = *(const uint32_t*)((const char*)s_thr + 56 + kk);
-                        const uint32_t acc = tie_bits(hh + (uint32_t)j, hs, stream, a.k0, a.k1) < lo ? 1u : 0u;
-                        if (j < 4) flo = (flo & ~(1u << (8 * j))) | (acc << (8 * j));
-                        else fhi = (fhi & ~(1u << (8 * (j - 4)))) | (acc << (8 * (j - 4)));
-                    }
-                }
-            }
-            const uint64_t nc8 = c8 ^ (flo | ((uint64_t)fhi << 32));
-            *(uint64_t*)(C + hh) = nc8;
-            if (need) {
-                const int sb = __builtin_popcountll(nc8);
-                msum += 2 * sb - 8;
-                if (color) {
-                    // sum_j (2 b'_j - 1)(2 n_j - 6) = 4 sum b'n - 12 sum b' - 2 sum n + 48
-                    const uint64_t bn = (nc8 * 0xffull) & ns;
-                    const int sbn = (int)((bn * 0x0101010101010101ull) >> 56);
-                    const int sn = (int)((ns * 0x0101010101010101ull) >> 56);
-                    bsum += 4 * sbn - 12 * sb - 2 * sn + 48;
-                }
-            }
-        } else {
-            // Generic even L: per-site addressing, same random-number assignment.
-#pragma unroll 1
-            for (int j = 0; j < 8; ++j) {
+    if (G.fast == 3) {
+        fast_tile<3>(a, G, tile, C, O, s_thr, color, hs, stream, need, q, lane, &msum, &bsum);
+    } else if (G.fast == 2) {
+        fast_tile<2>(a, G, tile, C, O, s_thr, color, hs, stream, need, q, lane, &msum, &bsum);
+    } else if (G.fast == 1) {
+        fast_tile<1>(a, G, tile, C, O, s_thr, color, hs, stream, need, q, lane, &msum, &bsum);
+    } else {
+        // Per-site addressing (Lh % 8 != 0, and 2D): same random-number assignment.
+        const int64_t h0 = (int64_t)tile * G.tsz + (int64_t)threadIdx.x * kSitesPerThread;
+        if (h0 < G.n_half) {
+            const int L = G.L, Lh = G.Lh;
+            const u32x4 w = philox((uint32_t)(h0 >> 4), hs, stream, tag(P_ISING, MODEL_ISING), a.k0, a.k1);
+#pragma unroll 4
+            for (int j = 0; j < kSitesPerThread; ++j) {
                 const int64_t h = h0 + j;
                 if (h >= G.n_half) break;
                 const int64_t row = h / Lh;
@@ -231,13 +407,13 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
                 uint32_t n = (uint32_t)O[row * Lh + xh] + O[row * Lh + xo] + O[((int64_t)z * L + ym) * Lh + xh] +
                              O[((int64_t)z * L + yp) * Lh + xh];
                 if (G.dims == 3) n += (uint32_t)O[((int64_t)zm * L + y) * Lh + xh] + O[((int64_t)zp * L + y) * Lh + xh];
-                const uint32_t b = C[h] & 1u;
-                const uint32_t k = b * (uint32_t)(2 * G.dims + 1) + n;  // table row: b * (z + 1) + n_up
+                const uint32_t bb = C[h] & 1u;
+                const uint32_t k = bb * (uint32_t)(2 * G.dims + 1) + n;  // table row: b * (z + 1) + n_up
                 const uint32_t thi = s_thr[k];
-                const uint32_t r16 = (word_of(w, j) >> (16 * (j & 1))) & 0xffffu;
-                bool acc = r16 < thi;
-                if (r16 == thi) acc = tie_bits((uint32_t)h, hs, stream, a.k0, a.k1) < s_thr[14 + k];
-                const uint32_t nb = b ^ (uint32_t)acc;
+                const uint32_t r7 = (word_of(w, j >> 2) >> (8 * (j & 3))) & 0x7fu;
+                bool acc = r7 < thi;
+                if (r7 == thi) acc = tie_bits((uint32_t)h, hs, stream, a.k0, a.k1) < s_thr[14 + k];
+                const uint32_t nb = bb ^ (uint32_t)acc;
                 C[h] = (uint8_t)nb;
                 if (need) {
                     const int sn = 2 * (int)nb - 1;
@@ -247,12 +423,10 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
             }
         }
     }
-    }
 
     // Fused measurement: wave -> block -> one int64 atomic per block and quantity.
     if (need) {
         const int wb = wave_sum(bsum), wm = wave_sum(msum);
-        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
         if (lane == 0) {
             s_red[0][wid] = wb;
             s_red[1][wid] = wm;
@@ -268,7 +442,6 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
             atomicAdd((unsigned long long*)&a.acc[2 * grep + 1], (unsigned long long)M);
         }
     }
-
 }
 
 // Closes a sweep: records, replica exchange, sweep counter.  One block,
@@ -290,8 +463,8 @@ __global__ __launch_bounds__(kThreads) void ising_init_kernel(const DevGroup* gr
     const uint32_t stream = stream_id((uint32_t)G.L, (uint32_t)r);
     for (int c = 0; c < 2; ++c) {
         uint8_t* C = pool + G.off + (int64_t)(2 * r + c) * G.stride;
-        for (int j = 0; j < kSitesPerThread * G.spt; ++j) {
-            const int64_t h = (int64_t)tile * kTile * G.spt + (int64_t)j * kThreads + threadIdx.x;
+        for (int j = 0; j < G.tsz / kThreads; ++j) {
+            const int64_t h = (int64_t)tile * G.tsz + (int64_t)j * kThreads + threadIdx.x;
             if (h >= G.n_half) break;
             const int64_t row = h / G.Lh;
             const int xh = (int)(h - row * G.Lh);
@@ -316,8 +489,8 @@ __global__ __launch_bounds__(kThreads) void ising_measure_kernel(const DevGroup*
     const uint8_t* C1 = pool + G.off + (int64_t)(2 * r + 1) * G.stride;
     const uint8_t* C0 = pool + G.off + (int64_t)(2 * r) * G.stride;
     int bsum = 0, msum = 0;
-    for (int j = 0; j < kSitesPerThread * G.spt; ++j) {
-        const int64_t h = (int64_t)tile * kTile * G.spt + (int64_t)j * kThreads + threadIdx.x;
+    for (int j = 0; j < G.tsz / kThreads; ++j) {
+        const int64_t h = (int64_t)tile * G.tsz + (int64_t)j * kThreads + threadIdx.x;
         if (h >= G.n_half) break;
         const int64_t row = h / Lh;
         const int xh = (int)(h - row * Lh);

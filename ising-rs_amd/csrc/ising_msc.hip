@@ -80,16 +80,7 @@ __global__ __launch_bounds__(kMT) void msc_half_kernel(IsingArgs a, int color) {
     const int nrep = min(32, G.R - 32 * wi);
     const uint32_t valid = nrep >= 32 ? 0xffffffffu : ((1u << nrep) - 1u);
     const int tid = threadIdx.x;
-    if (tid < 96) {
-        const int k = tid >> 5, j = tid & 31;
-        uint32_t v = 0;
-        for (int bit = 0; bit < nrep; ++bit) {
-            const int ti = a.r2t[G.rep0 + 32 * wi + bit];
-            const uint32_t T = a.mthr[3 * (G.rep0 + ti) + k];
-            v |= ((T >> (31 - j)) & 1u) << bit;
-        }
-        s_tp[k][j] = v;
-    }
+    if (tid < 96) s_tp[tid >> 5][tid & 31] = a.mplanes[(int64_t)(G.rep0 + 32 * wi) * 96 + tid];
     if (tid < 64) s_acc[tid >> 1][tid & 1] = 0;
     const int64_t t = a.ctl->t;
     const bool need = (a.ctl->flags & (F_MEAS | F_EXCH)) != 0;

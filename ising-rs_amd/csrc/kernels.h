@@ -20,8 +20,9 @@ struct IsingArgs {
     uint32_t k0, k1;
     double J;
     int32_t inline_thr;           // 1: thresholds come from thr_inline (single-lattice steps)
-    uint32_t thr_inline[28];
+    uint32_t thr_inline[32];
     const uint32_t* mthr;         // multispin coding: [rep0 + tidx][3] thresholds for a = 4, 5, 6
+    uint32_t* mplanes;            // multispin coding: per word, 3 x 32 threshold bit planes of its replicas
     int32_t acc_copies;           // accumulator copies (spread atomics), summed by the closing kernel
 };
 
@@ -93,6 +94,7 @@ hipError_t launch_ising_init(const DevGroup* groups, int ngroups, uint8_t* pool,
                              uint32_t k1, hipStream_t st);
 hipError_t launch_ising_measure(const DevGroup* groups, int ngroups, const uint8_t* pool, int64_t* acc,
                                 int nblocks, hipStream_t st);
-constexpr int kIsingTile = 2048;  // half-sites per block in Ising launches
+constexpr int kIsingTile = 4096;  // half-sites per block step in Ising launches (256 threads x 16)
+constexpr int kIsingTab = 32;     // acceptance-table words per temperature
 
 }  // namespace ising

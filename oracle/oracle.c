@@ -88,13 +88,14 @@ static double u24(uint32_t w) { return (double)(w >> 8) * (1.0 / 16777216.0); }
 /* Ising checkerboard                                                        */
 /* ------------------------------------------------------------------------ */
 static uint32_t thr_from_p(double p, uint32_t* lo) {
-    /* 33-bit threshold T = round-down(p * 2^32), T = 2^32 when p >= 1. */
+    /* 33-bit threshold T = round-down(p * 2^32), T = 2^32 when p >= 1, split
+     * as T = hi * 2^25 + lo with hi in 0..128 and lo < 2^25. */
     uint64_t t;
     if (!(p < 1.0)) t = 1ull << 32;
     else if (p <= 0.0) t = 0;
     else t = (uint64_t)ldexp(p, 32);
-    *lo = (uint32_t)(t & 0xffffu);
-    return (uint32_t)(t >> 16);
+    *lo = (uint32_t)(t & 0x1ffffffu);
+    return (uint32_t)(t >> 25);
 }
 
 static void thresholds_z(int z, double beta, double J, double h, uint32_t* hi, uint32_t* lo) {
@@ -125,15 +126,23 @@ void o_ising_init(int L, uint64_t seed, uint32_t stream, int8_t* spins) {
     }
 }
 
-/* Accept decision for one site: 16-bit draw from the group word, tie broken
- * by a second Philox call keyed by the site's half index. */
-static int ising_accept(uint32_t r16, uint32_t thi, uint32_t tlo, uint32_t h, uint32_t hs, uint32_t stream,
+/* Accept decision for one site, u < T for the 32-bit uniform
+ * u = r7 * 2^25 + u25: r7 is the low 7 bits of the site's byte of the group's
+ * Philox word (16 sites per call, site j of group h >> 4 takes byte j), and
+ * only when r7 equals T's high part is u25 drawn, as the top 25 bits of a
+ * second Philox call keyed by the site's half index. */
+static int ising_accept(uint32_t r7, uint32_t thi, uint32_t tlo, uint32_t h, uint32_t hs, uint32_t stream,
                         int model, uint64_t seed) {
-    if (r16 < thi) return 1;
-    if (r16 > thi) return 0;
+    if (r7 < thi) return 1;
+    if (r7 > thi) return 0;
     uint32_t w[4];
     ph(h, hs, stream, o_tag(O_P_TIE, model), seed, w);
-    return (w[0] & 0xffffu) < tlo;
+    return (w[0] >> 7) < tlo;
+}
+
+static uint32_t site_r7(const uint32_t w[4], uint32_t h) {
+    const uint32_t j = h & 15u;
+    return (w[j >> 2] >> (8 * (j & 3))) & 0x7fu;
 }
 
 void o_ising_halfsweep(int L, int8_t* spins, int color, int64_t t, uint64_t seed, uint32_t stream,
@@ -155,13 +164,12 @@ void o_ising_halfsweep(int L, int8_t* spins, int color, int64_t t, uint64_t seed
                 int b = spins[s] > 0;
                 int k = b * 7 + n_up;
                 uint32_t h = (uint32_t)(s >> 1);
-                uint32_t g = h >> 3, j = h & 7u;
+                uint32_t g = h >> 4;
                 if (g != cur_g) {
                     ph(g, hs, stream, o_tag(O_P_ISING, O_MODEL_ISING), seed, w);
                     cur_g = g;
                 }
-                uint32_t r16 = (w[j >> 1] >> (16 * (j & 1))) & 0xffffu;
-                if (ising_accept(r16, thr_hi[k], thr_lo[k], h, hs, stream, O_MODEL_ISING, seed))
+                if (ising_accept(site_r7(w, h), thr_hi[k], thr_lo[k], h, hs, stream, O_MODEL_ISING, seed))
                     spins[s] = (int8_t)-spins[s];
             }
 }
@@ -197,13 +205,12 @@ void o_ising2d_halfsweep(int L, int8_t* spins, int color, int64_t t, uint64_t se
             int b = spins[s] > 0;
             int k = b * 5 + n_up;
             uint32_t h = (uint32_t)(s >> 1);
-            uint32_t g = h >> 3, j = h & 7u;
+            uint32_t g = h >> 4;
             if (g != cur_g) {
                 ph(g, hs, stream, o_tag(O_P_ISING, O_MODEL_ISING), seed, w);
                 cur_g = g;
             }
-            uint32_t r16 = (w[j >> 1] >> (16 * (j & 1))) & 0xffffu;
-            if (ising_accept(r16, thr_hi[k], thr_lo[k], h, hs, stream, O_MODEL_ISING, seed))
+            if (ising_accept(site_r7(w, h), thr_hi[k], thr_lo[k], h, hs, stream, O_MODEL_ISING, seed))
                 spins[s] = (int8_t)-spins[s];
         }
 }

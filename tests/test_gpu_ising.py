@@ -119,6 +119,36 @@ def test_record_buffer_drain_and_overflow_guard():
     assert np.array_equal(got_b, rb)
 
 
+def test_large_lattice_sweeps_are_bit_exact():
+    """L=128 (16-site chunks in one row) and L=80 (two 8-site runs per chunk):
+    whole-lattice bytes after a few sweeps equal the oracle's, ties included
+    (about 1 site in 128 takes the tie path every half-sweep)."""
+    for L, sweeps in ((128, 3), (80, 4)):
+        seed = 7 + L
+        lat = _lat(L, seed)
+        ref = o.ising_init(L, seed, 0)
+        for t in range(sweeps):
+            lat.step(0.2216, 1.0, 0.0)
+            o.ising_sweep(L, ref, t, seed, float(np.float32(0.2216)))
+        got = lat.get_spins()
+        assert np.array_equal(got, ref), (L, int((got != ref).sum()))
+        assert lat.measure_raw() == o.ising_measure(L, ref)
+
+
+def test_chunks_per_thread_do_not_change_results():
+    temps = o.linspace(4.40, 4.62, 4)
+    out = []
+    for spt in (0, 1, 2, 3, 4):
+        pt = ising_hip.PTEnsemble("ising", [16, 32, 64, 96], 4, temps, 5, exchange_every=3, measure_every=2,
+                                  ising_spt=spt)
+        pt.warmup(5)
+        pt.sample(12)
+        out.append(pt.read_ising() + tuple(pt.get_spins(3, r) for r in range(4)))
+    for other in out[1:]:
+        for x, y in zip(out[0], other):
+            assert np.array_equal(x, y)
+
+
 def test_full_size_config_properties():
     """L=128 x 32 replicas (config 2's largest size): fused records vs standalone reduction."""
     R, L = 32, 128

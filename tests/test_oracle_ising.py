@@ -114,7 +114,8 @@ def test_antiferromagnet_energy_and_high_temperature_disorder():
 def test_thresholds_encode_metropolis_probabilities():
     beta = 1.0 / 4.5115
     hi, lo = o.thresholds(beta)
-    thr = (hi.astype(np.uint64) << 16) | lo.astype(np.uint64)
+    assert np.all(hi <= 128) and np.all(lo < 2 ** 25)
+    thr = (hi.astype(np.uint64) << 25) + lo.astype(np.uint64)
     for b in (0, 1):
         for n_up in range(7):
             s = 1 if b else -1
@@ -148,3 +149,25 @@ def test_sweep_is_deterministic_and_changes_state():
         o.ising_sweep(L, s1, t, 3, 0.5)
         o.ising_sweep(L, s2, t, 3, 0.5)
     assert np.array_equal(s1, s2) and not np.array_equal(s1, start)
+
+
+def test_split_acceptance_is_the_32bit_comparison():
+    """All-up lattice, colour 0: every site has dE = 12 J and the same threshold
+    T = hi * 2^25 + lo with lo / 2^25 = 1/2, so the tie path (r7 == hi, drawn
+    with probability 1/128) carries 1/256 of the acceptance.  The flip
+    frequency must be exp(-12 beta) to within 4 sigma; dropping the tie draw
+    (ties never or always accepted) moves it by about 11 sigma."""
+    L, reps = 16, 1000
+    p = 70.5 / 128
+    beta = -math.log(p) / 12.0
+    hi, lo = o.thresholds(beta)
+    assert hi[13] == 70 and abs(lo[13] / 2 ** 25 - 0.5) < 1e-6
+    st = o.lib.o_stream(L, 0)
+    flips = 0
+    for t in range(reps):
+        sp = np.ones(L ** 3, np.int8)
+        o.lib.o_ising_halfsweep(L, sp, 0, t, 77, st, hi, lo)
+        flips += int((sp < 0).sum())
+    n = reps * L ** 3 // 2
+    sigma = math.sqrt(p * (1 - p) / n)
+    assert abs(flips / n - p) < 4 * sigma, (flips / n, p, sigma)

@@ -52,14 +52,15 @@ def test_u01_range_and_resolution():
     assert o.lib.o_u01(0, 1 << 6) == 2.0 ** -53
 
 
-def test_philox_uniformity_of_16bit_draws():
-    # the 16-bit draws used for Ising acceptance are uniform over one sweep's counters
+def test_philox_uniformity_of_7bit_draws():
+    # the 7-bit draws used for Ising acceptance (low 7 bits of each byte) are
+    # uniform over one sweep's counters
     counts = np.zeros(16, np.int64)
     for g in range(4096):
         w = o.philox([g, 3, 7, 0x200], [42, 0])
         for word in w:
-            counts[(word & 0xFFFF) >> 12] += 1
-            counts[(word >> 16) >> 12] += 1
+            for j in range(4):
+                counts[((int(word) >> (8 * j)) & 0x7F) >> 3] += 1
     exp = counts.sum() / 16
     chi2 = float(((counts - exp) ** 2 / exp).sum())
     assert chi2 < 45.0  # 15 dof, p ~ 1e-4
