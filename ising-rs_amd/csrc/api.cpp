@@ -146,6 +146,7 @@ struct ising_pt {
     double* d_llpart = nullptr;
     int learn_cap_blocks = 0;
     float beta_inline = 0.f, delta_inline = 0.f, j_inline = 1.f;
+    int layout_width = 0, layout_spt = 0;  // Ising thread-step layout in use
     int64_t t_host = 0;        // sweeps enqueued
     int64_t s_host = -1;       // sample sweeps enqueued (-1 before sampling starts)
     int64_t pending_host = 0;  // measured sweeps the device will have written
@@ -378,6 +379,38 @@ int64_t measured_in(int64_t s0, int64_t n, int every) {
     return (s0 + n) / every - s0 / every;
 }
 
+// Ising thread-step layout of every group: 3 = 32 sites in one row
+// (L % 64 == 0), 2 = 16 sites in one row (L % 32 == 0), 1 = two 8-site runs
+// (L % 16 == 0), 0 = 16 sites addressed one by one (other L, and 2D).
+// width (16 or 32; 0 = 16) caps the step where L % 64 == 0; spt (1..4;
+// 0 = 2) is the number of thread steps per block tile.  Results do not
+// depend on either: only the grid changes.
+static int layout_ising(ising_pt* p, int width, int spt) {
+    if (width == 0) width = 16;
+    if (spt == 0) spt = 2;
+    int blk = 0;
+    for (DevGroup& G : p->groups) {
+        G.fast = G.dims == 2 ? 0 : ((G.Lh % 32) == 0 ? 3 : ((G.Lh % 16) == 0 ? 2 : ((G.Lh % 8) == 0 ? 1 : 0)));
+        if (G.fast == 3 && width == 16) G.fast = 2;
+        G.spt = G.fast ? spt : 1;
+        G.tsz = (int32_t)((G.fast == 3 ? 2 : 1) * kIsingTile * G.spt);
+        G.tiles = (int32_t)((G.n_half + G.tsz - 1) / G.tsz);
+        G.mag_T = G.sh_T = 0;
+        if (G.tiles >= 2) fastdiv_magic((uint32_t)G.tiles, &G.mag_T, &G.sh_T);
+        G.blk0 = blk;
+        if ((int64_t)blk + (int64_t)G.tiles * G.R > INT32_MAX / 2) return fail(ISING_E_ARG, "ensemble too large");
+        blk += G.tiles * G.R;
+    }
+    p->nblocks = blk;
+    p->layout_width = width;
+    p->layout_spt = spt;
+    if (p->d_groups) {
+        ST_COPY(p, p->d_groups, p->groups.data(), sizeof(DevGroup) * p->groups.size(), hipMemcpyHostToDevice);
+        p->drop_graph();
+    }
+    return ISING_OK;
+}
+
 int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, int R, const double* temps,
                    uint64_t seed, const ising_pt_opts_t* opts) {
     if (model != ISING_MODEL_ISING && model != ISING_MODEL_XY && model != ISING_MODEL_HEISENBERG &&
@@ -396,6 +429,8 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     if (o.record_capacity < 1) return fail(ISING_E_ARG, "record_capacity must be >= 1");
     if (o.graph_sweeps < 0 || o.graph_sweeps > 100000) return fail(ISING_E_ARG, "graph_sweeps must be in 0..100000");
     if (o.ising_spt < 0 || o.ising_spt > 4) return fail(ISING_E_ARG, "ising_spt must be in 0..4");
+    if (o.ising_width != 0 && o.ising_width != 16 && o.ising_width != 32)
+        return fail(ISING_E_ARG, "ising_width must be 0, 16 or 32");
     if (model == ISING_MODEL_ISING_MSC && !(o.h == 0.0 && o.j > 0.0))
         return fail(ISING_E_ARG, "multispin coding needs h = 0 and J > 0");
     if (model == ISING_MODEL_XY || model == ISING_MODEL_HEISENBERG) {
@@ -434,10 +469,6 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
         G.Lh = sizes[i] / 2;
         G.R = R;
         G.dims = p->dims;
-        // Ising thread-step layout: 3 = 32 sites in one row (L % 64 == 0),
-        // 2 = 16 sites in one row (L % 32 == 0), 1 = two 8-site runs (L % 16
-        // == 0), 0 = 16 sites addressed one by one (other L, and 2D).
-        G.fast = two_d ? 0 : ((G.Lh % 32) == 0 ? 3 : ((G.Lh % 16) == 0 ? 2 : ((G.Lh % 8) == 0 ? 1 : 0)));
         G.spt = 1;
         if (G.Lh >= 2) fastdiv_magic((uint32_t)G.Lh, &G.mag_Lh, &G.sh_Lh);
         fastdiv_magic((uint32_t)G.L, &G.mag_L, &G.sh_L);
@@ -445,8 +476,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
         G.n_half = (int64_t)sizes[i] * sizes[i] * (two_d ? 1 : sizes[i]) / 2;
         G.stride = ((G.n_half * elem + 255) / 256) * 256 / elem;
         G.off = off;
-        if (ising && G.fast && o.ising_spt > 0) G.spt = o.ising_spt;  // thread steps per block tile
-        G.tsz = (int32_t)((ising && G.fast == 3 ? 2 : 1) * tile * G.spt);
+        G.tsz = (int32_t)tile;
         G.tiles = (int32_t)((G.n_half + G.tsz - 1) / G.tsz);
         if (G.tiles >= 2) fastdiv_magic((uint32_t)G.tiles, &G.mag_T, &G.sh_T);
         G.blk0 = blk;
@@ -457,6 +487,10 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     }
     p->nblocks = blk;
     p->pool_bytes = off * elem;
+    if (ising) {
+        const int rc = layout_ising(p, o.ising_width, o.ising_spt);
+        if (rc) return rc;
+    }
 
     HIP_TRY(hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&p->ev0));
@@ -583,6 +617,7 @@ void ising_pt_default_opts(ising_pt_opts_t* o) {
     o->adapt_target = 0.5;
     o->graph_sweeps = 32;
     o->ising_spt = 0;
+    o->ising_width = 0;
 }
 
 int ising_pt_create(ising_pt_t** out, int model, const int32_t* sizes, int32_t n_sizes, int32_t replicas,
@@ -1057,6 +1092,44 @@ int ising_pt_metropolis_pass(ising_pt_t* p, int32_t color, int64_t t) {
     ORDER(p);
     if (p->learned_on) HIP_TRY(launch_cont_learned(p->cont_args(), p->nblocks, color, 0, t, p->st));
     else return fail(ISING_E_STATE, "enable the learned proposal first");
+    return ISING_OK;
+}
+
+int ising_pt_autotune(ising_pt_t* p, int32_t sweeps, int32_t* width, int32_t* spt) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    if (sweeps < 1 || sweeps > 100000) return fail(ISING_E_ARG, "sweeps per candidate must be in 1..100000");
+    if (p->model != ISING_MODEL_ISING && p->model != ISING_MODEL_ISING2D)
+        return fail(ISING_E_STATE, "the thread-step layout exists for int8 Ising ensembles only");
+    if (p->s_host >= 0) return fail(ISING_E_STATE, "autotune runs warm-up sweeps: call it before sampling");
+    if (p->inline_thr) return fail(ISING_E_STATE, "single-lattice handles are not tuned");
+    SYNC_ALL(p);
+    static const int kW[2] = {16, 32}, kS[3] = {1, 2, 4};
+    double best = 1e300;
+    int bw = p->layout_width, bs = p->layout_spt;
+    for (int wi = 0; wi < 2; ++wi)
+        for (int si = 0; si < 3; ++si) {
+            int rc = layout_ising(p, kW[wi], kS[si]);
+            if (rc) return rc;
+            rc = p->enqueue_sweeps(1);  // first launch of a shape is not timed
+            if (rc) return rc;
+            HIP_TRY(hipEventRecord(p->ev0, p->st));
+            rc = p->enqueue_sweeps(sweeps);
+            if (rc) return rc;
+            HIP_TRY(hipEventRecord(p->ev1, p->st));
+            HIP_TRY(hipEventSynchronize(p->ev1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+            if (ms < best) {
+                best = ms;
+                bw = kW[wi];
+                bs = kS[si];
+            }
+        }
+    const int rc = layout_ising(p, bw, bs);
+    if (rc) return rc;
+    p->timed = false;
+    if (width) *width = bw;
+    if (spt) *spt = bs;
     return ISING_OK;
 }
 

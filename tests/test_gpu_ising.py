@@ -135,18 +135,36 @@ def test_large_lattice_sweeps_are_bit_exact():
         assert lat.measure_raw() == o.ising_measure(L, ref)
 
 
-def test_chunks_per_thread_do_not_change_results():
+def test_thread_step_layouts_do_not_change_results():
     temps = o.linspace(4.40, 4.62, 4)
     out = []
-    for spt in (0, 1, 2, 3, 4):
-        pt = ising_hip.PTEnsemble("ising", [16, 32, 64, 96], 4, temps, 5, exchange_every=3, measure_every=2,
-                                  ising_spt=spt)
+    for width, spt in ((0, 0), (16, 1), (32, 1), (32, 2), (16, 4), (32, 3)):
+        pt = ising_hip.PTEnsemble("ising", [16, 32, 64, 96, 128], 4, temps, 5, exchange_every=3, measure_every=2,
+                                  ising_spt=spt, ising_width=width)
         pt.warmup(5)
         pt.sample(12)
-        out.append(pt.read_ising() + tuple(pt.get_spins(3, r) for r in range(4)))
+        out.append(pt.read_ising() + tuple(pt.get_spins(4, r) for r in range(4)))
     for other in out[1:]:
         for x, y in zip(out[0], other):
             assert np.array_equal(x, y)
+
+
+def test_autotune_keeps_the_chain():
+    """The autotuner's timed sweeps are ordinary warm-up sweeps: the tuned
+    ensemble continues exactly like an untuned one warmed up as long."""
+    temps = o.linspace(4.40, 4.62, 4)
+    tuned = ising_hip.PTEnsemble("ising", [32, 64], 4, temps, 9, exchange_every=3, measure_every=2)
+    width, spt = tuned.autotune(3)
+    assert width in (16, 32) and spt in (1, 2, 4)
+    assert tuned.sweeps() == 21
+    plain = ising_hip.PTEnsemble("ising", [32, 64], 4, temps, 9, exchange_every=3, measure_every=2)
+    plain.warmup(21)
+    for p in (tuned, plain):
+        p.sample(10)
+    for x, y in zip(tuned.read_ising(), plain.read_ising()):
+        assert np.array_equal(x, y)
+    with pytest.raises(ising_hip.IsingError):
+        tuned.autotune(1)  # sampling has started
 
 
 def test_full_size_config_properties():
