@@ -194,6 +194,9 @@ def main():
     temps = linspace(tlo, thi, R)
     me = opts["measure_every"]
     pt = ising_hip.PTEnsemble(model, sizes, R, temps, args.seed, record_capacity=args.steps // me + 8, **opts)
+    layout = None
+    if model == "ising":
+        layout = pt.autotune(20)  # warm-up sweeps that also pick the thread-step layout
     pt.warmup(args.warmup)
     pt.sync()
 
@@ -236,6 +239,8 @@ def main():
                      "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "device_updates_per_ns": round(updates / (ev_ms * 1e6), 3)},
     }
+    if layout is not None:
+        line["config"]["layout"] = {"run_width": layout[0], "runs_per_step": layout[1], "steps_per_tile": layout[2]}
     if passes > 1:
         line["config"]["or_updates_per_ns"] = round(updates * (passes - 1) / (wall * 1e9), 3)
         line["config"]["combined_updates_per_ns"] = round(updates * passes / (wall * 1e9), 3)

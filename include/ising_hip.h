@@ -154,13 +154,15 @@ typedef struct {
                                 (0 = launch every kernel directly) */
     int32_t ising_spt;       /* Ising: thread steps per block tile, 1..4 (0 = automatic);
                                 results do not depend on it */
-    int32_t ising_width;     /* Ising: sites per thread step where L % 64 == 0, 16 or 32
+    int32_t ising_width;     /* Ising: run width where L % 64 == 0, 16 or 32 sites
                                 (0 = automatic); results do not depend on it */
+    int32_t ising_runs;      /* Ising: 16-site runs per thread step, 1 or 2 (0 = automatic);
+                                results do not depend on it */
 } ising_pt_opts_t;
 
 /* Defaults: j=1, h=0, exchange_every=10, measure_every=1, delta=0.5,
  * n_overrelax=5, renorm_every=100, record_capacity=4096, adapt_every=0,
- * adapt_target=0.5, graph_sweeps=32, ising_spt=0, ising_width=0. */
+ * adapt_target=0.5, graph_sweeps=32, ising_spt=0, ising_width=0, ising_runs=0. */
 void ising_pt_default_opts(ising_pt_opts_t* opts);
 
 /* model: ISING_MODEL_*.  sizes: n_sizes even L values.  temps: `replicas`
@@ -224,11 +226,12 @@ int ising_pt_mh_stats(ising_pt_t* pt, int64_t* tries, int64_t* accepts);
 int ising_pt_metropolis_pass(ising_pt_t* pt, int32_t color, int64_t t);
 /* Sweeps applied so far (warm-up + sample). */
 /* Autotuner (no counterpart in the reference): times each int8 Ising
- * thread-step layout (width 16/32 x 1/2/4 steps per block tile) over `sweeps`
- * warm-up sweeps of this ensemble and keeps the fastest.  Results do not
- * depend on the layout, so these are ordinary warm-up sweeps (7 * sweeps of
- * them).  Before sampling only.  width/spt may be NULL. */
-int ising_pt_autotune(ising_pt_t* pt, int32_t sweeps, int32_t* width, int32_t* spt);
+ * thread-step layout (16-site runs, one or two per step, or 32-site runs, x
+ * 1/2/4 steps per block tile) over `sweeps` warm-up sweeps of this ensemble
+ * and keeps the fastest.  Results do not depend on the layout, so these are
+ * ordinary warm-up sweeps (9 * (sweeps + 1) of them: each layout's first
+ * sweep is not timed).  Before sampling only.  Outputs may be NULL. */
+int ising_pt_autotune(ising_pt_t* pt, int32_t sweeps, int32_t* width, int32_t* runs, int32_t* spt);
 int ising_pt_sweeps(const ising_pt_t* pt, int64_t* out);
 /* Bytes of spin storage on the device. */
 int ising_pt_device_bytes(const ising_pt_t* pt, int64_t* out);

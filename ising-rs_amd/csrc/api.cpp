@@ -146,7 +146,7 @@ struct ising_pt {
     double* d_llpart = nullptr;
     int learn_cap_blocks = 0;
     float beta_inline = 0.f, delta_inline = 0.f, j_inline = 1.f;
-    int layout_width = 0, layout_spt = 0;  // Ising thread-step layout in use
+    int layout_width = 0, layout_runs = 0, layout_spt = 0;  // Ising thread-step layout in use
     int64_t t_host = 0;        // sweeps enqueued
     int64_t s_host = -1;       // sample sweeps enqueued (-1 before sampling starts)
     int64_t pending_host = 0;  // measured sweeps the device will have written
@@ -210,6 +210,7 @@ struct ising_pt {
         a.mthr = d_mthr;
         a.mplanes = d_mplanes;
         a.acc_copies = model == ISING_MODEL_ISING_MSC ? kMscAccCopies : 1;
+        a.acc_stride = 2 * (int64_t)a.acc_copies * total_reps;
         return a;
     }
 
@@ -282,19 +283,21 @@ struct ising_pt {
 
     int enqueue_sweeps(int64_t n, int n_or = -1) {
         if (model == ISING_MODEL_ISING) {
+            // each colour-0 launch closes the sweep before it; the closing
+            // kernel closes the last one (see sweepctl.h)
             const IsingArgs a = ising_args();
             for (int64_t i = 0; i < n; ++i) {
                 HIP_TRY(launch_ising_half(a, nblocks, 0, st));
                 HIP_TRY(launch_ising_half(a, nblocks, 1, st));
-                HIP_TRY(launch_ising_close(a, st));
             }
+            if (n > 0) HIP_TRY(launch_ising_close(a, st));
         } else if (model == ISING_MODEL_ISING_MSC) {
             const IsingArgs a = ising_args();
             for (int64_t i = 0; i < n; ++i) {
                 HIP_TRY(launch_msc_half(a, nblocks, 0, st));
                 HIP_TRY(launch_msc_half(a, nblocks, 1, st));
-                HIP_TRY(launch_ising_close(a, st));
             }
+            if (n > 0) HIP_TRY(launch_ising_close(a, st));
         } else {
             // overrelaxation passes first, then one Metropolis sweep
             // (the order of ContinuousGpuLattice::sweep, gpu_lattice_continuous.rs:124-168)
@@ -379,21 +382,28 @@ int64_t measured_in(int64_t s0, int64_t n, int every) {
     return (s0 + n) / every - s0 / every;
 }
 
+// Accumulator words: two sweep parities of [copies][replica][2], then 8
+// slots for standalone measurements.
+static int64_t acc_words(int total_reps) { return 2 * 2 * (int64_t)kMscAccCopies * total_reps + 8; }
+
 // Ising thread-step layout of every group: 3 = 32 sites in one row
-// (L % 64 == 0), 2 = 16 sites in one row (L % 32 == 0), 1 = two 8-site runs
-// (L % 16 == 0), 0 = 16 sites addressed one by one (other L, and 2D).
-// width (16 or 32; 0 = 16) caps the step where L % 64 == 0; spt (1..4;
-// 0 = 2) is the number of thread steps per block tile.  Results do not
-// depend on either: only the grid changes.
-static int layout_ising(ising_pt* p, int width, int spt) {
+// (L % 64 == 0), 2 = 16-site runs in one row (L % 32 == 0; `runs` of them per
+// step, a block row apart), 1 = two 8-site runs (L % 16 == 0), 0 = 16 sites
+// addressed one by one (other L, and 2D).  width (16 or 32; 0 = 16) caps the
+// run where L % 64 == 0; runs (1 or 2; 0 = 2) applies to 16-site runs; spt
+// (1..4; 0 = 1) is the number of thread steps per block tile.  Results do not
+// depend on any of them: only the grid changes.
+static int layout_ising(ising_pt* p, int width, int runs, int spt) {
     if (width == 0) width = 16;
-    if (spt == 0) spt = 2;
+    if (runs == 0) runs = 2;
+    if (spt == 0) spt = 1;
     int blk = 0;
     for (DevGroup& G : p->groups) {
         G.fast = G.dims == 2 ? 0 : ((G.Lh % 32) == 0 ? 3 : ((G.Lh % 16) == 0 ? 2 : ((G.Lh % 8) == 0 ? 1 : 0)));
         if (G.fast == 3 && width == 16) G.fast = 2;
+        G.runs = G.fast == 2 ? runs : 1;
         G.spt = G.fast ? spt : 1;
-        G.tsz = (int32_t)((G.fast == 3 ? 2 : 1) * kIsingTile * G.spt);
+        G.tsz = (int32_t)((G.fast == 3 || G.runs == 2 ? 2 : 1) * kIsingTile * G.spt);
         G.tiles = (int32_t)((G.n_half + G.tsz - 1) / G.tsz);
         G.mag_T = G.sh_T = 0;
         if (G.tiles >= 2) fastdiv_magic((uint32_t)G.tiles, &G.mag_T, &G.sh_T);
@@ -403,6 +413,7 @@ static int layout_ising(ising_pt* p, int width, int spt) {
     }
     p->nblocks = blk;
     p->layout_width = width;
+    p->layout_runs = runs;
     p->layout_spt = spt;
     if (p->d_groups) {
         ST_COPY(p, p->d_groups, p->groups.data(), sizeof(DevGroup) * p->groups.size(), hipMemcpyHostToDevice);
@@ -431,6 +442,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     if (o.ising_spt < 0 || o.ising_spt > 4) return fail(ISING_E_ARG, "ising_spt must be in 0..4");
     if (o.ising_width != 0 && o.ising_width != 16 && o.ising_width != 32)
         return fail(ISING_E_ARG, "ising_width must be 0, 16 or 32");
+    if (o.ising_runs < 0 || o.ising_runs > 2) return fail(ISING_E_ARG, "ising_runs must be 0, 1 or 2");
     if (model == ISING_MODEL_ISING_MSC && !(o.h == 0.0 && o.j > 0.0))
         return fail(ISING_E_ARG, "multispin coding needs h = 0 and J > 0");
     if (model == ISING_MODEL_XY || model == ISING_MODEL_HEISENBERG) {
@@ -488,7 +500,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     p->nblocks = blk;
     p->pool_bytes = off * elem;
     if (ising) {
-        const int rc = layout_ising(p, o.ising_width, o.ising_spt);
+        const int rc = layout_ising(p, o.ising_width, o.ising_runs, o.ising_spt);
         if (rc) return rc;
     }
 
@@ -501,11 +513,11 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     if (msc) HIP_TRY(hipMalloc(&p->d_mthr, sizeof(uint32_t) * 3 * p->total_reps));
     if (msc) HIP_TRY(hipMalloc(&p->d_mplanes, sizeof(uint32_t) * 96 * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_thr, sizeof(uint32_t) * kIsingTab * p->total_reps));
-    HIP_TRY(hipMalloc(&p->d_r2t, sizeof(int32_t) * p->total_reps));
-    HIP_TRY(hipMalloc(&p->d_t2r, sizeof(int32_t) * p->total_reps));
+    HIP_TRY(hipMalloc(&p->d_r2t, sizeof(int32_t) * 2 * p->total_reps));  // two permutation parities
+    HIP_TRY(hipMalloc(&p->d_t2r, sizeof(int32_t) * 2 * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_betas, sizeof(double) * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_ctl, sizeof(DevCtl)));
-    HIP_TRY(hipMalloc(&p->d_acc, sizeof(int64_t) * 2 * kMscAccCopies * p->total_reps + 8 * sizeof(int64_t)));
+    HIP_TRY(hipMalloc(&p->d_acc, sizeof(int64_t) * acc_words(p->total_reps)));
     HIP_TRY(hipMalloc(&p->d_rec, ((ising || msc) ? sizeof(IsingRec) : sizeof(ContRec)) *
                                      (size_t)o.record_capacity * p->total_reps));
     if (!ising && !msc) {
@@ -518,7 +530,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     }
 
     std::vector<uint32_t> thr(kIsingTab * (size_t)p->total_reps);
-    std::vector<int32_t> ident(p->total_reps);
+    std::vector<int32_t> ident(2 * (size_t)p->total_reps);
     std::vector<double> betas(p->total_reps);
     std::vector<float> betas_f(p->total_reps), deltas(p->total_reps);
     std::vector<uint32_t> mthr(3 * (size_t)p->total_reps);
@@ -528,7 +540,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
             betas[g * R + i] = beta;
             betas_f[g * R + i] = (float)beta;
             deltas[g * R + i] = (float)o.delta;
-            ident[g * R + i] = i;
+            ident[g * R + i] = ident[p->total_reps + g * R + i] = i;
             ising_table(beta, o.j, o.h, &thr[kIsingTab * (size_t)(g * R + i)], p->dims);
             for (int k = 0; k < 3; ++k)  // multispin coding: T(a) for a = 4, 5, 6 aligned neighbours
                 mthr[3 * (size_t)(g * R + i) + k] = (uint32_t)std::ldexp(std::exp(-beta * o.j * (4.0 * (4 + k) - 12.0)), 32);
@@ -541,6 +553,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     ctl.rec_cap = o.record_capacity;
     ctl.renorm_every = (model == ISING_MODEL_XY || model == ISING_MODEL_HEISENBERG) ? o.renorm_every : 0;
     ctl.adapt_every = (model == ISING_MODEL_XY || model == ISING_MODEL_HEISENBERG) ? o.adapt_every : 0;
+    ctl.closed = -1;
     set_flags(&ctl, 0);
     ST_COPY(p, p->d_groups, p->groups.data(), sizeof(DevGroup) * p->groups.size(), hipMemcpyHostToDevice);
     ST_COPY(p, p->d_thr, thr.data(), sizeof(uint32_t) * thr.size(), hipMemcpyHostToDevice);
@@ -548,7 +561,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     ST_COPY(p, p->d_t2r, ident.data(), sizeof(int32_t) * ident.size(), hipMemcpyHostToDevice);
     ST_COPY(p, p->d_betas, betas.data(), sizeof(double) * betas.size(), hipMemcpyHostToDevice);
     ST_COPY(p, p->d_ctl, &ctl, sizeof(DevCtl), hipMemcpyHostToDevice);
-    ST_FILL(p, p->d_acc, 0, sizeof(int64_t) * 2 * kMscAccCopies * p->total_reps + 8 * sizeof(int64_t));
+    ST_FILL(p, p->d_acc, 0, sizeof(int64_t) * acc_words(p->total_reps));
     if (msc) {
         ST_COPY(p, p->d_mthr, mthr.data(), sizeof(uint32_t) * mthr.size(), hipMemcpyHostToDevice);
         // threshold bit planes per 32-replica word under the identity permutation
@@ -618,6 +631,7 @@ void ising_pt_default_opts(ising_pt_opts_t* o) {
     o->graph_sweeps = 32;
     o->ising_spt = 0;
     o->ising_width = 0;
+    o->ising_runs = 0;
 }
 
 int ising_pt_create(ising_pt_t** out, int model, const int32_t* sizes, int32_t n_sizes, int32_t replicas,
@@ -753,9 +767,14 @@ int ising_pt_get_perm(ising_pt_t* p, int32_t size_index, int32_t* r2t, int32_t* 
     if (!p) return fail(ISING_E_ARG, "null handle");
     const DevGroup* G = group_of(p, size_index);
     if (!G) return fail(ISING_E_ARG, "size_index out of range");
-    SYNC_ALL(p);
-    if (r2t) ST_COPY(p, r2t, p->d_r2t + G->rep0, sizeof(int32_t) * G->R, hipMemcpyDeviceToHost);
-    if (t2r) ST_COPY(p, t2r, p->d_t2r + G->rep0, sizeof(int32_t) * G->R, hipMemcpyDeviceToHost);
+    DevCtl c;
+    const int rc = p->read_ctl(&c);
+    if (rc) return rc;
+    // Ising ensembles keep two permutation buffers and use the one of the
+    // committed exchange count's parity; continuous ensembles update one in place
+    const int64_t off = p->ncomp == 0 ? (c.round & 1) * p->total_reps : 0;
+    if (r2t) ST_COPY(p, r2t, p->d_r2t + off + G->rep0, sizeof(int32_t) * G->R, hipMemcpyDeviceToHost);
+    if (t2r) ST_COPY(p, t2r, p->d_t2r + off + G->rep0, sizeof(int32_t) * G->R, hipMemcpyDeviceToHost);
     return ISING_OK;
 }
 
@@ -1095,7 +1114,7 @@ int ising_pt_metropolis_pass(ising_pt_t* p, int32_t color, int64_t t) {
     return ISING_OK;
 }
 
-int ising_pt_autotune(ising_pt_t* p, int32_t sweeps, int32_t* width, int32_t* spt) {
+int ising_pt_autotune(ising_pt_t* p, int32_t sweeps, int32_t* width, int32_t* runs, int32_t* spt) {
     if (!p) return fail(ISING_E_ARG, "null handle");
     if (sweeps < 1 || sweeps > 100000) return fail(ISING_E_ARG, "sweeps per candidate must be in 1..100000");
     if (p->model != ISING_MODEL_ISING && p->model != ISING_MODEL_ISING2D)
@@ -1103,12 +1122,13 @@ int ising_pt_autotune(ising_pt_t* p, int32_t sweeps, int32_t* width, int32_t* sp
     if (p->s_host >= 0) return fail(ISING_E_STATE, "autotune runs warm-up sweeps: call it before sampling");
     if (p->inline_thr) return fail(ISING_E_STATE, "single-lattice handles are not tuned");
     SYNC_ALL(p);
-    static const int kW[2] = {16, 32}, kS[3] = {1, 2, 4};
+    // (run width, runs per step) x steps per tile
+    static const int kL[3][2] = {{16, 1}, {16, 2}, {32, 1}}, kS[3] = {1, 2, 4};
     double best = 1e300;
-    int bw = p->layout_width, bs = p->layout_spt;
-    for (int wi = 0; wi < 2; ++wi)
+    int bw = p->layout_width, br = p->layout_runs, bs = p->layout_spt;
+    for (int li = 0; li < 3; ++li)
         for (int si = 0; si < 3; ++si) {
-            int rc = layout_ising(p, kW[wi], kS[si]);
+            int rc = layout_ising(p, kL[li][0], kL[li][1], kS[si]);
             if (rc) return rc;
             rc = p->enqueue_sweeps(1);  // first launch of a shape is not timed
             if (rc) return rc;
@@ -1121,14 +1141,16 @@ int ising_pt_autotune(ising_pt_t* p, int32_t sweeps, int32_t* width, int32_t* sp
             HIP_TRY(hipEventElapsedTime(&ms, p->ev0, p->ev1));
             if (ms < best) {
                 best = ms;
-                bw = kW[wi];
+                bw = kL[li][0];
+                br = kL[li][1];
                 bs = kS[si];
             }
         }
-    const int rc = layout_ising(p, bw, bs);
+    const int rc = layout_ising(p, bw, br, bs);
     if (rc) return rc;
     p->timed = false;
     if (width) *width = bw;
+    if (runs) *runs = br;
     if (spt) *spt = bs;
     return ISING_OK;
 }
@@ -1210,7 +1232,7 @@ int ising_lattice_set_spins(ising_lattice_t* l, const int8_t* in) {
 int ising_lattice_measure_raw(ising_lattice_t* l, int64_t* bond, int64_t* mag) {
     if (!l) return fail(ISING_E_ARG, "null handle");
     ising_pt* p = l->pt;
-    int64_t* acc = p->d_acc + 4;  // slots 4,5 of the accumulator block: standalone measurements
+    int64_t* acc = p->d_acc + acc_words(p->total_reps) - 8;  // standalone-measurement slots
     HIP_TRY(hipMemsetAsync(acc, 0, 2 * sizeof(int64_t), p->st));
     HIP_TRY(launch_ising_measure(p->d_groups, 1, p->d_pool, acc - 0, p->nblocks, p->st));
     int64_t v[2];

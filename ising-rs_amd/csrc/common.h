@@ -123,6 +123,8 @@ struct DevGroup {
     uint32_t mag_T, sh_T;    // n / tiles likewise (tiles >= 2)
     int32_t spt;             // Ising: thread steps per block tile
     int32_t tsz;             // Ising: half-sites per block tile (256 threads x step width x spt)
+    int32_t runs;            // Ising: 16-site runs per thread step (1 or 2, a block row apart)
+    int32_t pad2;
 };
 
 // Replica (or replica word) and tile of a block's local index.
@@ -155,6 +157,12 @@ struct DevCtl {
     int64_t s;             // t - sample_start (negative while warming up)
     int32_t ph_me, ph_ee;  // (s + 1) % measure_every, (s + 1) % exchange_every
     int32_t ph_rn, ph_ad;  // (t + 1) % renorm_every, (t + 1) % adapt_every
+    // Ising sweeps close in the next colour-0 launch (see sweepctl.h)
+    int64_t t1;            // sweep index seen by colour-1 launches
+    int64_t round1;        // exchanges committed, as seen by colour-1 launches
+    int64_t closed;        // last sweep closed by a closing kernel (-1: none)
+    int32_t fl1;           // flags seen by colour-1 launches
+    int32_t flp;           // flags of the previous sweep
 };
 
 enum : int32_t { F_MEAS = 1, F_EXCH = 2, F_RENORM = 4, F_ADAPT = 8, F_WARM = 16 };
@@ -194,6 +202,19 @@ __device__ __forceinline__ int32_t phase_step(int32_t ph, int32_t period) {
 // Advance the phases from sweep t to t + 1 (device side, no division).  Once
 // sampling has started, s counts up; during warm-up it stays negative (the
 // host re-derives every phase exactly when sampling starts).
+// Phase counters and flags of the next sweep; the exchange round is left to
+// the caller (Ising launches commit it when they close a sweep).
+__device__ __forceinline__ void advance_phases(DevCtl* c) {
+    if (c->s >= 0) {
+        c->s += 1;
+        c->ph_me = phase_step(c->ph_me, c->measure_every);
+        c->ph_ee = phase_step(c->ph_ee, c->exchange_every);
+    }
+    if (c->renorm_every > 0) c->ph_rn = phase_step(c->ph_rn, c->renorm_every);
+    if (c->adapt_every > 0) c->ph_ad = phase_step(c->ph_ad, c->adapt_every);
+    flags_from_phases(c);
+}
+
 __device__ __forceinline__ void advance_flags(DevCtl* c) {
     if (c->s >= 0) {
         if (c->ph_ee == 0) c->round += 1;

@@ -27,13 +27,12 @@
 // int64 accumulators by atomics; after the colour-1 launch a closing kernel writes
 // the records, performs the replica exchange and advances the sweep counter
 // in a one-block closing kernel.
-#include "kernels.h"
+#include "sweepctl.h"
 
 namespace ising {
 
 constexpr int kThreads = 256;
 constexpr int kSitesPerThread = 16;                // sites per Philox call (and per generic thread step)
-constexpr int kTieCap = 128;                       // tied thread steps queued per wave
 static_assert(kThreads * kSitesPerThread == kIsingTile, "host tiling must match the kernel");
 
 __device__ __forceinline__ int find_group(const DevGroup* groups, int ngroups, int b) {
@@ -51,95 +50,24 @@ __device__ __forceinline__ uint32_t word_of(const u32x4& w, int d) {
     return d == 0 ? w.x : (d == 1 ? w.y : (d == 2 ? w.z : w.w));
 }
 
-// Accumulated quantity q (0: bond sum, 1: magnetisation) of global replica gr,
-// summed over the accumulator copies.
-__device__ __forceinline__ int64_t acc_total(const IsingArgs& a, int gr, int q) {
-    int64_t v = 0;
-    for (int c = 0; c < a.acc_copies; ++c)
-        v += __hip_atomic_load(&a.acc[2 * ((int64_t)c * a.total_reps + gr) + q], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-    return v;
-}
-
 // Multispin coding: bit plane j (MSB first) of the class-k threshold of every
-// replica of a 32-replica word, bit r of the plane belonging to replica r.
-// Rebuilt after each exchange, since the thresholds follow the temperatures.
-__device__ void msc_planes(const IsingArgs& a, int tid, int nthreads) {
+// replica of a 32-replica word, bit r of the plane belonging to replica r,
+// for the permutation of parity `par`.  All threads of one block.
+__device__ void msc_planes(const IsingArgs& a, int par) {
+    const int32_t* r2t = a.r2t + (int64_t)par * a.total_reps;
     for (int g = 0; g < a.ngroups; ++g) {
         const DevGroup G = a.groups[g];
         const int nwords = (G.R + 31) / 32;
-        for (int e = tid; e < nwords * 96; e += nthreads) {
+        for (int e = threadIdx.x; e < nwords * 96; e += blockDim.x) {
             const int wi = e / 96, kj = e - wi * 96, kk = kj >> 5, j = kj & 31;
             const int nrep = min(32, G.R - 32 * wi);
             uint32_t v = 0;
             for (int bit = 0; bit < nrep; ++bit) {
-                const int ti = a.r2t[G.rep0 + 32 * wi + bit];
+                const int ti = r2t[G.rep0 + 32 * wi + bit];
                 v |= ((a.mthr[3 * (G.rep0 + ti) + kk] >> (31 - j)) & 1u) << bit;
             }
             a.mplanes[(int64_t)(G.rep0 + 32 * wi) * 96 + kj] = v;
         }
-    }
-}
-
-// Closing step of a sweep (one-block kernel after the colour-1 launch).
-__device__ void close_sweep(const IsingArgs& a, int64_t t, bool meas, bool exch, int64_t round) {
-    DevCtl* ctl = a.ctl;
-    const int tid = threadIdx.x;
-    const bool need = meas || exch;
-    const int nrec = ctl->n_rec;
-    const bool room = nrec < ctl->rec_cap;
-    if (need) {
-        for (int gr = tid; gr < a.total_reps; gr += blockDim.x) {
-            const int64_t B = acc_total(a, gr, 0);
-            const int64_t M = acc_total(a, gr, 1);
-            if (meas && room) {
-                IsingRec r;
-                r.bond = B;
-                r.mag = M;
-                r.tidx = a.r2t[gr];
-                r.pad = 0;
-                a.rec[(int64_t)nrec * a.total_reps + gr] = r;
-            }
-        }
-    }
-    __syncthreads();
-    if (exch) {
-        for (int g = 0; g < a.ngroups; ++g) {
-            const DevGroup G = a.groups[g];
-            const int npairs = (G.R - (int)(round & 1)) / 2;
-            for (int q = tid; q < npairs; q += blockDim.x) {
-                const int i = (int)(round & 1) + 2 * q;
-                const int ri = a.t2r[G.rep0 + i], rj = a.t2r[G.rep0 + i + 1];
-                const int64_t Bi = acc_total(a, G.rep0 + ri, 0);
-                const int64_t Bj = acc_total(a, G.rep0 + rj, 0);
-                const double Ei = -a.J * (double)Bi;
-                const double Ej = -a.J * (double)Bj;
-                const double delta = (a.betas[G.rep0 + i] - a.betas[G.rep0 + i + 1]) * (Ej - Ei);
-                const u32x4 w = philox((uint32_t)i, (uint32_t)round, stream_id((uint32_t)G.L, 0u),
-                                       tag(P_EXCH, MODEL_ISING), a.k0, a.k1);
-                const double u = u01(w.x, w.y);
-                if (delta <= 0.0 || u < det_exp(-delta)) {
-                    a.t2r[G.rep0 + i] = rj;
-                    a.t2r[G.rep0 + i + 1] = ri;
-                    a.r2t[G.rep0 + ri] = i + 1;
-                    a.r2t[G.rep0 + rj] = i;
-                    atomicAdd((unsigned long long*)&ctl->exch_accepts, 1ull);
-                }
-            }
-            if (tid == 0) atomicAdd((unsigned long long*)&ctl->exch_tries, (unsigned long long)npairs);
-        }
-    }
-    __syncthreads();
-    if (exch && a.mplanes) msc_planes(a, tid, blockDim.x);
-    if (need)
-        for (int64_t i = tid; i < 2 * (int64_t)a.total_reps * a.acc_copies; i += blockDim.x) a.acc[i] = 0;
-    if (tid == 0) {
-        if (meas) {
-            if (room) ctl->n_rec = nrec + 1;
-            else ctl->overflow = 1;
-        }
-        ctl->t = t + 1;
-        advance_flags(ctl);
     }
 }
 
@@ -227,64 +155,60 @@ __device__ __forceinline__ void resolve_tie(const IsingArgs& a, const DevGroup& 
     }
 }
 
-// Per-wave queue of tied thread steps (LDS): step base and tie mask, bit
-// 8j + 7 - d marking site 4d + j of the step.
-struct TieQueue {
-    uint32_t* qh;
-    uint32_t* qm;
-    int n;  // wave-uniform length
-};
+// Block queue of tied thread steps (LDS): step base and tie mask.  Bit
+// 8j + 7 - d marks byte j of word d of the step; word d lies in run d / RW at
+// offset 4 (d % RW).  A lane queues at most one entry per step, so the queue
+// holds kThreads * (steps per tile) entries and never overflows.
+constexpr int kTieCap = kThreads * 4;
 
-// Decides every queued tie: lane i takes entries i, i + 64, ...
+// Decides every queued tie of the block (all threads; after the tile).
+template <int RW, uint32_t RS>
 __device__ __forceinline__ void flush_ties(const IsingArgs& a, const DevGroup& G, uint8_t* C, const uint8_t* O,
                                            const uint32_t* s_thr, int color, uint32_t hs, uint32_t stream, bool need,
-                                           TieQueue& q, int lane, int* msum, int* bsum) {
-    __threadfence_block();  // the step stores are done before tied bytes are rewritten
-    __builtin_amdgcn_wave_barrier();
-    for (int base = 0; base < q.n; base += 64) {
-        const int e = base + lane;
-        if (e < q.n) {
-            const uint32_t h0 = q.qh[e];
-            uint32_t m = q.qm[e];
-            while (m) {
-                const uint32_t pb = (uint32_t)__builtin_ctz(m);
-                m &= m - 1u;
-                resolve_tie(a, G, C, O, s_thr, h0 + 4u * (7u - (pb & 7u)) + (pb >> 3), color, hs, stream, need, msum,
-                            bsum);
-            }
+                                           const uint32_t* qh, const uint32_t* qm, int n, int* msum, int* bsum) {
+    for (int e = threadIdx.x; e < n; e += kThreads) {
+        const uint32_t h0 = qh[e];
+        uint32_t m = qm[e];
+        while (m) {
+            const uint32_t pb = (uint32_t)__builtin_ctz(m);
+            m &= m - 1u;
+            const uint32_t d = 7u - (pb & 7u);
+            const uint32_t site = h0 + (d / RW) * RS + 4u * (d % RW) + (pb >> 3);
+            resolve_tie(a, G, C, O, s_thr, site, color, hs, stream, need, msum, bsum);
         }
     }
-    __builtin_amdgcn_wave_barrier();
-    q.n = 0;
 }
 
-// Thread steps of one block tile for a 3D lattice with Lh % 8 == 0.
-// MODE 1: 16 sites as two 8-site runs (each in one row); MODE 2: 16 sites in
-// one row; MODE 3: 32 sites in one row (two Philox calls).  Per 4-site word:
-// thresholds by byte permutes of the packed table, r7 < hi by the sign bit of
-// (r | 0x80) - hi, ties (byte exactly 0x80) queued for the wave.
-template <int MODE>
-__device__ __forceinline__ void fast_tile(const IsingArgs& a, const DevGroup& G, int tile, uint8_t* C,
-                                          const uint8_t* O, const uint32_t* s_thr, int color, uint32_t hs,
-                                          uint32_t stream, bool need, TieQueue& q, int lane, int* msum, int* bsum) {
-    constexpr int NW = MODE == 3 ? 8 : 4;  // 32-bit words (4 sites each) per thread step
-    constexpr int W = 4 * NW;
+// Thread steps of one block tile for a 3D lattice with Lh % 8 == 0.  A step
+// is NR runs of RW 4-site words, each run within one row; runs are RS
+// half-sites apart.  Layouts: (RW 2, NR 2, RS 8) two adjacent 8-site runs;
+// (4, 1) 16 sites; (4, 2, RS 256 * 16) two 16-site runs a block row apart;
+// (8, 1) 32 sites.  Per 4-site word: thresholds by byte permutes of the
+// packed table, r7 < hi by the sign bit of (r | 0x80) - hi, and ties (byte
+// exactly 0x80) queued for the block.
+template <int RW, int NR, uint32_t RS>
+__device__ __forceinline__ int fast_tile(const IsingArgs& a, const DevGroup& G, int tile, uint8_t* C,
+                                         const uint8_t* O, const uint32_t* s_thr, int color, uint32_t hs,
+                                         uint32_t stream, bool need, uint32_t* qh, uint32_t* qm, int* qn, int* msum,
+                                         int* bsum) {
+    constexpr int NW = RW * NR;                          // 32-bit words per thread step (<= 8)
+    constexpr uint32_t TS = RS == 8u ? 16u : 4u * RW;   // half-sites a thread step advances per thread
+    constexpr uint32_t SPAN = kThreads * TS * (RS == 8u ? 1u : NR);  // half-sites per block step
+    static_assert(NW <= 8, "tie masks hold 8 words");
     const uint32_t T0l = s_thr[28], T0h = s_thr[29], T1l = s_thr[30], T1h = s_thr[31];
     for (int part = 0; part < G.spt; ++part) {
-        const int64_t h0 = (int64_t)tile * G.tsz + (int64_t)part * (kThreads * W) + (int64_t)threadIdx.x * W;
+        const int64_t h0 = (int64_t)tile * G.tsz + (int64_t)part * SPAN + (int64_t)threadIdx.x * TS;
         uint32_t tie = 0;
         if (h0 < G.n_half) {
             const uint32_t hh = (uint32_t)h0;  // L^3/2 < 2^32 for every accepted L
             uint32_t c[NW], ns[NW], rw[NW];
-            if (MODE == 1) {
-                gather_row<2>(G, C, O, hh, color, c, ns);
-                gather_row<2>(G, C, O, hh + 8u, color, c + 2, ns + 2);
-            } else {
-                gather_row<NW>(G, C, O, hh, color, c, ns);
-            }
+#pragma unroll
+            for (int k = 0; k < NR; ++k) gather_row<RW>(G, C, O, hh + (uint32_t)k * RS, color, c + k * RW, ns + k * RW);
 #pragma unroll
             for (int k = 0; k < NW / 4; ++k) {
-                const u32x4 w = philox((hh >> 4) + (uint32_t)k, hs, stream, tag(P_ISING, MODEL_ISING), a.k0, a.k1);
+                // words 4k..4k+3 are the 16-site group of their first site
+                const uint32_t site = hh + (uint32_t)((4 * k) / RW) * RS + 4u * (uint32_t)((4 * k) % RW);
+                const u32x4 w = philox(site >> 4, hs, stream, tag(P_ISING, MODEL_ISING), a.k0, a.k1);
                 rw[4 * k] = w.x;
                 rw[4 * k + 1] = w.y;
                 rw[4 * k + 2] = w.z;
@@ -302,27 +226,23 @@ __device__ __forceinline__ void fast_tile(const IsingArgs& a, const DevGroup& G,
                 nc[d] = c[d] ^ (((zz >> 7) & 0x01010101u) ^ 0x01010101u);
                 // tie iff the byte is exactly 0x80 (r7 == hi)
                 const uint32_t u = (zz & 0x7f7f7f7fu) + 0x7f7f7f7fu;
-                tie |= (zz & ~u & 0x80808080u) >> (d & 7);
+                tie |= (zz & ~u & 0x80808080u) >> d;
             }
-            if (MODE == 1) {
-                typedef uint32_t vec2 __attribute__((ext_vector_type(2)));
-                vec2 o0 = {nc[0], nc[1]}, o1 = {nc[2], nc[3]};
-                *(vec2*)(C + hh) = o0;
-                *(vec2*)(C + hh + 8u) = o1;
-            } else {
-                typedef uint32_t vecw __attribute__((ext_vector_type(NW)));
-                vecw o;
+            typedef uint32_t vecr __attribute__((ext_vector_type(RW)));
 #pragma unroll
-                for (int d = 0; d < NW; ++d) o[d] = nc[d];
-                *(vecw*)(C + hh) = o;
+            for (int k = 0; k < NR; ++k) {
+                vecr o;
+#pragma unroll
+                for (int d = 0; d < RW; ++d) o[d] = nc[k * RW + d];
+                *(vecr*)(C + hh + (uint32_t)k * RS) = o;
             }
             if (need) {
                 int sb = 0;
 #pragma unroll
                 for (int d = 0; d < NW; ++d) sb += __builtin_popcount(nc[d]);
-                *msum += 2 * sb - W;
+                *msum += 2 * sb - 4 * NW;
                 if (color) {
-                    // sum_j (2 b'_j - 1)(2 n_j - 6) = 4 sum b'n - 12 sum b' - 2 sum n + 6 W
+                    // sum_j (2 b'_j - 1)(2 n_j - 6) = 4 sum b'n - 12 sum b' - 2 sum n + 6 (4 NW)
                     uint32_t bn = 0, sn = 0;  // byte sums stay below 256 (at most 8 words of 6)
 #pragma unroll
                     for (int d = 0; d < NW; ++d) {
@@ -331,24 +251,29 @@ __device__ __forceinline__ void fast_tile(const IsingArgs& a, const DevGroup& G,
                     }
                     const int sbn = (int)__builtin_amdgcn_sad_u8(bn, 0u, 0u);
                     const int snn = (int)__builtin_amdgcn_sad_u8(sn, 0u, 0u);
-                    *bsum += 4 * sbn - 12 * sb - 2 * snn + 6 * W;
+                    *bsum += 4 * sbn - 12 * sb - 2 * snn + 24 * NW;
                 }
             }
         }
-        // queue this lane's tied step
+        // queue this lane's tied step: one LDS add per wave, then lane slots
         const uint64_t bal = __ballot(tie != 0u);
         if (bal) {
+            int base = 0;
+            if ((threadIdx.x & 63) == 0) base = atomicAdd(qn, __popcll(bal));
+            base = __shfl(base, 0);
             if (tie) {
-                const uint32_t pos = (uint32_t)q.n + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
-                                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                q.qh[pos] = (uint32_t)h0;
-                q.qm[pos] = tie;
+                const uint32_t pos = (uint32_t)base + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                qh[pos] = (uint32_t)h0;
+                qm[pos] = tie;
             }
-            q.n += __popcll(bal);
-            if (q.n >= 64) flush_ties(a, G, C, O, s_thr, color, hs, stream, need, q, lane, msum, bsum);
         }
     }
-    if (q.n) flush_ties(a, G, C, O, s_thr, color, hs, stream, need, q, lane, msum, bsum);
+    __threadfence_block();  // this block's step stores are done before tied bytes are rewritten
+    __syncthreads();
+    const int n = *qn;
+    flush_ties<RW, RS>(a, G, C, O, s_thr, color, hs, stream, need, qh, qm, n, msum, bsum);
+    return n;
 }
 
 // One colour half-sweep of every replica of every group in the ensemble.
@@ -357,7 +282,8 @@ __device__ __forceinline__ void fast_tile(const IsingArgs& a, const DevGroup& G,
 __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int color) {
     __shared__ uint32_t s_thr[kIsingTab];
     __shared__ int64_t s_red[2][kThreads / 64];
-    __shared__ uint32_t s_qh[kThreads / 64][kTieCap], s_qm[kThreads / 64][kTieCap];
+    __shared__ uint32_t s_qh[kTieCap], s_qm[kTieCap];
+    __shared__ int s_qn;
     const int b = xcd_block(a.nblocks);
     if (b >= a.nblocks) return;
     const int g = find_group(a.groups, a.ngroups, b);
@@ -365,12 +291,22 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
     int r, tile;
     split_tiles(G, b - G.blk0, &r, &tile);
     const int grep = G.rep0 + r;
-    const int tidx = a.r2t[grep];
+    const SweepView v = view_of(a.ctl, color);
+    int tidx;
+    if (v.exch) {
+        const int64_t po = (v.round & 1) * a.total_reps;
+        tidx = exch_tidx(a, G, r, v.round, a.r2t + po, a.t2r + po, acc_of(a, v.t - 1));
+        if (b == G.blk0) exch_write_group(a, G, v.round, acc_of(a, v.t - 1));
+    } else {
+        tidx = a.r2t[(int64_t)v.par * a.total_reps + grep];
+    }
+    if (b == 0) sweep_bookkeeping(a, v, color);
     if (threadIdx.x < kIsingTab)
         s_thr[threadIdx.x] =
             a.inline_thr ? a.thr_inline[threadIdx.x] : a.thr[(int64_t)(G.rep0 + tidx) * kIsingTab + threadIdx.x];
-    const int64_t t = a.ctl->t;
-    const bool need = (a.ctl->flags & (F_MEAS | F_EXCH)) != 0;
+    if (threadIdx.x == 0) s_qn = 0;
+    const int64_t t = v.t;
+    const bool need = (v.flags & (F_MEAS | F_EXCH)) != 0;
     __syncthreads();
 
     const uint32_t hs = (uint32_t)(2 * t + color);
@@ -378,15 +314,17 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
     uint8_t* C = a.pool + G.off + (int64_t)(2 * r + color) * G.stride;
     const uint8_t* O = a.pool + G.off + (int64_t)(2 * r + (color ^ 1)) * G.stride;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    TieQueue q{s_qh[wid], s_qm[wid], 0};
 
     int bsum = 0, msum = 0;
     if (G.fast == 3) {
-        fast_tile<3>(a, G, tile, C, O, s_thr, color, hs, stream, need, q, lane, &msum, &bsum);
+        fast_tile<8, 1, 0u>(a, G, tile, C, O, s_thr, color, hs, stream, need, s_qh, s_qm, &s_qn, &msum, &bsum);
+    } else if (G.fast == 2 && G.runs == 2) {
+        fast_tile<4, 2, kThreads * 16u>(a, G, tile, C, O, s_thr, color, hs, stream, need, s_qh, s_qm, &s_qn, &msum,
+                                        &bsum);
     } else if (G.fast == 2) {
-        fast_tile<2>(a, G, tile, C, O, s_thr, color, hs, stream, need, q, lane, &msum, &bsum);
+        fast_tile<4, 1, 0u>(a, G, tile, C, O, s_thr, color, hs, stream, need, s_qh, s_qm, &s_qn, &msum, &bsum);
     } else if (G.fast == 1) {
-        fast_tile<1>(a, G, tile, C, O, s_thr, color, hs, stream, need, q, lane, &msum, &bsum);
+        fast_tile<2, 2, 8u>(a, G, tile, C, O, s_thr, color, hs, stream, need, s_qh, s_qm, &s_qn, &msum, &bsum);
     } else {
         // Per-site addressing (Lh % 8 != 0, and 2D): same random-number assignment.
         const int64_t h0 = (int64_t)tile * G.tsz + (int64_t)threadIdx.x * kSitesPerThread;
@@ -438,19 +376,35 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
                 B += s_red[0][i];
                 M += s_red[1][i];
             }
-            if (color) atomicAdd((unsigned long long*)&a.acc[2 * grep], (unsigned long long)B);
-            atomicAdd((unsigned long long*)&a.acc[2 * grep + 1], (unsigned long long)M);
+            int64_t* acc = acc_of(a, t);
+            if (color) atomicAdd((unsigned long long*)&acc[2 * grep], (unsigned long long)B);
+            atomicAdd((unsigned long long*)&acc[2 * grep + 1], (unsigned long long)M);
         }
     }
 }
 
-// Closes a sweep: records, replica exchange, sweep counter.  One block,
-// launched after the colour-1 half-sweep (a kernel boundary orders it after
-// every block's accumulator update).
+// Closes the last sweep of a host call (records, replica exchange), which no
+// colour-0 launch will close.  One block, after the colour-1 launch.
 __global__ __launch_bounds__(kThreads) void ising_close_kernel(IsingArgs a) {
-    const int64_t t = a.ctl->t;
-    const int32_t fl = a.ctl->flags;
-    close_sweep(a, t, (fl & F_MEAS) != 0, (fl & F_EXCH) != 0, a.ctl->round);
+    DevCtl* ctl = a.ctl;
+    const int64_t tl = ctl->t - 1;
+    if (ctl->closed >= tl) return;
+    const int32_t pfl = ctl->flp;
+    const int64_t round = ctl->round;
+    const int64_t* acc = acc_of(a, tl);
+    if (pfl & F_MEAS) write_records(a, acc, a.r2t + (round & 1) * a.total_reps);
+    if (pfl & F_EXCH) {
+        for (int g = 0; g < a.ngroups; ++g) {
+            exch_write_group(a, a.groups[g], round, acc);
+            __syncthreads();
+        }
+        if (a.mplanes) msc_planes(a, (int)((round + 1) & 1));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (pfl & F_EXCH) ctl->round = round + 1;
+        ctl->closed = tl;
+    }
 }
 
 // Hot start: i.i.d. +-1 with p = 1/2 from Philox(site, 0, stream, init tag).

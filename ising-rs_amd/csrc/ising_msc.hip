@@ -16,7 +16,7 @@
 // full 32-bit uniform, so the chain is the same Metropolis chain as the int8
 // path, with far fewer random bits per update.  Storage: 4 bytes per site for
 // 32 replicas, i.e. 1 bit per spin (a half-sweep moves 0.375 B per update).
-#include "kernels.h"
+#include "sweepctl.h"
 
 namespace ising {
 
@@ -80,11 +80,33 @@ __global__ __launch_bounds__(kMT) void msc_half_kernel(IsingArgs a, int color) {
     const int nrep = min(32, G.R - 32 * wi);
     const uint32_t valid = nrep >= 32 ? 0xffffffffu : ((1u << nrep) - 1u);
     const int tid = threadIdx.x;
-    if (tid < 96) s_tp[tid >> 5][tid & 31] = a.mplanes[(int64_t)(G.rep0 + 32 * wi) * 96 + tid];
+    const SweepView v = view_of(a.ctl, color);
+    if (v.exch) {
+        // this launch closes an exchange sweep: the word's threshold planes
+        // for the new temperatures, one ballot per plane (lane = replica bit)
+        const int64_t po = (v.round & 1) * a.total_reps;
+        const int64_t* pacc = acc_of(a, v.t - 1);
+        if (b == G.blk0) exch_write_group(a, G, v.round, pacc);
+        const int lane = tid & 63, wave = tid >> 6;
+        uint32_t T[3] = {0u, 0u, 0u};
+        if (lane < nrep) {
+            const int ti = exch_tidx(a, G, 32 * wi + lane, v.round, a.r2t + po, a.t2r + po, pacc);
+            for (int k = 0; k < 3; ++k) T[k] = a.mthr[3 * (G.rep0 + ti) + k];
+        }
+        for (int k = 0; k < 3; ++k)
+            for (int j = 8 * wave; j < 8 * wave + 8; ++j) {
+                const uint64_t m = __ballot((T[k] >> (31 - j)) & 1u);
+                if (lane == 0) s_tp[k][j] = (uint32_t)m;
+            }
+    } else if (tid < 96) {
+        s_tp[tid >> 5][tid & 31] = a.mplanes[(int64_t)(G.rep0 + 32 * wi) * 96 + tid];
+    }
+    if (b == 0) sweep_bookkeeping(a, v, color);
     if (tid < 64) s_acc[tid >> 1][tid & 1] = 0;
-    const int64_t t = a.ctl->t;
-    const bool need = (a.ctl->flags & (F_MEAS | F_EXCH)) != 0;
+    const int64_t t = v.t;
+    const bool need = (v.flags & (F_MEAS | F_EXCH)) != 0;
     __syncthreads();
+    if (v.exch && tile == 0 && tid < 96) a.mplanes[(int64_t)(G.rep0 + 32 * wi) * 96 + tid] = s_tp[tid >> 5][tid & 31];
 
     const uint32_t hs = (uint32_t)(2 * t + color);
     const uint32_t stream = ((uint32_t)G.L << 16) | (uint32_t)wi;
@@ -147,11 +169,11 @@ __global__ __launch_bounds__(kMT) void msc_half_kernel(IsingArgs a, int color) {
         __syncthreads();
         if (tid < 2 * nrep) {
             const int r = tid >> 1, q = tid & 1;
-            const int v = s_acc[r][q];
-            if (v != 0 && (q == 1 || color)) {
+            const int val = s_acc[r][q];
+            if (val != 0 && (q == 1 || color)) {
                 const int copy = b & (kMscAccCopies - 1);
-                atomicAdd((unsigned long long*)&a.acc[2 * ((int64_t)copy * a.total_reps + G.rep0 + 32 * wi + r) + q],
-                          (unsigned long long)(int64_t)v);
+                atomicAdd((unsigned long long*)&acc_of(a, t)[2 * ((int64_t)copy * a.total_reps + G.rep0 + 32 * wi + r) + q],
+                          (unsigned long long)(int64_t)val);
             }
         }
     }

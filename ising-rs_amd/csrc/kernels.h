@@ -10,12 +10,13 @@ struct IsingArgs {
     int32_t total_reps;
     int32_t nblocks;      // work blocks of a half-sweep (the grid is padded to a multiple of 8)
     uint8_t* pool;
-    const uint32_t* thr;  // [rep0 + tidx][28]: 14 high halves, then 14 low halves
-    int32_t* r2t;         // replica -> temperature index (per group, global replica index)
-    int32_t* t2r;         // temperature slot -> replica
+    const uint32_t* thr;  // [rep0 + tidx][32]: 14 high parts, 14 low parts, 4 packed high bytes
+    int32_t* r2t;         // [2][global replica] -> temperature index, by permutation parity
+    int32_t* t2r;         // [2][global temperature slot] -> replica
     const double* betas;  // [rep0 + tidx]
     DevCtl* ctl;
-    int64_t* acc;         // [global replica][2]: bond, mag
+    int64_t* acc;         // [2 sweep parities][acc_copies][global replica][2]: bond, mag
+    int64_t acc_stride;   // int64s per sweep parity
     IsingRec* rec;
     uint32_t k0, k1;
     double J;

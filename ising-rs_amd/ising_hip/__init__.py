@@ -42,7 +42,8 @@ class Opts(C.Structure):
     _fields_ = [("j", C.c_double), ("h", C.c_double), ("exchange_every", C.c_int32), ("measure_every", C.c_int32),
                 ("delta", C.c_double), ("n_overrelax", C.c_int32), ("renorm_every", C.c_int32),
                 ("record_capacity", C.c_int32), ("adapt_every", C.c_int32), ("adapt_target", C.c_double),
-                ("graph_sweeps", C.c_int32), ("ising_spt", C.c_int32), ("ising_width", C.c_int32)]
+                ("graph_sweeps", C.c_int32), ("ising_spt", C.c_int32), ("ising_width", C.c_int32),
+                ("ising_runs", C.c_int32)]
 
 
 class Observables(C.Structure):
@@ -110,7 +111,7 @@ _SIGS = {
     "ising_pt_metropolis_pass": (C.c_int, [_vp, C.c_int32, C.c_int64]),
     "ising_pt_sweeps": (C.c_int, [_vp, _i64p]),
     "ising_pt_device_bytes": (C.c_int, [_vp, _i64p]),
-    "ising_pt_autotune": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "ising_pt_autotune": (C.c_int, [_vp, C.c_int32] + [C.POINTER(C.c_int32)] * 3),
     "ising_batch_create": (C.c_int, [C.POINTER(_vp), _vp, C.c_int32, C.c_int32]),
     "ising_batch_destroy": (None, [_vp]),
     "ising_batch_warmup": (C.c_int, [_vp, C.c_int64]),
@@ -430,10 +431,10 @@ class PTEnsemble:
 
     def autotune(self, sweeps=20):
         """Times the int8 thread-step layouts over warm-up sweeps and keeps the
-        fastest; returns (width, steps per tile)."""
-        w, s = C.c_int32(), C.c_int32()
-        _check(lib.ising_pt_autotune(self._h, int(sweeps), C.byref(w), C.byref(s)))
-        return w.value, s.value
+        fastest; returns (run width, runs per step, steps per tile)."""
+        w, r, s = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib.ising_pt_autotune(self._h, int(sweeps), C.byref(w), C.byref(r), C.byref(s)))
+        return w.value, r.value, s.value
 
     def sweeps(self):
         v = C.c_int64()

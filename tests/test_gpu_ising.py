@@ -138,9 +138,9 @@ def test_large_lattice_sweeps_are_bit_exact():
 def test_thread_step_layouts_do_not_change_results():
     temps = o.linspace(4.40, 4.62, 4)
     out = []
-    for width, spt in ((0, 0), (16, 1), (32, 1), (32, 2), (16, 4), (32, 3)):
+    for width, runs, spt in ((0, 0, 0), (16, 1, 1), (16, 2, 1), (32, 1, 1), (32, 1, 2), (16, 2, 4), (16, 1, 3)):
         pt = ising_hip.PTEnsemble("ising", [16, 32, 64, 96, 128], 4, temps, 5, exchange_every=3, measure_every=2,
-                                  ising_spt=spt, ising_width=width)
+                                  ising_spt=spt, ising_width=width, ising_runs=runs)
         pt.warmup(5)
         pt.sample(12)
         out.append(pt.read_ising() + tuple(pt.get_spins(4, r) for r in range(4)))
@@ -154,11 +154,11 @@ def test_autotune_keeps_the_chain():
     ensemble continues exactly like an untuned one warmed up as long."""
     temps = o.linspace(4.40, 4.62, 4)
     tuned = ising_hip.PTEnsemble("ising", [32, 64], 4, temps, 9, exchange_every=3, measure_every=2)
-    width, spt = tuned.autotune(3)
-    assert width in (16, 32) and spt in (1, 2, 4)
-    assert tuned.sweeps() == 21
+    width, runs, spt = tuned.autotune(3)
+    assert (width, runs) in ((16, 1), (16, 2), (32, 1)) and spt in (1, 2, 4)
+    assert tuned.sweeps() == 9 * (3 + 1)  # nine layouts, one untimed sweep each
     plain = ising_hip.PTEnsemble("ising", [32, 64], 4, temps, 9, exchange_every=3, measure_every=2)
-    plain.warmup(21)
+    plain.warmup(36)
     for p in (tuned, plain):
         p.sample(10)
     for x, y in zip(tuned.read_ising(), plain.read_ising()):
