@@ -171,6 +171,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--workload", default="ising", choices=sorted(WORKLOADS) + ["batched"])
+    ap.add_argument("--layout", default="auto", metavar="W,R,S",
+                    help="int8 thread-step layout (run width, runs per step, steps per tile); "
+                         "'auto' times all layouts during warm-up and keeps the fastest")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -193,10 +196,16 @@ def main():
     model, sizes, (tlo, thi), bpu, dtype, opts, desc = WORKLOADS[args.workload]
     temps = linspace(tlo, thi, R)
     me = opts["measure_every"]
+    if model == "ising" and args.layout != "auto":
+        w, r, s = (int(v) for v in args.layout.split(","))
+        opts = dict(opts, ising_width=w, ising_runs=r, ising_spt=s)
     pt = ising_hip.PTEnsemble(model, sizes, R, temps, args.seed, record_capacity=args.steps // me + 8, **opts)
     layout = None
     if model == "ising":
-        layout = pt.autotune(20)  # warm-up sweeps that also pick the thread-step layout
+        if args.layout == "auto":
+            layout = pt.autotune(20)  # warm-up sweeps that also pick the thread-step layout
+        else:
+            layout = tuple(int(v) for v in args.layout.split(","))
     pt.warmup(args.warmup)
     pt.sync()
 
