@@ -17,6 +17,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ising-rs_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -163,6 +165,82 @@ def run_batched(args, ising_hip):
     print(json.dumps(line), flush=True)
 
 
+def temperature_series(tidx, x):
+    """Per (size, temperature) time series from per-replica records: at each
+    measured sweep, the value of the replica sitting at that temperature."""
+    n, ng, R = tidx.shape
+    order = np.argsort(tidx, axis=2)
+    return np.take_along_axis(x, order, axis=2)  # [sweep, size, temperature]
+
+
+def run_learned(args, ising_hip):
+    """Learned proposal against plain Metropolis + overrelaxation (config 5):
+    XY, L = 32, 64 x 32 replicas, T = linspace(2.18, 2.22).  Both chains sample
+    the same distribution; the figure of merit is independent samples per
+    second, sweeps/s / (2 tau_int(E)) summed over (size, temperature), with
+    tau_int by Sokal's automatic window (c = 6)."""
+    sizes, Rr = [32, 64], R
+    temps = linspace(2.18, 2.22, Rr)
+    steps = args.steps
+    kw = dict(exchange_every=10, measure_every=1, n_overrelax=2, delta=1.0, adapt_every=10, adapt_target=0.5,
+              renorm_every=100, record_capacity=steps + 8)
+    out = {}
+    for mode in ("plain", "learned"):
+        pt = ising_hip.PTEnsemble("xy", sizes, Rr, temps, args.seed, **kw)
+        pt.warmup(args.warmup)
+        train = None
+        if mode == "learned":
+            pt.learn_init(args.hidden, seed=7)
+            t0 = time.perf_counter()
+            loss = pt.learn_train(args.train_steps, batch=8192, lr=3e-3)
+            pt.sync()
+            train = {"steps": args.train_steps, "batch": 8192, "seconds": round(time.perf_counter() - t0, 3),
+                     "final_loss": loss}
+            pt.learn_enable(True)
+            pt.warmup(50)
+        pt.sync()
+        t0 = time.perf_counter()
+        pt.sample(steps)
+        pt.sync()
+        wall = time.perf_counter() - t0
+        tidx, e, mx, my, mz = pt.read_cont()
+        es = temperature_series(tidx, e)
+        m2 = temperature_series(tidx, mx * mx + my * my)
+        taus_e, taus_m = [], []
+        for g in range(len(sizes)):
+            for k in range(Rr):
+                taus_e.append(ising_hip.tau_int(es[:, g, k])[0])
+                taus_m.append(ising_hip.tau_int(m2[:, g, k])[0])
+        taus_e, taus_m = np.array(taus_e), np.array(taus_m)
+        isps = float(np.sum(steps / (2.0 * np.maximum(taus_e, 0.5))) / wall)
+        tries, acc = pt.mh_stats() if mode == "learned" else (0, 0)
+        out[mode] = {"sweeps_per_s": round(steps / wall, 2), "tau_int_E_mean": round(float(taus_e.mean()), 3),
+                     "tau_int_M2_mean": round(float(taus_m.mean()), 3), "independent_samples_per_s": round(isps, 1),
+                     "learned_acceptance": (round(acc / tries, 4) if tries else None), "training": train}
+        pt.close()
+    ratio = out["learned"]["independent_samples_per_s"] / out["plain"]["independent_samples_per_s"]
+    sites = sum(L ** 3 for L in sizes) * Rr
+    line = {
+        "metric": "independent samples/s (learned proposal)",
+        "value": out["learned"]["independent_samples_per_s"],
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 / out["learned"]["sweeps_per_s"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (MLP bf16 weights, f32 accumulate)",
+        "data": "synthetic",
+        "config": {"workload": "learned proposal vs plain (config 5): XY L=32,64 x 32 replicas, T=linspace(2.18,2.22,32), "
+                               "1 Metropolis + 2 overrelaxation passes per sweep, measure every sweep, exchange every 10",
+                   "hidden": args.hidden, "sites": sites, "plain": out["plain"], "learned": out["learned"],
+                   "learned_over_plain": round(ratio, 4)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,7 +248,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--workload", default="ising", choices=sorted(WORKLOADS) + ["batched"])
+    ap.add_argument("--workload", default="ising", choices=sorted(WORKLOADS) + ["batched", "learned"])
+    ap.add_argument("--hidden", type=int, default=16, help="learned workload: MLP hidden width (8, 16, 32 or 64)")
+    ap.add_argument("--train-steps", type=int, default=400, help="learned workload: Adam steps")
     ap.add_argument("--layout", default="auto", metavar="W,R,S",
                     help="int8 thread-step layout (run width, runs per step, steps per tile); "
                          "'auto' times all layouts during warm-up and keeps the fastest")
@@ -192,6 +272,9 @@ def main():
 
     if args.workload == "batched":
         run_batched(args, ising_hip)
+        return
+    if args.workload == "learned":
+        run_learned(args, ising_hip)
         return
     model, sizes, (tlo, thi), bpu, dtype, opts, desc = WORKLOADS[args.workload]
     temps = linspace(tlo, thi, R)

@@ -93,50 +93,46 @@ __device__ __forceinline__ void block_sum4(double v[4], double (*s_red)[kCT / 64
 // during warm-up, sweep counter.
 __device__ void close_cont(const ContArgs& a, int64_t t, int32_t fl) {
     const bool meas = (fl & F_MEAS) != 0, exch = (fl & F_EXCH) != 0;
-    __shared__ double s_part[4][kCT];
     DevCtl* ctl = a.ctl;
     const int tid = threadIdx.x;
     const bool need = meas || exch;
     const int nrec = ctl->n_rec;
     const bool room = nrec < ctl->rec_cap;
     if (need) {
-        for (int g = 0; g < a.ngroups; ++g) {
-            const DevGroup G = a.groups[g];
-            for (int r = 0; r < G.R; ++r) {
-                const int64_t b0 = G.blk0 + (int64_t)r * G.tiles;
-                double v[4] = {0.0, 0.0, 0.0, 0.0};
-                for (int64_t b = tid; b < G.tiles; b += kCT) {
-                    const double* p0 = a.part0 + 4 * (b0 + b);
-                    const double* p1 = a.part1 + 4 * (b0 + b);
-                    v[0] += p1[0];
-                    v[1] += p0[1] + p1[1];
-                    v[2] += p0[2] + p1[2];
-                    v[3] += p0[3] + p1[3];
+        // one wave per replica: lanes sum the block partials in a fixed
+        // assignment, then a fixed shuffle tree (deterministic FP64 order)
+        const int lane = tid & 63, wave = tid >> 6;
+        for (int gr = wave; gr < a.total_reps; gr += kCT / 64) {
+            int g = 0;
+            while (g + 1 < a.ngroups && gr >= a.groups[g + 1].rep0) ++g;
+            const int64_t tiles = a.groups[g].tiles;
+            const int64_t b0 = a.groups[g].blk0 + (int64_t)(gr - a.groups[g].rep0) * tiles;
+            double v[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int64_t b = lane; b < tiles; b += 64) {
+                const double* p0 = a.part0 + 4 * (b0 + b);
+                const double* p1 = a.part1 + 4 * (b0 + b);
+                v[0] += p1[0];
+                v[1] += p0[1] + p1[1];
+                v[2] += p0[2] + p1[2];
+                v[3] += p0[3] + p1[3];
+            }
+            for (int off = 32; off > 0; off >>= 1)
+                for (int k = 0; k < 4; ++k) v[k] += __shfl_down(v[k], off);
+            if (lane == 0) {
+                for (int k = 0; k < 4; ++k) a.dacc[4 * gr + k] = v[k];
+                if (meas && room) {
+                    ContRec rc;
+                    rc.e = v[0];
+                    rc.mx = v[1];
+                    rc.my = v[2];
+                    rc.mz = v[3];
+                    rc.tidx = a.r2t[gr];
+                    rc.pad = 0;
+                    a.rec[(int64_t)nrec * a.total_reps + gr] = rc;
                 }
-                for (int k = 0; k < 4; ++k) s_part[k][tid] = v[k];
-                __syncthreads();
-                for (int w = kCT / 2; w > 0; w >>= 1) {
-                    if (tid < w)
-                        for (int k = 0; k < 4; ++k) s_part[k][tid] += s_part[k][tid + w];
-                    __syncthreads();
-                }
-                if (tid == 0) {
-                    const int gr = G.rep0 + r;
-                    for (int k = 0; k < 4; ++k) a.dacc[4 * gr + k] = s_part[k][0];
-                    if (meas && room) {
-                        ContRec rc;
-                        rc.e = s_part[0][0];
-                        rc.mx = s_part[1][0];
-                        rc.my = s_part[2][0];
-                        rc.mz = s_part[3][0];
-                        rc.tidx = a.r2t[gr];
-                        rc.pad = 0;
-                        a.rec[(int64_t)nrec * a.total_reps + gr] = rc;
-                    }
-                }
-                __syncthreads();
             }
         }
+        __syncthreads();  // dacc complete before the exchange reads it
     }
     if (exch) {
         const int64_t round = ctl->round;
