@@ -3,7 +3,8 @@
 Follows analysis/scripts/reweighting.py:45-108 (wham_reweight) step for step
 (free-energy iteration, shift f_0 = 0, tolerance test, weights at the target),
 and returns reweighted means of given observables.
-Pinned by the closed-form two-level system in tests/test_gpu_reweight.py.
+Pinned by closed-form cases (two-level system, Gaussian energies, single
+histogram) in tests/test_oracle_reweight.py.
 """
 import numpy as np
 
