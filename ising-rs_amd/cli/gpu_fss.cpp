@@ -13,6 +13,18 @@
 // Additional flags (off by default, results identical either way):
 //   --batched 0|1      run every size in one ensemble (1, default) or one size at a time (0)
 //   --renorm-every k   continuous spins: renormalise every k sweeps (default 100)
+//   --binary 0|1       also write gpu_fss_{model}_N{n}_timeseries.bin: the exact
+//                      per-replica records (int64 bond sum and magnetisation for
+//                      Ising, FP64 E, Mx, My, Mz otherwise), which the 8-decimal CSV
+//                      rounds; format below
+//
+// Binary timeseries (little-endian), one file per size:
+//   char[8] "ISINGTS1"; int32 version (1), model (0 ising, 1 xy, 2 heisenberg),
+//   L, replicas R, measure_every, record kind (0: Ising, 1: continuous);
+//   int64 number of measured sweeps; double J; double T[R] (by temperature
+//   index); then, per measured sweep, R records in replica order:
+//     kind 0: int64 bond sum (E = -J * bond), int64 magnetisation, int32 temp_idx, int32 0
+//     kind 1: double E, Mx, My, Mz (totals), int32 temp_idx, int32 0
 //
 // Divergences from the reference, by design: initial states and exchange
 // draws come from the library's Philox stream (the reference's are unseeded /
@@ -121,7 +133,33 @@ struct Args {
     std::string outdir = "analysis/data";
     double delta = 0.5;
     unsigned long long n_overrelax = 5, measure_every = 1;
-    unsigned long long batched = 1, renorm_every = 100;
+    unsigned long long batched = 1, renorm_every = 100, binary = 0;
+};
+
+// Writer of the binary timeseries of one size.
+struct BinSeries {
+    FILE* f = nullptr;
+    int64_t n = 0;
+    void open(const std::string& path, int model, int L, int R, int measure_every, double J,
+              const std::vector<double>& temps) {
+        f = std::fopen(path.c_str(), "wb");
+        if (!f) die("create timeseries file failed: " + path);
+        std::setvbuf(f, nullptr, _IOFBF, 1 << 20);
+        const int32_t hdr[6] = {1, model, L, R, measure_every, model == ISING_MODEL_ISING ? 0 : 1};
+        const int64_t zero = 0;
+        if (std::fwrite("ISINGTS1", 1, 8, f) != 8 || std::fwrite(hdr, sizeof hdr, 1, f) != 1 ||
+            std::fwrite(&zero, sizeof zero, 1, f) != 1 || std::fwrite(&J, sizeof J, 1, f) != 1 ||
+            std::fwrite(temps.data(), sizeof(double), temps.size(), f) != temps.size())
+            die("write timeseries failed: " + path);
+    }
+    void put(const void* rec, size_t bytes) {
+        if (std::fwrite(rec, 1, bytes, f) != bytes) die("write timeseries failed");
+    }
+    void close(const std::string& path, int64_t sweeps) {
+        if (std::fseek(f, 32, SEEK_SET) != 0 || std::fwrite(&sweeps, sizeof sweeps, 1, f) != 1 || std::fclose(f) != 0)
+            die("write timeseries failed: " + path);
+        f = nullptr;
+    }
 };
 
 // One FSS run over `sizes` sharing one device ensemble.
@@ -159,7 +197,13 @@ void run(const Args& a, const std::vector<int>& sizes, int model, const char* mo
     }
 
     std::vector<FILE*> ts(ns);
-    std::vector<std::string> ts_path(ns), sum_path(ns);
+    std::vector<std::string> ts_path(ns), sum_path(ns), bin_path(ns);
+    std::vector<BinSeries> bin(a.binary ? ns : 0);
+    for (int g = 0; g < (int)bin.size(); ++g) {
+        bin_path[g] = a.outdir + "/gpu_fss_" + model_name + "_N" + std::to_string(sizes[g]) + "_timeseries.bin";
+        bin[g].open(bin_path[g], model, sizes[g], R, (int)a.measure_every, 1.0, temps);
+    }
+    int64_t measured = 0;
     for (int g = 0; g < ns; ++g) {
         ts_path[g] = a.outdir + "/gpu_fss_" + model_name + "_N" + std::to_string(sizes[g]) + "_timeseries.csv";
         sum_path[g] = a.outdir + "/gpu_fss_" + model_name + "_N" + std::to_string(sizes[g]) + "_summary.csv";
@@ -195,6 +239,7 @@ void run(const Args& a, const std::vector<int>& sizes, int model, const char* mo
         else
             check(ising_pt_read_cont(pt, cap, tidx.data(), ce.data(), cx.data(), cy.data(), cz.data(), &n),
                   "GPU measure failed");
+        measured += n;
         for (int64_t k = 0; k < n; ++k)
             for (int g = 0; g < ns; ++g) {
                 const double n3 = (double)sizes[g] * sizes[g] * sizes[g];
@@ -212,6 +257,17 @@ void run(const Args& a, const std::vector<int>& sizes, int model, const char* mo
                     se[g][t].push_back(e_per);
                     sm[g][t].push_back(m_per);
                     std::fprintf(ts[g], "%d,%.8f,%.8f\n", t, e_per, m_per);  // gpu_fss.rs:264
+                    if (a.binary) {
+                        if (model == ISING_MODEL_ISING) {
+                            const int64_t rec[3] = {bond[i], mag[i], (int64_t)(uint32_t)t};
+                            bin[g].put(rec, sizeof rec);
+                        } else {
+                            const double rec[4] = {ce[i], cx[i], cy[i], cz[i]};
+                            const int64_t ti = (int64_t)(uint32_t)t;
+                            bin[g].put(rec, sizeof rec);
+                            bin[g].put(&ti, sizeof ti);
+                        }
+                    }
                 }
             }
         if (s % 10000 == 0) std::fprintf(stderr, "    sweep %llu/%llu\n", s, a.samples);
@@ -219,6 +275,10 @@ void run(const Args& a, const std::vector<int>& sizes, int model, const char* mo
     for (int g = 0; g < ns; ++g) {
         if (std::fclose(ts[g]) != 0) die("flush timeseries failed");
         std::fprintf(stderr, "  Wrote %s\n", ts_path[g].c_str());
+        if (a.binary) {
+            bin[g].close(bin_path[g], measured);
+            std::fprintf(stderr, "  Wrote %s\n", bin_path[g].c_str());
+        }
         const double n3 = (double)sizes[g] * sizes[g] * sizes[g];
         std::string csv = "T,E,E_err,M,M_err,M2,M2_err,M4,M4_err,Cv,Cv_err,chi,chi_err\n";  // gpu_fss.rs:295
         for (int t = 0; t < R; ++t) {
@@ -267,6 +327,7 @@ int main(int argc, char** argv) {
         else if (f == "--batched") a.batched = parse_u64(get_arg(argc, argv, i, "--batched"), "--batched"), i += 2;
         else if (f == "--renorm-every")
             a.renorm_every = parse_u64(get_arg(argc, argv, i, "--renorm-every"), "--renorm-every"), i += 2;
+        else if (f == "--binary") a.binary = parse_u64(get_arg(argc, argv, i, "--binary"), "--binary"), i += 2;
         else i += 1;
     }
 

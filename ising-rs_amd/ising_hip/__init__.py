@@ -476,6 +476,38 @@ class Batch:
         return ms.value
 
 
+def read_timeseries(path):
+    """Reads a binary timeseries written by gpu_fss --binary 1.
+
+    Returns a dict: model, L, replicas, measure_every, J, temps (by
+    temperature index) and arrays shaped [measured sweep, replica]: temp_idx
+    plus bond and mag (int64, Ising) or e, mx, my, mz (FP64 totals)."""
+    raw = open(path, "rb").read()
+    if raw[:8] != b"ISINGTS1":
+        raise ValueError("not an ISINGTS1 file: %s" % path)
+    version, model, L, R, me, kind = np.frombuffer(raw, np.int32, 6, 8)
+    if version != 1:
+        raise ValueError("unsupported timeseries version %d" % version)
+    n = int(np.frombuffer(raw, np.int64, 1, 32)[0])
+    J = float(np.frombuffer(raw, np.float64, 1, 40)[0])
+    temps = np.frombuffer(raw, np.float64, R, 48).copy()
+    off = 48 + 8 * R
+    if kind == 0:
+        rec = np.frombuffer(raw, np.int64, 3 * n * R, off).reshape(n, R, 3)
+        fields = {"bond": rec[:, :, 0].copy(), "mag": rec[:, :, 1].copy(), "temp_idx": rec[:, :, 2].astype(np.int32)}
+    else:
+        rec = np.frombuffer(raw, np.float64, 5 * n * R, off).reshape(n, R, 5)
+        fields = {"e": rec[:, :, 0].copy(), "mx": rec[:, :, 1].copy(), "my": rec[:, :, 2].copy(),
+                  "mz": rec[:, :, 3].copy(),
+                  "temp_idx": np.frombuffer(raw, np.int64, 5 * n * R, off).reshape(n, R, 5)[:, :, 4].astype(np.int32)}
+    if len(raw) != off + (24 if kind == 0 else 40) * n * R:
+        raise ValueError("truncated timeseries file: %s" % path)
+    out = {"model": ["ising", "xy", "heisenberg"][model], "L": int(L), "replicas": int(R),
+           "measure_every": int(me), "J": J, "temps": temps}
+    out.update(fields)
+    return out
+
+
 def linspace_temperatures(t_min, t_max, n):
     out = np.empty(int(n), np.float64)
     _check(lib.ising_linspace_temperatures(t_min, t_max, int(n), _ptr(out)))

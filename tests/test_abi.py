@@ -85,3 +85,23 @@ def test_invalid_arguments_are_rejected_before_touching_the_device():
     assert ei.value.code == -1
     with pytest.raises(ising_hip.IsingError):
         ising_hip.PTEnsemble("ising", [8], 2, np.array([4.4, 4.6]), 1, exchange_every=0)
+
+
+def test_binary_timeseries_reader_parses_the_documented_layout(tmp_path):
+    """The reader of gpu_fss --binary output against a file written field by
+    field in the documented layout (no GPU needed)."""
+    import ising_hip
+    R, n, L = 3, 2, 4
+    temps = np.array([4.4, 4.5, 4.6])
+    recs = np.array([[[10, -4, 2], [12, 6, 0], [-2, 8, 1]], [[14, 2, 0], [16, -6, 2], [0, 0, 1]]], np.int64)
+    raw = (b"ISINGTS1" + np.array([1, 0, L, R, 5, 0], np.int32).tobytes() + np.array([n], np.int64).tobytes() +
+           np.array([1.0], np.float64).tobytes() + temps.tobytes() + recs.tobytes())
+    p = tmp_path / "x.bin"
+    p.write_bytes(raw)
+    ts = ising_hip.read_timeseries(str(p))
+    assert ts["model"] == "ising" and ts["L"] == L and ts["replicas"] == R and ts["measure_every"] == 5
+    assert np.array_equal(ts["bond"], recs[:, :, 0]) and np.array_equal(ts["mag"], recs[:, :, 1])
+    assert np.array_equal(ts["temp_idx"], recs[:, :, 2]) and np.array_equal(ts["temps"], temps)
+    p.write_bytes(raw[:-8])
+    with pytest.raises(ValueError):
+        ising_hip.read_timeseries(str(p))

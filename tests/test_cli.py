@@ -100,3 +100,36 @@ def test_continuous_smoke_outputs(tmp_path, model):
     assert ts[0] == "temp_idx,E,M" and len(ts) == 1 + 4 * 20
     assert summ[0].startswith("T,E,E_err") and len(summ) == 5
     assert "nan" not in "".join(ts + summ).lower()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model", ["ising", "xy"])
+def test_binary_timeseries_holds_the_exact_records(tmp_path, model):
+    """--binary 1 writes the unrounded records; rounding them as the CSV does
+    reproduces the CSV rows, and the Ising records equal the oracle's."""
+    import ising_hip
+    args = ["--model", model, "--sizes", "4,6", "--tmin", "4.4" if model == "ising" else "1.4", "--tmax",
+            "4.6" if model == "ising" else "2.2", "--replicas", "4", "--warmup", "50", "--samples", "60",
+            "--exchange-every", "10", "--outdir", str(tmp_path), "--binary", "1"]
+    if model != "ising":
+        args += ["--overrelax", "2", "--measure-every", "3"]
+    r = run(args)
+    assert r.returncode == 0, r.stderr
+    for L in (4, 6):
+        ts = ising_hip.read_timeseries(str(tmp_path / ("gpu_fss_%s_N%d_timeseries.bin" % (model, L))))
+        rows = open(tmp_path / ("gpu_fss_%s_N%d_timeseries.csv" % (model, L))).read().splitlines()[1:]
+        n3 = float(L ** 3)
+        assert ts["L"] == L and ts["replicas"] == 4 and ts["model"] == model
+        assert len(rows) == ts["temp_idx"].size
+        if model == "ising":
+            e = -ts["bond"] / n3
+            m = np.abs(ts["mag"]) / n3
+            rt, rb, rm, _ = o.ising_pt_run(L, 4, o.linspace(4.4, 4.6, 4), 42, 50, 60, 10, 1)
+            assert np.array_equal(ts["bond"].reshape(-1), rb) and np.array_equal(ts["mag"].reshape(-1), rm)
+            assert np.array_equal(ts["temp_idx"].reshape(-1), rt)
+        else:
+            e = ts["e"] / n3
+            m = np.sqrt(ts["mx"] ** 2 + ts["my"] ** 2 + ts["mz"] ** 2) / n3
+        got = ["%d,%.8f,%.8f" % (t, a, b) for t, a, b in zip(ts["temp_idx"].reshape(-1), e.reshape(-1), m.reshape(-1))]
+        assert got == rows
+        assert np.allclose(ts["temps"], np.linspace(float(args[5]), float(args[7]), 4))
