@@ -225,6 +225,51 @@ int ising_pt_mh_stats(ising_pt_t* pt, int64_t* tries, int64_t* accepts);
  * exchange or advancing the sweep counter (used by exactness tests). */
 int ising_pt_metropolis_pass(ising_pt_t* pt, int32_t color, int64_t t);
 /* Sweeps applied so far (warm-up + sample). */
+/* Annealing schedule (int8 Ising ensembles, before sampling): n sweeps, sweep
+ * k with inverse temperature betas[k] for every replica (acceptance tables
+ * built on the host in FP64 as for the ensemble).  When bond / mag are not
+ * NULL they receive [n][total replicas] int64 bond sums and magnetisations
+ * measured on the device after every sweep.  Serves the drivers below. */
+int ising_pt_anneal(ising_pt_t* pt, const double* betas, int64_t n, int64_t* bond, int64_t* mag);
+
+/* ------------------------------------------------------------------------
+ * The other LatticeGpu callers as batched drivers: every trial / size runs as
+ * a replica / group of one ensemble, so one launch per colour advances all.
+ * ------------------------------------------------------------------------ */
+
+/* One Kibble-Zurek quench of n_trials independent L = n lattices
+ * (src/cuda/kz_gpu.rs:9-45): warm up at t_start, ramp linearly to t_end over
+ * tau_q sweeps (kibble_zurek.rs:78-84), optionally freeze, and return each
+ * trial's domain-wall density (fraction of forward bonds across a wall,
+ * lattice_gpu.rs:254-280).  Trials are the Philox streams of one seed. */
+int ising_kz_run(int32_t n, double j, double t_start, double t_end, int64_t tau_q, int32_t n_trials,
+                 int64_t warmup_sweeps, int64_t freeze_sweeps, double freeze_temperature, uint64_t seed,
+                 double* rho);
+
+/* run_kz_sweep_gpu (src/cuda/kz_gpu.rs:49-95): for each tau_q, n_trials
+ * quenches keyed by base_seed + tau_q (kibble_zurek.rs:102-106); mean and
+ * standard error of the density (kibble_zurek.rs:86-100). */
+int ising_kz_sweep(int32_t n, double j, double t_start, double t_end, const int64_t* tau_q, int32_t n_tau,
+                   int32_t n_trials, int64_t warmup_sweeps, int64_t freeze_sweeps, double freeze_temperature,
+                   uint64_t base_seed, double* rho, double* rho_err);
+
+/* run_coarsening_gpu (src/cuda/coarsening_gpu.rs:6-31): warm up at t_high,
+ * quench to t_quench, sample the domain-wall density after steps 0,
+ * sample_every, ... < total_steps.  rho_out is [sample][trial]; n_trials = 1 is
+ * the reference's single lattice.  steps_out and rho_out hold at least
+ * ceil(total_steps / sample_every) (x n_trials) entries. */
+int ising_coarsening_run(int32_t n, double j, double t_high, double t_quench, int64_t warmup_sweeps,
+                         int64_t total_steps, int64_t sample_every, uint64_t seed, int32_t n_trials,
+                         int64_t* steps_out, double* rho_out, int64_t* n_samples);
+
+/* run_fss_gpu (src/cuda/fss_gpu.rs:7-39): annealed temperature sweep from
+ * t_max down to t_min in t_steps points; at each, warm_up then measure
+ * (LatticeGpu::measure, lattice_gpu.rs:193-252) with every size advancing
+ * together.  out is [size][temperature index, ascending T]. */
+int ising_fss_anneal(const int32_t* sizes, int32_t n_sizes, double j, double h, double t_min, double t_max,
+                     int32_t t_steps, int64_t warmup_sweeps, int64_t sample_sweeps, uint64_t seed,
+                     ising_observables_t* out);
+
 /* Autotuner (no counterpart in the reference): times each int8 Ising
  * thread-step layout (16-site runs, one or two per step, or 32-site runs, x
  * 1/2/4 steps per block tile) over `sweeps` warm-up sweeps of this ensemble

@@ -147,6 +147,12 @@ struct ising_pt {
     int learn_cap_blocks = 0;
     float beta_inline = 0.f, delta_inline = 0.f, j_inline = 1.f;
     int layout_width = 0, layout_runs = 0, layout_spt = 0;  // Ising thread-step layout in use
+    uint32_t* d_sched = nullptr;   // annealing schedule: table rows per sweep
+    int64_t sched_cap = 0;         // rows allocated
+    bool d_sched_on = false;
+    int64_t sched_t0 = 0;
+    int64_t* d_smeas = nullptr;    // annealing: per-sweep B, M of every replica
+    int64_t smeas_cap = 0;         // int64s allocated
     int64_t t_host = 0;        // sweeps enqueued
     int64_t s_host = -1;       // sample sweeps enqueued (-1 before sampling starts)
     int64_t pending_host = 0;  // measured sweeps the device will have written
@@ -161,6 +167,8 @@ struct ising_pt {
         (void)hipFree(d_thr);
         (void)hipFree(d_mthr);
         (void)hipFree(d_mplanes);
+        (void)hipFree(d_sched);
+        (void)hipFree(d_smeas);
         (void)hipFree(d_r2t);
         (void)hipFree(d_t2r);
         (void)hipFree(d_betas);
@@ -211,6 +219,8 @@ struct ising_pt {
         a.mplanes = d_mplanes;
         a.acc_copies = model == ISING_MODEL_ISING_MSC ? kMscAccCopies : 1;
         a.acc_stride = 2 * (int64_t)a.acc_copies * total_reps;
+        a.sched = d_sched_on ? d_sched : nullptr;
+        a.sched_t0 = sched_t0;
         return a;
     }
 
@@ -1111,6 +1121,260 @@ int ising_pt_metropolis_pass(ising_pt_t* p, int32_t color, int64_t t) {
     ORDER(p);
     if (p->learned_on) HIP_TRY(launch_cont_learned(p->cont_args(), p->nblocks, color, 0, t, p->st));
     else return fail(ISING_E_STATE, "enable the learned proposal first");
+    return ISING_OK;
+}
+
+int ising_pt_anneal(ising_pt_t* p, const double* betas, int64_t n, int64_t* bond, int64_t* mag) {
+    if (!p || (!betas && n > 0)) return fail(ISING_E_ARG, "null argument");
+    if (n < 0) return fail(ISING_E_ARG, "n must be >= 0");
+    if (p->model != ISING_MODEL_ISING && p->model != ISING_MODEL_ISING2D)
+        return fail(ISING_E_STATE, "annealing schedules exist for int8 Ising ensembles");
+    if (p->inline_thr) return fail(ISING_E_STATE, "single-lattice handles take beta per step");
+    if (p->s_host >= 0) return fail(ISING_E_STATE, "annealing sweeps are warm-up sweeps: call before sampling");
+    for (int64_t k = 0; k < n; ++k)
+        if (!(betas[k] > 0.0) || !std::isfinite(betas[k])) return fail(ISING_E_ARG, "betas must be positive and finite");
+    if (n == 0) return ISING_OK;
+    SYNC_ALL(p);
+    if (n > p->sched_cap) {
+        (void)hipFree(p->d_sched);
+        p->d_sched = nullptr;
+        p->sched_cap = 0;
+        HIP_TRY(hipMalloc(&p->d_sched, sizeof(uint32_t) * kIsingTab * (size_t)n));
+        p->sched_cap = n;
+    }
+    const bool meas = bond || mag;
+    const int64_t per = 2 * (int64_t)p->total_reps;
+    if (meas && n * per > p->smeas_cap) {
+        (void)hipFree(p->d_smeas);
+        p->d_smeas = nullptr;
+        p->smeas_cap = 0;
+        HIP_TRY(hipMalloc(&p->d_smeas, sizeof(int64_t) * (size_t)(n * per)));
+        p->smeas_cap = n * per;
+    }
+    std::vector<uint32_t> rows(kIsingTab * (size_t)n);
+    for (int64_t k = 0; k < n; ++k) ising_table(betas[k], p->opts.j, p->opts.h, &rows[kIsingTab * (size_t)k], p->dims);
+    ST_COPY(p, p->d_sched, rows.data(), sizeof(uint32_t) * rows.size(), hipMemcpyHostToDevice);
+    if (meas) ST_FILL(p, p->d_smeas, 0, sizeof(int64_t) * (size_t)(n * per));
+    p->d_sched_on = true;
+    p->sched_t0 = p->t_host;
+    int rc = ISING_OK;
+    for (int64_t k = 0; k < n && rc == ISING_OK; ++k) {
+        rc = p->enqueue_sweeps(1);
+        if (rc == ISING_OK && meas) {
+            const hipError_t e = launch_ising_measure(p->d_groups, (int)p->groups.size(), p->d_pool,
+                                                      p->d_smeas + k * per, p->nblocks, p->st);
+            if (e != hipSuccess) rc = fail(ISING_E_HIP, std::string("measure launch: ") + hipGetErrorString(e));
+        }
+    }
+    p->d_sched_on = false;
+    if (rc) return rc;
+    SYNC_ALL(p);
+    if (meas) {
+        std::vector<int64_t> v((size_t)(n * per));
+        ST_COPY(p, v.data(), p->d_smeas, sizeof(int64_t) * v.size(), hipMemcpyDeviceToHost);
+        for (int64_t k = 0; k < n; ++k)
+            for (int r = 0; r < p->total_reps; ++r) {
+                if (bond) bond[k * p->total_reps + r] = v[(size_t)(k * per + 2 * r)];
+                if (mag) mag[k * p->total_reps + r] = v[(size_t)(k * per + 2 * r + 1)];
+            }
+    }
+    return ISING_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The other LatticeGpu callers, as batched ensemble drivers
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// kibble_zurek.rs:78-84: inclusive linear ramp; the last sweep lands on t_end.
+double kz_ramp_temperature(double t_start, double t_end, int64_t step, int64_t tau_q) {
+    if (tau_q == 0) return t_end;
+    const double frac = (double)(step + 1) / (double)tau_q;
+    return t_start + (t_end - t_start) * frac;
+}
+
+// The reference's GPU callers pass beta to the device as f32 (kz_gpu.rs:25-35,
+// coarsening_gpu.rs:11-21, fss_gpu.rs:22): the same value is used here.
+double f32_beta(double T) { return (double)(float)(1.0 / T); }
+
+struct Handle {
+    ising_pt* p = nullptr;
+    ~Handle() { delete p; }
+};
+
+int one_size_ensemble(Handle& h, int n, int R, double j, double t, uint64_t seed) {
+    ising_pt_opts_t o;
+    ising_pt_default_opts(&o);
+    o.j = (double)(float)j;  // passed to the kernel as f32 in the reference
+    o.h = 0.0;
+    o.exchange_every = 1 << 30;
+    o.graph_sweeps = 0;
+    std::vector<double> temps((size_t)R, t);
+    std::vector<int32_t> sz(1, n);
+    return ising_pt_create(&h.p, ISING_MODEL_ISING, sz.data(), 1, R, temps.data(), seed, &o);
+}
+
+// Bond sum of every replica of the current state, on the device.
+int measure_bonds(ising_pt* p, int64_t* bond) {
+    const int64_t per = 2 * (int64_t)p->total_reps;
+    SYNC_ALL(p);
+    if (per > p->smeas_cap) {
+        (void)hipFree(p->d_smeas);
+        p->d_smeas = nullptr;
+        p->smeas_cap = 0;
+        HIP_TRY(hipMalloc(&p->d_smeas, sizeof(int64_t) * (size_t)per));
+        p->smeas_cap = per;
+    }
+    ST_FILL(p, p->d_smeas, 0, sizeof(int64_t) * (size_t)per);
+    HIP_TRY(launch_ising_measure(p->d_groups, (int)p->groups.size(), p->d_pool, p->d_smeas, p->nblocks, p->st));
+    std::vector<int64_t> v((size_t)per);
+    ST_COPY(p, v.data(), p->d_smeas, sizeof(int64_t) * v.size(), hipMemcpyDeviceToHost);
+    for (int r = 0; r < p->total_reps; ++r) bond[r] = v[(size_t)(2 * r)];
+    return ISING_OK;
+}
+
+int domain_walls(ising_pt* p, int n, const int64_t* bond, double* rho) {
+    const double bonds = 3.0 * (double)n * n * n;  // forward bonds (lattice_gpu.rs:254-280)
+    for (int r = 0; r < p->total_reps; ++r) rho[r] = (bonds - (double)bond[r]) / (2.0 * bonds);
+    return ISING_OK;
+}
+
+}  // namespace
+
+int ising_kz_run(int32_t n, double j, double t_start, double t_end, int64_t tau_q, int32_t n_trials,
+                 int64_t warmup_sweeps, int64_t freeze_sweeps, double freeze_temperature, uint64_t seed,
+                 double* rho) {
+    if (!rho || n_trials < 1 || tau_q < 0 || warmup_sweeps < 0 || freeze_sweeps < 0 || !(t_start > 0.0))
+        return fail(ISING_E_ARG, "need rho[n_trials], n_trials >= 1, tau_q/warmup/freeze >= 0, t_start > 0");
+    Handle h;
+    int rc = one_size_ensemble(h, n, n_trials, j, t_start, seed);
+    if (rc) return rc;
+    std::vector<double> betas;
+    betas.reserve((size_t)(warmup_sweeps + tau_q + freeze_sweeps + 1));
+    for (int64_t k = 0; k < warmup_sweeps; ++k) betas.push_back(f32_beta(t_start));        // kz_gpu.rs:18-24
+    for (int64_t k = 0; k < tau_q; ++k)                                                    // kz_gpu.rs:27-31
+        betas.push_back(f32_beta(std::max(kz_ramp_temperature(t_start, t_end, k, tau_q), 0.01)));
+    for (int64_t k = 0; k < freeze_sweeps; ++k) betas.push_back(f32_beta(std::max(freeze_temperature, 0.01)));
+    std::vector<int64_t> bond((size_t)n_trials);
+    rc = ising_pt_anneal(h.p, betas.data(), (int64_t)betas.size(), nullptr, nullptr);
+    if (rc) return rc;
+    rc = measure_bonds(h.p, bond.data());
+    if (rc) return rc;
+    return domain_walls(h.p, n, bond.data(), rho);
+}
+
+int ising_kz_sweep(int32_t n, double j, double t_start, double t_end, const int64_t* tau_q, int32_t n_tau,
+                   int32_t n_trials, int64_t warmup_sweeps, int64_t freeze_sweeps, double freeze_temperature,
+                   uint64_t base_seed, double* rho, double* rho_err) {
+    if (!tau_q || !rho || !rho_err || n_tau < 1) return fail(ISING_E_ARG, "need tau_q[n_tau], rho, rho_err");
+    std::vector<double> r((size_t)n_trials);
+    for (int k = 0; k < n_tau; ++k) {
+        // kibble_zurek.rs:102-106: trial seeds base + tau_q + trial; here the
+        // trials of one tau_q are replicas (Philox streams) of one ensemble keyed by base + tau_q
+        const uint64_t seed = base_seed + (uint64_t)tau_q[k];
+        const int rc = ising_kz_run(n, j, t_start, t_end, tau_q[k], n_trials, warmup_sweeps, freeze_sweeps,
+                                    freeze_temperature, seed, r.data());
+        if (rc) return rc;
+        // kibble_zurek.rs:86-100: mean and standard error (n - 1 variance)
+        double mean = 0.0;
+        for (double v : r) mean += v;
+        mean /= (double)n_trials;
+        double var = 0.0;
+        for (double v : r) var += (v - mean) * (v - mean);
+        rho[k] = mean;
+        rho_err[k] = n_trials > 1 ? std::sqrt(var / (double)(n_trials - 1)) / std::sqrt((double)n_trials) : 0.0;
+    }
+    return ISING_OK;
+}
+
+int ising_coarsening_run(int32_t n, double j, double t_high, double t_quench, int64_t warmup_sweeps,
+                         int64_t total_steps, int64_t sample_every, uint64_t seed, int32_t n_trials,
+                         int64_t* steps_out, double* rho_out, int64_t* n_samples) {
+    if (!steps_out || !rho_out || !n_samples || n_trials < 1 || sample_every < 1 || total_steps < 0 ||
+        warmup_sweeps < 0 || !(t_high > 0.0))
+        return fail(ISING_E_ARG, "need outputs, n_trials >= 1, sample_every >= 1, t_high > 0");
+    Handle h;
+    int rc = one_size_ensemble(h, n, n_trials, j, t_high, seed);
+    if (rc) return rc;
+    std::vector<double> warm((size_t)warmup_sweeps, f32_beta(t_high));  // coarsening_gpu.rs:9-13
+    rc = ising_pt_anneal(h.p, warm.data(), warmup_sweeps, nullptr, nullptr);
+    if (rc) return rc;
+    const double bq = f32_beta(std::max(t_quench, 0.01));               // coarsening_gpu.rs:15-16
+    int64_t ns = 0;
+    const int64_t chunk = 4096;
+    std::vector<double> betas((size_t)chunk, bq);
+    std::vector<int64_t> bond((size_t)(chunk * n_trials));
+    for (int64_t s0 = 0; s0 < total_steps; s0 += chunk) {
+        const int64_t m = std::min(chunk, total_steps - s0);
+        rc = ising_pt_anneal(h.p, betas.data(), m, bond.data(), nullptr);
+        if (rc) return rc;
+        for (int64_t k = 0; k < m; ++k) {
+            const int64_t step = s0 + k;
+            if (step % sample_every != 0) continue;                       // coarsening_gpu.rs:23-28
+            steps_out[ns] = step;
+            rc = domain_walls(h.p, n, &bond[(size_t)(k * n_trials)], &rho_out[(size_t)(ns * n_trials)]);
+            if (rc) return rc;
+            ++ns;
+        }
+    }
+    *n_samples = ns;
+    return ISING_OK;
+}
+
+int ising_fss_anneal(const int32_t* sizes, int32_t n_sizes, double j, double h, double t_min, double t_max,
+                     int32_t t_steps, int64_t warmup_sweeps, int64_t sample_sweeps, uint64_t seed,
+                     ising_observables_t* out) {
+    if (!sizes || !out || n_sizes < 1 || t_steps < 2 || sample_sweeps < 1 || warmup_sweeps < 0 ||
+        !(t_min > 0.0) || !(t_max > t_min))
+        return fail(ISING_E_ARG, "need sizes, out[n_sizes * t_steps], t_steps >= 2, samples >= 1, 0 < t_min < t_max");
+    ising_pt_opts_t o;
+    ising_pt_default_opts(&o);
+    o.j = (double)(float)j;  // the reference's kernel takes J and h as f32 (lattice_gpu.rs:85)
+    o.h = (double)(float)h;
+    o.exchange_every = 1 << 30;
+    o.graph_sweeps = 0;
+    std::vector<double> temps(1, t_max);
+    Handle hd;
+    int rc = ising_pt_create(&hd.p, ISING_MODEL_ISING, sizes, n_sizes, 1, temps.data(), seed, &o);
+    if (rc) return rc;
+    ising_pt* p = hd.p;
+    std::vector<int64_t> B((size_t)(sample_sweeps * n_sizes)), M((size_t)(sample_sweeps * n_sizes));
+    for (int32_t step = t_steps - 1; step >= 0; --step) {                 // fss_gpu.rs:17-33: high T to low T
+        const double T = t_min + (t_max - t_min) * (double)step / (double)(t_steps - 1);
+        const double beta = f32_beta(T);
+        std::vector<double> warm((size_t)warmup_sweeps, beta), meas((size_t)sample_sweeps, beta);
+        rc = ising_pt_anneal(p, warm.data(), warmup_sweeps, nullptr, nullptr);
+        if (rc) return rc;
+        rc = ising_pt_anneal(p, meas.data(), sample_sweeps, B.data(), M.data());
+        if (rc) return rc;
+        for (int g = 0; g < n_sizes; ++g) {                               // lattice_gpu.rs:193-252
+            const double n3 = (double)sizes[g] * sizes[g] * sizes[g], b = 1.0 / T;
+            double se = 0, se2 = 0, sm = 0, sm2 = 0, sm4 = 0, sms = 0, sms2 = 0;
+            for (int64_t k = 0; k < sample_sweeps; ++k) {
+                const double Bk = (double)B[(size_t)(k * n_sizes + g)], Mk = (double)M[(size_t)(k * n_sizes + g)];
+                const double e = (-j * Bk - h * Mk) / n3;
+                const double ms = Mk / n3, ma = std::fabs(ms);
+                se += e;
+                se2 += e * e;
+                sm += ma;
+                sm2 += ma * ma;
+                sm4 += ma * ma * ma * ma;
+                sms += ms;
+                sms2 += ms * ms;
+            }
+            const double s = (double)sample_sweeps;
+            const double ae = se / s, ae2 = se2 / s, ams = sms / s, ams2 = sms2 / s;
+            ising_observables_t& ob = out[(size_t)g * t_steps + step];
+            ob.temperature = T;
+            ob.energy = ae;
+            ob.magnetisation = sm / s;
+            ob.heat_capacity = b * b * (ae2 - ae * ae) * n3;
+            ob.susceptibility = b * (ams2 - ams * ams) * n3;
+            ob.m2 = sm2 / s;
+            ob.m4 = sm4 / s;
+        }
+    }
     return ISING_OK;
 }
 

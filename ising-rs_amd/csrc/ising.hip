@@ -302,8 +302,9 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
     }
     if (b == 0) sweep_bookkeeping(a, v, color);
     if (threadIdx.x < kIsingTab)
-        s_thr[threadIdx.x] =
-            a.inline_thr ? a.thr_inline[threadIdx.x] : a.thr[(int64_t)(G.rep0 + tidx) * kIsingTab + threadIdx.x];
+        s_thr[threadIdx.x] = a.sched        ? a.sched[(v.t - a.sched_t0) * kIsingTab + threadIdx.x]
+                             : a.inline_thr ? a.thr_inline[threadIdx.x]
+                                            : a.thr[(int64_t)(G.rep0 + tidx) * kIsingTab + threadIdx.x];
     if (threadIdx.x == 0) s_qn = 0;
     const int64_t t = v.t;
     const bool need = (v.flags & (F_MEAS | F_EXCH)) != 0;

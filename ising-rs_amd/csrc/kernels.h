@@ -25,6 +25,8 @@ struct IsingArgs {
     const uint32_t* mthr;         // multispin coding: [rep0 + tidx][3] thresholds for a = 4, 5, 6
     uint32_t* mplanes;            // multispin coding: per word, 3 x 32 threshold bit planes of its replicas
     int32_t acc_copies;           // accumulator copies (spread atomics), summed by the closing kernel
+    const uint32_t* sched;        // annealing: one table row per sweep for every replica (null: per temperature)
+    int64_t sched_t0;             // sweep index of schedule row 0
 };
 
 constexpr int kMscAccCopies = 16;

@@ -112,6 +112,15 @@ _SIGS = {
     "ising_pt_sweeps": (C.c_int, [_vp, _i64p]),
     "ising_pt_device_bytes": (C.c_int, [_vp, _i64p]),
     "ising_pt_autotune": (C.c_int, [_vp, C.c_int32] + [C.POINTER(C.c_int32)] * 3),
+    "ising_pt_anneal": (C.c_int, [_vp, _vp, C.c_int64, _vp, _vp]),
+    "ising_kz_run": (C.c_int, [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_int32, C.c_int64,
+                               C.c_int64, C.c_double, C.c_uint64, _vp]),
+    "ising_kz_sweep": (C.c_int, [C.c_int32, C.c_double, C.c_double, C.c_double, _vp, C.c_int32, C.c_int32,
+                                 C.c_int64, C.c_int64, C.c_double, C.c_uint64, _vp, _vp]),
+    "ising_coarsening_run": (C.c_int, [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_int64,
+                                       C.c_int64, C.c_uint64, C.c_int32, _vp, _vp, _i64p]),
+    "ising_fss_anneal": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32,
+                                   C.c_int64, C.c_int64, C.c_uint64, C.POINTER(Observables)]),
     "ising_batch_create": (C.c_int, [C.POINTER(_vp), _vp, C.c_int32, C.c_int32]),
     "ising_batch_destroy": (None, [_vp]),
     "ising_batch_warmup": (C.c_int, [_vp, C.c_int64]),
@@ -429,6 +438,18 @@ class PTEnsemble:
     def metropolis_pass(self, color, t):
         _check(lib.ising_pt_metropolis_pass(self._h, int(color), int(t)))
 
+    def anneal(self, betas, measure=False):
+        """Warm-up sweeps with inverse temperature betas[k] at sweep k for every
+        replica; with measure, returns ([n, total replicas] bond sums, mags)."""
+        b = np.ascontiguousarray(betas, np.float64)
+        if not measure:
+            _check(lib.ising_pt_anneal(self._h, _ptr(b), b.size, None, None))
+            return None
+        bond = np.empty((b.size, self.total_replicas), np.int64)
+        mag = np.empty_like(bond)
+        _check(lib.ising_pt_anneal(self._h, _ptr(b), b.size, _ptr(bond), _ptr(mag)))
+        return bond, mag
+
     def autotune(self, sweeps=20):
         """Times the int8 thread-step layouts over warm-up sweeps and keeps the
         fastest; returns (run width, runs per step, steps per tile)."""
@@ -474,6 +495,46 @@ class Batch:
         ms = C.c_double()
         _check(lib.ising_batch_sync(self._h, C.byref(ms)))
         return ms.value
+
+
+def kz_run(n, j, t_start, t_end, tau_q, n_trials, warmup_sweeps=200, freeze_sweeps=0, freeze_temperature=0.01,
+           seed=42):
+    """One Kibble-Zurek quench of n_trials lattices (kz_gpu.rs:9-45) -> domain-wall density per trial."""
+    rho = np.empty(n_trials, np.float64)
+    _check(lib.ising_kz_run(n, j, t_start, t_end, tau_q, n_trials, warmup_sweeps, freeze_sweeps,
+                            freeze_temperature, seed, _ptr(rho)))
+    return rho
+
+
+def run_kz_sweep(n, j, t_start, t_end, tau_q_values, n_trials, warmup_sweeps=200, freeze_sweeps=0,
+                 freeze_temperature=0.01, base_seed=42):
+    """run_kz_sweep_gpu (kz_gpu.rs:49-95) -> list of (tau_q, rho, rho_err, n_trials)."""
+    tq = np.ascontiguousarray(tau_q_values, np.int64)
+    rho, err = np.empty(tq.size), np.empty(tq.size)
+    _check(lib.ising_kz_sweep(n, j, t_start, t_end, _ptr(tq), tq.size, n_trials, warmup_sweeps, freeze_sweeps,
+                              freeze_temperature, base_seed, _ptr(rho), _ptr(err)))
+    return [(int(t), float(r), float(e), n_trials) for t, r, e in zip(tq, rho, err)]
+
+
+def run_coarsening(n, j=1.0, t_high=10.0, t_quench=0.5, warmup_sweeps=200, total_steps=50000, sample_every=100,
+                   seed=42, n_trials=1):
+    """run_coarsening_gpu (coarsening_gpu.rs:6-31) -> (steps, rho[sample, trial])."""
+    cap = (total_steps + sample_every - 1) // sample_every
+    steps = np.empty(max(cap, 1), np.int64)
+    rho = np.empty((max(cap, 1), n_trials), np.float64)
+    ns = C.c_int64()
+    _check(lib.ising_coarsening_run(n, j, t_high, t_quench, warmup_sweeps, total_steps, sample_every, seed, n_trials,
+                                    _ptr(steps), _ptr(rho), C.byref(ns)))
+    return steps[:ns.value], rho[:ns.value]
+
+
+def run_fss(sizes, j=1.0, h=0.0, t_min=4.0, t_max=5.0, t_steps=11, warmup_sweeps=1000, sample_sweeps=1000, seed=42):
+    """run_fss_gpu (fss_gpu.rs:7-39) -> list of (n, [Observables ascending in T])."""
+    sz = np.ascontiguousarray(sizes, np.int32)
+    out = (Observables * (len(sizes) * t_steps))()
+    _check(lib.ising_fss_anneal(_ptr(sz), sz.size, j, h, t_min, t_max, t_steps, warmup_sweeps, sample_sweeps, seed,
+                                out))
+    return [(int(L), [out[g * t_steps + k] for k in range(t_steps)]) for g, L in enumerate(sizes)]
 
 
 def read_timeseries(path):
