@@ -241,6 +241,41 @@ def run_learned(args, ising_hip):
     print(json.dumps(line), flush=True)
 
 
+def run_driver(args, ising_hip):
+    """The other LatticeGpu callers as batched drivers (int8 Ising, one launch
+    per colour for every trial / size): Kibble-Zurek sweep, coarsening and
+    annealed FSS at production-like sizes.  Metric: spin updates/ns over the
+    whole driver call (host-side schedule uploads and reductions included)."""
+    w = args.workload
+    t0 = time.perf_counter()
+    if w == "kz":
+        n, trials, taus, warm = 64, 64, [250, 500, 1000], 200
+        pts = ising_hip.run_kz_sweep(n, 1.0, 6.0, 1.0, taus, trials, warm, 0, 0.01, base_seed=args.seed)
+        updates = sum(n ** 3 * trials * (warm + t) for t in taus)
+        desc = "Kibble-Zurek sweep: L=64, 64 trials per tau_q, tau_q = 250, 500, 1000, 200 warm-up sweeps at T=6"
+        extra = {"rho": [round(p[1], 6) for p in pts]}
+    elif w == "coarsening":
+        n, trials, total, warm = 128, 16, 2000, 200
+        steps, rho = ising_hip.run_coarsening(n, 1.0, 10.0, 0.5, warm, total, 100, args.seed, trials)
+        updates = n ** 3 * trials * (warm + total)
+        desc = "coarsening: L=128, 16 trials, 200 sweeps at T=10, quench to T=0.5, 2000 steps, sample every 100"
+        extra = {"rho_last": round(float(rho[-1].mean()), 6)}
+    else:
+        sizes, tsteps, warm, samp = [16, 32, 64, 128], 8, 200, 200
+        res = ising_hip.run_fss(sizes, 1.0, 0.0, 4.3, 4.7, tsteps, warm, samp, args.seed)
+        updates = sum(L ** 3 for L in sizes) * tsteps * (warm + samp)
+        desc = "annealed FSS: L=16,32,64,128 together, 8 temperatures 4.7 -> 4.3, 200 warm-up + 200 measured sweeps each"
+        extra = {"binder_L128": [round(1 - o.m4 / (3 * o.m2 * o.m2), 4) for o in res[-1][1]]}
+    wall = time.perf_counter() - t0
+    line = {"metric": "spin updates/ns", "value": round(updates / (wall * 1e9), 3), "unit": "updates/ns",
+            "n_gpus": 1, "steps": None, "warmup": None, "ms_per_step": None, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+            "config": dict({"workload": desc, "updates": updates, "seconds": round(wall, 3)}, **extra)}
+    if args.cpu_seconds > 0:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds, "ising")
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,7 +283,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--workload", default="ising", choices=sorted(WORKLOADS) + ["batched", "learned"])
+    ap.add_argument("--workload", default="ising", choices=sorted(WORKLOADS) + ["batched", "learned", "kz", "coarsening", "fss"])
     ap.add_argument("--hidden", type=int, default=16, help="learned workload: MLP hidden width (8, 16, 32 or 64)")
     ap.add_argument("--train-steps", type=int, default=400, help="learned workload: Adam steps")
     ap.add_argument("--layout", default="auto", metavar="W,R,S",
@@ -275,6 +310,9 @@ def main():
         return
     if args.workload == "learned":
         run_learned(args, ising_hip)
+        return
+    if args.workload in ("kz", "coarsening", "fss"):
+        run_driver(args, ising_hip)
         return
     model, sizes, (tlo, thi), bpu, dtype, opts, desc = WORKLOADS[args.workload]
     temps = linspace(tlo, thi, R)

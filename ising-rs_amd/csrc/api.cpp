@@ -291,7 +291,7 @@ struct ising_pt {
         return n > 0 ? enqueue_sweeps(n) : ISING_OK;
     }
 
-    int enqueue_sweeps(int64_t n, int n_or = -1) {
+    int enqueue_sweeps(int64_t n, int n_or = -1, bool close_last = true) {
         if (model == ISING_MODEL_ISING) {
             // each colour-0 launch closes the sweep before it; the closing
             // kernel closes the last one (see sweepctl.h)
@@ -300,14 +300,14 @@ struct ising_pt {
                 HIP_TRY(launch_ising_half(a, nblocks, 0, st));
                 HIP_TRY(launch_ising_half(a, nblocks, 1, st));
             }
-            if (n > 0) HIP_TRY(launch_ising_close(a, st));
+            if (n > 0 && close_last) HIP_TRY(launch_ising_close(a, st));
         } else if (model == ISING_MODEL_ISING_MSC) {
             const IsingArgs a = ising_args();
             for (int64_t i = 0; i < n; ++i) {
                 HIP_TRY(launch_msc_half(a, nblocks, 0, st));
                 HIP_TRY(launch_msc_half(a, nblocks, 1, st));
             }
-            if (n > 0) HIP_TRY(launch_ising_close(a, st));
+            if (n > 0 && close_last) HIP_TRY(launch_ising_close(a, st));
         } else {
             // overrelaxation passes first, then one Metropolis sweep
             // (the order of ContinuousGpuLattice::sweep, gpu_lattice_continuous.rs:124-168)
@@ -1158,12 +1158,17 @@ int ising_pt_anneal(ising_pt_t* p, const double* betas, int64_t n, int64_t* bond
     p->d_sched_on = true;
     p->sched_t0 = p->t_host;
     int rc = ISING_OK;
-    for (int64_t k = 0; k < n && rc == ISING_OK; ++k) {
-        rc = p->enqueue_sweeps(1);
-        if (rc == ISING_OK && meas) {
-            const hipError_t e = launch_ising_measure(p->d_groups, (int)p->groups.size(), p->d_pool,
-                                                      p->d_smeas + k * per, p->nblocks, p->st);
-            if (e != hipSuccess) rc = fail(ISING_E_HIP, std::string("measure launch: ") + hipGetErrorString(e));
+    if (!meas) {
+        rc = p->enqueue_sweeps(n);
+    } else {
+        // each colour-0 launch closes the sweep before it; one closing kernel at the end
+        for (int64_t k = 0; k < n && rc == ISING_OK; ++k) {
+            rc = p->enqueue_sweeps(1, -1, k + 1 == n);
+            if (rc == ISING_OK) {
+                const hipError_t e = launch_ising_measure(p->d_groups, (int)p->groups.size(), p->d_pool,
+                                                          p->d_smeas + k * per, p->nblocks, p->st);
+                if (e != hipSuccess) rc = fail(ISING_E_HIP, std::string("measure launch: ") + hipGetErrorString(e));
+            }
         }
     }
     p->d_sched_on = false;
@@ -1302,20 +1307,20 @@ int ising_coarsening_run(int32_t n, double j, double t_high, double t_quench, in
     if (rc) return rc;
     const double bq = f32_beta(std::max(t_quench, 0.01));               // coarsening_gpu.rs:15-16
     int64_t ns = 0;
-    const int64_t chunk = 4096;
-    std::vector<double> betas((size_t)chunk, bq);
-    std::vector<int64_t> bond((size_t)(chunk * n_trials));
-    for (int64_t s0 = 0; s0 < total_steps; s0 += chunk) {
-        const int64_t m = std::min(chunk, total_steps - s0);
-        rc = ising_pt_anneal(h.p, betas.data(), m, bond.data(), nullptr);
+    std::vector<double> betas((size_t)sample_every, bq);
+    std::vector<int64_t> bond((size_t)n_trials);
+    for (int64_t step = 0; step < total_steps; step += sample_every) {
+        // sample after step (coarsening_gpu.rs:23-28), then run to the next sample point
+        rc = ising_pt_anneal(h.p, betas.data(), 1, bond.data(), nullptr);
         if (rc) return rc;
-        for (int64_t k = 0; k < m; ++k) {
-            const int64_t step = s0 + k;
-            if (step % sample_every != 0) continue;                       // coarsening_gpu.rs:23-28
-            steps_out[ns] = step;
-            rc = domain_walls(h.p, n, &bond[(size_t)(k * n_trials)], &rho_out[(size_t)(ns * n_trials)]);
+        steps_out[ns] = step;
+        rc = domain_walls(h.p, n, bond.data(), &rho_out[(size_t)(ns * n_trials)]);
+        if (rc) return rc;
+        ++ns;
+        const int64_t rest = std::min(sample_every, total_steps - step) - 1;
+        if (rest > 0) {
+            rc = ising_pt_anneal(h.p, betas.data(), rest, nullptr, nullptr);
             if (rc) return rc;
-            ++ns;
         }
     }
     *n_samples = ns;
