@@ -199,10 +199,27 @@ def run_learned(args, ising_hip):
             pt.learn_enable(True)
             pt.warmup(50)
         pt.sync()
-        t0 = time.perf_counter()
-        pt.sample(steps)
-        pt.sync()
-        wall = time.perf_counter() - t0
+        # sample in chunks; between chunks, the XY helicity sums per replica
+        # (outside the timed sampling)
+        chunk, wall, done = 10, 0.0, 0
+        hc = np.zeros((len(sizes), Rr))
+        hj2 = np.zeros((len(sizes), Rr))
+        nh = 0
+        while done < steps:
+            m = min(chunk, steps - done)
+            t0 = time.perf_counter()
+            pt.sample(m)
+            pt.sync()
+            wall += time.perf_counter() - t0
+            done += m
+            hel = pt.helicity()
+            for g in range(len(sizes)):
+                r2t, _ = pt.get_perm(g)
+                hg = hel[g * Rr:(g + 1) * Rr]
+                hc[g, r2t] += hg[:, :3].mean(axis=1)
+                hj2[g, r2t] += (hg[:, 3:] ** 2).mean(axis=1)
+            nh += 1
+        ups = [(hc[g] / nh - np.array([1.0 / t for t in temps]) * hj2[g] / nh) / L ** 3 for g, L in enumerate(sizes)]
         tidx, e, mx, my, mz = pt.read_cont()
         es = temperature_series(tidx, e)
         m2 = temperature_series(tidx, mx * mx + my * my)
@@ -214,7 +231,12 @@ def run_learned(args, ising_hip):
         taus_e, taus_m = np.array(taus_e), np.array(taus_m)
         isps = float(np.sum(steps / (2.0 * np.maximum(taus_e, 0.5))) / wall)
         tries, acc = pt.mh_stats() if mode == "learned" else (0, 0)
+        m2s = temperature_series(tidx, (mx * mx + my * my))
+        u4 = [float(np.mean(1 - (m2s[:, g, :] ** 2).mean(axis=0) / (3 * m2s[:, g, :].mean(axis=0) ** 2)))
+              for g in range(len(sizes))]
         out[mode] = {"sweeps_per_s": round(steps / wall, 2), "tau_int_E_mean": round(float(taus_e.mean()), 3),
+                     "helicity_modulus_mean": [round(float(u.mean()), 5) for u in ups],
+                     "binder_u4_mean": [round(v, 5) for v in u4],
                      "tau_int_M2_mean": round(float(taus_m.mean()), 3), "independent_samples_per_s": round(isps, 1),
                      "learned_acceptance": (round(acc / tries, 4) if tries else None), "training": train}
         pt.close()

@@ -225,6 +225,13 @@ int ising_pt_mh_stats(ising_pt_t* pt, int64_t* tries, int64_t* accepts);
  * exchange or advancing the sweep counter (used by exactness tests). */
 int ising_pt_metropolis_pass(ising_pt_t* pt, int32_t color, int64_t t);
 /* Sweeps applied so far (warm-up + sample). */
+/* XY spin-stiffness sums of every replica's current state (no counterpart in
+ * the reference): out[6 * replica + k] = C_x, C_y, C_z, J_x, J_y, J_z with, over
+ * the forward bonds (i, i + d), C_d = sum cos(theta_i - theta_j) and
+ * J_d = sum sin(theta_i - theta_j).  The helicity modulus along d at inverse
+ * temperature beta is (<C_d> - beta <J_d^2>) / N. */
+int ising_pt_helicity(ising_pt_t* pt, double* out);
+
 /* Annealing schedule (int8 Ising ensembles, before sampling): n sweeps, sweep
  * k with inverse temperature betas[k] for every replica (acceptance tables
  * built on the host in FP64 as for the ensemble).  When bond / mag are not

@@ -1124,6 +1124,31 @@ int ising_pt_metropolis_pass(ising_pt_t* p, int32_t color, int64_t t) {
     return ISING_OK;
 }
 
+int ising_pt_helicity(ising_pt_t* p, double* out) {
+    if (!p || !out) return fail(ISING_E_ARG, "null argument");
+    if (p->model != ISING_MODEL_XY) return fail(ISING_E_STATE, "the helicity sums are defined for XY ensembles");
+    SYNC_ALL(p);
+    std::vector<double> part(6 * (size_t)p->nblocks);
+    double* d_part = nullptr;
+    HIP_TRY(hipMalloc(&d_part, sizeof(double) * part.size()));
+    hipError_t e = launch_cont_helicity(p->d_groups, (int)p->groups.size(), p->d_fpool, d_part, p->nblocks, p->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(part.data(), d_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, p->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->st);
+    (void)hipFree(d_part);
+    if (e != hipSuccess) return fail(ISING_E_HIP, std::string("helicity: ") + hipGetErrorString(e));
+    // blocks of a replica are consecutive: sum them in block order
+    for (const DevGroup& G : p->groups)
+        for (int r = 0; r < G.R; ++r) {
+            double* o = out + 6 * (size_t)(G.rep0 + r);
+            for (int k = 0; k < 6; ++k) o[k] = 0.0;
+            for (int t = 0; t < G.tiles; ++t) {
+                const double* q = &part[6 * (size_t)(G.blk0 + (int64_t)r * G.tiles + t)];
+                for (int k = 0; k < 6; ++k) o[k] += q[k];
+            }
+        }
+    return ISING_OK;
+}
+
 int ising_pt_anneal(ising_pt_t* p, const double* betas, int64_t n, int64_t* bond, int64_t* mag) {
     if (!p || (!betas && n > 0)) return fail(ISING_E_ARG, "null argument");
     if (n < 0) return fail(ISING_E_ARG, "n must be >= 0");

@@ -403,6 +403,52 @@ __global__ __launch_bounds__(kCT) void cont_measure_kernel(const DevGroup* group
         for (int k = 0; k < 4; ++k) part[4 * (int64_t)blockIdx.x + k] = v[k];
 }
 
+// XY spin stiffness sums of every replica: over the forward bonds (i, i + d),
+// d = x, y, z, C_d = sum cos(theta_i - theta_j) = S_i . S_j and
+// J_d = sum sin(theta_i - theta_j) = S_i^y S_j^x - S_i^x S_j^y, as
+// part[6 * blk + {C_x, C_y, C_z, J_x, J_y, J_z}] block partials (FP64).
+// The helicity modulus along d is (<C_d> - beta <J_d^2>) / N.
+__global__ __launch_bounds__(kCT) void cont_helicity_kernel(const DevGroup* groups, int ngroups, const float* pool,
+                                                          double* part) {
+    __shared__ double s_red[4][kCT / 64];
+    const int g = find_group_c(groups, ngroups, blockIdx.x);
+    const DevGroup G = groups[g];
+    int r, tile;
+    split_tiles(G, (int)blockIdx.x - G.blk0, &r, &tile);
+    const int64_t h = (int64_t)tile * kCT + threadIdx.x;
+    double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (h < G.n_half) {
+        for (int c = 0; c < 2; ++c) {
+            const float* Ci = pool + G.off + (int64_t)(2 * r + c) * 2 * G.stride;
+            const float* Cj = pool + G.off + (int64_t)(2 * r + (c ^ 1)) * 2 * G.stride;
+            int64_t nb[6];
+            neighbours(G, h, c, nb);
+            const int64_t row = h / G.Lh;
+            const int z = (int)(row / G.L), y = (int)(row - (int64_t)z * G.L);
+            const int p = (y + z + c) & 1;
+            const int64_t fwd[3] = {p ? nb[1] : nb[0], nb[3], nb[5]};  // +x, +y, +z
+            const double xi = (double)Ci[h], yi = (double)Ci[G.stride + h];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double xj = (double)Cj[fwd[d]], yj = (double)Cj[G.stride + fwd[d]];
+                v[d] += xi * xj + yi * yj;
+                v[3 + d] += yi * xj - xi * yj;
+            }
+        }
+    }
+    block_sum4(v, s_red);
+    __syncthreads();
+    block_sum4(v + 4, s_red);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; ++k) part[6 * (int64_t)blockIdx.x + k] = v[k];
+}
+
+hipError_t launch_cont_helicity(const DevGroup* groups, int ngroups, const float* pool, double* part, int nblocks,
+                                hipStream_t st) {
+    hipLaunchKernelGGL(cont_helicity_kernel, dim3(nblocks), dim3(kCT), 0, st, groups, ngroups, pool, part);
+    return hipGetLastError();
+}
+
 hipError_t launch_cont_metro(const ContArgs& a, int nblocks, int color, int closes, hipStream_t st) {
     if (a.ncomp == 2) hipLaunchKernelGGL(cont_metro_kernel<2>, dim3(nblocks), dim3(kCT), 0, st, a, color, closes);
     else hipLaunchKernelGGL(cont_metro_kernel<3>, dim3(nblocks), dim3(kCT), 0, st, a, color, closes);

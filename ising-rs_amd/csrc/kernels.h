@@ -83,6 +83,8 @@ hipError_t launch_cont_close(const ContArgs& a, hipStream_t st);
 hipError_t launch_cont_or(const ContArgs& a, int nblocks, int color, hipStream_t st);
 hipError_t launch_cont_init(const DevGroup* groups, int ngroups, float* pool, int ncomp, int model, int nblocks,
                             uint32_t k0, uint32_t k1, hipStream_t st);
+hipError_t launch_cont_helicity(const DevGroup* groups, int ngroups, const float* pool, double* part, int nblocks,
+                                hipStream_t st);
 hipError_t launch_cont_measure(const DevGroup* groups, int ngroups, const float* pool, int ncomp, double J,
                                double* part, int nblocks, hipStream_t st);
 constexpr int kContTile = 256;  // half-sites per block in continuous launches

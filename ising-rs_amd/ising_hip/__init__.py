@@ -112,6 +112,7 @@ _SIGS = {
     "ising_pt_sweeps": (C.c_int, [_vp, _i64p]),
     "ising_pt_device_bytes": (C.c_int, [_vp, _i64p]),
     "ising_pt_autotune": (C.c_int, [_vp, C.c_int32] + [C.POINTER(C.c_int32)] * 3),
+    "ising_pt_helicity": (C.c_int, [_vp, _vp]),
     "ising_pt_anneal": (C.c_int, [_vp, _vp, C.c_int64, _vp, _vp]),
     "ising_kz_run": (C.c_int, [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_int32, C.c_int64,
                                C.c_int64, C.c_double, C.c_uint64, _vp]),
@@ -437,6 +438,12 @@ class PTEnsemble:
 
     def metropolis_pass(self, color, t):
         _check(lib.ising_pt_metropolis_pass(self._h, int(color), int(t)))
+
+    def helicity(self):
+        """[total replicas, 6] XY stiffness sums C_x, C_y, C_z, J_x, J_y, J_z of the current state."""
+        out = np.empty((self.total_replicas, 6), np.float64)
+        _check(lib.ising_pt_helicity(self._h, _ptr(out)))
+        return out
 
     def anneal(self, betas, measure=False):
         """Warm-up sweeps with inverse temperature betas[k] at sweep k for every

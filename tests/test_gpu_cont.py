@@ -161,3 +161,36 @@ def test_delta_adaptation_runs_in_warmup_only():
     assert np.all(d < 2.5) and np.all(d > 0.05)
     pt.sample(50)
     assert np.array_equal(pt.get_deltas(0), d)  # frozen while sampling
+
+
+def _helicity_numpy(sp, L):
+    s = sp.reshape(L, L, L, 2).astype(np.float64)  # [z, y, x, comp]
+    out = []
+    for ax in (2, 1, 0):  # +x, +y, +z
+        t = np.roll(s, -1, axis=ax)
+        out.append((s * t).sum())
+    for ax in (2, 1, 0):
+        t = np.roll(s, -1, axis=ax)
+        out.append((s[..., 1] * t[..., 0] - s[..., 0] * t[..., 1]).sum())
+    return np.array(out)
+
+
+def test_xy_helicity_sums_closed_forms_and_random_state():
+    L, R = 8, 3
+    pt = ising_hip.PTEnsemble("xy", [L], R, [2.2] * R, 4)
+    n = L ** 3
+    z, y, x = np.meshgrid(np.arange(L), np.arange(L), np.arange(L), indexing="ij")
+    aligned = np.stack([np.ones(n), np.zeros(n)], axis=1).astype(np.float32).reshape(-1)
+    theta = 2 * np.pi * 1 * x.reshape(-1) / L  # one full twist along x
+    twisted = np.stack([np.cos(theta), np.sin(theta)], axis=1).astype(np.float32).reshape(-1)
+    pt.set_spins(0, 0, aligned)
+    pt.set_spins(0, 1, twisted)
+    pt.sweeps()
+    hel = pt.helicity()
+    assert np.allclose(hel[0], [n, n, n, 0, 0, 0], atol=1e-9)
+    c, s_ = np.cos(2 * np.pi / L), np.sin(-2 * np.pi / L)
+    assert np.allclose(hel[1], [n * c, n, n, n * s_, 0, 0], rtol=1e-6, atol=1e-6)
+    rnd = pt.get_spins(0, 2)  # the random start
+    want = _helicity_numpy(rnd, L)
+    assert np.allclose(hel[2], want, rtol=1e-10, atol=1e-9)
+    assert np.allclose(hel[1], _helicity_numpy(pt.get_spins(0, 1), L), rtol=1e-10, atol=1e-9)
