@@ -240,6 +240,41 @@ int ising_pt_helicity(ising_pt_t* pt, double* out);
 int ising_pt_anneal(ising_pt_t* pt, const double* betas, int64_t n, int64_t* bond, int64_t* mag);
 
 /* ------------------------------------------------------------------------
+ * Cluster updates: the GPU Wolff the reference plans as a label-propagation
+ * step (docs/plans/2026-03-09-gpu-compute-optimizations-design.md:82-116;
+ * WolffGpuLattice, gpu-compute-optimizations-impl.md:1041-1330), with
+ * the CPU algorithms of src/wolff.rs:27-77 (Ising) and src/xy/wolff.rs:16-73
+ * (XY embedding).  One bond configuration per update (Fortuin-Kasteleyn bonds
+ * with probability 1 - exp(-2 beta |J|) between satisfied neighbours; for
+ * continuous spins 1 - exp(-2 beta J a_i a_j) with a = s . r along a random
+ * unit vector r per replica and update), labelled on the device by union-find;
+ * then either every cluster flips with probability 1/2 (Swendsen-Wang) or the
+ * cluster of one random seed site flips (Wolff).  Random draws are Philox,
+ * keyed by (seed, update index, size, replica): results are identical for any
+ * tiling.  Ising needs h = 0 and int8 storage (not ISING_MODEL_ISING_MSC).
+ * ------------------------------------------------------------------------ */
+#define ISING_CLUSTER_SW 0
+#define ISING_CLUSTER_WOLFF 1
+
+/* n cluster updates of every replica of an int8 Ising ensemble (3D or 2D) at
+ * its current temperature; may be interleaved with warm-up / sample calls.
+ * When not NULL, flipped / bond / mag receive [n][total replicas] int64:
+ * sites flipped by each update, then the bond sum and magnetisation after it. */
+int ising_pt_cluster(ising_pt_t* pt, int32_t kind, int64_t n, int64_t* flipped, int64_t* bond, int64_t* mag);
+/* The same for XY / Heisenberg ensembles; meas (when not NULL) receives
+ * [n][total replicas][4] doubles E, Mx, My, Mz after each update. */
+int ising_pt_cluster_cont(ising_pt_t* pt, int32_t kind, int64_t n, int64_t* flipped, double* meas);
+/* Cluster updates applied so far (the update index of the next one). */
+int ising_pt_cluster_count(const ising_pt_t* pt, int64_t* out);
+/* WolffGpuLattice::step(beta, j, rng) (gpu-compute-optimizations-impl.md:1150-1248)
+ * on a single lattice: one update of `kind`; *flipped (when not NULL) receives
+ * the cluster size (Wolff) or the number of sites flipped (Swendsen-Wang).
+ * Measurement, warm-up and spins use the lattice's other calls. */
+int ising_lattice_cluster_step(ising_lattice_t* lat, int32_t kind, float beta, float j, int64_t* flipped);
+/* The same for a single XY / Heisenberg lattice. */
+int ising_cont_cluster_step(ising_cont_t* lat, int32_t kind, float beta, float j, int64_t* flipped);
+
+/* ------------------------------------------------------------------------
  * The other LatticeGpu callers as batched drivers: every trial / size runs as
  * a replica / group of one ensemble, so one launch per colour advances all.
  * ------------------------------------------------------------------------ */

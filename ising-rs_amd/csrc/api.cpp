@@ -153,6 +153,18 @@ struct ising_pt {
     int64_t sched_t0 = 0;
     int64_t* d_smeas = nullptr;    // annealing: per-sweep B, M of every replica
     int64_t smeas_cap = 0;         // int64s allocated
+    // cluster updates (allocated at the first call)
+    std::vector<ClusterGroup> cgroups;
+    ClusterGroup* d_cg = nullptr;
+    int cl_blocks = 0;
+    int32_t* d_clabel = nullptr;
+    uint8_t* d_cbond = nullptr;
+    uint32_t* d_cbthr = nullptr;
+    int64_t* d_cflip = nullptr;   // [update][global replica] sites flipped
+    int64_t cflip_cap = 0;
+    double* d_cmeas = nullptr;    // continuous: [update][global replica][4] E, Mx, My, Mz
+    int64_t cmeas_cap = 0;
+    int64_t c_host = 0;           // cluster updates enqueued (Philox c1 of the next one)
     int64_t t_host = 0;        // sweeps enqueued
     int64_t s_host = -1;       // sample sweeps enqueued (-1 before sampling starts)
     int64_t pending_host = 0;  // measured sweeps the device will have written
@@ -189,6 +201,12 @@ struct ising_pt {
         (void)hipFree(d_lgpart);
         (void)hipFree(d_lgout);
         (void)hipFree(d_llpart);
+        (void)hipFree(d_cg);
+        (void)hipFree(d_clabel);
+        (void)hipFree(d_cbond);
+        (void)hipFree(d_cbthr);
+        (void)hipFree(d_cflip);
+        (void)hipFree(d_cmeas);
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (ext_ev) (void)hipEventDestroy(ext_ev);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -1212,6 +1230,176 @@ int ising_pt_anneal(ising_pt_t* p, const double* betas, int64_t n, int64_t* bond
 }
 
 // ---------------------------------------------------------------------------
+// Cluster updates (Swendsen-Wang, Wolff)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// Tiles, label and bond arrays, allocated at the first cluster update.
+int cluster_setup(ising_pt* p) {
+    if (p->d_cg) return ISING_OK;
+    std::vector<ClusterGroup> cg;
+    int64_t lab = 0, blk = 0;
+    for (const DevGroup& G : p->groups) {
+        ClusterGroup C{};
+        C.L = G.L;
+        C.dims = G.dims;
+        C.R = G.R;
+        C.rep0 = G.rep0;
+        C.tx = std::min(G.dims == 3 ? 32 : 64, G.L);
+        C.ty = std::min(G.dims == 3 ? 8 : 32, G.L);
+        C.tz = G.dims == 3 ? std::min(8, G.L) : 1;
+        C.ntx = (G.L + C.tx - 1) / C.tx;
+        C.nty = (G.L + C.ty - 1) / C.ty;
+        C.ntz = G.dims == 3 ? (G.L + C.tz - 1) / C.tz : 1;
+        C.tiles = C.ntx * C.nty * C.ntz;
+        C.nsites = G.dims == 3 ? (int64_t)G.L * G.L * G.L : (int64_t)G.L * G.L;
+        if (C.nsites >= (1ll << 31)) return fail(ISING_E_ARG, "cluster updates need L^d < 2^31 sites");
+        C.blk0 = (int32_t)blk;
+        blk += (int64_t)G.R * C.tiles;
+        C.lab0 = lab;
+        lab += (int64_t)G.R * C.nsites;
+        C.q4 = (G.L + 3) / 4;
+        cg.push_back(C);
+    }
+    if (blk >= (1ll << 31)) return fail(ISING_E_ARG, "too many cluster tiles");
+    HIP_TRY(hipMalloc(&p->d_cg, sizeof(ClusterGroup) * cg.size()));
+    HIP_TRY(hipMalloc(&p->d_clabel, sizeof(int32_t) * (size_t)lab));
+    HIP_TRY(hipMalloc(&p->d_cbond, (size_t)lab));
+    HIP_TRY(hipMalloc(&p->d_cbthr, sizeof(uint32_t) * (size_t)p->total_reps));
+    ST_COPY(p, p->d_cg, cg.data(), sizeof(ClusterGroup) * cg.size(), hipMemcpyHostToDevice);
+    p->cgroups = cg;
+    p->cl_blocks = (int)blk;
+    return ISING_OK;
+}
+
+// Bond threshold floor(2^32 p), p = 1 - exp(-2 beta |J|), at most 2^32 - 1.
+uint32_t bond_threshold(double beta, double J) {
+    const double pa = -std::expm1(-2.0 * beta * std::fabs(J));
+    if (!(pa > 0.0)) return 0u;
+    const double t = std::floor(std::ldexp(pa, 32));
+    return t >= 4294967295.0 ? 0xffffffffu : (uint32_t)t;
+}
+
+int grow(ising_pt* p, int64_t** buf, int64_t* cap, int64_t need) {
+    if (need <= *cap) return ISING_OK;
+    (void)hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    HIP_TRY(hipMalloc(buf, sizeof(int64_t) * (size_t)need));
+    *cap = need;
+    return ISING_OK;
+}
+
+// n cluster updates of every replica (kind 0 Swendsen-Wang, 1 Wolff), at each
+// replica's temperature (inl: beta and J of a single-lattice call).  After
+// each update, optionally: sites flipped, Ising bond sum and magnetisation,
+// continuous E, Mx, My, Mz; all indexed [update][global replica].
+int cluster_run(ising_pt* p, int kind, int64_t n, bool inl, double beta, double J, int64_t* flipped, int64_t* bond,
+                int64_t* mag, double* cmeas) {
+    if (kind != ISING_CLUSTER_SW && kind != ISING_CLUSTER_WOLFF) return fail(ISING_E_ARG, "unknown cluster kind");
+    if (n < 0) return fail(ISING_E_ARG, "n must be >= 0");
+    if (p->model == ISING_MODEL_ISING_MSC)
+        return fail(ISING_E_STATE, "cluster updates need int8 or continuous storage (not multispin coding)");
+    const bool ising = p->ncomp == 0;
+    if (ising && !inl && p->opts.h != 0.0) return fail(ISING_E_STATE, "cluster updates need h = 0");
+    if (n == 0) return ISING_OK;
+    SYNC_ALL(p);
+    int rc = cluster_setup(p);
+    if (rc) return rc;
+    const int64_t R = p->total_reps;
+    if (flipped) rc = grow(p, &p->d_cflip, &p->cflip_cap, n * R);
+    if (!rc && ising && (bond || mag)) rc = grow(p, &p->d_smeas, &p->smeas_cap, n * 2 * R);
+    if (!rc && !ising && cmeas) {
+        int64_t* q = (int64_t*)p->d_cmeas;
+        rc = grow(p, &q, &p->cmeas_cap, n * 4 * R);
+        p->d_cmeas = (double*)q;
+    }
+    if (rc) return rc;
+    if (ising && !inl) {
+        std::vector<uint32_t> thr((size_t)R);
+        // slot rep0 + i of every size holds temperature i
+        for (int64_t k = 0; k < R; ++k) thr[(size_t)k] = bond_threshold(1.0 / p->temps[(size_t)(k % p->R)], J);
+        ST_COPY(p, p->d_cbthr, thr.data(), sizeof(uint32_t) * thr.size(), hipMemcpyHostToDevice);
+    }
+    if (flipped) ST_FILL(p, p->d_cflip, 0, sizeof(int64_t) * (size_t)(n * R));
+    if (ising && (bond || mag)) ST_FILL(p, p->d_smeas, 0, sizeof(int64_t) * (size_t)(n * 2 * R));
+
+    ClusterArgs a{};
+    a.groups = p->d_groups;
+    a.cg = p->d_cg;
+    a.ngroups = (int32_t)p->groups.size();
+    a.total_reps = p->total_reps;
+    a.ipool = ising ? p->d_pool : nullptr;
+    a.fpool = ising ? nullptr : p->d_fpool;
+    a.ncomp = p->ncomp;
+    a.model = p->model == ISING_MODEL_XY ? MODEL_XY : (p->model == ISING_MODEL_HEISENBERG ? MODEL_HEIS : MODEL_ISING);
+    a.kind = kind;
+    a.k0 = (uint32_t)p->seed;
+    a.k1 = (uint32_t)(p->seed >> 32);
+    a.label = p->d_clabel;
+    a.bond = p->d_cbond;
+    a.r2t = p->d_r2t;
+    a.ctl = ising ? p->d_ctl : nullptr;
+    a.bthr = p->d_cbthr;
+    a.betas = p->d_betas;
+    a.J = J;
+    a.inline_p = inl ? 1 : 0;
+    a.bthr_inline = bond_threshold(beta, J);
+    a.beta_inline = beta;
+    for (int64_t k = 0; k < n; ++k) {
+        a.u = (uint32_t)(p->c_host + k);
+        a.flipped = flipped ? p->d_cflip + k * R : nullptr;
+        HIP_TRY(launch_cluster_update(a, p->cl_blocks, p->st));
+        if (ising && (bond || mag))
+            HIP_TRY(launch_ising_measure(p->d_groups, (int)p->groups.size(), p->d_pool, p->d_smeas + k * 2 * R,
+                                         p->nblocks, p->st));
+        if (!ising && cmeas) {
+            HIP_TRY(launch_cont_measure(p->d_groups, (int)p->groups.size(), p->d_fpool, p->ncomp, J, p->d_part0,
+                                        p->nblocks, p->st));
+            HIP_TRY(launch_cont_fold(p->d_groups, (int)p->groups.size(), (int)R, p->d_part0, p->d_cmeas + k * 4 * R,
+                                     p->st));
+        }
+    }
+    p->c_host += n;
+    HIP_TRY(hipStreamSynchronize(p->st));
+    if (flipped) ST_COPY(p, flipped, p->d_cflip, sizeof(int64_t) * (size_t)(n * R), hipMemcpyDeviceToHost);
+    if (ising && (bond || mag)) {
+        std::vector<int64_t> v((size_t)(n * 2 * R));
+        ST_COPY(p, v.data(), p->d_smeas, sizeof(int64_t) * v.size(), hipMemcpyDeviceToHost);
+        for (int64_t k = 0; k < n; ++k)
+            for (int64_t r = 0; r < R; ++r) {
+                if (bond) bond[k * R + r] = v[(size_t)(k * 2 * R + 2 * r)];
+                if (mag) mag[k * R + r] = v[(size_t)(k * 2 * R + 2 * r + 1)];
+            }
+    }
+    if (!ising && cmeas) ST_COPY(p, cmeas, p->d_cmeas, sizeof(double) * (size_t)(n * 4 * R), hipMemcpyDeviceToHost);
+    return ISING_OK;
+}
+
+}  // namespace
+
+int ising_pt_cluster(ising_pt_t* p, int32_t kind, int64_t n, int64_t* flipped, int64_t* bond, int64_t* mag) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    if (p->ncomp != 0) return fail(ISING_E_STATE, "not an Ising ensemble: use ising_pt_cluster_cont");
+    if (p->inline_thr) return fail(ISING_E_STATE, "single-lattice handles use ising_lattice_cluster_step");
+    return cluster_run(p, kind, n, false, 0.0, p->opts.j, flipped, bond, mag, nullptr);
+}
+
+int ising_pt_cluster_cont(ising_pt_t* p, int32_t kind, int64_t n, int64_t* flipped, double* meas) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    if (p->ncomp == 0) return fail(ISING_E_STATE, "not a continuous ensemble: use ising_pt_cluster");
+    if (p->inline_cont) return fail(ISING_E_STATE, "single-lattice handles use ising_cont_cluster_step");
+    return cluster_run(p, kind, n, false, 0.0, p->opts.j, flipped, nullptr, nullptr, meas);
+}
+
+int ising_pt_cluster_count(const ising_pt_t* p, int64_t* out) {
+    if (!p || !out) return fail(ISING_E_ARG, "null argument");
+    *out = p->c_host;
+    return ISING_OK;
+}
+
+// ---------------------------------------------------------------------------
 // The other LatticeGpu callers, as batched ensemble drivers
 // ---------------------------------------------------------------------------
 
@@ -1505,6 +1693,13 @@ int ising_lattice_step(ising_lattice_t* l, float beta, float j, float h) {
     return l->pt->enqueue_sweeps(1);
 }
 
+int ising_lattice_cluster_step(ising_lattice_t* l, int32_t kind, float beta, float j, int64_t* flipped) {
+    if (!l) return fail(ISING_E_ARG, "null handle");
+    if (!(beta >= 0.0f) || !std::isfinite(beta) || !std::isfinite(j))
+        return fail(ISING_E_ARG, "beta must be finite and >= 0; j finite");
+    return cluster_run(l->pt, kind, 1, true, (double)beta, (double)j, flipped, nullptr, nullptr, nullptr);
+}
+
 int ising_lattice_warm_up(ising_lattice_t* l, float beta, float j, float h, size_t sweeps) {
     for (size_t i = 0; i < sweeps; ++i) {
         const int rc = ising_lattice_step(l, beta, j, h);
@@ -1665,6 +1860,13 @@ int ising_cont_sweep(ising_cont_t* l, float beta, float j, float delta, size_t n
     if (rc) return rc;
     if (n_overrelax > 1000000) return fail(ISING_E_ARG, "n_overrelax too large");
     return l->pt->enqueue_sweeps(1, (int)n_overrelax);
+}
+
+int ising_cont_cluster_step(ising_cont_t* l, int32_t kind, float beta, float j, int64_t* flipped) {
+    if (!l) return fail(ISING_E_ARG, "null handle");
+    if (!(beta >= 0.0f) || !std::isfinite(beta) || !std::isfinite(j))
+        return fail(ISING_E_ARG, "beta must be finite and >= 0; j finite");
+    return cluster_run(l->pt, kind, 1, true, (double)beta, (double)j, flipped, nullptr, nullptr, nullptr);
 }
 
 int ising_cont_metropolis_pass(ising_cont_t* l, int color, float beta, float j, float delta) {

@@ -99,6 +99,47 @@ hipError_t launch_ising_init(const DevGroup* groups, int ngroups, uint8_t* pool,
                              uint32_t k1, hipStream_t st);
 hipError_t launch_ising_measure(const DevGroup* groups, int ngroups, const uint8_t* pool, int64_t* acc,
                                 int nblocks, hipStream_t st);
+// Cluster updates (Swendsen-Wang / Wolff).  One block per (replica, tile);
+// tiles are 32 x 8 x 8 sites (3D) or 64 x 32 (2D), clipped to L.
+constexpr int kClusterTile = 2048;  // sites per tile
+constexpr int kClusterFace = 256;   // largest + face of a tile
+struct ClusterGroup {
+    int32_t L, dims, R, rep0;
+    int32_t tx, ty, tz;       // tile extents
+    int32_t ntx, nty, ntz;    // tiles per axis
+    int32_t tiles, blk0;      // tiles per replica, first block of this size
+    int64_t nsites;           // L^dims
+    int64_t lab0;             // element of replica 0 in the label and bond arrays
+    int32_t q4;               // ceil(L / 4): Philox groups per lattice row
+    int32_t pad;
+};
+struct ClusterArgs {
+    const DevGroup* groups;
+    const ClusterGroup* cg;
+    int32_t ngroups, total_reps;
+    uint8_t* ipool;          // Ising pool (0/1 bytes), or null
+    float* fpool;            // continuous pool, or null
+    int32_t ncomp;           // 0 Ising, 2 XY, 3 Heisenberg
+    uint32_t model;          // random-stream tag of continuous models
+    int32_t kind;            // 0 Swendsen-Wang, 1 Wolff
+    uint32_t u;              // update index (Philox c1)
+    uint32_t k0, k1;
+    int32_t* label;          // [lab0 + replica * nsites + site]
+    uint8_t* bond;           // same indexing: bits 0-2 bonds +x/+y/+z, bit 3 coin
+    const int32_t* r2t;      // [2][total_reps]
+    const DevCtl* ctl;       // permutation parity = round & 1 (Ising); null: parity 0
+    const uint32_t* bthr;    // Ising: [rep0 + tidx] floor(2^32 (1 - exp(-2 beta |J|)))
+    const double* betas;     // continuous: [rep0 + tidx]
+    double J;
+    int32_t inline_p;        // single-lattice: bthr_inline / beta_inline
+    uint32_t bthr_inline;
+    double beta_inline;
+    int64_t* flipped;        // [global replica] sites flipped, accumulated (null: not counted)
+};
+hipError_t launch_cluster_update(const ClusterArgs& a, int nblocks, hipStream_t st);
+hipError_t launch_cont_fold(const DevGroup* groups, int ngroups, int total_reps, const double* part, double* out,
+                            hipStream_t st);
+
 constexpr int kIsingTile = 4096;  // half-sites per block step in Ising launches (256 threads x 16)
 constexpr int kIsingTab = 32;     // acceptance-table words per temperature
 

@@ -21,6 +21,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libising_hip.so")
 
 MODEL_ISING, MODEL_XY, MODEL_HEISENBERG, MODEL_ISING_MSC, MODEL_ISING2D = 0, 1, 2, 3, 4
+CLUSTER_SW, CLUSTER_WOLFF = 0, 1
+_CLUSTER = {"sw": CLUSTER_SW, "wolff": CLUSTER_WOLFF, CLUSTER_SW: CLUSTER_SW, CLUSTER_WOLFF: CLUSTER_WOLFF}
 _MODELS = {"ising": MODEL_ISING, "xy": MODEL_XY, "heisenberg": MODEL_HEISENBERG, "ising_msc": MODEL_ISING_MSC,
            "ising2d": MODEL_ISING2D}
 
@@ -114,6 +116,11 @@ _SIGS = {
     "ising_pt_autotune": (C.c_int, [_vp, C.c_int32] + [C.POINTER(C.c_int32)] * 3),
     "ising_pt_helicity": (C.c_int, [_vp, _vp]),
     "ising_pt_anneal": (C.c_int, [_vp, _vp, C.c_int64, _vp, _vp]),
+    "ising_pt_cluster": (C.c_int, [_vp, C.c_int32, C.c_int64, _vp, _vp, _vp]),
+    "ising_pt_cluster_cont": (C.c_int, [_vp, C.c_int32, C.c_int64, _vp, _vp]),
+    "ising_pt_cluster_count": (C.c_int, [_vp, _i64p]),
+    "ising_lattice_cluster_step": (C.c_int, [_vp, C.c_int32, C.c_float, C.c_float, _i64p]),
+    "ising_cont_cluster_step": (C.c_int, [_vp, C.c_int32, C.c_float, C.c_float, _i64p]),
     "ising_kz_run": (C.c_int, [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_int32, C.c_int64,
                                C.c_int64, C.c_double, C.c_uint64, _vp]),
     "ising_kz_sweep": (C.c_int, [C.c_int32, C.c_double, C.c_double, C.c_double, _vp, C.c_int32, C.c_int32,
@@ -222,6 +229,26 @@ class LatticeGpu:
         _check(lib.ising_lattice_sweeps(self._h, C.byref(v)))
         return v.value
 
+    def cluster_step(self, kind, beta, j=1.0):
+        """One Swendsen-Wang ("sw") or Wolff ("wolff") update; returns the sites flipped."""
+        f = C.c_int64()
+        _check(lib.ising_lattice_cluster_step(self._h, _CLUSTER[kind], beta, j, C.byref(f)))
+        return f.value
+
+
+class WolffGpuLattice(LatticeGpu):
+    """GPU Wolff lattice the reference plans (WolffGpuLattice,
+    docs/plans/2026-03-09-gpu-compute-optimizations-impl.md:1041-1330): `step`
+    flips one cluster and returns its size; measurement and spins are the
+    LatticeGpu calls on the same int8 lattice."""
+
+    def step(self, beta, j=1.0):
+        return self.cluster_step("wolff", beta, j)
+
+    def warm_up(self, beta, j, steps):
+        for _ in range(int(steps)):
+            self.step(beta, j)
+
 
 class ContinuousGpuLattice:
     """One XY (n_comp=2) or Heisenberg (n_comp=3) lattice on the GPU
@@ -266,6 +293,12 @@ class ContinuousGpuLattice:
 
     def metropolis_pass(self, color, beta, j, delta):
         _check(lib.ising_cont_metropolis_pass(self._h, int(color), beta, j, delta))
+
+    def cluster_step(self, kind, beta, j=1.0):
+        """One embedded Swendsen-Wang ("sw") or Wolff ("wolff") update; returns the sites flipped."""
+        f = C.c_int64()
+        _check(lib.ising_cont_cluster_step(self._h, _CLUSTER[kind], beta, j, C.byref(f)))
+        return f.value
 
     def overrelax_pass(self, color, j=1.0):
         _check(lib.ising_cont_overrelax_pass(self._h, int(color), j))
@@ -456,6 +489,31 @@ class PTEnsemble:
         mag = np.empty_like(bond)
         _check(lib.ising_pt_anneal(self._h, _ptr(b), b.size, _ptr(bond), _ptr(mag)))
         return bond, mag
+
+    def cluster(self, kind, n=1, measure=False):
+        """n Swendsen-Wang ("sw") or Wolff ("wolff") updates of every replica at
+        its current temperature.  Returns [n, total replicas] sites flipped and,
+        with measure, the state after each update: (bond sums, magnetisations)
+        for Ising ensembles, [n, total replicas, 4] E, Mx, My, Mz otherwise."""
+        n = int(n)
+        k = _CLUSTER[kind]
+        flipped = np.empty((n, self.total_replicas), np.int64)
+        if self.model in (MODEL_XY, MODEL_HEISENBERG):
+            meas = np.empty((n, self.total_replicas, 4), np.float64) if measure else None
+            _check(lib.ising_pt_cluster_cont(self._h, k, n, _ptr(flipped), _ptr(meas) if measure else None))
+            return (flipped, meas) if measure else flipped
+        if not measure:
+            _check(lib.ising_pt_cluster(self._h, k, n, _ptr(flipped), None, None))
+            return flipped
+        bond = np.empty((n, self.total_replicas), np.int64)
+        mag = np.empty_like(bond)
+        _check(lib.ising_pt_cluster(self._h, k, n, _ptr(flipped), _ptr(bond), _ptr(mag)))
+        return flipped, bond, mag
+
+    def cluster_count(self):
+        v = C.c_int64()
+        _check(lib.ising_pt_cluster_count(self._h, C.byref(v)))
+        return v.value
 
     def autotune(self, sweeps=20):
         """Times the int8 thread-step layouts over warm-up sweeps and keeps the

@@ -974,3 +974,187 @@ int64_t o_msc_pt_run(int L, int R, const double* temps, uint64_t seed, double J,
     free(sp); free(r2t); free(t2r); free(betas); free(en); free(tt); free(tr);
     return nrec;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Cluster updates (library: csrc/cluster.hip)                              */
+/* ------------------------------------------------------------------------ */
+static int64_t uf_find(int32_t* lab, int64_t x) {
+    while (lab[x] != x) {
+        lab[x] = lab[lab[x]];
+        x = lab[x];
+    }
+    return x;
+}
+
+static void uf_union(int32_t* lab, int64_t a, int64_t b) {
+    a = uf_find(lab, a);
+    b = uf_find(lab, b);
+    if (a == b) return;
+    if (a < b) lab[b] = (int32_t)a;
+    else lab[a] = (int32_t)b;
+}
+
+uint32_t o_bond_threshold(double beta, double J) {
+    double p = -expm1(-2.0 * beta * fabs(J));
+    if (!(p > 0.0)) return 0u;
+    double t = floor(ldexp(p, 32));
+    return t >= 4294967295.0 ? 0xffffffffu : (uint32_t)t;
+}
+
+static int64_t fwd(int dim, int L, int x, int y, int z, int d) {
+    if (d == 0) x = (x + 1) % L;
+    else if (d == 1) y = (y + 1) % L;
+    else z = (z + 1) % L;
+    (void)dim;
+    return ((int64_t)z * L + y) * L + x;
+}
+
+static uint64_t mulhi64(uint64_t a, uint64_t b) {
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
+}
+
+/* Flip step shared by both spin kinds: returns the sites whose cluster flips in mark[]. */
+static int64_t choose_flips(int64_t N, int kind, uint32_t u, uint64_t seed, uint32_t stream, int model,
+                            int32_t* lab, const uint8_t* bonds, uint8_t* mark) {
+    int64_t rs = -1, cnt = 0;
+    if (kind == 1) {
+        uint32_t w[4];
+        ph(0u, u, stream, o_tag(10, model), seed, w);
+        const uint64_t v = ((uint64_t)w[0] << 32) | w[1];
+        rs = uf_find(lab, (int64_t)mulhi64(v, (uint64_t)N));
+    }
+    for (int64_t i = 0; i < N; ++i) {
+        const int64_t r = uf_find(lab, i);
+        mark[i] = (uint8_t)(kind == 1 ? r == rs : ((bonds[r] >> 3) & 1u));
+        cnt += mark[i];
+    }
+    return cnt;
+}
+
+int64_t o_ising_cluster(int dim, int L, int8_t* sp, int kind, uint32_t u, uint64_t seed, uint32_t stream,
+                        uint32_t thr, double J, int32_t* lab, uint8_t* bonds) {
+    const int Lz = dim == 3 ? L : 1, q4 = (L + 3) / 4;
+    const int64_t N = (int64_t)Lz * L * L;
+    const uint32_t hi = thr >> 24, lo = thr & 0xffffffu;
+    uint8_t* own = NULL;
+    if (!bonds) bonds = own = (uint8_t*)malloc((size_t)N);
+    for (int64_t i = 0; i < N; ++i) lab[i] = (int32_t)i;
+    for (int z = 0; z < Lz; ++z)
+        for (int y = 0; y < L; ++y)
+            for (int x = 0; x < L; ++x) {
+                const int64_t i = ((int64_t)z * L + y) * L + x;
+                const uint32_t grp = (uint32_t)(((int64_t)z * L + y) * q4 + x / 4);
+                uint32_t w[4];
+                ph(grp, u, stream, o_tag(7, 0), seed, w);
+                const uint32_t wk = w[x & 3];
+                uint32_t bits = (wk >> 31) << 3;
+                for (int d = 0; d < dim; ++d) {
+                    const int64_t j = fwd(dim, L, x, y, z, d);
+                    const int sat = J > 0.0 ? sp[i] == sp[j] : sp[i] != sp[j];
+                    if (!sat) continue;
+                    const uint32_t r8 = (wk >> (8 * d)) & 0xffu;
+                    int act = r8 < hi;
+                    if (r8 == hi && lo != 0u) {
+                        uint32_t t[4];
+                        ph(4u * grp + (uint32_t)(x & 3), u, stream, o_tag(8, 0), seed, t);
+                        act = (t[d] >> 8) < lo;
+                    }
+                    if (act) {
+                        bits |= 1u << d;
+                        uf_union(lab, i, j);
+                    }
+                }
+                bonds[i] = (uint8_t)bits;
+            }
+    uint8_t* mark = (uint8_t*)malloc((size_t)N);
+    const int64_t cnt = choose_flips(N, kind, u, seed, stream, 0, lab, bonds, mark);
+    for (int64_t i = 0; i < N; ++i)
+        if (mark[i]) sp[i] = (int8_t)-sp[i];
+    free(mark);
+    free(own);
+    return cnt;
+}
+
+int64_t o_cont_cluster(int L, int nc, double* sp, int kind, uint32_t u, uint64_t seed, uint32_t stream, int model,
+                       double beta, double J, double margin, int64_t* n_marginal, int32_t* lab, uint8_t* bonds) {
+    const int64_t N = (int64_t)L * L * L;
+    uint8_t* own = NULL;
+    if (!bonds) bonds = own = (uint8_t*)malloc((size_t)N);
+    double r[3] = {0.0, 0.0, 0.0};
+    {
+        uint32_t w[4];
+        ph(0u, u, stream, o_tag(9, model), seed, w);
+        const double two_pi = 6.283185307179586;
+        if (nc == 2) {
+            const double phi = two_pi * o_u01(w[0], w[1]);
+            r[0] = cos(phi);
+            r[1] = sin(phi);
+        } else {
+            const double zc = 2.0 * o_u01(w[0], w[1]) - 1.0, phi = two_pi * o_u01(w[2], w[3]);
+            const double s = sqrt(fmax(0.0, 1.0 - zc * zc));
+            r[0] = s * cos(phi);
+            r[1] = s * sin(phi);
+            r[2] = zc;
+        }
+    }
+    double* a = (double*)malloc(sizeof(double) * (size_t)N);
+    for (int64_t i = 0; i < N; ++i) {
+        double v = 0.0;
+        for (int k = 0; k < nc; ++k) v += sp[i * nc + k] * r[k];
+        a[i] = v;
+        lab[i] = (int32_t)i;
+    }
+    int64_t marg = 0;
+    for (int z = 0; z < L; ++z)
+        for (int y = 0; y < L; ++y)
+            for (int x = 0; x < L; ++x) {
+                const int64_t i = ((int64_t)z * L + y) * L + x;
+                uint32_t w[4];
+                ph((uint32_t)i, u, stream, o_tag(7, model), seed, w);
+                uint32_t bits = (w[3] >> 31) << 3;
+                for (int d = 0; d < 3; ++d) {
+                    const int64_t j = fwd(3, L, x, y, z, d);
+                    const double xx = 2.0 * beta * J * a[i] * a[j];
+                    const double uu = u24(w[d]);
+                    const double p = xx > 0.0 ? -expm1(-xx) : 0.0;
+                    if (fabs(uu - p) < margin) ++marg;
+                    if (xx > 0.0 && uu < p) {
+                        bits |= 1u << d;
+                        uf_union(lab, i, j);
+                    }
+                }
+                bonds[i] = (uint8_t)bits;
+            }
+    uint8_t* mark = (uint8_t*)malloc((size_t)N);
+    const int64_t cnt = choose_flips(N, kind, u, seed, stream, model, lab, bonds, mark);
+    for (int64_t i = 0; i < N; ++i)
+        if (mark[i])
+            for (int k = 0; k < nc; ++k) sp[i * nc + k] -= 2.0 * a[i] * r[k];
+    if (n_marginal) *n_marginal = marg;
+    free(mark);
+    free(a);
+    free(own);
+    return cnt;
+}
+
+void o_ising_cluster_chain(int dim, int L, uint64_t seed, int kind, double beta, double J, int64_t warmup, int64_t n,
+                           int64_t* out_bond, int64_t* out_mag, int64_t* out_flipped) {
+    const int64_t N = dim == 3 ? (int64_t)L * L * L : (int64_t)L * L;
+    /* the hot start is keyed by site index: a square lattice takes the first L^2 of the cube's */
+    int8_t* sp = (int8_t*)malloc((size_t)L * L * L);
+    int32_t* lab = (int32_t*)malloc(sizeof(int32_t) * (size_t)N);
+    const uint32_t stream = o_stream(L, 0), thr = o_bond_threshold(beta, J);
+    o_ising_init(L, seed, stream, sp);
+    for (int64_t k = 0; k < warmup + n; ++k) {
+        const int64_t f = o_ising_cluster(dim, L, sp, kind, (uint32_t)k, seed, stream, thr, J, lab, NULL);
+        if (k < warmup) continue;
+        int64_t b, m;
+        if (dim == 3) o_ising_measure(L, sp, &b, &m);
+        else o_ising2d_measure(L, sp, &b, &m);
+        if (out_bond) out_bond[k - warmup] = b;
+        if (out_mag) out_mag[k - warmup] = m;
+        if (out_flipped) out_flipped[k - warmup] = f;
+    }
+    free(lab);
+    free(sp);
+}

@@ -168,6 +168,47 @@ double o_tau_int(const double* x, int64_t n, double c, int64_t* window);
 void o_ising_chain(int dim, int L, uint64_t seed, double beta, double J, double h, int64_t warmup,
                    int64_t n_sweeps, int64_t* out_bond, int64_t* out_mag);
 
+/* ---------------- Cluster updates (Swendsen-Wang / Wolff) ------------------
+ * One bond configuration per update, as the library draws it: Fortuin-
+ * Kasteleyn bonds on the forward bonds (+x, +y[, +z]) of every site, active
+ * between satisfied neighbours (aligned for J > 0, anti-aligned for J < 0)
+ * with probability floor(2^32 p) / 2^32, p = 1 - exp(-2 beta |J|)
+ * (src/wolff.rs:27-77 grows the same clusters from one seed).  Draws:
+ * Philox(c0 = row * ceil(L/4) + x/4, u, stream, tag(7, 0)), word x & 3,
+ * byte d against the high 8 bits of the threshold; on equality the 24-bit
+ * draw Philox(4 c0 + (x & 3), u, stream, tag(8, 0)) word d >> 8 against the
+ * low 24 bits.  Bit 31 of the site's word is its cluster coin.  Clusters by
+ * union-find (root = smallest site index); Swendsen-Wang (kind 0) flips a
+ * cluster when its root's coin is set, Wolff (kind 1) flips the cluster of
+ * the site floor(N ((w0 << 32) | w1) / 2^64), w = Philox(0, u, stream, tag(10, 0)).
+ * Returns the number of sites flipped; bonds[i] (when not NULL) receives the
+ * site's bond bits (bit d) and coin (bit 3).
+ * Pinned by: exact stationarity of the Boltzmann distribution on 2x2x2 and
+ * 4x4 lattices (full transition matrices from enumerated bond draws, not from
+ * this code), clusters = connected components found by breadth-first search,
+ * Wolff mean cluster size = <M^2> / N, and equilibrium against o_ref_*
+ * (tests/test_oracle_cluster.py). */
+uint32_t o_bond_threshold(double beta, double J);
+int64_t o_ising_cluster(int dim, int L, int8_t* spins, int kind, uint32_t u, uint64_t seed, uint32_t stream,
+                        uint32_t thr, double J, int32_t* lab, uint8_t* bonds);
+/* Continuous spins, Wolff embedding (src/xy/wolff.rs:16-73): r from
+ * Philox(0, u, stream, tag(9, model)) (XY: phi = 2 pi u01(w0, w1); Heisenberg:
+ * z = 2 u01(w0, w1) - 1, phi = 2 pi u01(w2, w3)), a_i = s_i . r, bond d of
+ * site i active iff x = 2 beta J a_i a_j > 0 and u24(w_d) < 1 - exp(-x) with
+ * w = Philox(i, u, stream, tag(7, model)); bit 31 of w_3 is the coin; a flip
+ * is s -> s - 2 a r.  FP64 throughout; *n_marginal counts bonds whose uniform
+ * lies within `margin` of the probability (where an FP32 device may differ).
+ * Pinned by: energy conservation of every cluster-reflected configuration's
+ * embedded Ising weights (reflection is an isometry), unit norm, and
+ * equilibrium against o_ref_xy_chain. */
+int64_t o_cont_cluster(int L, int ncomp, double* spins, int kind, uint32_t u, uint64_t seed, uint32_t stream,
+                       int model, double beta, double J, double margin, int64_t* n_marginal, int32_t* lab,
+                       uint8_t* bonds);
+/* Cluster-update chains for equilibrium checks: `warmup` then `n` updates of
+ * one lattice, recording the bond sum and magnetisation after each. */
+void o_ising_cluster_chain(int dim, int L, uint64_t seed, int kind, double beta, double J, int64_t warmup, int64_t n,
+                           int64_t* out_bond, int64_t* out_mag, int64_t* out_flipped);
+
 #ifdef __cplusplus
 }
 #endif

@@ -252,3 +252,49 @@ def ising2d_pt_run(L, R, temps, seed, warmup, samples, exchange_every, measure_e
     n = lib.o_ising_pt_run_dim(2, L, R, np.asarray(temps, np.float64), seed, J, h, warmup, samples, exchange_every,
                                measure_every, tidx, bond, mag, final)
     return tidx[:n], bond[:n], mag[:n], final.reshape(R, L ** 2)
+
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C")
+lib.o_bond_threshold.restype = C.c_uint32
+lib.o_bond_threshold.argtypes = [C.c_double, C.c_double]
+lib.o_ising_cluster.restype = C.c_int64
+lib.o_ising_cluster.argtypes = [C.c_int, C.c_int, _i8p, C.c_int, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32,
+                                C.c_double, _i32p, _u8p]
+lib.o_cont_cluster.restype = C.c_int64
+lib.o_cont_cluster.argtypes = [C.c_int, C.c_int, _f64p, C.c_int, C.c_uint32, C.c_uint64, C.c_uint32, C.c_int,
+                               C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int64), _i32p, _u8p]
+lib.o_ising_cluster_chain.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_double, C.c_double, C.c_int64,
+                                      C.c_int64, _i64p, _i64p, _i64p]
+
+
+def bond_threshold(beta, J=1.0):
+    return int(lib.o_bond_threshold(beta, J))
+
+
+def ising_cluster(dim, L, spins, kind, u, seed, stream, beta=None, J=1.0, thr=None):
+    """One cluster update of `spins` (int8 +-1, modified in place); returns (flipped, bonds)."""
+    n = spins.size
+    if thr is None:
+        thr = bond_threshold(beta, J)
+    lab = np.zeros(n, np.int32)
+    bonds = np.zeros(n, np.uint8)
+    f = lib.o_ising_cluster(dim, L, spins, kind, u, seed, stream, thr, J, lab, bonds)
+    return int(f), bonds
+
+
+def cont_cluster(L, ncomp, spins, kind, u, seed, stream, model, beta, J=1.0, margin=1e-5):
+    """One embedded cluster update of `spins` ([L^3 * ncomp] float64, in place):
+    returns (flipped, bonds, bonds within `margin` of their probability)."""
+    n = L ** 3
+    lab = np.zeros(n, np.int32)
+    bonds = np.zeros(n, np.uint8)
+    marg = C.c_int64()
+    f = lib.o_cont_cluster(L, ncomp, spins, kind, u, seed, stream, model, beta, J, margin, C.byref(marg), lab, bonds)
+    return int(f), bonds, marg.value
+
+
+def ising_cluster_chain(dim, L, seed, kind, beta, warmup, n, J=1.0):
+    b = np.zeros(n, np.int64)
+    m = np.zeros(n, np.int64)
+    f = np.zeros(n, np.int64)
+    lib.o_ising_cluster_chain(dim, L, seed, kind, beta, J, warmup, n, b, m, f)
+    return b, m, f
