@@ -185,9 +185,13 @@ def run_learned(args, ising_hip):
     kw = dict(exchange_every=10, measure_every=1, n_overrelax=2, delta=1.0, adapt_every=10, adapt_target=0.5,
               renorm_every=100, record_capacity=steps + 8)
     out = {}
-    for mode in ("plain", "learned"):
+    for mode in ("plain", "learned", "cluster"):
         pt = ising_hip.PTEnsemble("xy", sizes, Rr, temps, args.seed, **kw)
         pt.warmup(args.warmup)
+        if mode == "cluster":
+            out[mode] = xy_cluster_leg(pt, sizes, Rr, temps, steps, ising_hip)
+            pt.close()
+            continue
         train = None
         if mode == "learned":
             pt.learn_init(args.hidden, seed=7)
@@ -241,6 +245,7 @@ def run_learned(args, ising_hip):
                      "learned_acceptance": (round(acc / tries, 4) if tries else None), "training": train}
         pt.close()
     ratio = out["learned"]["independent_samples_per_s"] / out["plain"]["independent_samples_per_s"]
+    cratio = out["cluster"]["independent_samples_per_s"] / out["plain"]["independent_samples_per_s"]
     sites = sum(L ** 3 for L in sizes) * Rr
     line = {
         "metric": "independent samples/s (learned proposal)",
@@ -258,7 +263,119 @@ def run_learned(args, ising_hip):
         "config": {"workload": "learned proposal vs plain (config 5): XY L=32,64 x 32 replicas, T=linspace(2.18,2.22,32), "
                                "1 Metropolis + 2 overrelaxation passes per sweep, measure every sweep, exchange every 10",
                    "hidden": args.hidden, "sites": sites, "plain": out["plain"], "learned": out["learned"],
-                   "learned_over_plain": round(ratio, 4)},
+                   "cluster": out["cluster"], "learned_over_plain": round(ratio, 4),
+                   "cluster_over_plain": round(cratio, 4)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def xy_cluster_leg(pt, sizes, Rr, temps, steps, ising_hip):
+    """Embedded Swendsen-Wang updates (Wolff's reflection along a random axis,
+    every cluster flipped with probability 1/2) of the same XY ensemble: one
+    update per step, E and M after each; the statistical-efficiency comparator
+    of the learned proposal."""
+    pt.cluster("sw", 100)
+    chunk, wall, done = 10, 0.0, 0
+    E = np.zeros((steps, len(sizes), Rr))
+    M2 = np.zeros_like(E)
+    hc = np.zeros((len(sizes), Rr))
+    hj2 = np.zeros((len(sizes), Rr))
+    nh = 0
+    while done < steps:
+        m = min(chunk, steps - done)
+        t0 = time.perf_counter()
+        _, meas = pt.cluster("sw", m, measure=True)
+        wall += time.perf_counter() - t0
+        hel = pt.helicity()
+        for g in range(len(sizes)):
+            r2t, _ = pt.get_perm(g)
+            sl = slice(g * Rr, (g + 1) * Rr)
+            E[done:done + m, g, r2t] = meas[:, sl, 0]
+            M2[done:done + m, g, r2t] = meas[:, sl, 1] ** 2 + meas[:, sl, 2] ** 2
+            hg = hel[sl]
+            hc[g, r2t] += hg[:, :3].mean(axis=1)
+            hj2[g, r2t] += (hg[:, 3:] ** 2).mean(axis=1)
+        nh += 1
+        done += m
+    ups = [(hc[g] / nh - np.array([1.0 / t for t in temps]) * hj2[g] / nh) / L ** 3 for g, L in enumerate(sizes)]
+    taus_e = np.array([ising_hip.tau_int(E[:, g, k])[0] for g in range(len(sizes)) for k in range(Rr)])
+    taus_m = np.array([ising_hip.tau_int(M2[:, g, k])[0] for g in range(len(sizes)) for k in range(Rr)])
+    isps = float(np.sum(steps / (2.0 * np.maximum(taus_e, 0.5))) / wall)
+    u4 = [float(np.mean(1 - (M2[:, g, :] ** 2).mean(axis=0) / (3 * M2[:, g, :].mean(axis=0) ** 2)))
+          for g in range(len(sizes))]
+    return {"updates_per_s": round(steps / wall, 2), "tau_int_E_mean": round(float(taus_e.mean()), 3),
+            "tau_int_M2_mean": round(float(taus_m.mean()), 3), "independent_samples_per_s": round(isps, 1),
+            "helicity_modulus_mean": [round(float(u.mean()), 5) for u in ups],
+            "binder_u4_mean": [round(v, 5) for v in u4],
+            "note": "one embedded Swendsen-Wang update per step, measured after each (no Metropolis / overrelaxation)"}
+
+
+def run_cluster(args, ising_hip):
+    """Cluster updates (Swendsen-Wang, Wolff) against checkerboard Metropolis
+    near T_c, 3D Ising.  (a) Throughput: Swendsen-Wang updates of the config-2
+    ensemble (L = 8..128 x 32 replicas, T = linspace(4.40, 4.62)), in sites
+    per ns.  (b) Efficiency at T_c = 4.5115, L = 32 and 64 x 32 independent
+    replicas: independent samples/s of |M| = steps/s / (2 tau_int(|M|)) summed
+    over replicas, tau_int by Sokal's window (c = 6)."""
+    temps = linspace(4.40, 4.62, R)
+    pt = ising_hip.PTEnsemble("ising", SIZES, R, temps, args.seed)
+    pt.cluster("sw", 3)
+    n_thr = 200
+    t0 = time.perf_counter()
+    pt.cluster("sw", n_thr)
+    wall_thr = time.perf_counter() - t0
+    sites = sum(L ** 3 for L in SIZES) * R
+    pt.close()
+    thr = {"sizes": SIZES, "replicas": R, "sw_updates": n_thr, "sites_per_ns": round(sites * n_thr / (wall_thr * 1e9), 3),
+           "ms_per_update": round(wall_thr * 1e3 / n_thr, 4)}
+    if args.cluster_part == "throughput":
+        print(json.dumps({"metric": "sites/ns (Swendsen-Wang updates)", "value": thr["sites_per_ns"],
+                          "unit": "sites/ns", "n_gpus": 1, "config": thr}), flush=True)
+        return
+    counts = {"metropolis": 40000, "sw": 4000, "wolff": 20000}
+    eff = {}
+    for L in (32, 64):
+        res = {}
+        for mode, n in counts.items():
+            pt = ising_hip.PTEnsemble("ising", [L], R, [4.5115] * R, args.seed, exchange_every=1 << 30,
+                                      measure_every=1, record_capacity=n + 8)
+            if mode == "metropolis":
+                pt.warmup(2000)
+                pt.sync()
+                t0 = time.perf_counter()
+                pt.sample(n)
+                pt.sync()
+                wall = time.perf_counter() - t0
+                _, _, mag = pt.read_ising()
+                series = np.abs(mag[:, 0, :]) / L ** 3
+                flipped = None
+            else:
+                pt.cluster(mode, 2000 if mode == "wolff" else 300)
+                t0 = time.perf_counter()
+                flipped, _, mag = pt.cluster(mode, n, measure=True)
+                wall = time.perf_counter() - t0
+                series = np.abs(mag) / L ** 3
+            taus = np.array([ising_hip.tau_int(series[:, r])[0] for r in range(R)])
+            isps = float(np.sum(n / (2.0 * np.maximum(taus, 0.5))) / wall)
+            res[mode] = {"steps": n, "steps_per_s": round(n / wall, 1), "tau_int_absM_mean": round(float(taus.mean()), 3),
+                         "independent_samples_per_s": round(isps, 1), "absM_mean": round(float(series.mean()), 5)}
+            if flipped is not None:
+                res[mode]["mean_sites_flipped"] = round(float(flipped.mean()), 1)
+            pt.close()
+        for mode in ("sw", "wolff"):
+            res[mode + "_over_metropolis"] = round(res[mode]["independent_samples_per_s"]
+                                                   / res["metropolis"]["independent_samples_per_s"], 3)
+        eff["L%d" % L] = res
+    value = eff["L64"]["sw"]["independent_samples_per_s"]
+    line = {
+        "metric": "independent samples/s (Swendsen-Wang, |M|, 3D Ising L=64 x 32 replicas at T_c)",
+        "value": value, "unit": "samples/s", "n_gpus": 1, "steps": counts["sw"], "warmup": 300,
+        "ms_per_step": round(1e3 / eff["L64"]["sw"]["steps_per_s"], 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": "cluster updates vs checkerboard Metropolis near T_c: 3D Ising, 32 independent "
+                               "replicas at T=4.5115, L=32 and 64; throughput on the config-2 ensemble",
+                   "throughput": thr,
+                   "efficiency": eff},
     }
     print(json.dumps(line), flush=True)
 
@@ -305,9 +422,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--workload", default="ising", choices=sorted(WORKLOADS) + ["batched", "learned", "kz", "coarsening", "fss"])
+    ap.add_argument("--workload", default="ising", choices=sorted(WORKLOADS) + ["batched", "learned", "kz", "coarsening", "fss",
+                                                                  "cluster"])
     ap.add_argument("--hidden", type=int, default=16, help="learned workload: MLP hidden width (8, 16, 32 or 64)")
     ap.add_argument("--train-steps", type=int, default=400, help="learned workload: Adam steps")
+    ap.add_argument("--cluster-part", default="all", choices=["all", "throughput"],
+                    help="cluster workload: 'throughput' times only the Swendsen-Wang updates of the config-2 ensemble")
     ap.add_argument("--layout", default="auto", metavar="W,R,S",
                     help="int8 thread-step layout (run width, runs per step, steps per tile); "
                          "'auto' times all layouts during warm-up and keeps the fastest")
@@ -335,6 +455,9 @@ def main():
         return
     if args.workload in ("kz", "coarsening", "fss"):
         run_driver(args, ising_hip)
+        return
+    if args.workload == "cluster":
+        run_cluster(args, ising_hip)
         return
     model, sizes, (tlo, thi), bpu, dtype, opts, desc = WORKLOADS[args.workload]
     temps = linspace(tlo, thi, R)

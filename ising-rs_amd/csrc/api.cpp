@@ -1260,6 +1260,12 @@ int cluster_setup(ising_pt* p) {
         C.lab0 = lab;
         lab += (int64_t)G.R * C.nsites;
         C.q4 = (G.L + 3) / 4;
+        // every extent is even (L even, tiles of even width), so >= 2 as fastdiv needs
+        fastdiv_magic((uint32_t)G.L, &C.mag_L, &C.sh_L);
+        fastdiv_magic((uint32_t)C.tx, &C.mag_x[0], &C.sh_x[0]);
+        fastdiv_magic((uint32_t)(G.L - (C.ntx - 1) * C.tx), &C.mag_x[1], &C.sh_x[1]);
+        fastdiv_magic((uint32_t)C.ty, &C.mag_y[0], &C.sh_y[0]);
+        fastdiv_magic((uint32_t)(G.L - (C.nty - 1) * C.ty), &C.mag_y[1], &C.sh_y[1]);
         cg.push_back(C);
     }
     if (blk >= (1ll << 31)) return fail(ISING_E_ARG, "too many cluster tiles");

@@ -34,12 +34,14 @@
 // src/wolff.rs:27-77 and src/xy/wolff.rs:16-73.
 //
 // Random numbers (Philox4x32-10, c1 = update index u, c2 = stream(L, replica)):
-//   Ising bond   c0 = row * ceil(L/4) + x/4, tag P_BOND; word x & 3 belongs to
-//                site x: byte d is the 8-bit draw of bond d, bit 31 the coin.
+//   Ising bond   c0 = site / 2, tag P_BOND: words (w0, w1) belong to the even
+//                site of the pair, (w2, w3) to the odd one; with (wa, wb) the
+//                site's words the 16-bit draws of bonds +x, +y, +z are
+//                wa & 0xffff, wa >> 16, wb & 0xffff and bit 31 of wb is the coin.
 //                Bond active iff aligned (J > 0) / anti-aligned (J < 0) and
-//                draw < hi, or draw == hi and the 24-bit tie draw < lo, where
-//                hi * 2^24 + lo = floor(2^32 (1 - exp(-2 beta |J|))).
-//   Ising tie    c0 = 4 * (Philox group) + (x & 3), tag P_BTIE; word d >> 8.
+//                draw < hi, or draw == hi and the 16-bit tie draw < lo, where
+//                hi * 2^16 + lo = floor(2^32 (1 - exp(-2 beta |J|))).
+//   Ising tie    c0 = site, tag (P_BTIE << 8) | d: word 0 >> 16.
 //   cont. bond   c0 = site index, tag P_BOND; word d -> 24-bit uniform, bit 31 of word 3 the coin.
 //   cont. r      c0 = 0, tag P_CDIR: XY phi = 2 pi u01(w0, w1); Heisenberg
 //                z = 2 u01(w0, w1) - 1, phi = 2 pi u01(w2, w3).
@@ -51,6 +53,8 @@ namespace ising {
 namespace {
 
 constexpr int kCB = 256;  // threads per block
+constexpr int kPer = kClusterTile / kCB;  // sites per thread
+static_assert(kClusterFace <= kCB, "one face site per thread");
 
 __device__ __forceinline__ int find_cgroup(const ClusterGroup* cg, int ngroups, int b) {
     int g = 0;
@@ -58,9 +62,10 @@ __device__ __forceinline__ int find_cgroup(const ClusterGroup* cg, int ngroups, 
     return g;
 }
 
-// A block's tile: replica, origin and extents.
+// A block's tile: replica, origin, extents and the divisions by them.
 struct Tile {
     int r, x0, y0, z0, ex, ey, ez, n;
+    uint32_t mx, sx, my, sy;  // n / ex, n / ey by fastdiv
 };
 
 __device__ __forceinline__ Tile tile_of(const ClusterGroup& C, int local) {
@@ -77,30 +82,33 @@ __device__ __forceinline__ Tile tile_of(const ClusterGroup& C, int local) {
     T.ey = min(C.ty, C.L - T.y0);
     T.ez = C.dims == 3 ? min(C.tz, C.L - T.z0) : 1;
     T.n = T.ex * T.ey * T.ez;
+    const int lx = itx == C.ntx - 1, ly = ity == C.nty - 1;
+    T.mx = C.mag_x[lx];
+    T.sx = C.sh_x[lx];
+    T.my = C.mag_y[ly];
+    T.sy = C.sh_y[ly];
     return T;
 }
 
-__device__ __forceinline__ int64_t site_of(const ClusterGroup& C, int x, int y, int z) {
-    return ((int64_t)z * C.L + y) * C.L + x;
+// Tile-local index li = (zl * ey + yl) * ex + xl -> (xl, yl, zl).
+__device__ __forceinline__ void decode(const Tile& T, uint32_t li, int* xl, int* yl, int* zl) {
+    const uint32_t row = fastdiv(li, T.mx, T.sx);
+    *xl = (int)(li - row * (uint32_t)T.ex);
+    const uint32_t z = fastdiv(row, T.my, T.sy);
+    *yl = (int)(row - z * (uint32_t)T.ey);
+    *zl = (int)z;
 }
 
-// Half-lattice element of site (x, y, z): colour and index within the colour.
-__device__ __forceinline__ int64_t half_index(const DevGroup& G, int x, int y, int z, int* colour) {
-    *colour = (x + y + z) & 1;
-    return ((int64_t)z * G.L + y) * G.Lh + (x >> 1);
+// Site index within a replica (L^dims < 2^31).
+__device__ __forceinline__ uint32_t site_of(const ClusterGroup& C, int x, int y, int z) {
+    return ((uint32_t)z * (uint32_t)C.L + (uint32_t)y) * (uint32_t)C.L + (uint32_t)x;
 }
 
-__device__ __forceinline__ uint8_t* ising_byte(const ClusterArgs& a, const DevGroup& G, int r, int x, int y, int z) {
-    int c;
-    const int64_t h = half_index(G, x, y, z, &c);
-    return a.ipool + G.off + (2 * (int64_t)r + c) * G.stride + h;
-}
-
-template <int NC>
-__device__ __forceinline__ float* cont_ptr(const ClusterArgs& a, const DevGroup& G, int r, int x, int y, int z) {
-    int c;
-    const int64_t h = half_index(G, x, y, z, &c);
-    return a.fpool + G.off + (2 * (int64_t)r + c) * NC * G.stride + h;
+// Element of site (x, y, z) in the colour half-lattice base[(x + y + z) & 1].
+template <typename P>
+__device__ __forceinline__ P* at(P* const base[2], const DevGroup& G, int x, int y, int z) {
+    const uint32_t h = ((uint32_t)z * (uint32_t)G.L + (uint32_t)y) * (uint32_t)G.Lh + ((uint32_t)x >> 1);
+    return base[(x + y + z) & 1] + h;
 }
 
 // Random reflection direction of replica r for update u (FP32).
@@ -137,11 +145,15 @@ __device__ __forceinline__ int tslot(const ClusterArgs& a, const ClusterGroup& C
 
 // --- union-find in LDS -------------------------------------------------------
 
+// Path halving: a parent is only ever replaced by one of its ancestors, which
+// keeps every set intact under concurrent unions.
 __device__ __forceinline__ uint32_t lds_find(uint32_t* lab, uint32_t x) {
     uint32_t p = __hip_atomic_load(&lab[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     while (p != x) {
+        const uint32_t gp = __hip_atomic_load(&lab[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (gp != p) __hip_atomic_store(&lab[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         x = p;
-        p = __hip_atomic_load(&lab[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        p = gp;
     }
     return x;
 }
@@ -163,19 +175,22 @@ __device__ __forceinline__ void lds_union(uint32_t* lab, uint32_t a, uint32_t b)
 }
 
 // --- union-find on the global labels -----------------------------------------
-// Loads bypass the non-coherent vector L1 (agent-scope atomics), so links made
-// by other compute units are seen.  Path halving stores only ever replace a
-// parent by one of its ancestors.
+// Parents only decrease and always stay inside their set, so a stale (cached)
+// parent is still an ancestor: finds may read through the caches, and a union
+// that acts on a stale root is corrected by the atomicMin's return value
+// (the link fails, and the union retries with the parent that won).  Loads
+// and the path-halving stores are relaxed atomics so the compiler keeps every
+// access; only the link itself is a device-wide atomic.
 
 __device__ __forceinline__ int32_t g_load(const int32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 __device__ __forceinline__ int32_t g_find(int32_t* lab, int32_t x) {
     int32_t p = g_load(&lab[x]);
     while (p != x) {
         const int32_t gp = g_load(&lab[p]);
-        if (gp != p) __hip_atomic_store(&lab[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (gp != p) __hip_atomic_store(&lab[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         x = p;
         p = gp;
     }
@@ -198,6 +213,17 @@ __device__ __forceinline__ void g_union(int32_t* lab, int32_t a, int32_t b) {
     }
 }
 
+// Wave-aggregated append of `v` (lanes with act set) to an LDS list.
+__device__ __forceinline__ void append(uint16_t* list, int* n, bool act, uint16_t v) {
+    const uint64_t m = __ballot(act);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    if (lane == __ffsll((unsigned long long)m) - 1) base = atomicAdd(n, __popcll(m));
+    base = __shfl(base, __ffsll((unsigned long long)m) - 1, 64);
+    if (act) list[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+
 template <int NC>
 struct SpinT {
     using type = float;
@@ -215,10 +241,13 @@ struct SpinT<0> {
 template <int NC>
 __global__ __launch_bounds__(kCB) void cluster_bond_kernel(ClusterArgs a) {
     using S = typename SpinT<NC>::type;
+    using P = typename std::conditional<NC == 0, uint8_t, float>::type;
     __shared__ S s_v[kClusterTile];
     __shared__ S s_h[3][kClusterFace];
     __shared__ uint8_t s_bond[kClusterTile];
     __shared__ uint32_t s_lab[kClusterTile];
+    __shared__ uint16_t s_list[kClusterTile];
+    __shared__ int s_nlist;
 
     const int g = find_cgroup(a.cg, a.ngroups, blockIdx.x);
     const ClusterGroup C = a.cg[g];
@@ -230,33 +259,59 @@ __global__ __launch_bounds__(kCB) void cluster_bond_kernel(ClusterArgs a) {
 
     float rv[3] = {0.f, 0.f, 0.f};
     if constexpr (NC > 0) direction<NC>(a, stream, rv);
-
+    P* base[2];
+    if constexpr (NC == 0) {
+        base[0] = a.ipool + G.off + 2 * (int64_t)T.r * G.stride;
+        base[1] = base[0] + G.stride;
+    } else {
+        base[0] = a.fpool + G.off + 2 * (int64_t)T.r * NC * G.stride;
+        base[1] = base[0] + NC * G.stride;
+    }
     auto value = [&](int x, int y, int z) -> S {
-        if constexpr (NC == 0) return (S)*ising_byte(a, G, T.r, x, y, z);
-        else return project<NC>(cont_ptr<NC>(a, G, T.r, x, y, z), G.stride, rv);
+        if constexpr (NC == 0) return (S)*at(base, G, x, y, z);
+        else return project<NC>(at(base, G, x, y, z), G.stride, rv);
     };
 
-    for (int li = tid; li < T.n; li += kCB) {
-        const int row = li / T.ex, xl = li - row * T.ex;
-        const int zl = row / T.ey, yl = row - zl * T.ey;
-        s_v[li] = value(T.x0 + xl, T.y0 + yl, T.z0 + zl);
-        s_lab[li] = (uint32_t)li;
-    }
+    // every load of the tile and its three + faces first (a face has at most
+    // kCB sites), then the stores to LDS
     {
-        const int xn = (T.x0 + T.ex) % L, yn = (T.y0 + T.ey) % L, zn = (T.z0 + T.ez) % L;
-        for (int k = tid; k < T.ey * T.ez; k += kCB) {
-            const int zl = k / T.ey, yl = k - zl * T.ey;
-            s_h[0][k] = value(xn, T.y0 + yl, T.z0 + zl);
-        }
-        for (int k = tid; k < T.ex * T.ez; k += kCB) {
-            const int zl = k / T.ex, xl = k - zl * T.ex;
-            s_h[1][k] = value(T.x0 + xl, yn, T.z0 + zl);
-        }
-        if (dims == 3)
-            for (int k = tid; k < T.ex * T.ey; k += kCB) {
-                const int yl = k / T.ex, xl = k - yl * T.ex;
-                s_h[2][k] = value(T.x0 + xl, T.y0 + yl, zn);
+        S v[kPer], hv[3];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int li = tid + k * kCB;
+            if (li < T.n) {
+                int xl, yl, zl;
+                decode(T, (uint32_t)li, &xl, &yl, &zl);
+                v[k] = value(T.x0 + xl, T.y0 + yl, T.z0 + zl);
             }
+        }
+        const int xn = T.x0 + T.ex == L ? 0 : T.x0 + T.ex;
+        const int yn = T.y0 + T.ey == L ? 0 : T.y0 + T.ey;
+        const int zn = T.z0 + T.ez == L ? 0 : T.z0 + T.ez;
+        const int f0 = T.ey * T.ez, f1 = T.ex * T.ez, f2 = dims == 3 ? T.ex * T.ey : 0;
+        if (tid < f0) {
+            const int zl = (int)fastdiv((uint32_t)tid, T.my, T.sy), yl = tid - zl * T.ey;
+            hv[0] = value(xn, T.y0 + yl, T.z0 + zl);
+        }
+        if (tid < f1) {
+            const int zl = (int)fastdiv((uint32_t)tid, T.mx, T.sx), xl = tid - zl * T.ex;
+            hv[1] = value(T.x0 + xl, yn, T.z0 + zl);
+        }
+        if (tid < f2) {
+            const int yl = (int)fastdiv((uint32_t)tid, T.mx, T.sx), xl = tid - yl * T.ex;
+            hv[2] = value(T.x0 + xl, T.y0 + yl, zn);
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int li = tid + k * kCB;
+            if (li < T.n) {
+                s_v[li] = v[k];
+                s_lab[li] = (uint32_t)li;
+            }
+        }
+        if (tid < f0) s_h[0][tid] = hv[0];
+        if (tid < f1) s_h[1][tid] = hv[1];
+        if (tid < f2) s_h[2][tid] = hv[2];
     }
     __syncthreads();
 
@@ -270,47 +325,40 @@ __global__ __launch_bounds__(kCB) void cluster_bond_kernel(ClusterArgs a) {
     uint8_t* bond = a.bond + C.lab0 + (int64_t)T.r * C.nsites;
     if constexpr (NC == 0) {
         const uint32_t thr = a.inline_p ? a.bthr_inline : a.bthr[tslot(a, C, T.r)];
-        const uint32_t hi = thr >> 24, lo = thr & 0xffffffu;
+        const uint32_t hi = thr >> 16, lo = thr & 0xffffu;
         const bool ferro = a.J > 0.0;
-        const int gq = (T.ex + 3) >> 2;  // Philox groups per tile row (x0 is a multiple of 4)
-        for (int k = tid; k < T.ey * T.ez * gq; k += kCB) {
-            const int row = k / gq, q = k - row * gq;
-            const int zl = row / T.ey, yl = row - zl * T.ey;
-            const int y = T.y0 + yl, z = T.z0 + zl;
-            const uint32_t grp = (uint32_t)(((int64_t)z * L + y) * C.q4 + (T.x0 >> 2) + q);
-            const u32x4 w4 = philox(grp, a.u, stream, tag(P_BOND, MODEL_ISING), a.k0, a.k1);
-            const uint32_t wk[4] = {w4.x, w4.y, w4.z, w4.w};
-            for (int s = 0; s < 4; ++s) {
-                const int xl = 4 * q + s;
-                if (xl >= T.ex) break;
-                const int li = (zl * T.ey + yl) * T.ex + xl;
+        // one Philox call per pair of sites (x even, x + 1): ex and x0 are even
+        for (int k = tid; 2 * k < T.n; k += kCB) {
+            const int li0 = 2 * k;
+            int xl, yl, zl;
+            decode(T, (uint32_t)li0, &xl, &yl, &zl);
+            const uint32_t site0 = site_of(C, T.x0 + xl, T.y0 + yl, T.z0 + zl);
+            const u32x4 w4 = philox(site0 >> 1, a.u, stream, tag(P_BOND, MODEL_ISING), a.k0, a.k1);
+            for (int s = 0; s < 2; ++s) {
+                const int li = li0 + s;
+                const uint32_t wa = s ? w4.z : w4.x, wb = s ? w4.w : w4.y;
                 const S si = s_v[li];
-                uint32_t bits = (wk[s] >> 31) << 3;
+                uint32_t bits = (wb >> 31) << 3;
                 for (int d = 0; d < dims; ++d) {
-                    const S sj = nbr(li, xl, yl, zl, d);
+                    const S sj = nbr(li, xl + s, yl, zl, d);
                     if ((si == sj) != ferro) continue;
-                    const uint32_t r8 = (wk[s] >> (8 * d)) & 0xffu;
-                    bool act = r8 < hi;
-                    if (r8 == hi && lo != 0u) {
-                        const u32x4 t4 = philox(4u * grp + (uint32_t)s, a.u, stream, tag(P_BTIE, MODEL_ISING), a.k0,
-                                                a.k1);
-                        const uint32_t tw = d == 0 ? t4.x : (d == 1 ? t4.y : t4.z);
-                        act = (tw >> 8) < lo;
-                    }
+                    const uint32_t r16 = d == 0 ? wa & 0xffffu : (d == 1 ? wa >> 16 : wb & 0xffffu);
+                    bool act = r16 < hi;
+                    if (r16 == hi && lo != 0u)  // 1 in 65536: 16 more bits decide
+                        act = (philox(site0 + s, a.u, stream, tag(P_BTIE, d), a.k0, a.k1).x >> 16) < lo;
                     bits |= (uint32_t)act << d;
                 }
                 s_bond[li] = (uint8_t)bits;
-                bond[site_of(C, T.x0 + xl, y, z)] = (uint8_t)bits;
             }
         }
     } else {
         const double beta = a.inline_p ? a.beta_inline : a.betas[tslot(a, C, T.r)];
         const float kappa = (float)(2.0 * beta * a.J);
         for (int li = tid; li < T.n; li += kCB) {
-            const int row = li / T.ex, xl = li - row * T.ex;
-            const int zl = row / T.ey, yl = row - zl * T.ey;
-            const int64_t site = site_of(C, T.x0 + xl, T.y0 + yl, T.z0 + zl);
-            const u32x4 w4 = philox((uint32_t)site, a.u, stream, tag(P_BOND, a.model), a.k0, a.k1);
+            int xl, yl, zl;
+            decode(T, (uint32_t)li, &xl, &yl, &zl);
+            const uint32_t site = site_of(C, T.x0 + xl, T.y0 + yl, T.z0 + zl);
+            const u32x4 w4 = philox(site, a.u, stream, tag(P_BOND, a.model), a.k0, a.k1);
             const uint32_t wk[3] = {w4.x, w4.y, w4.z};
             const float ai = s_v[li];
             uint32_t bits = (w4.w >> 31) << 3;
@@ -319,78 +367,122 @@ __global__ __launch_bounds__(kCB) void cluster_bond_kernel(ClusterArgs a) {
                 if (x > 0.f && u24f(wk[d]) < -expm1f(-x)) bits |= 1u << d;
             }
             s_bond[li] = (uint8_t)bits;
-            bond[site] = (uint8_t)bits;
         }
     }
     __syncthreads();
 
-    // union-find over the bonds inside the tile
-    const int step[3] = {1, T.ex, T.ex * T.ey};
-    for (int li = tid; li < T.n; li += kCB) {
-        const uint32_t bits = s_bond[li];
-        if (!(bits & 7u)) continue;
-        const int row = li / T.ex, xl = li - row * T.ex;
-        const int zl = row / T.ey, yl = row - zl * T.ey;
-        const int cc[3] = {xl + 1 < T.ex, yl + 1 < T.ey, zl + 1 < T.ez};
-        for (int d = 0; d < dims; ++d)
-            if (((bits >> d) & 1u) && cc[d]) lds_union(s_lab, (uint32_t)li, (uint32_t)(li + step[d]));
+    // union-find over the bonds inside the tile, one direction per round: the
+    // active bonds of the round are first compacted into a list (so every
+    // lane of a wave has a union to do), then united
+    uint32_t inner[kPer];  // bit d: the +d neighbour lies in the tile
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int li = tid + k * kCB;
+        inner[k] = 0u;
+        if (li < T.n) {
+            int xl, yl, zl;
+            decode(T, (uint32_t)li, &xl, &yl, &zl);
+            inner[k] = (uint32_t)(xl + 1 < T.ex) | ((uint32_t)(yl + 1 < T.ey) << 1) | ((uint32_t)(zl + 1 < T.ez) << 2);
+        }
     }
-    __syncthreads();
+    const int step[3] = {1, T.ex, T.ex * T.ey};
+    for (int d = 0; d < dims; ++d) {
+        if (tid == 0) s_nlist = 0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int li = tid + k * kCB;
+            const bool act = li < T.n && ((s_bond[li] & inner[k]) >> d) & 1u;
+            append(s_list, &s_nlist, act, (uint16_t)li);
+        }
+        __syncthreads();
+        const int n = s_nlist;
+        for (int e = tid; e < n; e += kCB) {
+            const uint32_t li = s_list[e];
+            lds_union(s_lab, li, li + (uint32_t)step[d]);
+        }
+        __syncthreads();
+    }
 
+    // every site's label: the site index of its tile-local root (the smallest
+    // index of its tile-local cluster); the bond byte gains bit 4 at those roots
     int32_t* lab = a.label + C.lab0 + (int64_t)T.r * C.nsites;
     for (int li = tid; li < T.n; li += kCB) {
-        const int row = li / T.ex, xl = li - row * T.ex;
-        const int zl = row / T.ey, yl = row - zl * T.ey;
-        const int rt = (int)lds_find(s_lab, (uint32_t)li);
-        const int rrow = rt / T.ex, rxl = rt - rrow * T.ex;
-        const int rzl = rrow / T.ey, ryl = rrow - rzl * T.ey;
-        lab[site_of(C, T.x0 + xl, T.y0 + yl, T.z0 + zl)] =
-            (int32_t)site_of(C, T.x0 + rxl, T.y0 + ryl, T.z0 + rzl);
+        int xl, yl, zl, rx, ry, rz;
+        decode(T, (uint32_t)li, &xl, &yl, &zl);
+        const uint32_t rt = lds_find(s_lab, (uint32_t)li);
+        decode(T, rt, &rx, &ry, &rz);
+        const uint32_t site = site_of(C, T.x0 + xl, T.y0 + yl, T.z0 + zl);
+        lab[site] = (int32_t)site_of(C, T.x0 + rx, T.y0 + ry, T.z0 + rz);
+        bond[site] = (uint8_t)(s_bond[li] | (rt == (uint32_t)li ? 16u : 0u));
     }
 }
 
 // ---------------------------------------------------------------------------
-// Bonds that leave a tile through its + faces.
+// Bonds that leave a tile through its + faces: union of the two tile-local
+// roots (finds start at the roots, so the labels of the other sites keep
+// pointing at their tile-local root).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kCB) void cluster_merge_kernel(ClusterArgs a) {
     const int g = find_cgroup(a.cg, a.ngroups, blockIdx.x);
     const ClusterGroup C = a.cg[g];
     const Tile T = tile_of(C, (int)blockIdx.x - C.blk0);
-    const int L = C.L;
+    const int L = C.L, tid = threadIdx.x;
     const uint8_t* bond = a.bond + C.lab0 + (int64_t)T.r * C.nsites;
     int32_t* lab = a.label + C.lab0 + (int64_t)T.r * C.nsites;
-    const int tid = threadIdx.x;
-    {
-        const int x = T.x0 + T.ex - 1, xn = (x + 1) % L;
-        for (int k = tid; k < T.ey * T.ez; k += kCB) {
-            const int zl = k / T.ey, yl = k - zl * T.ey;
-            const int64_t i = site_of(C, x, T.y0 + yl, T.z0 + zl);
-            if (bond[i] & 1u) g_union(lab, (int32_t)i, (int32_t)site_of(C, xn, T.y0 + yl, T.z0 + zl));
+    // this thread's site on each + face (a face has at most kCB sites) and its neighbour across
+    uint32_t fi[3], fj[3];
+    bool on[3] = {false, false, false};
+    if (tid < T.ey * T.ez) {
+        const int zl = (int)fastdiv((uint32_t)tid, T.my, T.sy), yl = tid - zl * T.ey;
+        const int x = T.x0 + T.ex - 1;
+        fi[0] = site_of(C, x, T.y0 + yl, T.z0 + zl);
+        fj[0] = site_of(C, x + 1 == L ? 0 : x + 1, T.y0 + yl, T.z0 + zl);
+        on[0] = true;
+    }
+    if (tid < T.ex * T.ez) {
+        const int zl = (int)fastdiv((uint32_t)tid, T.mx, T.sx), xl = tid - zl * T.ex;
+        const int y = T.y0 + T.ey - 1;
+        fi[1] = site_of(C, T.x0 + xl, y, T.z0 + zl);
+        fj[1] = site_of(C, T.x0 + xl, y + 1 == L ? 0 : y + 1, T.z0 + zl);
+        on[1] = true;
+    }
+    if (C.dims == 3 && tid < T.ex * T.ey) {
+        const int yl = (int)fastdiv((uint32_t)tid, T.mx, T.sx), xl = tid - yl * T.ex;
+        const int z = T.z0 + T.ez - 1;
+        fi[2] = site_of(C, T.x0 + xl, T.y0 + yl, z);
+        fj[2] = site_of(C, T.x0 + xl, T.y0 + yl, z + 1 == L ? 0 : z + 1);
+        on[2] = true;
+    }
+    uint32_t bb[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) bb[d] = on[d] ? bond[fi[d]] : 0u;
+    int32_t ra[3], rb[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        on[d] = on[d] && ((bb[d] >> d) & 1u);
+        if (on[d]) {
+            ra[d] = g_load(&lab[fi[d]]);
+            rb[d] = g_load(&lab[fj[d]]);
         }
     }
-    {
-        const int y = T.y0 + T.ey - 1, yn = (y + 1) % L;
-        for (int k = tid; k < T.ex * T.ez; k += kCB) {
-            const int zl = k / T.ex, xl = k - zl * T.ex;
-            const int64_t i = site_of(C, T.x0 + xl, y, T.z0 + zl);
-            if (bond[i] & 2u) g_union(lab, (int32_t)i, (int32_t)site_of(C, T.x0 + xl, yn, T.z0 + zl));
-        }
-    }
-    if (C.dims == 3) {
-        const int z = T.z0 + T.ez - 1, zn = (z + 1) % L;
-        for (int k = tid; k < T.ex * T.ey; k += kCB) {
-            const int yl = k / T.ex, xl = k - yl * T.ex;
-            const int64_t i = site_of(C, T.x0 + xl, T.y0 + yl, z);
-            if (bond[i] & 4u) g_union(lab, (int32_t)i, (int32_t)site_of(C, T.x0 + xl, T.y0 + yl, zn));
-        }
-    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (on[d]) g_union(lab, ra[d], rb[d]);
 }
 
 // ---------------------------------------------------------------------------
-// Roots and flips.
+// Roots and flips.  Phase 1 resolves the cluster root of every tile-local root
+// and decides its flip (memoised in LDS); phase 2 applies it to every site
+// through its tile-local root.  Each thread's sites are processed together so
+// their loads overlap.
 // ---------------------------------------------------------------------------
 template <int NC>
 __global__ __launch_bounds__(kCB) void cluster_flip_kernel(ClusterArgs a) {
+    using P = typename std::conditional<NC == 0, uint8_t, float>::type;
+    __shared__ uint8_t s_flip[kClusterTile];
+    __shared__ uint16_t s_list[kClusterTile];
+    __shared__ int s_nlist;
     __shared__ int32_t s_seed;
     __shared__ int64_t s_cnt[kCB / 64];
     const int g = find_cgroup(a.cg, a.ngroups, blockIdx.x);
@@ -401,34 +493,119 @@ __global__ __launch_bounds__(kCB) void cluster_flip_kernel(ClusterArgs a) {
     const uint8_t* bond = a.bond + C.lab0 + (int64_t)T.r * C.nsites;
     int32_t* lab = a.label + C.lab0 + (int64_t)T.r * C.nsites;
     const int tid = threadIdx.x;
-    if (a.kind == 1) {
-        if (tid == 0) {
+    if (tid == 0) {
+        s_nlist = 0;
+        if (a.kind == 1) {
             const u32x4 w = philox(0u, a.u, stream, tag(P_WSEED, NC == 0 ? MODEL_ISING : a.model), a.k0, a.k1);
             const uint64_t v = ((uint64_t)w.x << 32) | w.y;
             const int32_t seed = (int32_t)__umul64hi(v, (uint64_t)C.nsites);
-            s_seed = g_find(lab, seed);
+            s_seed = g_find(lab, g_load(&lab[seed]));
         }
-        __syncthreads();
     }
+    __syncthreads();
+    uint32_t site[kPer], bb[kPer];
+    int xs[kPer], ys[kPer], zs[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int li = tid + k * kCB;
+        bb[k] = 0u;
+        if (li < T.n) {
+            decode(T, (uint32_t)li, &xs[k], &ys[k], &zs[k]);
+            site[k] = site_of(C, T.x0 + xs[k], T.y0 + ys[k], T.z0 + zs[k]);
+            bb[k] = bond[site[k]] | 32u;  // bit 5: in the tile
+        }
+    }
+    // phase 1: the tile-local roots, compacted into a list, walk their parent
+    // chains in step (kPer per thread at most: a tile has n <= kPer * kCB sites)
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) append(s_list, &s_nlist, (bb[k] & 16u) != 0u, (uint16_t)(tid + k * kCB));
+    __syncthreads();
+    const int nroots = s_nlist;
+    int32_t cur[kPer], own[kPer];
+    bool walk[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int e = tid + k * kCB;
+        walk[k] = e < nroots;
+        if (walk[k]) {
+            int xl, yl, zl;
+            decode(T, s_list[e], &xl, &yl, &zl);
+            own[k] = cur[k] = (int32_t)site_of(C, T.x0 + xl, T.y0 + yl, T.z0 + zl);
+        }
+    }
+    bool mine[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) mine[k] = walk[k];
+    for (;;) {
+        int32_t par[kPer];
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            if (walk[k]) par[k] = g_load(&lab[cur[k]]);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            if (walk[k]) {
+                if (par[k] == cur[k]) walk[k] = false;
+                else cur[k] = par[k];
+                any |= walk[k];
+            }
+        if (!any) break;
+    }
+    // point each tile-local root straight at its cluster root (later walks of
+    // other tiles through it get shorter), and decide its flip
+    uint32_t coin[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+        if (mine[k]) {
+            if (cur[k] != own[k])
+                __hip_atomic_store(&lab[own[k]], cur[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (a.kind == 0) coin[k] = bond[cur[k]];
+        }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+        if (mine[k])
+            s_flip[s_list[tid + k * kCB]] = (uint8_t)(a.kind == 1 ? cur[k] == s_seed : (coin[k] >> 3) & 1u);
+    __syncthreads();
+
+    // phase 2: every site takes the decision of its tile-local root
     float rv[3] = {0.f, 0.f, 0.f};
     if constexpr (NC > 0) direction<NC>(a, stream, rv);
+    P* base[2];
+    if constexpr (NC == 0) {
+        base[0] = a.ipool + G.off + 2 * (int64_t)T.r * G.stride;
+        base[1] = base[0] + G.stride;
+    } else {
+        base[0] = a.fpool + G.off + 2 * (int64_t)T.r * NC * G.stride;
+        base[1] = base[0] + NC * G.stride;
+    }
+    const uint32_t tbase = site_of(C, T.x0, T.y0, T.z0);
+    const uint32_t L = (uint32_t)C.L;
+    int32_t lr[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+        if ((bb[k] & 48u) == 32u) lr[k] = lab[site[k]];  // not a tile-local root: its label names that root
     int cnt = 0;
-    for (int li = tid; li < T.n; li += kCB) {
-        const int row = li / T.ex, xl = li - row * T.ex;
-        const int zl = row / T.ey, yl = row - zl * T.ey;
-        const int x = T.x0 + xl, y = T.y0 + yl, z = T.z0 + zl;
-        const int32_t rt = g_find(lab, (int32_t)site_of(C, x, y, z));
-        const bool flip = a.kind == 1 ? rt == s_seed : ((bond[rt] >> 3) & 1u) != 0u;
-        if (!flip) continue;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        if (!(bb[k] & 32u)) continue;
+        int li = tid + k * kCB;
+        if (!(bb[k] & 16u)) {
+            const uint32_t d = (uint32_t)lr[k] - tbase;
+            const uint32_t q = fastdiv(d, C.mag_L, C.sh_L);
+            const uint32_t dz = fastdiv(q, C.mag_L, C.sh_L);
+            li = (int)((dz * (uint32_t)T.ey + (q - dz * L)) * (uint32_t)T.ex + (d - q * L));
+        }
+        if (!s_flip[li]) continue;
         ++cnt;
+        const int x = T.x0 + xs[k], y = T.y0 + ys[k], z = T.z0 + zs[k];
         if constexpr (NC == 0) {
-            uint8_t* p = ising_byte(a, G, T.r, x, y, z);
+            uint8_t* p = at(base, G, x, y, z);
             *p ^= 1u;
         } else {
-            float* p = cont_ptr<NC>(a, G, T.r, x, y, z);
+            float* p = at(base, G, x, y, z);
             const float s2 = 2.f * project<NC>(p, G.stride, rv);
 #pragma unroll
-            for (int k = 0; k < NC; ++k) p[k * G.stride] -= s2 * rv[k];
+            for (int c = 0; c < NC; ++c) p[c * G.stride] -= s2 * rv[c];
         }
     }
     if (!a.flipped) return;

@@ -112,6 +112,11 @@ struct ClusterGroup {
     int64_t lab0;             // element of replica 0 in the label and bond arrays
     int32_t q4;               // ceil(L / 4): Philox groups per lattice row
     int32_t pad;
+    // n / d == fastdiv(n, mag, sh) for d = L, and for the x / y extents of a
+    // full tile ([0]) and of the last tile along the axis ([1])
+    uint32_t mag_L, sh_L;
+    uint32_t mag_x[2], sh_x[2];
+    uint32_t mag_y[2], sh_y[2];
 };
 struct ClusterArgs {
     const DevGroup* groups;

@@ -1033,9 +1033,9 @@ static int64_t choose_flips(int64_t N, int kind, uint32_t u, uint64_t seed, uint
 
 int64_t o_ising_cluster(int dim, int L, int8_t* sp, int kind, uint32_t u, uint64_t seed, uint32_t stream,
                         uint32_t thr, double J, int32_t* lab, uint8_t* bonds) {
-    const int Lz = dim == 3 ? L : 1, q4 = (L + 3) / 4;
+    const int Lz = dim == 3 ? L : 1;
     const int64_t N = (int64_t)Lz * L * L;
-    const uint32_t hi = thr >> 24, lo = thr & 0xffffffu;
+    const uint32_t hi = thr >> 16, lo = thr & 0xffffu;
     uint8_t* own = NULL;
     if (!bonds) bonds = own = (uint8_t*)malloc((size_t)N);
     for (int64_t i = 0; i < N; ++i) lab[i] = (int32_t)i;
@@ -1043,21 +1043,21 @@ int64_t o_ising_cluster(int dim, int L, int8_t* sp, int kind, uint32_t u, uint64
         for (int y = 0; y < L; ++y)
             for (int x = 0; x < L; ++x) {
                 const int64_t i = ((int64_t)z * L + y) * L + x;
-                const uint32_t grp = (uint32_t)(((int64_t)z * L + y) * q4 + x / 4);
                 uint32_t w[4];
-                ph(grp, u, stream, o_tag(7, 0), seed, w);
-                const uint32_t wk = w[x & 3];
-                uint32_t bits = (wk >> 31) << 3;
+                ph((uint32_t)(i >> 1), u, stream, o_tag(7, 0), seed, w);
+                const uint32_t wa = w[2 * (x & 1)], wb = w[2 * (x & 1) + 1];
+                uint32_t bits = (wb >> 31) << 3;
                 for (int d = 0; d < dim; ++d) {
                     const int64_t j = fwd(dim, L, x, y, z, d);
                     const int sat = J > 0.0 ? sp[i] == sp[j] : sp[i] != sp[j];
                     if (!sat) continue;
-                    const uint32_t r8 = (wk >> (8 * d)) & 0xffu;
-                    int act = r8 < hi;
-                    if (r8 == hi && lo != 0u) {
+                    const uint32_t r16 = d == 0 ? wa & 0xffffu : (d == 1 ? wa >> 16 : wb & 0xffffu);
+                    int act = r16 < hi;
+                    if (r16 == hi && lo != 0u) {
                         uint32_t t[4];
-                        ph(4u * grp + (uint32_t)(x & 3), u, stream, o_tag(8, 0), seed, t);
-                        act = (t[d] >> 8) < lo;
+                        ph((uint32_t)i, u, stream, o_tag(8, d), seed, t);
+                        act = (t[0] >> 16) < lo;
+                        bits |= 32u << d; /* decided by the tie draw (reported, not used) */
                     }
                     if (act) {
                         bits |= 1u << d;

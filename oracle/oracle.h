@@ -174,15 +174,18 @@ void o_ising_chain(int dim, int L, uint64_t seed, double beta, double J, double 
  * between satisfied neighbours (aligned for J > 0, anti-aligned for J < 0)
  * with probability floor(2^32 p) / 2^32, p = 1 - exp(-2 beta |J|)
  * (src/wolff.rs:27-77 grows the same clusters from one seed).  Draws:
- * Philox(c0 = row * ceil(L/4) + x/4, u, stream, tag(7, 0)), word x & 3,
- * byte d against the high 8 bits of the threshold; on equality the 24-bit
- * draw Philox(4 c0 + (x & 3), u, stream, tag(8, 0)) word d >> 8 against the
- * low 24 bits.  Bit 31 of the site's word is its cluster coin.  Clusters by
+ * w = Philox(c0 = site / 2, u, stream, tag(7, 0)); the site's words are
+ * (wa, wb) = (w0, w1) for even x, (w2, w3) for odd x; the 16-bit draws of
+ * bonds +x, +y, +z are wa & 0xffff, wa >> 16, wb & 0xffff, compared with the
+ * high 16 bits of the threshold; on equality the 16-bit draw
+ * Philox(site, u, stream, tag(8, d)) word 0 >> 16 against the low 16 bits.
+ * Bit 31 of wb is the site's cluster coin.  Clusters by
  * union-find (root = smallest site index); Swendsen-Wang (kind 0) flips a
  * cluster when its root's coin is set, Wolff (kind 1) flips the cluster of
  * the site floor(N ((w0 << 32) | w1) / 2^64), w = Philox(0, u, stream, tag(10, 0)).
  * Returns the number of sites flipped; bonds[i] (when not NULL) receives the
- * site's bond bits (bit d) and coin (bit 3).
+ * site's bond bits (bit d), coin (bit 3) and, in bit 5 + d, whether bond d
+ * was decided by the tie draw.
  * Pinned by: exact stationarity of the Boltzmann distribution on 2x2x2 and
  * 4x4 lattices (full transition matrices from enumerated bond draws, not from
  * this code), clusters = connected components found by breadth-first search,

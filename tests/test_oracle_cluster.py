@@ -8,8 +8,9 @@ Checks, none of which restates the oracle's own formulas:
   breadth-first search here; bonds only on satisfied pairs; the flip rule
   (Swendsen-Wang: the component's smallest site's coin; Wolff: the component
   of the Philox-drawn seed site);
-* the frequency of active bonds among satisfied pairs equals floor(2^32 p) / 2^32
-  at a temperature where the 24-bit tie path carries a visible share;
+* the frequency of active bonds among satisfied pairs equals floor(2^32 p) / 2^32,
+  and among the bonds decided by the 16-bit tie draw equals the threshold's
+  low half / 2^16;
 * the Boltzmann distribution is stationary: the joint (B, M) histogram of long
   chains on the 2x2x2 cube and the 4x4 square matches exact enumeration, also
   for an antiferromagnet (src/wolff.rs:27-77 samples the same distribution);
@@ -125,13 +126,9 @@ def test_clusters_are_components_and_flip_rule(dim, L, J, kind):
 
 
 def test_bond_frequency_and_coins():
-    # beta chosen so that the low 24 bits of the threshold are about half full:
-    # dropping the tie draw would move the frequency by ~2e-3 (over 10 sigma here)
     L, dim = 16, 3
     beta = 0.2216544
     thr = o.bond_threshold(beta)
-    lo_share = (thr & 0xFFFFFF) / 2 ** 24
-    assert 0.2 < lo_share < 0.8
     p = thr / 2 ** 32
     rng = np.random.default_rng(7)
     roots = coins = 0
@@ -156,6 +153,30 @@ def test_bond_frequency_and_coins():
         sd = math.sqrt(p * (1 - p) / per_dir[0, d])
         assert abs(per_dir[1, d] / per_dir[0, d] - p) < 4.5 * sd
     assert abs(coins / roots - 0.5) < 4 * math.sqrt(0.25 / roots)
+
+
+def test_tie_draw_decides_with_the_low_threshold_bits():
+    # a 16-bit draw equal to the threshold's high half (1 bond in 65536) is
+    # decided by 16 more bits against the low half: the active fraction among
+    # those bonds is lo / 65536 (0.81 here; 0 or 1 if the tie path were dropped
+    # or forced, 0.19 if its comparison were reversed)
+    L, dim, beta = 64, 3, 0.211
+    thr = o.bond_threshold(beta)
+    q = (thr & 0xFFFF) / 65536
+    assert 0.75 < q < 0.85
+    rng = np.random.default_rng(3)
+    ties = active = 0
+    u = 0
+    while ties < 800:
+        sp = (rng.integers(0, 2, L ** 3) * 2 - 1).astype(np.int8)
+        _, bonds = o.ising_cluster(dim, L, sp, 0, u, SEED, (L << 16) | 5, beta=beta)
+        for d in range(3):
+            t = ((bonds >> (5 + d)) & 1).astype(bool)
+            ties += int(t.sum())
+            active += int((((bonds >> d) & 1).astype(bool) & t).sum())
+        u += 1
+    f = active / ties
+    assert abs(f - q) < 4 * math.sqrt(q * (1 - q) / ties), (f, q, ties)
 
 
 # --- stationarity against exact enumeration ---------------------------------
