@@ -185,15 +185,16 @@ def run_learned(args, ising_hip):
     kw = dict(exchange_every=10, measure_every=1, n_overrelax=2, delta=1.0, adapt_every=10, adapt_target=0.5,
               renorm_every=100, record_capacity=steps + 8)
     out = {}
-    for mode in ("plain", "learned", "cluster"):
-        pt = ising_hip.PTEnsemble("xy", sizes, Rr, temps, args.seed, **kw)
+    for mode in ("plain", "learned", "learned_or0", "cluster"):
+        nor = {"plain": 2, "learned": args.learned_or, "learned_or0": 0, "cluster": 2}[mode]
+        pt = ising_hip.PTEnsemble("xy", sizes, Rr, temps, args.seed, **dict(kw, n_overrelax=nor))
         pt.warmup(args.warmup)
         if mode == "cluster":
             out[mode] = xy_cluster_leg(pt, sizes, Rr, temps, steps, ising_hip)
             pt.close()
             continue
         train = None
-        if mode == "learned":
+        if mode.startswith("learned"):
             pt.learn_init(args.hidden, seed=7)
             t0 = time.perf_counter()
             loss = pt.learn_train(args.train_steps, batch=8192, lr=3e-3)
@@ -234,7 +235,7 @@ def run_learned(args, ising_hip):
                 taus_m.append(ising_hip.tau_int(m2[:, g, k])[0])
         taus_e, taus_m = np.array(taus_e), np.array(taus_m)
         isps = float(np.sum(steps / (2.0 * np.maximum(taus_e, 0.5))) / wall)
-        tries, acc = pt.mh_stats() if mode == "learned" else (0, 0)
+        tries, acc = pt.mh_stats() if mode.startswith("learned") else (0, 0)
         m2s = temperature_series(tidx, (mx * mx + my * my))
         u4 = [float(np.mean(1 - (m2s[:, g, :] ** 2).mean(axis=0) / (3 * m2s[:, g, :].mean(axis=0) ** 2)))
               for g in range(len(sizes))]
@@ -261,9 +262,13 @@ def run_learned(args, ising_hip):
         "dtype": "f32 (MLP bf16 weights, f32 accumulate)",
         "data": "synthetic",
         "config": {"workload": "learned proposal vs plain (config 5): XY L=32,64 x 32 replicas, T=linspace(2.18,2.22,32), "
-                               "1 Metropolis + 2 overrelaxation passes per sweep, measure every sweep, exchange every 10",
+                               "plain = 1 Metropolis + 2 overrelaxation passes per sweep, learned = the learned "
+                               "proposal's pass + learned_overrelaxation_passes, measure every sweep, exchange every 10",
                    "hidden": args.hidden, "sites": sites, "plain": out["plain"], "learned": out["learned"],
-                   "cluster": out["cluster"], "learned_over_plain": round(ratio, 4),
+                   "learned_or0": out["learned_or0"], "cluster": out["cluster"],
+                   "learned_overrelaxation_passes": args.learned_or, "learned_over_plain": round(ratio, 4),
+                   "learned_or0_over_plain": round(out["learned_or0"]["independent_samples_per_s"]
+                                                   / out["plain"]["independent_samples_per_s"], 4),
                    "cluster_over_plain": round(cratio, 4)},
     }
     print(json.dumps(line), flush=True)
@@ -426,6 +431,8 @@ def main():
                                                                   "cluster"])
     ap.add_argument("--hidden", type=int, default=16, help="learned workload: MLP hidden width (8, 16, 32 or 64)")
     ap.add_argument("--train-steps", type=int, default=400, help="learned workload: Adam steps")
+    ap.add_argument("--learned-or", type=int, default=2,
+                    help="learned workload: overrelaxation passes per sweep of the learned chain (plain keeps 2)")
     ap.add_argument("--cluster-part", default="all", choices=["all", "throughput"],
                     help="cluster workload: 'throughput' times only the Swendsen-Wang updates of the config-2 ensemble")
     ap.add_argument("--layout", default="auto", metavar="W,R,S",

@@ -217,6 +217,10 @@ int ising_pt_learn_set_params(ising_pt_t* pt, const float* in, int64_t n);
 int ising_pt_learn_train(ising_pt_t* pt, int32_t steps, int32_t batch, double lr, double* mean_loss);
 /* Loss and mean gradient on explicit samples (x: n * 6 features, c: n values of s.h^). */
 int ising_pt_learn_eval(ising_pt_t* pt, int64_t n, const float* x, const float* c, double* mean_loss, float* grad);
+/* Network outputs (n * 3 floats: two concentration logits and the mixture
+ * logit) for n feature rows, evaluated as the fused sweep does: BF16 weights,
+ * inputs and activations on the matrix cores, FP32 accumulation. */
+int ising_pt_learn_infer(ising_pt_t* pt, int64_t n, const float* x, float* out);
 /* 1: sweeps use the learned proposal for the Metropolis passes; 0: the cap/rotation proposal. */
 int ising_pt_learn_enable(ising_pt_t* pt, int32_t on);
 /* Learned Metropolis-Hastings moves attempted / accepted since creation. */

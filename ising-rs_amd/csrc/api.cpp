@@ -145,6 +145,7 @@ struct ising_pt {
     float *d_lm1 = nullptr, *d_lm2 = nullptr, *d_lgpart = nullptr, *d_lgout = nullptr;
     double* d_llpart = nullptr;
     int learn_cap_blocks = 0;
+    int64_t* d_mh = nullptr;  // [kMhSlots][8] learned-move counters
     float beta_inline = 0.f, delta_inline = 0.f, j_inline = 1.f;
     int layout_width = 0, layout_runs = 0, layout_spt = 0;  // Ising thread-step layout in use
     uint32_t* d_sched = nullptr;   // annealing schedule: table rows per sweep
@@ -201,6 +202,7 @@ struct ising_pt {
         (void)hipFree(d_lgpart);
         (void)hipFree(d_lgout);
         (void)hipFree(d_llpart);
+        (void)hipFree(d_mh);
         (void)hipFree(d_cg);
         (void)hipFree(d_clabel);
         (void)hipFree(d_cbond);
@@ -272,6 +274,7 @@ struct ising_pt {
         a.beta_inline = beta_inline;
         a.delta_inline = delta_inline;
         a.net = d_lnet;
+        a.mh = d_mh;
         a.learn_H = learn_H;
         return a;
     }
@@ -985,6 +988,10 @@ int ising_pt_learn_init(ising_pt_t* p, int32_t hidden, uint64_t seed) {
     HIP_TRY(hipMalloc(&p->d_lgpart, sizeof(float) * (size_t)P * cap));
     HIP_TRY(hipMalloc(&p->d_lgout, sizeof(float) * P));
     HIP_TRY(hipMalloc(&p->d_llpart, sizeof(double) * cap));
+    if (!p->d_mh) {
+        HIP_TRY(hipMalloc(&p->d_mh, sizeof(int64_t) * 8 * kMhSlots));
+        ST_FILL(p, p->d_mh, 0, sizeof(int64_t) * 8 * kMhSlots);
+    }
     p->learn_H = hidden;
     p->learn_P = P;
     p->learn_cap_blocks = cap;
@@ -1107,6 +1114,25 @@ int ising_pt_learn_eval(ising_pt_t* p, int64_t n, const float* x, const float* c
     return ISING_OK;
 }
 
+int ising_pt_learn_infer(ising_pt_t* p, int64_t n, const float* x, float* out) {
+    int rc = need_learn(p);
+    if (rc) return rc;
+    if (n < 1 || !x || !out) return fail(ISING_E_ARG, "need n >= 1 feature rows and an output buffer");
+    if (n > (1ll << 28)) return fail(ISING_E_ARG, "too many feature rows");
+    ORDER(p);
+    float *dx = nullptr, *dout = nullptr;
+    HIP_TRY(hipMalloc(&dx, sizeof(float) * 6 * n));
+    hipError_t e = hipMalloc(&dout, sizeof(float) * 3 * n);
+    if (e == hipSuccess) e = hipMemcpyAsync(dx, x, sizeof(float) * 6 * n, hipMemcpyHostToDevice, p->st);
+    if (e == hipSuccess) e = launch_learn_infer(p->d_lnet, p->learn_H, dx, n, dout, p->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, p->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->st);
+    (void)hipFree(dx);
+    (void)hipFree(dout);
+    HIP_TRY(e);
+    return ISING_OK;
+}
+
 int ising_pt_learn_enable(ising_pt_t* p, int32_t on) {
     if (on) {
         const int rc = need_learn(p);
@@ -1127,8 +1153,17 @@ int ising_pt_mh_stats(ising_pt_t* p, int64_t* tries, int64_t* accepts) {
     DevCtl c;
     const int rc = p->read_ctl(&c);
     if (rc) return rc;
-    if (tries) *tries = c.mh_tries;
-    if (accepts) *accepts = c.mh_accepts;
+    int64_t t = c.mh_tries, acc = c.mh_accepts;
+    if (p->d_mh) {
+        std::vector<int64_t> v(8 * kMhSlots);
+        ST_COPY(p, v.data(), p->d_mh, sizeof(int64_t) * v.size(), hipMemcpyDeviceToHost);
+        for (int k = 0; k < kMhSlots; ++k) {
+            t += v[8 * (size_t)k];
+            acc += v[8 * (size_t)k + 1];
+        }
+    }
+    if (tries) *tries = t;
+    if (accepts) *accepts = acc;
     return ISING_OK;
 }
 

@@ -37,6 +37,7 @@ hipError_t launch_msc_init(const DevGroup* groups, int ngroups, uint32_t* pool, 
 hipError_t launch_msc_measure(const DevGroup* groups, int ngroups, const uint32_t* pool, int total_reps, int nblocks,
                               int64_t* acc, hipStream_t st);
 
+constexpr int kMhSlots = 64;  // counter copies, so that waves do not contend on one address
 struct ContArgs {
     const DevGroup* groups;
     int32_t ngroups;
@@ -64,6 +65,7 @@ struct ContArgs {
     int32_t inline_params; // 1: beta/delta from the two fields below (single-lattice passes)
     float beta_inline, delta_inline;
     const uint16_t* net;   // learned proposal: BF16 parameters (null when off)
+    int64_t* mh;           // learned proposal: [kMhSlots][8] moves tried (0) / accepted (1), spread by wave
     int32_t learn_H;       // hidden width of the learned proposal's MLP
 };
 
@@ -76,6 +78,7 @@ hipError_t launch_learn_grad(const ContArgs& a, int nblk, int64_t step, int mode
 hipError_t launch_learn_adam(float* master, uint16_t* net, float* m1, float* m2, const float* gpart, int nblk, int P,
                              float inv_n, float lr, int64_t step, float* gout, hipStream_t st);
 hipError_t launch_learn_pack(const float* master, uint16_t* net, int P, hipStream_t st);
+hipError_t launch_learn_infer(const uint16_t* net, int H, const float* x, int64_t n, float* out, hipStream_t st);
 
 hipError_t launch_ising_close(const IsingArgs& a, hipStream_t st);
 hipError_t launch_cont_metro(const ContArgs& a, int nblocks, int color, int closes, hipStream_t st);

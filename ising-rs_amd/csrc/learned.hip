@@ -38,32 +38,34 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 // exp(-x) I0(x) and exp(-x) I1(x), x >= 0: Abramowitz & Stegun 9.8.1-9.8.4.
 __device__ __forceinline__ float i0e_f(float x) {
     if (x <= 3.75f) {
-        const float t = (x / 3.75f) * (x / 3.75f);
+        const float t = (x * (1.0f / 3.75f)) * (x * (1.0f / 3.75f));
         const float p = 1.0f + t * (3.5156229f + t * (3.0899424f + t * (1.2067492f + t * (0.2659732f + t * (0.0360768f + t * 0.0045813f)))));
         return p * __expf(-x);
     }
-    const float y = 3.75f / x;
+    const float y = 3.75f * __builtin_amdgcn_rcpf(x);
     const float p = 0.39894228f + y * (0.01328592f + y * (0.00225319f + y * (-0.00157565f + y * (0.00916281f + y * (-0.02057706f + y * (0.02635537f + y * (-0.01647633f + y * 0.00392377f)))))));
     return p * rsqrtf(x);
 }
 __device__ __forceinline__ float i1e_f(float x) {
     if (x <= 3.75f) {
-        const float t = (x / 3.75f) * (x / 3.75f);
+        const float t = (x * (1.0f / 3.75f)) * (x * (1.0f / 3.75f));
         const float p = 0.5f + t * (0.87890594f + t * (0.51498869f + t * (0.15084934f + t * (0.02658733f + t * (0.00301532f + t * 0.00032411f)))));
         return x * p * __expf(-x);
     }
-    const float y = 3.75f / x;
+    const float y = 3.75f * __builtin_amdgcn_rcpf(x);
     const float p = 0.39894228f + y * (-0.03988024f + y * (-0.00362018f + y * (0.00163801f + y * (-0.01031555f + y * (0.02282967f + y * (-0.02895312f + y * (0.01787654f + y * -0.00420059f)))))));
     return p * rsqrtf(x);
 }
 
+// tanh from one exponential and one hardware reciprocal (v_rcp_f32, 1 ulp):
+// e = exp(-2|x|) in (0, 1] never overflows; absolute error ~1e-7.
 __device__ __forceinline__ float tanh_f(float x) {
     const float e = __expf(-2.0f * fabsf(x));
-    const float r = (1.0f - e) / (1.0f + e);
+    const float r = (1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e);
     return copysignf(r, x);
 }
-__device__ __forceinline__ float softplus_f(float x) { return x > 15.0f ? x : log1pf(__expf(x)); }
-__device__ __forceinline__ float sigmoid_f(float x) { return 1.0f / (1.0f + __expf(-x)); }
+__device__ __forceinline__ float softplus_f(float x) { return x > 15.0f ? x : __logf(1.0f + __expf(x)); }
+__device__ __forceinline__ float sigmoid_f(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
 
 // log f_k(c) per component (see header).
 template <int N>
@@ -90,6 +92,28 @@ __device__ __forceinline__ Mix mixture(const float o[3]) {
     m.k1 = softplus_f(o[1]) + 1e-3f;
     m.w0 = sigmoid_f(o[2]);
     return m;
+}
+
+// log q(c) = logsumexp(A_0 + kappa_0 c, A_1 + kappa_1 c): the constants of a
+// site's mixture, computed once and shared by the forward and reverse moves.
+struct MixLog {
+    float k0, k1, a0, a1;
+};
+
+template <int N>
+__device__ __forceinline__ MixLog mix_log(const Mix& m) {
+    MixLog q;
+    q.k0 = m.k0;
+    q.k1 = m.k1;
+    q.a0 = __logf(m.w0) + log_comp<N>(m.k0, 0.0f);
+    q.a1 = __logf(1.0f - m.w0) + log_comp<N>(m.k1, 0.0f);
+    return q;
+}
+
+__device__ __forceinline__ float log_q_fast(const MixLog& q, float c) {
+    const float l0 = q.a0 + q.k0 * c, l1 = q.a1 + q.k1 * c;
+    const float mx = fmaxf(l0, l1);
+    return mx + __logf(__expf(l0 - mx) + __expf(l1 - mx));
 }
 
 template <int N>
@@ -156,7 +180,7 @@ __device__ __forceinline__ float site_features(float beta, const float nv[3][6],
 #pragma unroll
     for (int k = 0; k < N; ++k) hn2 += h[k] * h[k];
     const float hn = sqrtf(hn2);
-    const float inv = hn > 1e-20f ? 1.0f / hn : 0.0f;
+    const float inv = hn > 1e-20f ? __builtin_amdgcn_rcpf(hn) : 0.0f;
 #pragma unroll
     for (int k = 0; k < 3; ++k) hhat[k] = k < N ? h[k] * inv : 0.0f;
     float m2 = 0.f, m3 = 0.f, m4 = 0.f;
@@ -180,6 +204,137 @@ __device__ __forceinline__ float site_features(float beta, const float nv[3][6],
 }
 
 // ---------------------------------------------------------------------------
+// Inference on the matrix cores.  A wave evaluates the MLP for its 64 sites as
+// four tiles of 16 sites with v_mfma_f32_16x16x16_bf16, in the transposed form
+// D[neuron][site] = W[neuron][k] X[k][site]: each lane's accumulator (rows
+// 4 (lane / 16) + r, column lane % 16) is exactly the B operand the next layer
+// needs (rows k = 4 (lane / 16) + r of column lane % 16), so activations stay
+// in registers between layers.  Layer 1 takes the 6 features plus a constant 1
+// (its bias column) padded to K = 16; later biases seed the accumulators.
+// Weights are BF16 as stored; inputs and activations are rounded to BF16 for
+// the products, accumulation is FP32.
+// ---------------------------------------------------------------------------
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <int H>
+struct NetFrag {
+    static constexpr int T = (H + 15) / 16;  // 16-wide tiles of the hidden width
+    v4s w1[T], w2[T][T], w3[T][T], w4[T];
+    v4f b2[T], b3[T], b4;
+};
+
+// This lane's operand fragments of every layer (zero padding past H and 3).
+template <int H>
+__device__ __forceinline__ void load_net(const uint16_t* __restrict__ w, NetFrag<H>& F) {
+    using O = Off<H>;
+    constexpr int T = NetFrag<H>::T;
+    const int l = threadIdx.x & 63, col = l & 15, kq = 4 * (l >> 4);
+#pragma unroll
+    for (int mt = 0; mt < T; ++mt) {
+        const int j = 16 * mt + col;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = kq + q;
+            uint16_t v = 0;
+            if (j < H) v = k < kNIn ? w[O::W1 + j * kNIn + k] : (k == kNIn ? w[O::b1 + j] : (uint16_t)0);
+            F.w1[mt][q] = (short)v;
+        }
+#pragma unroll
+        for (int kt = 0; kt < T; ++kt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = 16 * kt + kq + q;
+                const bool ok = j < H && i < H;
+                F.w2[mt][kt][q] = (short)(ok ? w[O::W2 + j * H + i] : (uint16_t)0);
+                F.w3[mt][kt][q] = (short)(ok ? w[O::W3 + j * H + i] : (uint16_t)0);
+            }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int n = 16 * mt + kq + r;
+            F.b2[mt][r] = n < H ? bf2f(w[O::b2 + n]) : 0.0f;
+            F.b3[mt][r] = n < H ? bf2f(w[O::b3 + n]) : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int kt = 0; kt < T; ++kt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 16 * kt + kq + q;
+            F.w4[kt][q] = (short)((col < 3 && i < H) ? w[O::W4 + col * H + i] : (uint16_t)0);
+        }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) F.b4[r] = kq + r < 3 ? bf2f(w[O::b4 + kq + r]) : 0.0f;
+}
+
+__device__ __forceinline__ short bf16_bits(float f) { return __builtin_bit_cast(short, (__bf16)f); }  // RNE
+
+__device__ __forceinline__ v4s tanh_bf16(v4f d) {
+    v4s r;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) r[q] = bf16_bits(tanh_f(d[q]));
+    return r;
+}
+
+// One tile of 16 sites: xb is this lane's input fragment (features 4 (lane / 16) .. + 3
+// of site lane % 16); returns the output accumulator (lanes 0-15: o_0..o_2 in rows 0-2).
+template <int H>
+__device__ __forceinline__ v4f mlp_tile(const NetFrag<H>& F, v4s xb) {
+    constexpr int T = NetFrag<H>::T;
+    const v4f zero = {0.f, 0.f, 0.f, 0.f};
+    v4s a1[T], a2[T];
+#pragma unroll
+    for (int mt = 0; mt < T; ++mt) a1[mt] = tanh_bf16(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.w1[mt], xb, zero, 0, 0, 0));
+#pragma unroll
+    for (int mt = 0; mt < T; ++mt) {
+        v4f d = F.b2[mt];
+#pragma unroll
+        for (int kt = 0; kt < T; ++kt) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.w2[mt][kt], a1[kt], d, 0, 0, 0);
+        a2[mt] = tanh_bf16(d);
+    }
+#pragma unroll
+    for (int mt = 0; mt < T; ++mt) {
+        v4f d = F.b3[mt];
+#pragma unroll
+        for (int kt = 0; kt < T; ++kt) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.w3[mt][kt], a2[kt], d, 0, 0, 0);
+        a1[mt] = tanh_bf16(d);
+    }
+    v4f d = F.b4;
+#pragma unroll
+    for (int kt = 0; kt < T; ++kt) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.w4[kt], a1[kt], d, 0, 0, 0);
+    return d;
+}
+
+// Outputs o[3] for the site of every lane of the block's waves.  All threads of
+// the block call it together.  feat: 8 halves per thread; out: 3 floats per thread.
+template <int H>
+__device__ __forceinline__ void mlp_block(const NetFrag<H>& F, const float x[kNIn], float o[3], uint16_t* feat,
+                                          float* out) {
+    const int t = threadIdx.x, l = t & 63, w0 = t - l;
+#pragma unroll
+    for (int i = 0; i < kNIn; ++i) feat[8 * t + i] = (uint16_t)bf16_bits(x[i]);
+    feat[8 * t + kNIn] = 0x3f80;  // 1.0
+    feat[8 * t + kNIn + 1] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+        v4s xb = {0, 0, 0, 0};
+        if ((l >> 4) < 2) {
+            const uint16_t* p = feat + 8 * (w0 + 16 * st + (l & 15)) + 4 * (l >> 4);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) xb[q] = (short)p[q];
+        }
+        const v4f d = mlp_tile<H>(F, xb);
+        if (l < 16)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) out[3 * (w0 + 16 * st + l) + r] = d[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 3; ++r) o[r] = out[3 * t + r];
+}
+
+// ---------------------------------------------------------------------------
 // Learned Metropolis-Hastings pass over one colour (replaces the cap/rotation
 // proposal when the ensemble's learned mode is on).  Same measurement fusion
 // as the plain pass.
@@ -187,7 +342,8 @@ __device__ __forceinline__ float site_features(float beta, const float nv[3][6],
 template <int N, int H>
 __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int color, int closes, int64_t t_override) {
     __shared__ double s_red[4][4];
-    __shared__ uint16_t s_w[Off<H>::P];
+    __shared__ uint16_t s_feat[8 * 256];
+    __shared__ float s_out[3 * 256];
     int g = 0;
     while (g + 1 < a.ngroups && (int)blockIdx.x >= a.groups[g + 1].blk0) ++g;
     const DevGroup G = a.groups[g];
@@ -195,7 +351,8 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int color
     split_tiles(G, (int)blockIdx.x - G.blk0, &r, &tile);
     const int grep = G.rep0 + r;
     const int ti = G.rep0 + a.r2t[grep];
-    for (int i = threadIdx.x; i < Off<H>::P; i += blockDim.x) s_w[i] = a.net[i];
+    NetFrag<H> F;
+    load_net<H>(a.net, F);
     const float beta = a.betas_f[ti];
     const int64_t t = t_override >= 0 ? t_override : a.ctl->t;
     const int32_t fl = t_override >= 0 ? 0 : a.ctl->flags;
@@ -203,14 +360,18 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int color
     const bool renorm = (fl & F_RENORM) != 0;
     const uint32_t hs = (uint32_t)(2 * t + color);
     const uint32_t stream = stream_id((uint32_t)G.L, (uint32_t)r);
-    __syncthreads();
 
     float* C = a.pool + G.off + (int64_t)(2 * r + color) * N * G.stride;
     const float* O = a.pool + G.off + (int64_t)(2 * r + (color ^ 1)) * N * G.stride;
     const int64_t h = (int64_t)tile * 256 + threadIdx.x;
     double part[4] = {0.0, 0.0, 0.0, 0.0};
     int acc_n = 0, tried = 0;
-    if (h < G.n_half) {
+    const bool live = h < G.n_half;
+    float nv[3][6] = {};
+    float S[3] = {0.f, 0.f, 0.f};
+    float x[kNIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, hh[3] = {0.f, 0.f, 0.f};
+    float hn = 0.f;
+    if (live) {
         // neighbours (same addressing as the plain pass)
         const int L = G.L, Lh = G.Lh;
         const int64_t row = h / Lh;
@@ -223,21 +384,21 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int color
         const int64_t nb[6] = {row * Lh + xh, row * Lh + xo, ((int64_t)z * L + ym) * Lh + xh,
                                ((int64_t)z * L + yp) * Lh + xh, ((int64_t)zm * L + y) * Lh + xh,
                                ((int64_t)zp * L + y) * Lh + xh};
-        float nv[3][6];
 #pragma unroll
         for (int k = 0; k < N; ++k)
 #pragma unroll
             for (int q = 0; q < 6; ++q) nv[k][q] = O[k * G.stride + nb[q]];
-        float S[3] = {0.f, 0.f, 0.f};
 #pragma unroll
         for (int k = 0; k < N; ++k) S[k] = C[k * G.stride + h];
-        float x[kNIn], hh[3];
-        const float hn = site_features<N>(beta, nv, x, hh);
+        hn = site_features<N>(beta, nv, x, hh);
+    }
+    // the proposal's parameters for every site of the block, on the matrix cores
+    float o[3];
+    mlp_block<H>(F, x, o, s_feat, s_out);
+    if (live) {
         bool accept = false;
         float nw[3] = {S[0], S[1], S[2]};
         if (hn > 1e-6f) {
-            float o[3];
-            mlp_forward<H>(s_w, x, o, nullptr, 0);
             const Mix m = mixture(o);
             const u32x4 w = philox((uint32_t)h, hs, stream, (uint32_t)((7u << 8) | a.model), a.k0, a.k1);
             const float kap = u24f(w.x) < m.w0 ? m.k0 : m.k1;
@@ -246,7 +407,7 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int color
             if (N == 3) {
                 // Fisher: c = 1 + log(u + (1 - u) e^{-2 kappa}) / kappa, azimuth uniform
                 const float u = u24f(w.y);
-                cn = 1.0f + __logf(u + (1.0f - u) * __expf(-2.0f * kap)) / kap;
+                cn = 1.0f + __logf(u + (1.0f - u) * __expf(-2.0f * kap)) * __builtin_amdgcn_rcpf(kap);
                 cn = fminf(1.0f, fmaxf(-1.0f, cn));
                 float sp, cp;
                 sincosf(6.28318530717958647692f * u24f(w.z), &sp, &cp);
@@ -264,31 +425,60 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int color
                 nw[2] = cn * hh[2] + sn * (cp * e2 + sp * f2);
                 have = true;
             } else {
-                // von Mises by Best-Fisher rejection
-                const float sr = 0.5f / kap;
+                // von Mises by Best-Fisher rejection.  The first four attempts
+                // draw from independent Philox calls and are evaluated together
+                // (no chain of dependent draws); the first accepted one wins, as
+                // if they had been made in turn.  Later attempts run one by one.
+                const float sr = 0.5f * __builtin_amdgcn_rcpf(kap);
                 const float rr = sr + sqrtf(1.0f + sr * sr);
-#pragma unroll 1
-                for (int at = 1; at <= kVmAttempts && !have; ++at) {
-                    const u32x4 v = philox((uint32_t)h, hs, stream, (uint32_t)((7u << 8) | a.model | (at << 16)),
-                                           a.k0, a.k1);
+                auto try_vm = [&](const u32x4& v, float* W) -> bool {
                     const float zc = __cosf(3.14159265358979f * u24f(v.x));
-                    const float W = (1.0f + rr * zc) / (rr + zc);
-                    const float Y = kap * (rr - W);
+                    *W = (1.0f + rr * zc) * __builtin_amdgcn_rcpf(rr + zc);
+                    const float Y = kap * (rr - *W);
                     const float V = fmaxf(u24f(v.y), 1e-30f);
-                    if (Y * (2.0f - Y) - V >= 0.0f || __logf(Y / V) + 1.0f - Y >= 0.0f) {
-                        cn = fminf(1.0f, fmaxf(-1.0f, W));
-                        const float sn = sqrtf(fmaxf(0.0f, 1.0f - cn * cn)) * (u24f(v.z) < 0.5f ? -1.0f : 1.0f);
-                        nw[0] = cn * hh[0] - sn * hh[1];
-                        nw[1] = sn * hh[0] + cn * hh[1];
+                    return Y * (2.0f - Y) - V >= 0.0f || __logf(Y) - __logf(V) + 1.0f - Y >= 0.0f;
+                };
+                constexpr int kBatch = 4;
+                u32x4 vb[kBatch];
+#pragma unroll
+                for (int k = 0; k < kBatch; ++k)
+                    vb[k] = philox((uint32_t)h, hs, stream, (uint32_t)((7u << 8) | a.model | ((k + 1) << 16)), a.k0,
+                                   a.k1);
+                float Wsel = 0.f;
+                uint32_t zsel = 0u;
+#pragma unroll
+                for (int k = kBatch - 1; k >= 0; --k) {
+                    float Wk;
+                    if (try_vm(vb[k], &Wk)) {
+                        Wsel = Wk;
+                        zsel = vb[k].z;
                         have = true;
                     }
+                }
+#pragma unroll 1
+                for (int at = kBatch + 1; at <= kVmAttempts && !have; ++at) {
+                    const u32x4 v = philox((uint32_t)h, hs, stream, (uint32_t)((7u << 8) | a.model | (at << 16)),
+                                           a.k0, a.k1);
+                    float Wk;
+                    if (try_vm(v, &Wk)) {
+                        Wsel = Wk;
+                        zsel = v.z;
+                        have = true;
+                    }
+                }
+                if (have) {
+                    cn = fminf(1.0f, fmaxf(-1.0f, Wsel));
+                    const float sn = sqrtf(fmaxf(0.0f, 1.0f - cn * cn)) * (u24f(zsel) < 0.5f ? -1.0f : 1.0f);
+                    nw[0] = cn * hh[0] - sn * hh[1];
+                    nw[1] = sn * hh[0] + cn * hh[1];
                 }
             }
             if (have) {
                 float co = 0.f;
 #pragma unroll
                 for (int k = 0; k < N; ++k) co += S[k] * hh[k];
-                const float lr = beta * a.Jf * hn * (cn - co) + log_q<N>(m, co) - log_q<N>(m, cn);
+                const MixLog ql = mix_log<N>(m);
+                const float lr = beta * a.Jf * hn * (cn - co) + log_q_fast(ql, co) - log_q_fast(ql, cn);
                 accept = lr >= 0.0f || __logf(fmaxf(u24f(w.w), 1e-30f)) < lr;
                 tried = 1;
             }
@@ -329,8 +519,9 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int color
     {
         const int wa = wave_sum(acc_n), wt = wave_sum(tried);
         if ((threadIdx.x & 63) == 0 && wt) {
-            atomicAdd((unsigned long long*)&a.ctl->mh_tries, (unsigned long long)wt);
-            atomicAdd((unsigned long long*)&a.ctl->mh_accepts, (unsigned long long)wa);
+            int64_t* slot = a.mh + 8 * ((blockIdx.x * 4 + (threadIdx.x >> 6)) & (kMhSlots - 1));
+            atomicAdd((unsigned long long*)&slot[0], (unsigned long long)wt);
+            atomicAdd((unsigned long long*)&slot[1], (unsigned long long)wa);
         }
     }
     if (need) {
@@ -540,6 +731,39 @@ int learn_params(int H) {
         case 64: return Off<64>::P;
     }
     return -1;
+}
+
+// Network outputs o[3] for explicit feature rows, evaluated exactly as the
+// fused sweep does (checks of the matrix-core path).
+template <int H>
+__global__ __launch_bounds__(256) void learn_infer_kernel(const uint16_t* __restrict__ net, const float* __restrict__ xin,
+                                                          int64_t n, float* __restrict__ out) {
+    __shared__ uint16_t s_feat[8 * 256];
+    __shared__ float s_out[3 * 256];
+    NetFrag<H> F;
+    load_net<H>(net, F);
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    float x[kNIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (i < n)
+#pragma unroll
+        for (int k = 0; k < kNIn; ++k) x[k] = xin[i * kNIn + k];
+    float o[3];
+    mlp_block<H>(F, x, o, s_feat, s_out);
+    if (i < n)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) out[i * 3 + r] = o[r];
+}
+
+hipError_t launch_learn_infer(const uint16_t* net, int H, const float* x, int64_t n, float* out, hipStream_t st) {
+    const int nb = (int)((n + 255) / 256);
+    switch (H) {
+        case 8: hipLaunchKernelGGL(learn_infer_kernel<8>, dim3(nb), dim3(256), 0, st, net, x, n, out); break;
+        case 16: hipLaunchKernelGGL(learn_infer_kernel<16>, dim3(nb), dim3(256), 0, st, net, x, n, out); break;
+        case 32: hipLaunchKernelGGL(learn_infer_kernel<32>, dim3(nb), dim3(256), 0, st, net, x, n, out); break;
+        case 64: hipLaunchKernelGGL(learn_infer_kernel<64>, dim3(nb), dim3(256), 0, st, net, x, n, out); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 template <int N, int H>

@@ -109,6 +109,7 @@ _SIGS = {
     "ising_pt_learn_train": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_double, _f64p]),
     "ising_pt_learn_eval": (C.c_int, [_vp, C.c_int64, _vp, _vp, _f64p, _vp]),
     "ising_pt_learn_enable": (C.c_int, [_vp, C.c_int32]),
+    "ising_pt_learn_infer": (C.c_int, [_vp, C.c_int64, _vp, _vp]),
     "ising_pt_mh_stats": (C.c_int, [_vp, _i64p, _i64p]),
     "ising_pt_metropolis_pass": (C.c_int, [_vp, C.c_int32, C.c_int64]),
     "ising_pt_sweeps": (C.c_int, [_vp, _i64p]),
@@ -460,6 +461,13 @@ class PTEnsemble:
         loss = C.c_double()
         _check(lib.ising_pt_learn_eval(self._h, len(c), _ptr(x), _ptr(c), C.byref(loss), _ptr(g)))
         return loss.value, g
+
+    def learn_infer(self, x):
+        """Network outputs [n, 3] for feature rows x [n, 6], as the fused sweep evaluates them."""
+        xa = np.ascontiguousarray(x, np.float32).reshape(-1, 6)
+        out = np.empty((xa.shape[0], 3), np.float32)
+        _check(lib.ising_pt_learn_infer(self._h, xa.shape[0], _ptr(xa), _ptr(out)))
+        return out
 
     def learn_enable(self, on=True):
         _check(lib.ising_pt_learn_enable(self._h, 1 if on else 0))

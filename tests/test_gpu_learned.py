@@ -64,6 +64,37 @@ def test_gradients_match_fp64_finite_differences(model, nc):
     assert cos > 0.9999
 
 
+def forward_bf16(p, x):
+    """FP64 forward with the fused sweep's roundings: BF16 inputs and
+    activations (weights are BF16 already)."""
+    a = bf16_round(x)
+    for k in (1, 2, 3):
+        a = bf16_round(np.tanh(a @ p["W%d" % k].T + p["b%d" % k]))
+    return a @ p["W4"].T + p["b4"]
+
+
+@pytest.mark.parametrize("H", [8, 16, 32, 64])
+def test_matrix_core_inference_matches_the_bf16_forward(H):
+    nc = 2
+    pt = ensemble("xy")
+    pt.learn_init(H, seed=3)
+    rng = np.random.default_rng(H)
+    params = (rng.normal(size=ref.n_params(H)) * (1.5 / np.sqrt(H))).astype(np.float32)
+    pt.learn_set_params(params)
+    x, _ = random_samples(nc, 1000, 0.45, 21)
+    got = pt.learn_infer(x).astype(np.float64)
+    p64 = ref.unpack_params(bf16_round(params), H)
+    want = forward_bf16(p64, x.astype(np.float64))
+    full, _ = ref.forward(p64, x.astype(np.float64))
+    err = np.abs(got - want)
+    scale = np.max(np.abs(want))
+    # accumulation order and the device tanh can move an activation across a
+    # BF16 rounding boundary now and then; anything systematic (a layout or
+    # padding mistake) shows as errors of the order of the outputs themselves
+    assert np.median(err) < 1e-3 * scale and np.max(err) < 3e-2 * scale, (np.median(err), np.max(err), scale)
+    assert np.max(np.abs(got - full)) < 0.1 * scale
+
+
 def test_hidden64_gradients_on_a_parameter_subset():
     H, nc = 64, 3
     pt = ensemble("heisenberg")
