@@ -119,6 +119,7 @@ struct ising_pt {
     uint8_t* d_pool = nullptr;
     int64_t pool_bytes = 0;
     uint32_t* d_thr = nullptr;
+    uint32_t* d_thr_rep = nullptr;  // int8 Ising: thresholds by replica under the permutation in force
     uint32_t* d_mthr = nullptr;  // multispin coding thresholds
     uint32_t* d_mplanes = nullptr;  // multispin coding threshold bit planes per word
     int32_t* d_r2t = nullptr;
@@ -178,6 +179,7 @@ struct ising_pt {
         (void)hipFree(d_groups);
         (void)hipFree(d_pool);
         (void)hipFree(d_thr);
+        (void)hipFree(d_thr_rep);
         (void)hipFree(d_mthr);
         (void)hipFree(d_mplanes);
         (void)hipFree(d_sched);
@@ -224,6 +226,9 @@ struct ising_pt {
         a.nblocks = nblocks;
         a.pool = d_pool;
         a.thr = d_thr;
+        a.thr_rep = d_thr_rep;
+        a.gk_on = groups.size() <= (size_t)kArgGroups ? 1 : 0;
+        if (a.gk_on) std::memcpy(a.gk, groups.data(), sizeof(DevGroup) * groups.size());
         a.r2t = d_r2t;
         a.t2r = d_t2r;
         a.betas = d_betas;
@@ -544,6 +549,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     if (msc) HIP_TRY(hipMalloc(&p->d_mthr, sizeof(uint32_t) * 3 * p->total_reps));
     if (msc) HIP_TRY(hipMalloc(&p->d_mplanes, sizeof(uint32_t) * 96 * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_thr, sizeof(uint32_t) * kIsingTab * p->total_reps));
+    if (ising) HIP_TRY(hipMalloc(&p->d_thr_rep, sizeof(uint32_t) * kIsingTab * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_r2t, sizeof(int32_t) * 2 * p->total_reps));  // two permutation parities
     HIP_TRY(hipMalloc(&p->d_t2r, sizeof(int32_t) * 2 * p->total_reps));
     HIP_TRY(hipMalloc(&p->d_betas, sizeof(double) * p->total_reps));
@@ -588,6 +594,8 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     set_flags(&ctl, 0);
     ST_COPY(p, p->d_groups, p->groups.data(), sizeof(DevGroup) * p->groups.size(), hipMemcpyHostToDevice);
     ST_COPY(p, p->d_thr, thr.data(), sizeof(uint32_t) * thr.size(), hipMemcpyHostToDevice);
+    if (p->d_thr_rep)  // the starting permutation is the identity
+        ST_COPY(p, p->d_thr_rep, thr.data(), sizeof(uint32_t) * thr.size(), hipMemcpyHostToDevice);
     ST_COPY(p, p->d_r2t, ident.data(), sizeof(int32_t) * ident.size(), hipMemcpyHostToDevice);
     ST_COPY(p, p->d_t2r, ident.data(), sizeof(int32_t) * ident.size(), hipMemcpyHostToDevice);
     ST_COPY(p, p->d_betas, betas.data(), sizeof(double) * betas.size(), hipMemcpyHostToDevice);

@@ -74,11 +74,6 @@ __device__ void msc_planes(const IsingArgs& a, int par) {
 // Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
 // Remap so each XCD walks one contiguous range of tiles: the z-neighbour
 // planes a tile reads are then mostly in the same XCD's L2.
-__device__ __forceinline__ int xcd_block(int nblocks) {
-    const int per = (int)(gridDim.x >> 3);
-    return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-}
-
 // 24-bit multiply (one full-rate instruction); both operands are below 2^24
 // for every lattice the library accepts (L <= 2046, so L^2 < 2^22).
 __device__ __forceinline__ uint32_t mul24(uint32_t x, uint32_t y) { return (x & 0xffffffu) * (y & 0xffffffu); }
@@ -279,32 +274,49 @@ __device__ __forceinline__ int fast_tile(const IsingArgs& a, const DevGroup& G, 
 // One colour half-sweep of every replica of every group in the ensemble.
 // Grid: one block per (group, replica, tile of G.tsz half-sites), padded to
 // a multiple of 8 blocks.
+// A colour launch: one block per tile; block i runs on XCD i % 8, and the
+// tiles are renumbered so each XCD walks one contiguous range of them (the z
+// neighbour planes of a tile then stay in that XCD's L2).
 __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int color) {
     __shared__ uint32_t s_thr[kIsingTab];
     __shared__ int64_t s_red[2][kThreads / 64];
     __shared__ uint32_t s_qh[kTieCap], s_qm[kTieCap];
     __shared__ int s_qn;
-    const int b = xcd_block(a.nblocks);
+    const int per = (int)(gridDim.x >> 3);
+    const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if (b >= a.nblocks) return;
-    const int g = find_group(a.groups, a.ngroups, b);
-    const DevGroup G = a.groups[g];
+    int g;
+    DevGroup G;
+    if (a.gk_on) {
+        g = 0;
+        while (g + 1 < a.ngroups && b >= a.gk[g + 1].blk0) ++g;
+        G = a.gk[g];
+    } else {
+        g = find_group(a.groups, a.ngroups, b);
+        G = a.groups[g];
+    }
     int r, tile;
     split_tiles(G, b - G.blk0, &r, &tile);
     const int grep = G.rep0 + r;
+    // issued before the control block is read: the table by replica is right
+    // unless this launch closes an exchange
+    uint32_t thr_pre = 0u;
+    if (a.thr_rep && threadIdx.x < kIsingTab) thr_pre = a.thr_rep[(int64_t)grep * kIsingTab + threadIdx.x];
     const SweepView v = view_of(a.ctl, color);
-    int tidx;
+    int tidx = 0;
     if (v.exch) {
         const int64_t po = (v.round & 1) * a.total_reps;
         tidx = exch_tidx(a, G, r, v.round, a.r2t + po, a.t2r + po, acc_of(a, v.t - 1));
         if (b == G.blk0) exch_write_group(a, G, v.round, acc_of(a, v.t - 1));
-    } else {
+    } else if (!a.thr_rep) {
         tidx = a.r2t[(int64_t)v.par * a.total_reps + grep];
     }
     if (b == 0) sweep_bookkeeping(a, v, color);
     if (threadIdx.x < kIsingTab)
-        s_thr[threadIdx.x] = a.sched        ? a.sched[(v.t - a.sched_t0) * kIsingTab + threadIdx.x]
-                             : a.inline_thr ? a.thr_inline[threadIdx.x]
-                                            : a.thr[(int64_t)(G.rep0 + tidx) * kIsingTab + threadIdx.x];
+        s_thr[threadIdx.x] = a.sched                  ? a.sched[(v.t - a.sched_t0) * kIsingTab + threadIdx.x]
+                             : a.inline_thr           ? a.thr_inline[threadIdx.x]
+                             : (a.thr_rep && !v.exch) ? thr_pre
+                                                      : a.thr[(int64_t)(G.rep0 + tidx) * kIsingTab + threadIdx.x];
     if (threadIdx.x == 0) s_qn = 0;
     const int64_t t = v.t;
     const bool need = (v.flags & (F_MEAS | F_EXCH)) != 0;

@@ -4,6 +4,7 @@
 
 namespace ising {
 
+constexpr int kArgGroups = 12;  // size descriptors carried in the kernel arguments
 struct IsingArgs {
     const DevGroup* groups;
     int32_t ngroups;
@@ -27,6 +28,15 @@ struct IsingArgs {
     int32_t acc_copies;           // accumulator copies (spread atomics), summed by the closing kernel
     const uint32_t* sched;        // annealing: one table row per sweep for every replica (null: per temperature)
     int64_t sched_t0;             // sweep index of schedule row 0
+    // Thresholds by replica: thr_rep[global replica] = thr[rep0 + its temperature index]
+    // under the permutation in force, rewritten wherever an exchange writes a new
+    // permutation; lets a launch that closes no exchange read its table without
+    // first reading the permutation (null: read through the permutation).
+    uint32_t* thr_rep;
+    // The size descriptors again, in the kernel arguments (gk_on when ngroups <= kArgGroups):
+    // a block finds its size and tile without a dependent global load.
+    int32_t gk_on;
+    DevGroup gk[kArgGroups];
 };
 
 constexpr int kMscAccCopies = 16;

@@ -91,6 +91,15 @@ __device__ __forceinline__ void exch_write_group(const IsingArgs& a, const DevGr
         }
     }
     if (threadIdx.x == 0) atomicAdd((unsigned long long*)&a.ctl->exch_tries, (unsigned long long)npairs);
+    if (a.thr_rep) {
+        // thresholds by replica under the new permutation
+        __syncthreads();
+        for (int k = threadIdx.x; k < G.R * kIsingTab; k += blockDim.x) {
+            const int rr = k / kIsingTab, e = k - rr * kIsingTab;
+            a.thr_rep[(int64_t)(G.rep0 + rr) * kIsingTab + e] =
+                a.thr[(int64_t)(G.rep0 + r2t_n[G.rep0 + rr]) * kIsingTab + e];
+        }
+    }
 }
 
 // Measurement records of sweep `ts` (one per replica, in global replica
