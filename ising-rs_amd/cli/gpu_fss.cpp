@@ -134,6 +134,10 @@ struct Args {
     double delta = 0.5;
     unsigned long long n_overrelax = 5, measure_every = 1;
     unsigned long long batched = 1, renorm_every = 100, binary = 0;
+    // --algorithm (the flag the reference plans, gpu-compute-optimizations-impl.md:1348-1380):
+    // metropolis; msc (Ising, multispin coded); sw / wolff (a Swendsen-Wang or Wolff cluster
+    // update after every Metropolis sweep)
+    std::string algorithm = "metropolis";
 };
 
 // Writer of the binary timeseries of one size.
@@ -145,7 +149,8 @@ struct BinSeries {
         f = std::fopen(path.c_str(), "wb");
         if (!f) die("create timeseries file failed: " + path);
         std::setvbuf(f, nullptr, _IOFBF, 1 << 20);
-        const int32_t hdr[6] = {1, model, L, R, measure_every, model == ISING_MODEL_ISING ? 0 : 1};
+        const bool ints = model == ISING_MODEL_ISING || model == ISING_MODEL_ISING_MSC;
+        const int32_t hdr[6] = {1, model, L, R, measure_every, ints ? 0 : 1};
         const int64_t zero = 0;
         if (std::fwrite("ISINGTS1", 1, 8, f) != 8 || std::fwrite(hdr, sizeof hdr, 1, f) != 1 ||
             std::fwrite(&zero, sizeof zero, 1, f) != 1 || std::fwrite(&J, sizeof J, 1, f) != 1 ||
@@ -163,13 +168,25 @@ struct BinSeries {
 };
 
 // One FSS run over `sizes` sharing one device ensemble.
-void run(const Args& a, const std::vector<int>& sizes, int model, const char* model_name) {
+void run(const Args& a, const std::vector<int>& sizes, int model, const char* model_name, int cluster) {
     const int R = (int)a.replicas;
+    const bool ints = model == ISING_MODEL_ISING || model == ISING_MODEL_ISING_MSC;  // int64 records
+    // `n` sweeps of the chosen algorithm: Metropolis sweeps, each followed by a cluster update when asked
+    auto sweeps = [&](ising_pt_t* pt, int64_t n, bool sample) {
+        if (cluster < 0) return sample ? ising_pt_sample(pt, n) : ising_pt_warmup(pt, n);
+        for (int64_t k = 0; k < n; ++k) {
+            int rc = sample ? ising_pt_sample(pt, 1) : ising_pt_warmup(pt, 1);
+            if (!rc) rc = ints ? ising_pt_cluster(pt, cluster, 1, nullptr, nullptr, nullptr)
+                               : ising_pt_cluster_cont(pt, cluster, 1, nullptr, nullptr);
+            if (rc) return rc;
+        }
+        return 0;
+    };
     std::vector<double> temps(R);
     check(ising_linspace_temperatures(a.t_min, a.t_max, R, temps.data()), "linspace");
     const int ns = (int)sizes.size();
     for (int L : sizes) std::fprintf(stderr, "  N=%d: creating %d %sreplicas...\n", L, R,
-                                     model == ISING_MODEL_ISING ? "" : (std::string(model_name) + " ").c_str());
+                                     ints ? "" : (std::string(model_name) + " ").c_str());
     ising_pt_opts_t o;
     ising_pt_default_opts(&o);
     o.j = 1.0;
@@ -188,7 +205,7 @@ void run(const Args& a, const std::vector<int>& sizes, int model, const char* mo
     for (int L : sizes) std::fprintf(stderr, "  N=%d: warming up %llu sweeps...\n", L, a.warmup);
     for (unsigned long long w = 0; w < a.warmup;) {
         const unsigned long long step = std::min<unsigned long long>(1000 - (w % 1000), a.warmup - w);
-        check(ising_pt_warmup(pt, (int64_t)step), "GPU step failed");
+        check(sweeps(pt, (int64_t)step, false), "GPU step failed");
         w += step;
         if (w % 1000 == 0) {
             check(ising_pt_sync(pt, nullptr), "GPU step failed");
@@ -216,7 +233,7 @@ void run(const Args& a, const std::vector<int>& sizes, int model, const char* mo
     std::vector<std::vector<std::vector<double>>> se(ns, std::vector<std::vector<double>>(R)),
         sm(ns, std::vector<std::vector<double>>(R));
     for (int L : sizes) {
-        if (model == ISING_MODEL_ISING)  // gpu_fss.rs:234 / :493
+        if (ints)  // gpu_fss.rs:234 / :493
             std::fprintf(stderr, "  N=%d: sampling %llu sweeps with PT exchange every %llu...\n", L, a.samples,
                          a.exchange_every);
         else
@@ -231,10 +248,10 @@ void run(const Args& a, const std::vector<int>& sizes, int model, const char* mo
     std::vector<double> ce(cap * tot), cx(cap * tot), cy(cap * tot), cz(cap * tot);
     for (unsigned long long s = 0; s < a.samples;) {
         const unsigned long long step = std::min<unsigned long long>(chunk - (s % chunk), a.samples - s);
-        check(ising_pt_sample(pt, (int64_t)step), "GPU step failed");
+        check(sweeps(pt, (int64_t)step, true), "GPU step failed");
         s += step;
         int64_t n = 0;
-        if (model == ISING_MODEL_ISING)
+        if (ints)
             check(ising_pt_read_ising(pt, cap, tidx.data(), bond.data(), mag.data(), &n), "GPU measure failed");
         else
             check(ising_pt_read_cont(pt, cap, tidx.data(), ce.data(), cx.data(), cy.data(), cz.data(), &n),
@@ -246,7 +263,7 @@ void run(const Args& a, const std::vector<int>& sizes, int model, const char* mo
                 for (int r = 0; r < R; ++r) {
                     const int64_t i = k * tot + (int64_t)g * R + r;
                     double e_per, m_per;
-                    if (model == ISING_MODEL_ISING) {
+                    if (ints) {
                         e_per = -1.0 * (double)bond[i] / n3;  // gpu_fss.rs:250-251
                         m_per = std::fabs((double)mag[i]) / n3;
                     } else {
@@ -258,7 +275,7 @@ void run(const Args& a, const std::vector<int>& sizes, int model, const char* mo
                     sm[g][t].push_back(m_per);
                     std::fprintf(ts[g], "%d,%.8f,%.8f\n", t, e_per, m_per);  // gpu_fss.rs:264
                     if (a.binary) {
-                        if (model == ISING_MODEL_ISING) {
+                        if (ints) {
                             const int64_t rec[3] = {bond[i], mag[i], (int64_t)(uint32_t)t};
                             bin[g].put(rec, sizeof rec);
                         } else {
@@ -328,6 +345,7 @@ int main(int argc, char** argv) {
         else if (f == "--renorm-every")
             a.renorm_every = parse_u64(get_arg(argc, argv, i, "--renorm-every"), "--renorm-every"), i += 2;
         else if (f == "--binary") a.binary = parse_u64(get_arg(argc, argv, i, "--binary"), "--binary"), i += 2;
+        else if (f == "--algorithm") a.algorithm = get_arg(argc, argv, i, "--algorithm"), i += 2;
         else i += 1;
     }
 
@@ -370,10 +388,22 @@ int main(int argc, char** argv) {
     else if (a.model == "xy") model = ISING_MODEL_XY;
     else if (a.model == "heisenberg") model = ISING_MODEL_HEISENBERG;
     else die("Error: --model must be ising, xy, or heisenberg");
+    int cluster = -1;
+    if (a.algorithm == "msc") {
+        if (model != ISING_MODEL_ISING) die("Error: --algorithm msc needs --model ising");
+        model = ISING_MODEL_ISING_MSC;
+    } else if (a.algorithm == "sw") {
+        cluster = ISING_CLUSTER_SW;
+    } else if (a.algorithm == "wolff") {
+        cluster = ISING_CLUSTER_WOLFF;
+    } else if (a.algorithm != "metropolis") {
+        die("Error: --algorithm must be metropolis, msc, sw, or wolff");
+    }
+    if (a.algorithm != "metropolis") std::fprintf(stderr, "  algorithm: %s\n", a.algorithm.c_str());
     if (sizes.empty()) return 0;
 
-    if (a.batched) run(a, sizes, model, a.model.c_str());
+    if (a.batched) run(a, sizes, model, a.model.c_str(), cluster);
     else
-        for (int L : sizes) run(a, std::vector<int>{L}, model, a.model.c_str());
+        for (int L : sizes) run(a, std::vector<int>{L}, model, a.model.c_str(), cluster);
     return 0;
 }

@@ -636,7 +636,7 @@ def read_timeseries(path):
                   "temp_idx": np.frombuffer(raw, np.int64, 5 * n * R, off).reshape(n, R, 5)[:, :, 4].astype(np.int32)}
     if len(raw) != off + (24 if kind == 0 else 40) * n * R:
         raise ValueError("truncated timeseries file: %s" % path)
-    out = {"model": ["ising", "xy", "heisenberg"][model], "L": int(L), "replicas": int(R),
+    out = {"model": ["ising", "xy", "heisenberg", "ising_msc", "ising2d"][model], "L": int(L), "replicas": int(R),
            "measure_every": int(me), "J": J, "temps": temps}
     out.update(fields)
     return out

@@ -44,15 +44,25 @@ def test_unknown_model_and_unknown_flags(tmp_path):
     assert "Error: --model must be ising, xy, or heisenberg" in r.stderr
 
 
+@pytest.mark.parametrize("args,msg", [
+    (["--algorithm", "annealing"], "Error: --algorithm must be metropolis, msc, sw, or wolff"),
+    (["--algorithm", "msc", "--model", "xy"], "Error: --algorithm msc needs --model ising"),
+])
+def test_algorithm_flag_validation(args, msg, tmp_path):
+    r = run(["--outdir", str(tmp_path)] + args)
+    assert r.returncode == 1 and msg in r.stderr
+
+
 def test_invalid_sizes_are_dropped_like_the_reference(tmp_path):
     # filter_map(parse().ok()) drops "x" and " " entries; an empty list runs nothing
     r = run(["--sizes", "x,,y", "--outdir", str(tmp_path)])
     assert r.returncode == 0 and "sizes=[]" in r.stderr
 
 
-def expected_ising_csvs(L, R, tmin, tmax, warmup, samples, ee, me, seed):
+def expected_ising_csvs(L, R, tmin, tmax, warmup, samples, ee, me, seed, msc=False):
     temps = o.linspace(tmin, tmax, R)
-    tidx, bond, mag, _ = o.ising_pt_run(L, R, temps, seed, warmup, samples, ee, me)
+    run_pt = o.msc_pt_run if msc else o.ising_pt_run
+    tidx, bond, mag, _ = run_pt(L, R, temps, seed, warmup, samples, ee, me)
     n3 = float(L ** 3)
     ts = ["temp_idx,E,M"]
     series = {k: ([], []) for k in range(R)}
@@ -133,3 +143,32 @@ def test_binary_timeseries_holds_the_exact_records(tmp_path, model):
         got = ["%d,%.8f,%.8f" % (t, a, b) for t, a, b in zip(ts["temp_idx"].reshape(-1), e.reshape(-1), m.reshape(-1))]
         assert got == rows
         assert np.allclose(ts["temps"], np.linspace(float(args[5]), float(args[7]), 4))
+
+
+@pytest.mark.gpu
+def test_multispin_algorithm_writes_the_oracle_files(tmp_path):
+    # --algorithm msc: the same files from the multispin-coded chain, record for record
+    r = run(["--model", "ising", "--sizes", "4,8", "--tmin", "4.4", "--tmax", "4.6", "--replicas", "4", "--warmup",
+             "50", "--samples", "100", "--exchange-every", "10", "--outdir", str(tmp_path), "--algorithm", "msc"])
+    assert r.returncode == 0, r.stderr
+    assert "algorithm: msc" in r.stderr
+    for L in (4, 8):
+        ts_ref, sum_ref = expected_ising_csvs(L, 4, 4.4, 4.6, 50, 100, 10, 1, 42, msc=True)
+        assert open(tmp_path / ("gpu_fss_ising_N%d_timeseries.csv" % L)).read() == ts_ref
+        assert open(tmp_path / ("gpu_fss_ising_N%d_summary.csv" % L)).read() == sum_ref
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("algorithm,model", [("sw", "ising"), ("wolff", "ising"), ("sw", "xy")])
+def test_cluster_algorithms_run_the_hybrid_chain(tmp_path, algorithm, model):
+    # a cluster update after every Metropolis sweep: same files, one record per measured sweep
+    r = run(["--model", model, "--sizes", "6", "--tmin", "4.4" if model == "ising" else "1.8", "--tmax",
+             "4.6" if model == "ising" else "2.4", "--replicas", "4", "--warmup", "40", "--samples", "60",
+             "--outdir", str(tmp_path), "--algorithm", algorithm, "--overrelax", "1"])
+    assert r.returncode == 0, r.stderr
+    ts = open(tmp_path / ("gpu_fss_%s_N6_timeseries.csv" % model)).read().splitlines()
+    summ = open(tmp_path / ("gpu_fss_%s_N6_summary.csv" % model)).read().splitlines()
+    assert ts[0] == "temp_idx,E,M" and len(ts) == 1 + 4 * 60
+    assert len(summ) == 5 and "nan" not in "".join(ts + summ).lower()
+    e = np.array([float(x.split(",")[1]) for x in ts[1:]])
+    assert np.all(e < 0) and np.all(e >= -3.0)
