@@ -433,6 +433,8 @@ def main():
     ap.add_argument("--train-steps", type=int, default=400, help="learned workload: Adam steps")
     ap.add_argument("--learned-or", type=int, default=2,
                     help="learned workload: overrelaxation passes per sweep of the learned chain (plain keeps 2)")
+    ap.add_argument("--fused", type=int, default=0, choices=[0, 8, 16],
+                    help="int8 Ising: one launch per sweep with boxes of this many y/z rows (0: one launch per colour)")
     ap.add_argument("--cluster-part", default="all", choices=["all", "throughput"],
                     help="cluster workload: 'throughput' times only the Swendsen-Wang updates of the config-2 ensemble")
     ap.add_argument("--layout", default="auto", metavar="W,R,S",
@@ -472,6 +474,8 @@ def main():
     if model == "ising" and args.layout != "auto":
         w, r, s = (int(v) for v in args.layout.split(","))
         opts = dict(opts, ising_width=w, ising_runs=r, ising_spt=s)
+    if model == "ising" and args.fused:
+        opts = dict(opts, ising_fused=args.fused)
     pt = ising_hip.PTEnsemble(model, sizes, R, temps, args.seed, record_capacity=args.steps // me + 8, **opts)
     layout = None
     if model == "ising":
@@ -494,11 +498,13 @@ def main():
     passes = 1 + opts.get("n_overrelax", 0) if not model.startswith("ising") else 1
     updates = sites * args.steps                   # Metropolis updates (the reported metric)
     value = updates / (wall * 1e9)
-    launches = 2 * passes * args.steps              # colour passes (Metropolis + overrelaxation)
+    fused = model == "ising" and args.fused > 0
+    launches = (1 if fused else 2) * passes * args.steps  # colour passes, or whole sweeps when fused
     avg_launch_s = ev_ms * 1e-3 / launches
-    alg_bytes_per_launch = int(bpu * sites / 2)     # bytes per update x updates per colour pass
+    alg_bytes_per_launch = int(bpu * sites / (1 if fused else 2))  # bytes per update x updates per launch
     achieved = alg_bytes_per_launch / avg_launch_s / 1e9
-    kernel = {"ising": "ising_half_kernel", "ising_msc": "msc_half_kernel"}.get(model, "cont_metro_kernel+cont_or_kernel")
+    kernel = {"ising": "ising_fused_kernel" if fused else "ising_half_kernel",
+              "ising_msc": "msc_half_kernel"}.get(model, "cont_metro_kernel+cont_or_kernel")
     line = {
         "metric": "spin updates/ns",
         "value": round(value, 3),
@@ -516,7 +522,7 @@ def main():
                    "site_passes_per_step": passes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic() if args.workload == "ising" else None,
+                     "traffic": load_traffic() if args.workload == "ising" and not fused else None,
                      "kernel": kernel, "alg_bytes_per_launch": alg_bytes_per_launch,
                      "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "device_updates_per_ns": round(updates / (ev_ms * 1e6), 3)},

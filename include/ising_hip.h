@@ -158,11 +158,17 @@ typedef struct {
                                 (0 = automatic); results do not depend on it */
     int32_t ising_runs;      /* Ising: 16-site runs per thread step, 1 or 2 (0 = automatic);
                                 results do not depend on it */
+    /* int8 Ising: 0 = two launches per sweep (one per colour); 8 or 16 = one
+     * launch per sweep, each block updating both colours of a box of that many
+     * y and z rows (plus a recomputed halo) staged in LDS; lattices with
+     * L <= 32 are staged whole.  Results are identical either way.  Needs every
+     * size to be 3D with L <= 32 or L % 32 == 0 (ignored otherwise). */
+    int32_t ising_fused;
 } ising_pt_opts_t;
 
 /* Defaults: j=1, h=0, exchange_every=10, measure_every=1, delta=0.5,
  * n_overrelax=5, renorm_every=100, record_capacity=4096, adapt_every=0,
- * adapt_target=0.5, graph_sweeps=32, ising_spt=0, ising_width=0, ising_runs=0. */
+ * adapt_target=0.5, graph_sweeps=32, ising_spt=0, ising_width=0, ising_runs=0, ising_fused=0. */
 void ising_pt_default_opts(ising_pt_opts_t* opts);
 
 /* model: ISING_MODEL_*.  sizes: n_sizes even L values.  temps: `replicas`

@@ -149,6 +149,10 @@ struct ising_pt {
     int64_t* d_mh = nullptr;  // [kMhSlots][8] learned-move counters
     float beta_inline = 0.f, delta_inline = 0.f, j_inline = 1.f;
     int layout_width = 0, layout_runs = 0, layout_spt = 0;  // Ising thread-step layout in use
+    int fused_box = 0;       // one-launch sweeps: box edge asked for (0: two launches per sweep)
+    bool fused_on = false;   // one-launch sweeps in use (every size qualifies)
+    int fnblocks = 0, fused_lds = 0;
+    uint8_t* d_pool2 = nullptr;  // one-launch sweeps: the second lattice buffer
     uint32_t* d_sched = nullptr;   // annealing schedule: table rows per sweep
     int64_t sched_cap = 0;         // rows allocated
     bool d_sched_on = false;
@@ -180,6 +184,7 @@ struct ising_pt {
         (void)hipFree(d_pool);
         (void)hipFree(d_thr);
         (void)hipFree(d_thr_rep);
+        (void)hipFree(d_pool2);
         (void)hipFree(d_mthr);
         (void)hipFree(d_mplanes);
         (void)hipFree(d_sched);
@@ -227,6 +232,7 @@ struct ising_pt {
         a.pool = d_pool;
         a.thr = d_thr;
         a.thr_rep = d_thr_rep;
+        a.fnblocks = fnblocks;
         a.gk_on = groups.size() <= (size_t)kArgGroups ? 1 : 0;
         if (a.gk_on) std::memcpy(a.gk, groups.data(), sizeof(DevGroup) * groups.size());
         a.r2t = d_r2t;
@@ -318,7 +324,20 @@ struct ising_pt {
     }
 
     int enqueue_sweeps(int64_t n, int n_or = -1, bool close_last = true) {
-        if (model == ISING_MODEL_ISING) {
+        if (model == ISING_MODEL_ISING && fused_on) {
+            // one launch per sweep, alternating the two lattice buffers; it
+            // closes the sweep before it
+            if (!d_pool2) return fail(ISING_E_STATE, "second lattice buffer missing");
+            IsingArgs a = ising_args();
+            for (int64_t i = 0; i < n; ++i) {
+                a.pool = (i & 1) ? d_pool2 : d_pool;
+                a.pool_out = (i & 1) ? d_pool : d_pool2;
+                HIP_TRY(launch_ising_fused(a, fused_lds, st));
+            }
+            if (n & 1) HIP_TRY(hipMemcpyAsync(d_pool, d_pool2, pool_bytes, hipMemcpyDeviceToDevice, st));
+            a.pool = d_pool;
+            if (n > 0 && close_last) HIP_TRY(launch_ising_close(a, st));
+        } else if (model == ISING_MODEL_ISING) {
             // each colour-0 launch closes the sweep before it; the closing
             // kernel closes the last one (see sweepctl.h)
             const IsingArgs a = ising_args();
@@ -448,6 +467,35 @@ static int layout_ising(ising_pt* p, int width, int runs, int spt) {
         blk += G.tiles * G.R;
     }
     p->nblocks = blk;
+    // one-launch sweeps: boxes of fused_box y and z rows (L % 32 == 0), or
+    // the whole replica (L <= 32); every size must qualify
+    {
+        int fb = 0;
+        int64_t lds = 0;
+        bool ok = p->fused_box > 0;
+        for (DevGroup& G : p->groups) {
+            G.fbox = 0;
+            G.fboxes = 1;
+            G.fblk0 = fb;
+            if (G.dims != 3) {
+                ok = false;
+            } else if (G.L <= 32) {
+                if ((G.Lh % 4) != 0 || ((int64_t)G.L * G.L * G.Lh) % 16 != 0) ok = false;
+                lds = std::max<int64_t>(lds, 2 * (int64_t)G.L * G.L * G.Lh);
+            } else if (ok && G.L % 32 == 0 && G.L % p->fused_box == 0) {
+                G.fbox = p->fused_box;
+                G.fboxes = (G.L / G.fbox) * (G.L / G.fbox);
+                lds = std::max<int64_t>(lds, 2 * (int64_t)(G.fbox + 4) * (G.fbox + 4) * G.Lh);
+            } else {
+                ok = false;
+            }
+            fb += G.fboxes * G.R;
+        }
+        p->fused_on = ok && lds <= 65536;
+        p->fnblocks = fb;
+        p->fused_lds = (int)lds;
+        if (p->fused_on && p->d_pool && !p->d_pool2) HIP_TRY(hipMalloc(&p->d_pool2, p->pool_bytes));
+    }
     p->layout_width = width;
     p->layout_runs = runs;
     p->layout_spt = spt;
@@ -476,6 +524,8 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     if (o.record_capacity < 1) return fail(ISING_E_ARG, "record_capacity must be >= 1");
     if (o.graph_sweeps < 0 || o.graph_sweeps > 100000) return fail(ISING_E_ARG, "graph_sweeps must be in 0..100000");
     if (o.ising_spt < 0 || o.ising_spt > 4) return fail(ISING_E_ARG, "ising_spt must be in 0..4");
+    if (o.ising_fused != 0 && o.ising_fused != 8 && o.ising_fused != 16)
+        return fail(ISING_E_ARG, "ising_fused must be 0, 8 or 16");
     if (o.ising_width != 0 && o.ising_width != 16 && o.ising_width != 32)
         return fail(ISING_E_ARG, "ising_width must be 0, 16 or 32");
     if (o.ising_runs < 0 || o.ising_runs > 2) return fail(ISING_E_ARG, "ising_runs must be 0, 1 or 2");
@@ -536,6 +586,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     p->nblocks = blk;
     p->pool_bytes = off * elem;
     if (ising) {
+        p->fused_box = model == ISING_MODEL_ISING ? o.ising_fused : 0;
         const int rc = layout_ising(p, o.ising_width, o.ising_runs, o.ising_spt);
         if (rc) return rc;
     }
@@ -545,6 +596,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     HIP_TRY(hipEventCreate(&p->ev1));
     HIP_TRY(hipMalloc(&p->d_groups, sizeof(DevGroup) * p->groups.size()));
     if (ising || msc) HIP_TRY(hipMalloc(&p->d_pool, p->pool_bytes));
+    if (p->fused_on) HIP_TRY(hipMalloc(&p->d_pool2, p->pool_bytes));
     else HIP_TRY(hipMalloc(&p->d_fpool, p->pool_bytes));
     if (msc) HIP_TRY(hipMalloc(&p->d_mthr, sizeof(uint32_t) * 3 * p->total_reps));
     if (msc) HIP_TRY(hipMalloc(&p->d_mplanes, sizeof(uint32_t) * 96 * p->total_reps));
@@ -671,6 +723,7 @@ void ising_pt_default_opts(ising_pt_opts_t* o) {
     o->ising_spt = 0;
     o->ising_width = 0;
     o->ising_runs = 0;
+    o->ising_fused = 0;
 }
 
 int ising_pt_create(ising_pt_t** out, int model, const int32_t* sizes, int32_t n_sizes, int32_t replicas,

@@ -128,6 +128,9 @@ struct DevGroup {
     int32_t tsz;             // Ising: half-sites per block tile (256 threads x step width x spt)
     int32_t runs;            // Ising: 16-site runs per thread step (1 or 2, a block row apart)
     int32_t pad2;
+    // one-launch sweeps (int8 Ising): box edge in y and z rows (0: the whole
+    // replica in one box), boxes per replica, first block of this size
+    int32_t fbox, fboxes, fblk0, fpad;
 };
 
 // Replica (or replica word) and tile of a block's local index.
@@ -166,6 +169,8 @@ struct DevCtl {
     int64_t closed;        // last sweep closed by a closing kernel (-1: none)
     int32_t fl1;           // flags seen by colour-1 launches
     int32_t flp;           // flags of the previous sweep
+    int32_t fdone;         // one-launch sweeps: blocks finished in the running launch
+    int32_t fpad;
 };
 
 enum : int32_t { F_MEAS = 1, F_EXCH = 2, F_RENORM = 4, F_ADAPT = 8, F_WARM = 16 };

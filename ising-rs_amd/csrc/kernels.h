@@ -36,6 +36,8 @@ struct IsingArgs {
     // The size descriptors again, in the kernel arguments (gk_on when ngroups <= kArgGroups):
     // a block finds its size and tile without a dependent global load.
     int32_t gk_on;
+    int32_t fnblocks;  // one-launch sweeps: boxes over all sizes and replicas
+    uint8_t* pool_out; // one-launch sweeps: lattice buffer written (pool is read)
     DevGroup gk[kArgGroups];
 };
 
@@ -108,6 +110,7 @@ hipError_t launch_wham_reduce(const double* E, int64_t n, const double* logD, co
                               int n_obs, const double* obs, double* part, int nchunk, double* out, hipStream_t st);
 
 hipError_t launch_ising_half(const IsingArgs& a, int nblocks, int color, hipStream_t st);
+hipError_t launch_ising_fused(const IsingArgs& a, int lds_bytes, hipStream_t st);
 hipError_t launch_ising_init(const DevGroup* groups, int ngroups, uint8_t* pool, int nblocks, uint32_t k0,
                              uint32_t k1, hipStream_t st);
 hipError_t launch_ising_measure(const DevGroup* groups, int ngroups, const uint8_t* pool, int64_t* acc,

@@ -45,7 +45,7 @@ class Opts(C.Structure):
                 ("delta", C.c_double), ("n_overrelax", C.c_int32), ("renorm_every", C.c_int32),
                 ("record_capacity", C.c_int32), ("adapt_every", C.c_int32), ("adapt_target", C.c_double),
                 ("graph_sweeps", C.c_int32), ("ising_spt", C.c_int32), ("ising_width", C.c_int32),
-                ("ising_runs", C.c_int32)]
+                ("ising_runs", C.c_int32), ("ising_fused", C.c_int32)]
 
 
 class Observables(C.Structure):
