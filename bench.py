@@ -433,8 +433,8 @@ def main():
     ap.add_argument("--train-steps", type=int, default=400, help="learned workload: Adam steps")
     ap.add_argument("--learned-or", type=int, default=2,
                     help="learned workload: overrelaxation passes per sweep of the learned chain (plain keeps 2)")
-    ap.add_argument("--fused", type=int, default=0, choices=[0, 8, 16],
-                    help="int8 Ising: one launch per sweep with boxes of this many y/z rows (0: one launch per colour)")
+    ap.add_argument("--fused", type=int, default=0, choices=[0, 8, 16, 32],
+                    help="int8 Ising: one launch per sweep, blocks streaming this many z planes (0: one launch per colour)")
     ap.add_argument("--cluster-part", default="all", choices=["all", "throughput"],
                     help="cluster workload: 'throughput' times only the Swendsen-Wang updates of the config-2 ensemble")
     ap.add_argument("--layout", default="auto", metavar="W,R,S",

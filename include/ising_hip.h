@@ -158,11 +158,13 @@ typedef struct {
                                 (0 = automatic); results do not depend on it */
     int32_t ising_runs;      /* Ising: 16-site runs per thread step, 1 or 2 (0 = automatic);
                                 results do not depend on it */
-    /* int8 Ising: 0 = two launches per sweep (one per colour); 8 or 16 = one
-     * launch per sweep, each block updating both colours of a box of that many
-     * y and z rows (plus a recomputed halo) staged in LDS; lattices with
-     * L <= 32 are staged whole.  Results are identical either way.  Needs every
-     * size to be 3D with L <= 32 or L % 32 == 0 (ignored otherwise). */
+    /* int8 Ising: 0 = two launches per sweep (one per colour); 8, 16 or 32 =
+     * one launch per sweep, each block streaming that many z planes (at most
+     * L) of its replicas through LDS and updating both colours (colour 0 of
+     * the two planes beyond its ends is recomputed); lattices with L < 32 are
+     * staged whole.  Results are identical either way.  Needs every size to
+     * be 3D with L < 32 (L % 8 == 0) or L % 32 == 0 and L <= 128 (ignored
+     * otherwise). */
     int32_t ising_fused;
 } ising_pt_opts_t;
 

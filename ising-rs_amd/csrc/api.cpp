@@ -149,7 +149,7 @@ struct ising_pt {
     int64_t* d_mh = nullptr;  // [kMhSlots][8] learned-move counters
     float beta_inline = 0.f, delta_inline = 0.f, j_inline = 1.f;
     int layout_width = 0, layout_runs = 0, layout_spt = 0;  // Ising thread-step layout in use
-    int fused_box = 0;       // one-launch sweeps: box edge asked for (0: two launches per sweep)
+    int fused_z = 0;         // one-launch sweeps: z planes per block asked for (0: two launches per sweep)
     bool fused_on = false;   // one-launch sweeps in use (every size qualifies)
     int fnblocks = 0, fused_lds = 0;
     uint8_t* d_pool2 = nullptr;  // one-launch sweeps: the second lattice buffer
@@ -467,29 +467,39 @@ static int layout_ising(ising_pt* p, int width, int runs, int spt) {
         blk += G.tiles * G.R;
     }
     p->nblocks = blk;
-    // one-launch sweeps: boxes of fused_box y and z rows (L % 32 == 0), or
-    // the whole replica (L <= 32); every size must qualify
+    // one-launch sweeps: sizes with L % 32 == 0 (L <= 128) streamed in z,
+    // fused_z planes (at most L) of P = 16384 / L^2 replicas per block; sizes
+    // with L < 32 whole, one replica per block; every size must qualify.  The
+    // largest size takes the first blocks (its blocks run longest).
     {
-        int fb = 0;
         int64_t lds = 0;
-        bool ok = p->fused_box > 0;
+        bool ok = p->fused_z > 0;
         for (DevGroup& G : p->groups) {
-            G.fbox = 0;
-            G.fboxes = 1;
-            G.fblk0 = fb;
+            G.fz = 0;
+            G.frep = 1;
+            G.fblocks = G.R;
             if (G.dims != 3) {
                 ok = false;
-            } else if (G.L <= 32) {
+            } else if (G.L < 32) {
                 if ((G.Lh % 4) != 0 || ((int64_t)G.L * G.L * G.Lh) % 16 != 0) ok = false;
-                lds = std::max<int64_t>(lds, 2 * (int64_t)G.L * G.L * G.Lh);
-            } else if (ok && G.L % 32 == 0 && G.L % p->fused_box == 0) {
-                G.fbox = p->fused_box;
-                G.fboxes = (G.L / G.fbox) * (G.L / G.fbox);
-                lds = std::max<int64_t>(lds, 2 * (int64_t)(G.fbox + 4) * (G.fbox + 4) * G.Lh);
+                lds = std::max<int64_t>(lds, (int64_t)G.L * G.L * G.L);
+            } else if (ok && G.L % 32 == 0 && G.L <= 128) {
+                G.fz = std::min(p->fused_z, G.L);
+                G.frep = std::min(G.R, 16384 / (G.L * G.L));
+                G.fblocks = (G.R + G.frep - 1) / G.frep * (G.L / G.fz);
+                lds = std::max<int64_t>(lds, 4 * (int64_t)G.frep * G.L * G.L);
             } else {
                 ok = false;
             }
-            fb += G.fboxes * G.R;
+        }
+        std::vector<int> order(p->groups.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return p->groups[x].L > p->groups[y].L; });
+        int fb = 0;
+        for (int i : order) {
+            p->groups[i].fblk0 = fb;
+            fb += p->groups[i].fblocks;
         }
         p->fused_on = ok && lds <= 65536;
         p->fnblocks = fb;
@@ -524,8 +534,8 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     if (o.record_capacity < 1) return fail(ISING_E_ARG, "record_capacity must be >= 1");
     if (o.graph_sweeps < 0 || o.graph_sweeps > 100000) return fail(ISING_E_ARG, "graph_sweeps must be in 0..100000");
     if (o.ising_spt < 0 || o.ising_spt > 4) return fail(ISING_E_ARG, "ising_spt must be in 0..4");
-    if (o.ising_fused != 0 && o.ising_fused != 8 && o.ising_fused != 16)
-        return fail(ISING_E_ARG, "ising_fused must be 0, 8 or 16");
+    if (o.ising_fused != 0 && o.ising_fused != 8 && o.ising_fused != 16 && o.ising_fused != 32)
+        return fail(ISING_E_ARG, "ising_fused must be 0, 8, 16 or 32");
     if (o.ising_width != 0 && o.ising_width != 16 && o.ising_width != 32)
         return fail(ISING_E_ARG, "ising_width must be 0, 16 or 32");
     if (o.ising_runs < 0 || o.ising_runs > 2) return fail(ISING_E_ARG, "ising_runs must be 0, 1 or 2");
@@ -586,7 +596,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     p->nblocks = blk;
     p->pool_bytes = off * elem;
     if (ising) {
-        p->fused_box = model == ISING_MODEL_ISING ? o.ising_fused : 0;
+        p->fused_z = model == ISING_MODEL_ISING ? o.ising_fused : 0;
         const int rc = layout_ising(p, o.ising_width, o.ising_runs, o.ising_spt);
         if (rc) return rc;
     }

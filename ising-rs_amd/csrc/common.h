@@ -128,9 +128,9 @@ struct DevGroup {
     int32_t tsz;             // Ising: half-sites per block tile (256 threads x step width x spt)
     int32_t runs;            // Ising: 16-site runs per thread step (1 or 2, a block row apart)
     int32_t pad2;
-    // one-launch sweeps (int8 Ising): box edge in y and z rows (0: the whole
-    // replica in one box), boxes per replica, first block of this size
-    int32_t fbox, fboxes, fblk0, fpad;
+    // one-launch sweeps (int8 Ising): z planes per block (0: the whole replica
+    // in one block), blocks of this size, its first block, replicas per block
+    int32_t fz, fblocks, fblk0, frep;
 };
 
 // Replica (or replica word) and tile of a block's local index.

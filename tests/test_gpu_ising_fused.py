@@ -1,6 +1,8 @@
-"""One launch per int8 Ising sweep (both colours of a box staged in LDS with a
-recomputed halo): records, permutations and lattices bit-exact against the
-oracle and against the two-launch sweeps, with graphs and annealing."""
+"""One launch per int8 Ising sweep (z planes of several replicas streamed
+through LDS, both colours updated, colour 0 of the planes beyond a block's ends
+recomputed; small lattices whole): records, permutations and lattices
+bit-exact against the oracle and against the two-launch sweeps, with graphs
+and annealing."""
 import numpy as np
 import pytest
 
@@ -10,11 +12,12 @@ pytestmark = pytest.mark.gpu
 ising_hip = pytest.importorskip("ising_hip")
 
 
-@pytest.mark.parametrize("box,sizes", [(8, [8, 16, 32, 64, 128]), (16, [16, 64, 128]), (8, [24, 96])])
-def test_fused_pt_records_are_bit_exact(box, sizes):
-    R, seed, warmup, samples, ee, me = 3, 11, 4, 12, 3, 2
+@pytest.mark.parametrize("planes,sizes,R", [(8, [8, 16, 32, 64, 128], 3), (16, [16, 64, 128], 3), (8, [24, 96], 3),
+                                           (32, [8, 32, 96], 3), (16, [32, 64], 21)])
+def test_fused_pt_records_are_bit_exact(planes, sizes, R):
+    seed, warmup, samples, ee, me = 11, 4, 12, 3, 2
     temps = o.linspace(4.40, 4.62, R)
-    pt = ising_hip.PTEnsemble("ising", sizes, R, temps, seed, exchange_every=ee, measure_every=me, ising_fused=box)
+    pt = ising_hip.PTEnsemble("ising", sizes, R, temps, seed, exchange_every=ee, measure_every=me, ising_fused=planes)
     pt.warmup(warmup)
     pt.sample(samples)
     tidx, bond, mag = pt.read_ising()
@@ -31,7 +34,7 @@ def test_fused_equals_two_launches_with_graphs_and_host_calls():
     temps = o.linspace(4.4, 4.62, 6)
     kw = dict(exchange_every=3, measure_every=2)
     outs = []
-    for fused, graph in ((0, 0), (8, 0), (8, 4), (16, 4)):
+    for fused, graph in ((0, 0), (8, 0), (8, 4), (16, 4), (32, 4)):
         p = ising_hip.PTEnsemble("ising", [16, 32, 64], 6, temps, 21, graph_sweeps=graph, ising_fused=fused, **kw)
         p.warmup(9)
         p.sample(10)
