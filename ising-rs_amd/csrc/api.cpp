@@ -487,7 +487,7 @@ static int layout_ising(ising_pt* p, int width, int runs, int spt) {
                 G.fz = std::min(p->fused_z, G.L);
                 G.frep = std::min(G.R, 16384 / (G.L * G.L));
                 G.fblocks = (G.R + G.frep - 1) / G.frep * (G.L / G.fz);
-                lds = std::max<int64_t>(lds, 4 * (int64_t)G.frep * G.L * G.L);
+                lds = std::max<int64_t>(lds, 9 * (int64_t)G.frep * G.L * G.L / 2);  // 4 + 5 colour-plane slots
             } else {
                 ok = false;
             }
@@ -501,7 +501,7 @@ static int layout_ising(ising_pt* p, int width, int runs, int spt) {
             p->groups[i].fblk0 = fb;
             fb += p->groups[i].fblocks;
         }
-        p->fused_on = ok && lds <= 65536;
+        p->fused_on = ok && lds <= 73728;
         p->fnblocks = fb;
         p->fused_lds = (int)lds;
         if (p->fused_on && p->d_pool && !p->d_pool2) HIP_TRY(hipMalloc(&p->d_pool2, p->pool_bytes));

@@ -2,29 +2,27 @@
 //
 // Sizes with L % 32 == 0 (L <= 128) are streamed in z.  A block owns Z
 // consecutive z planes of P replicas (P = 16384 / L^2, capped by R, so that
-// every thread takes one 16-site chunk of a row per colour pass) and walks them
-// through a ring of four plane slots in LDS, each slot holding both colours of
-// one plane of its P replicas.  With j the plane's index relative to z0 - 2,
-// step s
-//   - writes plane j = s + 2 (loaded two steps earlier into registers) into its
-//     slot and starts the load of plane s + 4;
-//   - updates colour 0 of plane s + 1, then colour 1 of plane s (s >= 2), and
-//     writes plane s back.
-// Colour 0 of the planes z0 - 1 and z0 + Z, which the neighbouring blocks own,
-// is recomputed here from the same inputs with the same Philox counters, so
-// the sweep equals the two-launch sweep bit for bit.  A draw equal to its
-// threshold's high part (probability 2^-7 per site) is queued in LDS and
-// decided after the pass by one lane each, so the pass itself has no divergent
-// second Philox call.  Sizes with L < 32 are staged whole, one replica per
-// block, and updated site by site.
+// every thread takes one 16-site chunk of a row per colour pass and one
+// 16-byte unit of a colour plane per load) and walks them through LDS: colour
+// 0 in a ring of four plane slots, colour 1 in a ring of five, loading each
+// plane two steps ahead into registers.  Colour 1 lags colour 0 by two planes,
+// so a step has two phases: plane writes into LDS plus the decisions of the
+// ties queued in the previous phase, then the two colour passes and the write
+// back of a finished plane (strm_sweep).  Colour 0 of the planes z0 - 1 and
+// z0 + Z, which the neighbouring blocks own, is recomputed here from the same
+// inputs with the same Philox counters, so the sweep equals the two-launch
+// sweep bit for bit.  A draw equal to its threshold's high part (probability
+// 2^-7 per site) is queued in LDS and decided in the next phase by one lane
+// each, so the pass itself has no divergent second Philox call.  Sizes with
+// L < 32 are staged whole, one replica per block, four sites per thread.
 //
 // With no grid-wide dependency between the colours the sweep needs one launch
 // instead of two: the previous sweep is closed (records, replica exchange) at
 // the start as in a colour-0 launch, and the block that finishes last writes
 // the next sweep's control fields (a relaxed counter; every other block has
 // read them by then).  Blocks read one lattice buffer and write the other (a
-// neighbour's halo planes must be the old sweep's); the host alternates the
-// two and copies back after an odd number of sweeps, so between calls the
+// neighbour's boundary planes must be the old sweep's); the host alternates
+// the two and copies back after an odd number of sweeps, so between calls the
 // lattice is where it always is.
 //
 // Bytes per site and sweep: (Z + 4) / Z read + 1 written, against 2 + 1 for two
@@ -66,61 +64,69 @@ struct Shared {
 // What one thread of a streaming block keeps through all steps.
 struct Strm {
     int L, Lh, PB, P, z0, rg;
-    int slot_bytes;  // P * L * L
+    int hs_bytes;    // one colour of one plane of the block's replicas: P * L * L / 2
     // the thread's chunk
     bool cv;         // a chunk of a valid replica
     int lane, y, x0, rym, ryp;
     uint32_t stream;
-    // the thread's two 16-byte load/store units of a plane
-    bool uv[2];
+    // the thread's 16-byte unit of a colour plane
+    bool uv;
     int64_t gbase;   // byte offset of the block's first replica in the lattice buffer
-    uint32_t ug[2];  // byte offset from there at z = 0
-    int ul[2];       // byte offset in the slot
+    uint32_t ug;     // byte offset of the unit's colour-0 plane at z = 0 from there
+    int64_t stride;  // colour 1 lies one stride further
 };
 
-__device__ __forceinline__ uint8_t* slot_of(uint8_t* ring, const Strm& S, int j) {
-    return ring + (j & 3) * S.slot_bytes;
+// LDS: colour 0 in a ring of four plane slots, colour 1 in a ring of five.
+__device__ __forceinline__ uint8_t* slot0(uint8_t* ring, const Strm& S, int j) { return ring + (j & 3) * S.hs_bytes; }
+__device__ __forceinline__ uint8_t* slot1(uint8_t* ring, const Strm& S, int j) {
+    return ring + (4 + j % 5) * S.hs_bytes;
+}
+__device__ __forceinline__ uint8_t* slotc(uint8_t* ring, const Strm& S, int j, int c) {
+    return c ? slot1(ring, S, j) : slot0(ring, S, j);
 }
 
-__device__ __forceinline__ void plane_load(const uint8_t* P0, const Strm& S, int j, v4u (&r)[2]) {
+__device__ __forceinline__ v4u unit_load(const uint8_t* P0, const Strm& S, int j, int c) {
     const int64_t zo = (int64_t)wrap(S.z0 - 2 + j, S.L) * S.PB;
-#pragma unroll
-    for (int k = 0; k < 2; ++k)
-        if (S.uv[k]) r[k] = *(const v4u*)(P0 + S.gbase + zo + S.ug[k]);
+    v4u r = {0u, 0u, 0u, 0u};
+    if (S.uv) r = *(const v4u*)(P0 + S.gbase + (c ? S.stride : 0) + zo + S.ug);
+    return r;
 }
 
-__device__ __forceinline__ void plane_put(uint8_t* ring, const Strm& S, int j, const v4u (&r)[2]) {
-    uint8_t* sl = slot_of(ring, S, j);
-#pragma unroll
-    for (int k = 0; k < 2; ++k)
-        if (S.uv[k]) *(v4u*)(sl + S.ul[k]) = r[k];
+__device__ __forceinline__ void unit_put(uint8_t* ring, const Strm& S, int j, int c, const v4u& r) {
+    if (S.uv) *(v4u*)(slotc(ring, S, j, c) + 16 * threadIdx.x) = r;
 }
 
-__device__ __forceinline__ void plane_store(uint8_t* P1, uint8_t* ring, const Strm& S, int j) {
+// Writes both colours of plane j back; returns the sum of their spins (+-1).
+__device__ __forceinline__ int unit_store(uint8_t* P1, uint8_t* ring, const Strm& S, int j) {
+    if (!S.uv) return 0;
     const int64_t zo = (int64_t)wrap(S.z0 - 2 + j, S.L) * S.PB;
-    const uint8_t* sl = slot_of(ring, S, j);
+    const v4u c0 = *(const v4u*)(slot0(ring, S, j) + 16 * threadIdx.x);
+    const v4u c1 = *(const v4u*)(slot1(ring, S, j) + 16 * threadIdx.x);
+    *(v4u*)(P1 + S.gbase + zo + S.ug) = c0;
+    *(v4u*)(P1 + S.gbase + S.stride + zo + S.ug) = c1;
+    int m = 0;
 #pragma unroll
-    for (int k = 0; k < 2; ++k)
-        if (S.uv[k]) *(v4u*)(P1 + S.gbase + zo + S.ug[k]) = *(const v4u*)(sl + S.ul[k]);
+    for (int k = 0; k < 4; ++k) m += __builtin_popcount(c0[k]) + __builtin_popcount(c1[k]);
+    return 2 * m - 32;
 }
 
 // Colour pass over this thread's chunk of plane j: four sites per 32-bit word,
 // up-neighbour counts per byte from five row words and the x-shifted word
 // (edge byte across the chunk), thresholds by byte permutes of the packed
-// table, all four 7-bit draws compared at once (a byte of exactly 0x80 after
+// table, all four 7-bit draws compared at once.  A byte of exactly 0x80 after
 // the subtraction is a tie: the provisional byte keeps the spin and the site
-// is queued).  count_m: add the chunk's spins to msum; count_b: its bonds to
-// bsum.
+// is queued (LDS offset, table index, lane, colour) for the next phase.
+// count_b: add the chunk's bonds to bsum.
 __device__ __forceinline__ void strm_pass(const IsingArgs& a, uint8_t* ring, const Strm& S, Shared& sh, int j, int c,
-                                          uint32_t hs, int tq, bool count_m, bool count_b, int& msum, int& bsum) {
+                                          uint32_t hs, int tq, bool count_b, int& bsum) {
     if (!S.cv) return;
     const int L = S.L, Lh = S.Lh;
     const int zq = wrap(S.z0 - 2 + j, L);
-    const int lo = 2 * S.lane * S.PB;
-    uint8_t* Cs = slot_of(ring, S, j) + lo + c * S.PB;
-    const uint8_t* Os = slot_of(ring, S, j) + lo + (c ^ 1) * S.PB;
-    const uint8_t* Om = slot_of(ring, S, j - 1) + lo + (c ^ 1) * S.PB;
-    const uint8_t* Op = slot_of(ring, S, j + 1) + lo + (c ^ 1) * S.PB;
+    const int lo = S.lane * S.PB;
+    uint8_t* Cs = slotc(ring, S, j, c) + lo;
+    const uint8_t* Os = slotc(ring, S, j, c ^ 1) + lo;
+    const uint8_t* Om = slotc(ring, S, j - 1, c ^ 1) + lo;
+    const uint8_t* Op = slotc(ring, S, j + 1, c ^ 1) + lo;
     const int rb = S.y * Lh + S.x0;
     const int p = (S.y + zq + c) & 1;
     const v4u vc = *(const v4u*)(Cs + rb);
@@ -161,88 +167,86 @@ __device__ __forceinline__ void strm_pass(const IsingArgs& a, uint8_t* ring, con
             const uint32_t hh = h0 + 4u * (uint32_t)d + jj;
             const int q = atomicAdd(&sh.nt[tq], 1);
             if (q < kTieCap) {
-                sh.tie_w[q] = (uint32_t)(Cs - ring + rb + 4 * d + (int)jj) | (kk << 16) | ((uint32_t)S.lane << 20);
+                sh.tie_w[q] = (uint32_t)(Cs - ring + rb + 4 * d + (int)jj) | (kk << 17) |
+                              ((uint32_t)S.lane << 21) | ((uint32_t)c << 25);
                 sh.tie_h[q] = hh;
             } else if (tie_draw(hh, hs, S.stream, a.k0, a.k1) < tb[14 + kk]) {
-                n2 ^= 1u << (8 * jj);
+                n2 ^= 1u << (8 * jj);  // queue full: decided on the spot
             }
         }
         nc[d] = n2;
-        ups += __builtin_popcount(n2);
-        bn += __builtin_amdgcn_perm(0u, 0x0000ff00u, n2) & ns;
-        sn += ns;
+        if (count_b) {
+            ups += __builtin_popcount(n2);
+            bn += __builtin_amdgcn_perm(0u, 0x0000ff00u, n2) & ns;
+            sn += ns;
+        }
     }
     *(v4u*)(Cs + rb) = nc;
-    if (count_m) msum += 2 * ups - 16;
     if (count_b)
         bsum += 4 * (int)__builtin_amdgcn_sad_u8(bn, 0u, 0u) - 12 * ups - 2 * (int)__builtin_amdgcn_sad_u8(sn, 0u, 0u) + 96;
 }
 
-// Decides the queued ties of the last pass (one lane each) and patches their
-// bytes; the measurement sums move by the flips.
+// Decides the ties queued in the last phase (one lane each) and patches their
+// bytes; the bond sums move by the colour-1 flips.
 __device__ __forceinline__ void strm_ties(const IsingArgs& a, uint8_t* ring, const Strm& S, Shared& sh, int nt,
-                                          uint32_t hs, bool count_m, bool count_b) {
+                                          uint32_t hs0, bool need) {
     for (int k = threadIdx.x; k < nt; k += kFT) {
         const uint32_t w = sh.tie_w[k];
-        const uint32_t kk = (w >> 16) & 15u, lane = (w >> 20) & 15u;
+        const uint32_t kk = (w >> 17) & 15u, lane = (w >> 21) & 15u, c = (w >> 25) & 1u;
         const uint32_t stream = stream_id((uint32_t)S.L, (uint32_t)(S.rg * S.P + (int)lane));
-        if (tie_draw(sh.tie_h[k], hs, stream, a.k0, a.k1) < sh.thr[lane * kIsingTab + 14 + kk]) {
-            ring[w & 0xffffu] ^= 1u;
-            const int b = kk >= 7u ? 1 : 0, n = (int)kk - 7 * b;
-            const int dm = 2 - 4 * b;  // spin change of the flip
-            if (count_m) atomicAdd(&sh.acc[lane][1], dm);
-            if (count_b) atomicAdd(&sh.acc[lane][0], dm * (2 * n - 6));
+        if (tie_draw(sh.tie_h[k], hs0 + c, stream, a.k0, a.k1) < sh.thr[lane * kIsingTab + 14 + kk]) {
+            ring[w & 0x1ffffu] ^= 1u;
+            if (need && c) {
+                const int b = kk >= 7u ? 1 : 0, n = (int)kk - 7 * b;
+                atomicAdd(&sh.acc[lane][0], (2 - 4 * b) * (2 * n - 6));
+            }
         }
     }
 }
 
-// The steps of a streaming block (see the top of the file).
+// The steps of a streaming block.  With j the plane index relative to z0 - 2,
+// step s has two phases separated by barriers:
+//   A: colour 0 of plane s + 1 and colour 1 of plane s + 2 go from registers
+//      into their slots (loads of the planes two steps ahead start), and the
+//      ties queued in the previous phase B are decided;
+//   B: colour 0 of plane s + 1 is updated (j = 1 .. Z + 2: the planes z0 - 1
+//      and z0 + Z are recomputed), colour 1 of plane s - 1 (j = 2 .. Z + 1),
+//      and plane s - 2 (final in both colours) is written back.
+// Colour 1 of plane j reads colour 0 of j - 1 .. j + 1, updated at steps
+// j - 2 .. j with their ties decided by step j + 1; colour 0 of plane j reads
+// colour 1 of j - 1 .. j + 1 before any of them is updated.
 __device__ __forceinline__ void strm_sweep(const IsingArgs& a, uint8_t* s_ring, const Strm& S, Shared& sh, int Z,
                                            uint32_t hs0, uint32_t hs1, bool need, int& msum, int& bsum) {
     const uint8_t* P0 = a.pool;
     uint8_t* P1 = a.pool_out;
-    v4u ra[2], rb[2];
-    // planes z0 - 2 and z0 - 1 straight into their slots, z0 and z0 + 1 into registers
-    plane_load(P0, S, 0, ra);
-    plane_load(P0, S, 1, rb);
-    plane_put(s_ring, S, 0, ra);
-    plane_put(s_ring, S, 1, rb);
-    plane_load(P0, S, 2, ra);
-    plane_load(P0, S, 3, rb);
-    int tq = 0;
-    auto step = [&](int s, v4u(&r)[2], bool back) {
-        plane_put(s_ring, S, s + 2, r);
-        // the last two loads fetch plane Z + 3 again, unused: no branch around them
-        plane_load(P0, S, min(s + 4, Z + 3), r);
+    // colour 1 of planes 0 and 1 straight into their slots; registers hold
+    // (colour 0 of j + 1, colour 1 of j + 2) for the next two steps
+    unit_put(s_ring, S, 0, 1, unit_load(P0, S, 0, 1));
+    unit_put(s_ring, S, 1, 1, unit_load(P0, S, 1, 1));
+    v4u a0 = unit_load(P0, S, 1, 0), a1 = unit_load(P0, S, 2, 1);
+    v4u b0 = unit_load(P0, S, 2, 0), b1 = unit_load(P0, S, 3, 1);
+    auto step = [&](int s, v4u& r0, v4u& r1) {
+        // phase A
+        if (s + 1 <= Z + 2) unit_put(s_ring, S, s + 1, 0, r0);
+        if (s + 2 <= Z + 3) unit_put(s_ring, S, s + 2, 1, r1);
+        r0 = unit_load(P0, S, min(s + 3, Z + 2), 0);
+        r1 = unit_load(P0, S, min(s + 4, Z + 3), 1);
+        const int nt = min(sh.nt[(s + 1) & 1], kTieCap);
+        strm_ties(a, s_ring, S, sh, nt, hs0, need);
         __syncthreads();
-        // colour 0 of plane s + 1 (owned when it lies in [z0, z0 + Z))
-        const bool own0 = s + 1 >= 2 && s + 1 <= Z + 1;
-        strm_pass(a, s_ring, S, sh, s + 1, 0, hs0, tq, need && own0, false, msum, bsum);
-        if (threadIdx.x == 0) sh.nt[tq ^ 1] = 0;
+        // phase B
+        if (threadIdx.x == 0) sh.nt[(s + 1) & 1] = 0;
+        if (s <= Z + 1) strm_pass(a, s_ring, S, sh, s + 1, 0, hs0, s & 1, false, bsum);
+        if (s >= 3 && s <= Z + 2) strm_pass(a, s_ring, S, sh, s - 1, 1, hs1, s & 1, need, bsum);
+        if (s >= 4) {
+            const int m = unit_store(P1, s_ring, S, s - 2);
+            if (need) msum += m;
+        }
         __syncthreads();
-        int nt = min(sh.nt[tq], kTieCap);
-        if (nt) {
-            strm_ties(a, s_ring, S, sh, nt, hs0, need && own0, false);
-            __syncthreads();
-        }
-        tq ^= 1;
-        if (back) {
-            // colour 1 of plane s, then the plane goes back
-            strm_pass(a, s_ring, S, sh, s, 1, hs1, tq, need, need, msum, bsum);
-            if (threadIdx.x == 0) sh.nt[tq ^ 1] = 0;
-            __syncthreads();
-            nt = min(sh.nt[tq], kTieCap);
-            if (nt) {
-                strm_ties(a, s_ring, S, sh, nt, hs1, need, need);
-                __syncthreads();
-            }
-            tq ^= 1;
-            plane_store(P1, s_ring, S, s);
-        }
     };
-    for (int s = 0; s < Z + 2; s += 2) {
-        step(s, ra, s >= 2);
-        step(s + 1, rb, s + 1 >= 2);
+    for (int s = 0; s < Z + 4; s += 2) {
+        step(s, a0, a1);
+        step(s + 1, b0, b1);
     }
 }
 
@@ -355,7 +359,7 @@ __global__ __launch_bounds__(kFT) void ising_fused_kernel(IsingArgs a) {
         S.P = P;
         S.z0 = zc * G.fz;
         S.rg = rg;
-        S.slot_bytes = P * 2 * S.PB;
+        S.hs_bytes = P * S.PB;
         {
             const int cpr = Lh >> 4, cpl = L * cpr;
             const int i = (int)threadIdx.x;
@@ -370,18 +374,14 @@ __global__ __launch_bounds__(kFT) void ising_fused_kernel(IsingArgs a) {
             S.stream = stream_id((uint32_t)L, (uint32_t)(rg * P + S.lane));
             my_lane = S.lane;
         }
-        S.gbase = G.off + 2 * (int64_t)rg * P * G.stride;
         {
-            const int upc = S.PB >> 4;  // 16-byte units per colour plane
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const int u = (int)threadIdx.x + k * kFT;
-                const int pc = u / upc, o = u - pc * upc;
-                const int ln = pc >> 1, c = pc & 1;
-                S.uv[k] = ln < nlanes;
-                S.ug[k] = (uint32_t)((2 * ln + c) * G.stride + 16 * o);  // < 2 P stride <= 16384 L
-                S.ul[k] = 16 * u;
-            }
+            const int upc = S.PB >> 4;  // 16-byte units per colour plane of one replica
+            const int u = (int)threadIdx.x;
+            const int ln = u / upc, o = u - ln * upc;
+            S.uv = ln < nlanes;
+            S.gbase = G.off + 2 * (int64_t)rg * P * G.stride;
+            S.ug = (uint32_t)(2 * ln * G.stride + 16 * o);  // < 2 P stride <= 16384 L
+            S.stride = G.stride;
         }
         strm_sweep(a, s_ring, S, sh, G.fz, hs0, hs1, need, msum, bsum);
     } else {
