@@ -526,7 +526,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     for (int i = 0; i < n_sizes; ++i)
         if (!valid_L(sizes[i])) return fail(ISING_E_ARG, "lattice sizes must be even, 2..2046");
     for (int i = 0; i < R; ++i)
-        if (!(temps[i] > 0.0) || !std::isfinite(temps[i])) return fail(ISING_E_ARG, "temperatures must be positive");
+        if (!(temps[i] > 0.0) || !std::isfinite(temps[i])) return fail(ISING_E_ARG, "temperatures must be positive and finite");
     ising_pt_opts_t o;
     if (opts) o = *opts;
     else ising_pt_default_opts(&o);

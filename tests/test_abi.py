@@ -85,6 +85,24 @@ def test_invalid_arguments_are_rejected_before_touching_the_device():
     assert ei.value.code == -1
     with pytest.raises(ising_hip.IsingError):
         ising_hip.PTEnsemble("ising", [8], 2, np.array([4.4, 4.6]), 1, exchange_every=0)
+    bad = [
+        lambda: ising_hip.LatticeGpu(0, 1),
+        lambda: ising_hip.LatticeGpu(4096, 1),  # beyond 2..2046
+        lambda: ising_hip.ContinuousGpuLattice(8, 4, 1),
+        lambda: ising_hip.PTEnsemble("ising", [], 2, [4.4, 4.5], 1),
+        lambda: ising_hip.PTEnsemble("ising", [8], 0, [], 1),
+        lambda: ising_hip.PTEnsemble("ising", [8], 2, [4.4, float("nan")], 1),
+        lambda: ising_hip.PTEnsemble("ising", [8], 2, [4.4, float("inf")], 1),
+        lambda: ising_hip.PTEnsemble("ising", [8, 9], 2, [4.4, 4.5], 1),
+        lambda: ising_hip.PTEnsemble("ising", [8], 2, [4.4, 4.5], 1, measure_every=0),
+        lambda: ising_hip.PTEnsemble("ising", [8], 2, [4.4, 4.5], 1, record_capacity=0),
+        lambda: ising_hip.PTEnsemble("ising", [8], 2, [4.4, 4.5], 1, ising_spt=9),
+        lambda: ising_hip.PTEnsemble("ising", [8], 2, [4.4, 4.5], 1, ising_fused=12),
+    ]
+    for make in bad:
+        with pytest.raises(ising_hip.IsingError) as ei:
+            make()
+        assert ei.value.code == -1
 
 
 def test_binary_timeseries_reader_parses_the_documented_layout(tmp_path):
