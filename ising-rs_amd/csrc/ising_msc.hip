@@ -43,6 +43,36 @@ __device__ __forceinline__ void count6(const uint32_t v[6], uint32_t* x0, uint32
     full_add(k1, k2, k0, x1, x2);
 }
 
+// Column sums of the wave's 64 x 32 bit matrix (one word per lane): lane l < 32
+// returns the number of lanes whose word has bit l set.  Each 32-lane half is
+// transposed as a 32 x 32 bit matrix in five butterfly stages (partner lane
+// l ^ j by ds_swizzle, a rotation by j, a masked merge that swaps the j x j
+// off-diagonal blocks), after which lane l holds column l of its half; one
+// popcount per half and a cross-half add give the counts.  All 64 lanes must
+// be active.
+template <int J>
+__device__ __forceinline__ uint32_t transpose_stage(uint32_t w, uint32_t lane) {
+    constexpr uint32_t m = J == 16 ? 0x0000ffffu : J == 8 ? 0x00ff00ffu : J == 4 ? 0x0f0f0f0fu
+                                                       : J == 2 ? 0x33333333u : 0x55555555u;
+    const uint32_t p = (uint32_t)__builtin_amdgcn_ds_swizzle((int)w, 0x1f | (J << 10));  // lane ^ J
+    const bool upper = (lane & (uint32_t)J) != 0u;
+    // lower lanes take the partner's low block shifted up, upper lanes its high block shifted down
+    const uint32_t rot = __builtin_amdgcn_alignbit(p, p, upper ? (uint32_t)J : 32u - (uint32_t)J);
+    const uint32_t keep = upper ? ~m : m;
+    return (w & keep) | (rot & ~keep);
+}
+
+__device__ __forceinline__ int column_count(uint32_t w) {
+    const uint32_t lane = threadIdx.x & 63u;
+    w = transpose_stage<16>(w, lane);
+    w = transpose_stage<8>(w, lane);
+    w = transpose_stage<4>(w, lane);
+    w = transpose_stage<2>(w, lane);
+    w = transpose_stage<1>(w, lane);
+    const int c = __builtin_popcount(w);
+    return c + __shfl_xor(c, 32);
+}
+
 __device__ __forceinline__ int find_group_m(const DevGroup* groups, int ngroups, int b) {
     int g = 0;
     while (g + 1 < ngroups && b >= groups[g + 1].blk0) ++g;
@@ -151,20 +181,16 @@ __global__ __launch_bounds__(kMT) void msc_half_kernel(IsingArgs a, int color) {
     }
     if (need) {
         // per replica r: M_r += 2 * #up - #sites, B_r += sum over sites of (6 - 2 * anti_r)
+        // (ns and x0..x2 are 0 on lanes without a site); lane r of the wave gets replica r's counts
         const uint64_t lv = __ballot(live);
         const int nl = __popcll(lv);
-        for (int r = 0; r < nrep; ++r) {
-            const int up = __popcll(__ballot(live && ((ns >> r) & 1u)));
-            int bsum = 0;
-            if (color) {
-                const int anti = __popcll(__ballot((x0 >> r) & 1u)) + 2 * __popcll(__ballot((x1 >> r) & 1u)) +
-                                 4 * __popcll(__ballot((x2 >> r) & 1u));
-                bsum = 6 * nl - 2 * anti;
-            }
-            if ((tid & 63) == 0) {
-                atomicAdd(&s_acc[r][1], 2 * up - nl);
-                if (color) atomicAdd(&s_acc[r][0], bsum);
-            }
+        const int up = column_count(ns);
+        int anti = 0;
+        if (color) anti = column_count(x0) + 2 * column_count(x1) + 4 * column_count(x2);
+        const int r = tid & 63;
+        if (r < nrep) {
+            atomicAdd(&s_acc[r][1], 2 * up - nl);
+            if (color) atomicAdd(&s_acc[r][0], 6 * nl - 2 * anti);
         }
         __syncthreads();
         if (tid < 2 * nrep) {
