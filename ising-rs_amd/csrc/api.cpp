@@ -248,7 +248,7 @@ struct ising_pt {
         if (inline_thr) std::memcpy(a.thr_inline, thr_inline, sizeof(thr_inline));
         a.mthr = d_mthr;
         a.mplanes = d_mplanes;
-        a.acc_copies = model == ISING_MODEL_ISING_MSC ? kMscAccCopies : 1;
+        a.acc_copies = model == ISING_MODEL_ISING_MSC ? kMscAccCopies : kIsingAccCopies;
         a.acc_stride = 2 * (int64_t)a.acc_copies * total_reps;
         a.sched = d_sched_on ? d_sched : nullptr;
         a.sched_t0 = sched_t0;

@@ -389,7 +389,9 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
                 B += s_red[0][i];
                 M += s_red[1][i];
             }
-            int64_t* acc = acc_of(a, t);
+            // accumulator copy by block (copies are summed where read): fewer
+            // atomics per address than one copy taking every tile of a replica
+            int64_t* acc = acc_of(a, t) + 2 * (int64_t)(b & (a.acc_copies - 1)) * a.total_reps;
             if (color) atomicAdd((unsigned long long*)&acc[2 * grep], (unsigned long long)B);
             atomicAdd((unsigned long long*)&acc[2 * grep + 1], (unsigned long long)M);
         }
