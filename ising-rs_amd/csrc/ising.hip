@@ -279,7 +279,8 @@ __device__ __forceinline__ int fast_tile(const IsingArgs& a, const DevGroup& G, 
 // neighbour planes of a tile then stay in that XCD's L2).
 __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int color) {
     __shared__ uint32_t s_thr[kIsingTab];
-    __shared__ int64_t s_red[2][kThreads / 64];
+    __shared__ int s_red[2];  // block sums of the bond sum and magnetisation
+    __shared__ int s_waves;    // waves done with the block sums
     __shared__ uint32_t s_qh[kTieCap], s_qm[kTieCap];
     __shared__ int s_qn;
     const int per = (int)(gridDim.x >> 3);
@@ -317,7 +318,11 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
                              : a.inline_thr           ? a.thr_inline[threadIdx.x]
                              : (a.thr_rep && !v.exch) ? thr_pre
                                                       : a.thr[(int64_t)(G.rep0 + tidx) * kIsingTab + threadIdx.x];
-    if (threadIdx.x == 0) s_qn = 0;
+    if (threadIdx.x == 0) {
+        s_qn = 0;
+        s_red[0] = s_red[1] = 0;
+        s_waves = 0;
+    }
     const int64_t t = v.t;
     const bool need = (v.flags & (F_MEAS | F_EXCH)) != 0;
     __syncthreads();
@@ -326,7 +331,7 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
     const uint32_t stream = stream_id((uint32_t)G.L, (uint32_t)r);
     uint8_t* C = a.pool + G.off + (int64_t)(2 * r + color) * G.stride;
     const uint8_t* O = a.pool + G.off + (int64_t)(2 * r + (color ^ 1)) * G.stride;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
 
     int bsum = 0, msum = 0;
     if (G.fast == 3) {
@@ -375,25 +380,25 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
         }
     }
 
-    // Fused measurement: wave -> block -> one int64 atomic per block and quantity.
+    // Fused measurement: wave -> block -> one int64 atomic per block and
+    // quantity.  No barrier: each wave adds its sums in LDS and the wave that
+    // finishes last (workgroup-scope acquire-release counter) issues the
+    // atomics, so waves without tied sites need not wait for the tie pass.
     if (need) {
         const int wb = wave_sum(bsum), wm = wave_sum(msum);
         if (lane == 0) {
-            s_red[0][wid] = wb;
-            s_red[1][wid] = wm;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int64_t B = 0, M = 0;
-            for (int i = 0; i < kThreads / 64; ++i) {
-                B += s_red[0][i];
-                M += s_red[1][i];
+            atomicAdd(&s_red[0], wb);
+            atomicAdd(&s_red[1], wm);
+            const int done = __hip_atomic_fetch_add(&s_waves, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (done == kThreads / 64 - 1) {
+                const int64_t B = __hip_atomic_load(&s_red[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                const int64_t M = __hip_atomic_load(&s_red[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                // accumulator copy by block (copies are summed where read): fewer
+                // atomics per address than one copy taking every tile of a replica
+                int64_t* acc = acc_of(a, t) + 2 * (int64_t)(b & (a.acc_copies - 1)) * a.total_reps;
+                if (color) atomicAdd((unsigned long long*)&acc[2 * grep], (unsigned long long)B);
+                atomicAdd((unsigned long long*)&acc[2 * grep + 1], (unsigned long long)M);
             }
-            // accumulator copy by block (copies are summed where read): fewer
-            // atomics per address than one copy taking every tile of a replica
-            int64_t* acc = acc_of(a, t) + 2 * (int64_t)(b & (a.acc_copies - 1)) * a.total_reps;
-            if (color) atomicAdd((unsigned long long*)&acc[2 * grep], (unsigned long long)B);
-            atomicAdd((unsigned long long*)&acc[2 * grep + 1], (unsigned long long)M);
         }
     }
 }
