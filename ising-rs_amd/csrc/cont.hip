@@ -23,7 +23,8 @@
 
 namespace ising {
 
-constexpr int kCT = 256;  // threads per block, one site per thread
+constexpr int kCT = 256;        // threads per block, one site per thread
+constexpr int kCloseT = 1024;  // threads of the one-block closing kernel (one wave per replica at a time)
 
 __device__ __forceinline__ int find_group_c(const DevGroup* groups, int ngroups, int b) {
     int g = 0;
@@ -102,12 +103,14 @@ __device__ void close_cont(const ContArgs& a, int64_t t, int32_t fl) {
         // one wave per replica: lanes sum the block partials in a fixed
         // assignment, then a fixed shuffle tree (deterministic FP64 order)
         const int lane = tid & 63, wave = tid >> 6;
-        for (int gr = wave; gr < a.total_reps; gr += kCT / 64) {
+        for (int gr = wave; gr < a.total_reps; gr += kCloseT / 64) {
             int g = 0;
             while (g + 1 < a.ngroups && gr >= a.groups[g + 1].rep0) ++g;
             const int64_t tiles = a.groups[g].tiles;
             const int64_t b0 = a.groups[g].blk0 + (int64_t)(gr - a.groups[g].rep0) * tiles;
             double v[4] = {0.0, 0.0, 0.0, 0.0};
+            // unrolled so that a lane's loads are in flight together (the sums keep their order)
+#pragma unroll 8
             for (int64_t b = lane; b < tiles; b += 64) {
                 const double* p0 = a.part0 + 4 * (b0 + b);
                 const double* p1 = a.part1 + 4 * (b0 + b);
@@ -139,7 +142,7 @@ __device__ void close_cont(const ContArgs& a, int64_t t, int32_t fl) {
         for (int g = 0; g < a.ngroups; ++g) {
             const DevGroup G = a.groups[g];
             const int npairs = (G.R - (int)(round & 1)) / 2;
-            for (int q = tid; q < npairs; q += kCT) {
+            for (int q = tid; q < npairs; q += kCloseT) {
                 const int i = (int)(round & 1) + 2 * q;
                 const int ri = a.t2r[G.rep0 + i], rj = a.t2r[G.rep0 + i + 1];
                 const double Ei = a.dacc[4 * (G.rep0 + ri)];
@@ -162,7 +165,7 @@ __device__ void close_cont(const ContArgs& a, int64_t t, int32_t fl) {
     // Warm-up only: scale each temperature's proposal size towards the target
     // acceptance.  Frozen during sampling, so the sampled chain is stationary.
     if (a.adapt_every > 0 && (fl & F_ADAPT)) {
-        for (int gr = tid; gr < a.total_reps; gr += kCT) {
+        for (int gr = tid; gr < a.total_reps; gr += kCloseT) {
             int g = 0;
             while (g + 1 < a.ngroups && gr >= a.groups[g + 1].rep0) ++g;
             const DevGroup G = a.groups[g];
@@ -302,7 +305,7 @@ __global__ __launch_bounds__(kCT) void cont_metro_kernel(ContArgs a, int color, 
 }
 
 // Closes a continuous sweep (one block, after the colour-1 Metropolis pass).
-__global__ __launch_bounds__(kCT) void cont_close_kernel(ContArgs a) {
+__global__ __launch_bounds__(kCloseT) void cont_close_kernel(ContArgs a) {
     close_cont(a, a.ctl->t, a.ctl->flags);
 }
 
@@ -456,7 +459,7 @@ hipError_t launch_cont_metro(const ContArgs& a, int nblocks, int color, int clos
 }
 
 hipError_t launch_cont_close(const ContArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(cont_close_kernel, dim3(1), dim3(kCT), 0, st, a);
+    hipLaunchKernelGGL(cont_close_kernel, dim3(1), dim3(kCloseT), 0, st, a);
     return hipGetLastError();
 }
 
