@@ -134,6 +134,27 @@ struct DevGroup {
     int32_t fz, fblocks, fblk0, frep;
 };
 
+// The six neighbour half-indices (in the other colour) of half-site h of
+// colour `color` of group G: same row, x-shifted, y -/+ 1, z -/+ 1.  32-bit
+// throughout (L^3 / 2 < 2^32 for every accepted L), divisions by the group's
+// magic numbers (Lh = 1 only for L = 2).
+__device__ __forceinline__ void site_nbrs(const DevGroup& G, uint32_t h, int color, uint32_t nb[6]) {
+    const uint32_t L = (uint32_t)G.L, Lh = (uint32_t)G.Lh;
+    const uint32_t row = Lh >= 2u ? fastdiv(h, G.mag_Lh, G.sh_Lh) : h;
+    const uint32_t xh = h - row * Lh;
+    const uint32_t z = fastdiv(row, G.mag_L, G.sh_L), y = row - z * L;
+    const uint32_t p = (y + z + (uint32_t)color) & 1u;
+    const uint32_t ym = y == 0u ? L - 1u : y - 1u, yp = y + 1u == L ? 0u : y + 1u;
+    const uint32_t zm = z == 0u ? L - 1u : z - 1u, zp = z + 1u == L ? 0u : z + 1u;
+    const uint32_t xo = p ? (xh + 1u == Lh ? 0u : xh + 1u) : (xh == 0u ? Lh - 1u : xh - 1u);
+    nb[0] = row * Lh + xh;
+    nb[1] = row * Lh + xo;
+    nb[2] = (z * L + ym) * Lh + xh;
+    nb[3] = (z * L + yp) * Lh + xh;
+    nb[4] = (zm * L + y) * Lh + xh;
+    nb[5] = (zp * L + y) * Lh + xh;
+}
+
 // Replica (or replica word) and tile of a block's local index.
 __device__ __forceinline__ void split_tiles(const DevGroup& G, int local, int* unit, int* tile) {
     const uint32_t u = G.tiles >= 2 ? fastdiv((uint32_t)local, G.mag_T, G.sh_T) : (uint32_t)local;

@@ -32,25 +32,12 @@ __device__ __forceinline__ int find_group_c(const DevGroup* groups, int ngroups,
     return g;
 }
 
-__device__ __forceinline__ void neighbours(const DevGroup& G, int64_t h, int color, int64_t nb[6]) {
-    const int L = G.L, Lh = G.Lh;
-    const int64_t row = h / Lh;
-    const int xh = (int)(h - row * Lh);
-    const int z = (int)(row / L), y = (int)(row - (int64_t)z * L);
-    const int p = (y + z + color) & 1;
-    const int ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
-    const int zm = z == 0 ? L - 1 : z - 1, zp = z == L - 1 ? 0 : z + 1;
-    const int xo = p ? (xh + 1 == Lh ? 0 : xh + 1) : (xh == 0 ? Lh - 1 : xh - 1);
-    nb[0] = row * Lh + xh;
-    nb[1] = row * Lh + xo;
-    nb[2] = ((int64_t)z * L + ym) * Lh + xh;
-    nb[3] = ((int64_t)z * L + yp) * Lh + xh;
-    nb[4] = ((int64_t)zm * L + y) * Lh + xh;
-    nb[5] = ((int64_t)zp * L + y) * Lh + xh;
+__device__ __forceinline__ void neighbours(const DevGroup& G, int64_t h, int color, uint32_t nb[6]) {
+    site_nbrs(G, (uint32_t)h, color, nb);
 }
 
 template <int N>
-__device__ __forceinline__ void load_nbrs(const float* O, int64_t stride, const int64_t nb[6], float nv[3][6]) {
+__device__ __forceinline__ void load_nbrs(const float* O, int64_t stride, const uint32_t nb[6], float nv[3][6]) {
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         const float* Ok = O + k * stride;
@@ -67,7 +54,7 @@ __device__ __forceinline__ void field_f32(const float nv[3][6], float hf[3]) {
 }
 
 template <int N>
-__device__ __forceinline__ void load_field(const float* O, int64_t stride, const int64_t nb[6], float hf[3]) {
+__device__ __forceinline__ void load_field(const float* O, int64_t stride, const uint32_t nb[6], float hf[3]) {
     float nv[3][6];
     load_nbrs<N>(O, stride, nb, nv);
     field_f32<N>(nv, hf);
@@ -216,7 +203,7 @@ __global__ __launch_bounds__(kCT) void cont_metro_kernel(ContArgs a, int color, 
     double part[4] = {0.0, 0.0, 0.0, 0.0};
     int acc_n = 0;
     if (h < G.n_half) {
-        int64_t nb[6];
+        uint32_t nb[6];
         neighbours(G, h, color, nb);
         float nv[3][6];
         load_nbrs<N>(O, G.stride, nb, nv);
@@ -320,7 +307,7 @@ __global__ __launch_bounds__(kCT) void cont_or_kernel(ContArgs a, int color) {
     const float* O = a.pool + G.off + (int64_t)(2 * r + (color ^ 1)) * N * G.stride;
     const int64_t h = (int64_t)tile * kCT + threadIdx.x;
     if (h >= G.n_half) return;
-    int64_t nb[6];
+    uint32_t nb[6];
     neighbours(G, h, color, nb);
     float hf[3];
     load_field<N>(O, G.stride, nb, hf);
@@ -388,7 +375,7 @@ __global__ __launch_bounds__(kCT) void cont_measure_kernel(const DevGroup* group
     const int64_t h = (int64_t)tile * kCT + threadIdx.x;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     if (h < G.n_half) {
-        int64_t nb[6];
+        uint32_t nb[6];
         neighbours(G, h, 1, nb);
         double e = 0.0;
 #pragma unroll
@@ -424,7 +411,7 @@ __global__ __launch_bounds__(kCT) void cont_helicity_kernel(const DevGroup* grou
         for (int c = 0; c < 2; ++c) {
             const float* Ci = pool + G.off + (int64_t)(2 * r + c) * 2 * G.stride;
             const float* Cj = pool + G.off + (int64_t)(2 * r + (c ^ 1)) * 2 * G.stride;
-            int64_t nb[6];
+            uint32_t nb[6];
             neighbours(G, h, c, nb);
             const int64_t row = h / G.Lh;
             const int z = (int)(row / G.L), y = (int)(row - (int64_t)z * G.L);

@@ -373,17 +373,8 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int color
     float hn = 0.f;
     if (live) {
         // neighbours (same addressing as the plain pass)
-        const int L = G.L, Lh = G.Lh;
-        const int64_t row = h / Lh;
-        const int xh = (int)(h - row * Lh);
-        const int z = (int)(row / L), y = (int)(row - (int64_t)z * L);
-        const int p = (y + z + color) & 1;
-        const int ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
-        const int zm = z == 0 ? L - 1 : z - 1, zp = z == L - 1 ? 0 : z + 1;
-        const int xo = p ? (xh + 1 == Lh ? 0 : xh + 1) : (xh == 0 ? Lh - 1 : xh - 1);
-        const int64_t nb[6] = {row * Lh + xh, row * Lh + xo, ((int64_t)z * L + ym) * Lh + xh,
-                               ((int64_t)z * L + yp) * Lh + xh, ((int64_t)zm * L + y) * Lh + xh,
-                               ((int64_t)zp * L + y) * Lh + xh};
+        uint32_t nb[6];
+        site_nbrs(G, (uint32_t)h, color, nb);
 #pragma unroll
         for (int k = 0; k < N; ++k)
 #pragma unroll
@@ -575,17 +566,8 @@ __global__ __launch_bounds__(kLearnThreads) void learn_grad_kernel(ContArgs a, i
         const int r = gr - G.rep0;
         const int color = (int)(w.z & 1u);
         const int64_t h = (int64_t)(((uint64_t)w.y * (uint64_t)G.n_half) >> 32);
-        const int L = G.L, Lh = G.Lh;
-        const int64_t row = h / Lh;
-        const int xh = (int)(h - row * Lh);
-        const int z = (int)(row / L), y = (int)(row - (int64_t)z * L);
-        const int p = (y + z + color) & 1;
-        const int ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
-        const int zm = z == 0 ? L - 1 : z - 1, zp = z == L - 1 ? 0 : z + 1;
-        const int xo = p ? (xh + 1 == Lh ? 0 : xh + 1) : (xh == 0 ? Lh - 1 : xh - 1);
-        const int64_t nb[6] = {row * Lh + xh, row * Lh + xo, ((int64_t)z * L + ym) * Lh + xh,
-                               ((int64_t)z * L + yp) * Lh + xh, ((int64_t)zm * L + y) * Lh + xh,
-                               ((int64_t)zp * L + y) * Lh + xh};
+        uint32_t nb[6];
+        site_nbrs(G, (uint32_t)h, color, nb);
         const float* Cp = a.pool + G.off + (int64_t)(2 * r + color) * N * G.stride;
         const float* Op = a.pool + G.off + (int64_t)(2 * r + (color ^ 1)) * N * G.stride;
         float nv[3][6];
