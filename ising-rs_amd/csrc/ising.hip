@@ -353,9 +353,9 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
             for (int j = 0; j < kSitesPerThread; ++j) {
                 const int64_t h = h0 + j;
                 if (h >= G.n_half) break;
-                const int64_t row = h / Lh;
+                const int64_t row = Lh >= 2 ? (int64_t)fastdiv((uint32_t)h, G.mag_Lh, G.sh_Lh) : h;
                 const int xh = (int)(h - row * Lh);
-                const int z = (int)(row / L), y = (int)(row - (int64_t)z * L);
+                const int z = (int)fastdiv((uint32_t)row, G.mag_L, G.sh_L), y = (int)(row - (int64_t)z * L);
                 const int p = (y + z + color) & 1;
                 const int ym = y == 0 ? L - 1 : y - 1, yp = y == L - 1 ? 0 : y + 1;
                 const int zm = z == 0 ? L - 1 : z - 1, zp = z == L - 1 ? 0 : z + 1;
