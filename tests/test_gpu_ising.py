@@ -68,7 +68,7 @@ def test_ground_state_preserved_and_measure_observables():
 
 
 @pytest.mark.parametrize("sizes,R,warmup,samples,ee,me", [([4, 8], 4, 10, 60, 10, 5), ([8, 16, 6], 8, 20, 100, 10, 1),
-                                                          ([32], 6, 5, 40, 3, 2)])
+                                                          ([32], 6, 5, 40, 3, 2), ([24, 40], 3, 4, 12, 3, 2)])
 def test_pt_ensemble_records_are_bit_exact(sizes, R, warmup, samples, ee, me):
     seed = 42
     temps = o.linspace(4.40, 4.62, R)
@@ -139,11 +139,11 @@ def test_thread_step_layouts_do_not_change_results():
     temps = o.linspace(4.40, 4.62, 4)
     out = []
     for width, runs, spt in ((0, 0, 0), (16, 1, 1), (16, 2, 1), (32, 1, 1), (32, 1, 2), (16, 2, 4), (16, 1, 3)):
-        pt = ising_hip.PTEnsemble("ising", [16, 32, 64, 96, 128], 4, temps, 5, exchange_every=3, measure_every=2,
-                                  ising_spt=spt, ising_width=width, ising_runs=runs)
+        pt = ising_hip.PTEnsemble("ising", [8, 16, 24, 32, 64, 96, 128], 4, temps, 5, exchange_every=3,
+                                  measure_every=2, ising_spt=spt, ising_width=width, ising_runs=runs)
         pt.warmup(5)
         pt.sample(12)
-        out.append(pt.read_ising() + tuple(pt.get_spins(4, r) for r in range(4)))
+        out.append(pt.read_ising() + tuple(pt.get_spins(g, r) for g in (0, 2, 6) for r in range(4)))
     for other in out[1:]:
         for x, y in zip(out[0], other):
             assert np.array_equal(x, y)
