@@ -1,12 +1,12 @@
 """Throughput of the 3D Ising parallel-tempering path on one MI355X.
 
-Workload (the reference's config-2 PT run at full size): 3D Ising, L in
+Workload (the reference's 3D Ising parallel-tempering run at full size): L in
 {8,16,32,64,128}, 32 replicas per L on T = linspace(4.40, 4.62, 32), int8
 spins from a random +-1 start, Philox per (seed, replica, sweep, site),
 exchange every 10 sweeps, measure every 5.  One step = one PT sweep of every
 lattice (76,693,504 spin updates), measurement and exchange included.
 
-Prints one JSON line (the driver's contract) with a `roofline` object for the
+Prints one JSON line (metric, value, configuration) with a `roofline` object for the
 dominant kernel (ising_half_kernel) and a `cpu_baseline` from the oracle's
 restatement of the reference's random-site Metropolis (metropolis.rs:8-27)
 timed on a bounded sample of the same workload.
@@ -106,14 +106,14 @@ def copy_calibration(nbytes):
 WORKLOADS = {
     # name: (model, sizes, temps, bytes/update, dtype, options, description)
     "ising": ("ising", SIZES, (4.40, 4.62), 3, "u8", dict(exchange_every=10, measure_every=5),
-              "3D Ising PT (config 2): L=8,16,32,64,128 x 32 replicas, T=linspace(4.40,4.62,32), exchange every 10, "
+              "3D Ising PT: L=8,16,32,64,128 x 32 replicas, T=linspace(4.40,4.62,32), exchange every 10, "
               "measure every 5, int8 spins, Philox4x32-10"),
     "ising_msc": ("ising_msc", SIZES, (4.40, 4.62), 0.375, "u32 (1 bit/spin)", dict(exchange_every=10, measure_every=5),
-                  "3D Ising PT (config 2), multispin coded (32 replicas per 32-bit word), exchange every 10, "
+                  "3D Ising PT L=8..128 x 32 replicas, multispin coded (32 replicas per 32-bit word), exchange every 10, "
                   "measure every 5"),
     "heisenberg": ("heisenberg", [16, 32, 64], (1.40, 1.50), 36, "f32",
                    dict(exchange_every=10, measure_every=5, n_overrelax=2, delta=1.0, renorm_every=100),
-                   "3D Heisenberg PT (config 3): L=16,32,64 x 32 replicas, T=linspace(1.40,1.50,32), cap Metropolis "
+                   "3D Heisenberg PT: L=16,32,64 x 32 replicas, T=linspace(1.40,1.50,32), cap Metropolis "
                    "+ 2 overrelaxation passes per sweep, FP64 sums"),
     "xy": ("xy", [16, 32, 64], (2.15, 2.25), 24, "f32",
            dict(exchange_every=10, measure_every=5, n_overrelax=2, delta=1.5, renorm_every=100),
@@ -171,7 +171,7 @@ def run_batched(args, ising_hip):
         "vs_baseline": None,
         "dtype": "u8 + f32",
         "data": "synthetic",
-        "config": {"workload": "batched PT (config 4): Ising int8 T=4.40-4.62 + XY T=2.15-2.25 + Heisenberg "
+        "config": {"workload": "batched PT: Ising int8 T=4.40-4.62 + XY T=2.15-2.25 + Heisenberg "
                                "T=1.40-1.50, L=8,12,16,24,32,48,64,96,128 x 32 replicas each, exchange every 10, "
                                "measure every 5, continuous models 1 Metropolis + 2 overrelaxation passes",
                    "sizes": BATCH_SIZES, "replicas": R, "models": [m for m, _, _, _ in BATCH_MODELS],
@@ -197,7 +197,7 @@ def temperature_series(tidx, x):
 
 
 def run_learned(args, ising_hip):
-    """Learned proposal against plain Metropolis + overrelaxation (config 5):
+    """Learned proposal against plain Metropolis + overrelaxation:
     XY, L = 32, 64 x 32 replicas, T = linspace(2.18, 2.22).  Both chains sample
     the same distribution; the figure of merit is independent samples per
     second, sweeps/s / (2 tau_int(E)) summed over (size, temperature), with
@@ -284,7 +284,7 @@ def run_learned(args, ising_hip):
         "vs_baseline": None,
         "dtype": "f32 (MLP bf16 weights, f32 accumulate)",
         "data": "synthetic",
-        "config": {"workload": "learned proposal vs plain (config 5): XY L=32,64 x 32 replicas, T=linspace(2.18,2.22,32), "
+        "config": {"workload": "learned proposal vs plain: XY L=32,64 x 32 replicas, T=linspace(2.18,2.22,32), "
                                "plain = 1 Metropolis + 2 overrelaxation passes per sweep, learned = the learned "
                                "proposal's pass + learned_overrelaxation_passes, measure every sweep, exchange every 10",
                    "hidden": args.hidden, "sites": sites, "plain": out["plain"], "learned": out["learned"],
@@ -340,7 +340,7 @@ def xy_cluster_leg(pt, sizes, Rr, temps, steps, ising_hip):
 
 def run_cluster(args, ising_hip):
     """Cluster updates (Swendsen-Wang, Wolff) against checkerboard Metropolis
-    near T_c, 3D Ising.  (a) Throughput: Swendsen-Wang updates of the config-2
+    near T_c, 3D Ising.  (a) Throughput: Swendsen-Wang updates of the L=8..128 x 32
     ensemble (L = 8..128 x 32 replicas, T = linspace(4.40, 4.62)), in sites
     per ns.  (b) Efficiency at T_c = 4.5115, L = 32 and 64 x 32 independent
     replicas: independent samples/s of |M| = steps/s / (2 tau_int(|M|)) summed
@@ -401,7 +401,7 @@ def run_cluster(args, ising_hip):
         "ms_per_step": round(1e3 / eff["L64"]["sw"]["steps_per_s"], 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u8", "data": "synthetic",
         "config": {"workload": "cluster updates vs checkerboard Metropolis near T_c: 3D Ising, 32 independent "
-                               "replicas at T=4.5115, L=32 and 64; throughput on the config-2 ensemble",
+                               "replicas at T=4.5115, L=32 and 64; throughput on the L=8..128 x 32 Ising ensemble",
                    "throughput": thr,
                    "efficiency": eff},
     }
@@ -459,7 +459,7 @@ def main():
     ap.add_argument("--fused", type=int, default=0, choices=[0, 8, 16, 32],
                     help="int8 Ising: one launch per sweep, blocks streaming this many z planes (0: one launch per colour)")
     ap.add_argument("--cluster-part", default="all", choices=["all", "throughput"],
-                    help="cluster workload: 'throughput' times only the Swendsen-Wang updates of the config-2 ensemble")
+                    help="cluster workload: 'throughput' times only the Swendsen-Wang updates of the L=8..128 x 32 ensemble")
     ap.add_argument("--layout", default="auto", metavar="W,R,S",
                     help="int8 thread-step layout (run width, runs per step, steps per tile); "
                          "'auto' times all layouts during warm-up and keeps the fastest")
@@ -470,7 +470,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
+        print("bench.py runs on exactly one GPU: the library has no multi-GPU path (an ensemble would be "
+              "sharded by replica across GPUs, with the exchange energies gathered every exchange round)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -336,7 +336,7 @@ int ising_pt_sweeps(const ising_pt_t* pt, int64_t* out);
 int ising_pt_device_bytes(const ising_pt_t* pt, int64_t* out);
 
 /* ------------------------------------------------------------------------
- * Batch: several ensembles (e.g. Ising, XY and Heisenberg of config 4)
+ * Batch: several ensembles (e.g. Ising, XY and Heisenberg of one batched run)
  * advanced together.  One captured HIP graph per `graph_sweeps` sweeps holds
  * a branch per member on the member's own stream, so one graph launch runs
  * every model concurrently.  Members keep their own records, permutations

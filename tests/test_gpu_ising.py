@@ -1,6 +1,6 @@
 """GPU parity of the Ising path against the oracle (bit-exact), through the C ABI.
 
-Small sizes are replayed sweep for sweep by the oracle; the full config-2 size
+Small sizes are replayed sweep for sweep by the oracle; the full size (L = 8 .. 128 x 32 replicas)
 (L=128 x 32 replicas) is checked through size-independent properties: fused
 records equal a standalone reduction of the downloaded lattice, spins stay
 +-1, the temperature permutation stays a bijection.
@@ -168,7 +168,7 @@ def test_autotune_keeps_the_chain():
 
 
 def test_full_size_config_properties():
-    """L=128 x 32 replicas (config 2's largest size): fused records vs standalone reduction."""
+    """L=128 x 32 replicas (the largest size of the benchmark ensemble): fused records vs standalone reduction."""
     R, L = 32, 128
     temps = o.linspace(4.40, 4.62, R)
     pt = ising_hip.PTEnsemble("ising", [L], R, temps, 42, exchange_every=2, measure_every=1)
