@@ -383,6 +383,18 @@ int ising_wham(int32_t K, const int64_t* n_k, const double* betas, const double*
  * report independent samples per second.  *window receives W. */
 int ising_tau_int(const double* x, int64_t n, double c, double* tau, int64_t* window);
 
+/* Crossing temperature of two Binder-cumulant curves U1(T1), U2(T2)
+ * (analysis/scripts/analyze_gpu_fss.py:256-277, find_binder_crossings): both
+ * curves are interpolated by not-a-knot cubic splines (the interpolant of
+ * scipy's interp1d(kind="cubic")) on 2000 equally spaced temperatures of the
+ * common range [max(min T1, min T2), min(max T1, max T2)]; the first sign
+ * change of U1 - U2 between grid points i, i + 1 gives the linear root
+ * T_i - d_i (T_{i+1} - T_i) / (d_{i+1} - d_i).  Temperatures may come in any
+ * order (they are sorted); each curve needs >= 4 distinct temperatures.
+ * *found = 0 (and *tc untouched) when the curves do not cross. */
+int ising_binder_crossing(const double* t1, const double* u1, int32_t n1, const double* t2, const double* u2,
+                          int32_t n2, double* tc, int32_t* found);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2316,6 +2316,107 @@ int ising_tau_int(const double* x, int64_t n, double c, double* tau, int64_t* wi
     return ISING_OK;
 }
 
+// Not-a-knot cubic spline through (x[i], y[i]), x strictly increasing, n >= 4:
+// second derivatives m[] from the n x n system (interior continuity of the
+// first derivative, third-derivative continuity at x[1] and x[n-2]), solved by
+// Gaussian elimination with partial pivoting (n is a temperature count).
+static bool spline_not_a_knot(const std::vector<double>& x, const std::vector<double>& y, std::vector<double>* m) {
+    const int n = (int)x.size();
+    std::vector<double> A((size_t)n * n, 0.0), b(n, 0.0);
+    auto a = [&](int r, int c) -> double& { return A[(size_t)r * n + c]; };
+    const double h0 = x[1] - x[0], h1 = x[2] - x[1];
+    a(0, 0) = h1;
+    a(0, 1) = -(h0 + h1);
+    a(0, 2) = h0;
+    for (int i = 1; i < n - 1; ++i) {
+        const double hl = x[i] - x[i - 1], hr = x[i + 1] - x[i];
+        a(i, i - 1) = hl;
+        a(i, i) = 2.0 * (hl + hr);
+        a(i, i + 1) = hr;
+        b[i] = 6.0 * ((y[i + 1] - y[i]) / hr - (y[i] - y[i - 1]) / hl);
+    }
+    const double ha = x[n - 2] - x[n - 3], hb = x[n - 1] - x[n - 2];
+    a(n - 1, n - 3) = hb;
+    a(n - 1, n - 2) = -(ha + hb);
+    a(n - 1, n - 1) = ha;
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(a(r, c)) > std::fabs(a(piv, c))) piv = r;
+        if (!(std::fabs(a(piv, c)) > 0.0)) return false;
+        if (piv != c) {
+            for (int k = 0; k < n; ++k) std::swap(a(c, k), a(piv, k));
+            std::swap(b[c], b[piv]);
+        }
+        for (int r = c + 1; r < n; ++r) {
+            const double f = a(r, c) / a(c, c);
+            if (f == 0.0) continue;
+            for (int k = c; k < n; ++k) a(r, k) -= f * a(c, k);
+            b[r] -= f * b[c];
+        }
+    }
+    m->assign(n, 0.0);
+    for (int r = n - 1; r >= 0; --r) {
+        double v = b[r];
+        for (int k = r + 1; k < n; ++k) v -= a(r, k) * (*m)[k];
+        (*m)[r] = v / a(r, r);
+    }
+    return true;
+}
+
+static double spline_eval(const std::vector<double>& x, const std::vector<double>& y, const std::vector<double>& m,
+                          double t) {
+    const int n = (int)x.size();
+    int i = (int)(std::upper_bound(x.begin(), x.end(), t) - x.begin()) - 1;
+    i = std::max(0, std::min(n - 2, i));
+    const double h = x[i + 1] - x[i], A = x[i + 1] - t, B = t - x[i];
+    return m[i] * A * A * A / (6.0 * h) + m[i + 1] * B * B * B / (6.0 * h) + (y[i] / h - m[i] * h / 6.0) * A +
+           (y[i + 1] / h - m[i + 1] * h / 6.0) * B;
+}
+
+static bool sorted_curve(const double* t, const double* u, int32_t n, std::vector<double>* x, std::vector<double>* y) {
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::sort(idx.begin(), idx.end(), [&](int p, int q) { return t[p] < t[q]; });
+    x->resize(n);
+    y->resize(n);
+    for (int i = 0; i < n; ++i) {
+        (*x)[i] = t[idx[i]];
+        (*y)[i] = u[idx[i]];
+        if (!std::isfinite((*x)[i]) || !std::isfinite((*y)[i])) return false;
+        if (i > 0 && !((*x)[i] > (*x)[i - 1])) return false;
+    }
+    return true;
+}
+
+int ising_binder_crossing(const double* t1, const double* u1, int32_t n1, const double* t2, const double* u2,
+                          int32_t n2, double* tc, int32_t* found) {
+    if (!t1 || !u1 || !t2 || !u2 || !tc || !found) return fail(ISING_E_ARG, "null argument");
+    if (n1 < 4 || n2 < 4) return fail(ISING_E_ARG, "each curve needs at least 4 temperatures");
+    std::vector<double> x1, y1, x2, y2, m1, m2;
+    if (!sorted_curve(t1, u1, n1, &x1, &y1) || !sorted_curve(t2, u2, n2, &x2, &y2))
+        return fail(ISING_E_ARG, "temperatures must be finite and distinct, cumulants finite");
+    if (!spline_not_a_knot(x1, y1, &m1) || !spline_not_a_knot(x2, y2, &m2))
+        return fail(ISING_E_ARG, "singular spline system");
+    *found = 0;
+    const double lo = std::max(x1.front(), x2.front()), hi = std::min(x1.back(), x2.back());
+    if (!(hi > lo)) return ISING_OK;
+    constexpr int kGrid = 2000;
+    double tp = lo, dp = spline_eval(x1, y1, m1, lo) - spline_eval(x2, y2, m2, lo);
+    for (int j = 1; j < kGrid; ++j) {
+        const double t = lo + (hi - lo) * (double)j / (double)(kGrid - 1);
+        const double d = spline_eval(x1, y1, m1, t) - spline_eval(x2, y2, m2, t);
+        if (dp * d < 0.0) {
+            *tc = tp - dp * (t - tp) / (d - dp);
+            *found = 1;
+            return ISING_OK;
+        }
+        tp = t;
+        dp = d;
+    }
+    return ISING_OK;
+}
+
 int ising_wham(int32_t K, const int64_t* n_k, const double* betas, const double* energies, int32_t n_obs,
                const double* obs, int32_t max_iter, double tol, double* f_out, int32_t* iters, int32_t n_targets,
                const double* beta_targets, double* out) {

@@ -139,6 +139,7 @@ _SIGS = {
     "ising_replica_exchange": (C.c_int, [_vp, _vp, C.c_int32, _vp, _vp, _vp, C.c_int32, C.POINTER(C.c_int32)]),
     "ising_jackknife": (C.c_int, [_vp, _vp, C.c_int64, C.c_double, C.c_double, C.c_int32, _vp]),
     "ising_tau_int": (C.c_int, [_vp, C.c_int64, C.c_double, _f64p, _i64p]),
+    "ising_binder_crossing": (C.c_int, [_vp, _vp, C.c_int32, _vp, _vp, C.c_int32, _f64p, C.POINTER(C.c_int32)]),
     "ising_wham": (C.c_int, [C.c_int32, _vp, _vp, _vp, C.c_int32, _vp, C.c_int32, C.c_double, _vp,
                              C.POINTER(C.c_int32), C.c_int32, _vp, _vp]),
 }
@@ -676,6 +677,32 @@ def tau_int(x, c=6.0):
     t, w = C.c_double(), C.c_int64()
     _check(lib.ising_tau_int(_ptr(a), len(a), c, C.byref(t), C.byref(w)))
     return t.value, w.value
+
+
+def binder_crossing(t1, u1, t2, u2):
+    """Crossing temperature of two Binder-cumulant curves (the rule of
+    analyze_gpu_fss.py:256-277: not-a-knot cubic splines on a 2000-point common
+    grid, first sign change, linear root); None when they do not cross."""
+    a1, b1 = np.ascontiguousarray(t1, np.float64), np.ascontiguousarray(u1, np.float64)
+    a2, b2 = np.ascontiguousarray(t2, np.float64), np.ascontiguousarray(u2, np.float64)
+    if a1.shape != b1.shape or a2.shape != b2.shape:
+        raise ValueError("each curve needs as many cumulants as temperatures")
+    tc, found = C.c_double(), C.c_int32()
+    _check(lib.ising_binder_crossing(_ptr(a1), _ptr(b1), len(a1), _ptr(a2), _ptr(b2), len(a2), C.byref(tc),
+                                     C.byref(found)))
+    return tc.value if found.value else None
+
+
+def binder_crossings(curves):
+    """find_binder_crossings (analyze_gpu_fss.py:256-277): curves maps L to
+    (temperatures, U4); returns [(L1, L2, T_c)] for adjacent sizes that cross."""
+    sizes = sorted(curves)
+    out = []
+    for n1, n2 in zip(sizes, sizes[1:]):
+        tc = binder_crossing(curves[n1][0], curves[n1][1], curves[n2][0], curves[n2][1])
+        if tc is not None:
+            out.append((n1, n2, tc))
+    return out
 
 
 def wham(energy_lists, betas, observables=(), beta_targets=(), max_iter=100, tol=1e-8):

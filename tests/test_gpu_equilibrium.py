@@ -122,6 +122,8 @@ def test_binder_crossing_reproduces_3d_ising_tc():
     tc = reference_crossing(np.array(temps), U[16], U[32])
     assert tc is not None
     assert abs(tc - 4.5115) < 0.02, tc
+    # the library's crossing finder gives the same temperature
+    assert ising_hip.binder_crossing(temps, U[16], temps, U[32]) == pytest.approx(tc, abs=1e-10)
 
 
 def cont_pt_series(model, L, R, temps, seed, warmup, samples, **kw):

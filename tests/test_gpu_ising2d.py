@@ -66,3 +66,4 @@ def test_2d_binder_crossing_at_onsager_point():
         U[L] = np.array([binder(m[t == k])[0] for k in range(R)])
     tc = reference_crossing(np.array(temps), U[16], U[32])
     assert tc is not None and abs(tc - T_ONSAGER) < 0.01, (tc, T_ONSAGER)
+    assert ising_hip.binder_crossings({16: (temps, U[16]), 32: (temps, U[32])})[0][2] == pytest.approx(tc, abs=1e-10)
