@@ -679,6 +679,82 @@ def tau_int(x, c=6.0):
     return t.value, w.value
 
 
+def series_by_temperature(tidx, bond, mag, L, J=1.0, dims=3):
+    """Per-temperature series of one lattice size from ensemble records
+    (read_ising()[k][:, g, :] for size g): [total energies], [|m| per spin],
+    the inputs of the reference's reweighting (analyze_gpu_fss.py:60-84:
+    E = -J * bond, |M| / L^dims), indexed by temperature slot."""
+    t = np.asarray(tidx).reshape(-1)
+    e = -J * np.asarray(bond, np.float64).reshape(-1)
+    m = np.abs(np.asarray(mag, np.float64).reshape(-1)) / float(L) ** dims
+    k_max = int(t.max()) + 1 if t.size else 0
+    return [e[t == k] for k in range(k_max)], [m[t == k] for k in range(k_max)]
+
+
+def _closest_run_errors(e_sim, m_sim, delta_beta, beta_t, V, n_blocks):
+    # single-histogram jackknife from one run (analyze_gpu_fss.py:118-141, 212-235)
+    bs = len(e_sim) // n_blocks
+    jk = np.zeros((3, n_blocks))
+    for b in range(n_blocks):
+        idx = np.concatenate([np.arange(0, b * bs), np.arange((b + 1) * bs, n_blocks * bs)])
+        lw = -delta_beta * e_sim[idx]
+        lw -= lw.max()
+        w = np.exp(lw)
+        w /= w.sum()
+        mj = m_sim[idx]
+        jm, jm2, jm4 = w @ mj, w @ mj ** 2, w @ mj ** 4
+        jk[:, b] = (beta_t * V * (jm2 - jm ** 2), jm, 1.0 - jm4 / (3.0 * jm2 ** 2) if jm2 > 1e-15 else 0.0)
+    return jk
+
+
+def reweight_fine(energy_lists, mag_lists, betas, temps_fine, n, method="wham", n_blocks=20, max_iter=100,
+                  tol=1e-8):
+    """Observables on a fine temperature grid (analyze_gpu_fss.py:86-253):
+    method "wham" (wham_reweight_fine: all runs, the WHAM solve on the GPU) or
+    "single" (reweight_single_histogram: the run closest in temperature).
+    energy_lists[k] / mag_lists[k]: total energies and |m| per spin of run k at
+    betas[k]; n: linear size (V = n^3).  Returns a dict of arrays T, M, M2,
+    M4, chi_conn, chi_err, Cv, U, E (per spin), M_err, U_err; the errors are the
+    reference's single-histogram jackknife over n_blocks blocks of the closest
+    run."""
+    V = float(n) ** 3
+    betas = np.asarray(betas, np.float64)
+    T = np.asarray(temps_fine, np.float64)
+    e_l = [np.asarray(e, np.float64) for e in energy_lists]
+    m_l = [np.asarray(m, np.float64) for m in mag_lists]
+    sim_T = 1.0 / betas
+    if method == "wham":
+        obs = [m_l, [m ** 2 for m in m_l], [m ** 4 for m in m_l], e_l, [e ** 2 for e in e_l]]
+        _, _, means = wham(e_l, betas, obs, 1.0 / T, max_iter, tol)
+        M, M2, M4, E, E2 = (means[:, j] for j in range(5))
+    elif method == "single":
+        M, M2, M4, E, E2 = (np.zeros(T.size) for _ in range(5))
+        for i, t in enumerate(T):
+            k = int(np.argmin(np.abs(sim_T - t)))
+            lw = -(1.0 / t - betas[k]) * e_l[k]
+            lw -= lw.max()
+            w = np.exp(lw)
+            w /= w.sum()
+            M[i], M2[i], M4[i] = w @ m_l[k], w @ m_l[k] ** 2, w @ m_l[k] ** 4
+            E[i], E2[i] = w @ e_l[k], w @ e_l[k] ** 2
+    else:
+        raise ValueError("method must be 'wham' or 'single'")
+    bt = 1.0 / T
+    chi = bt * V * (M2 - M ** 2)
+    cv = bt ** 2 * (E2 - E ** 2) / V
+    U = np.where(M2 > 1e-15, 1.0 - M4 / (3.0 * np.where(M2 > 1e-15, M2, 1.0) ** 2), 0.0)
+    chi_err, M_err, U_err = (np.zeros(T.size) for _ in range(3))
+    f = (n_blocks - 1.0) / n_blocks
+    for i, t in enumerate(T):
+        k = int(np.argmin(np.abs(sim_T - t)))
+        jk = _closest_run_errors(e_l[k], m_l[k], 1.0 / t - betas[k], 1.0 / t, V, n_blocks)
+        chi_err[i] = np.sqrt(f * np.sum((jk[0] - chi[i]) ** 2))
+        M_err[i] = np.sqrt(f * np.sum((jk[1] - M[i]) ** 2))
+        U_err[i] = np.sqrt(f * np.sum((jk[2] - U[i]) ** 2))
+    return {"T": T, "M": M, "M2": M2, "M4": M4, "chi_conn": chi, "chi_err": chi_err, "Cv": cv, "U": U,
+            "E": E / V, "M_err": M_err, "U_err": U_err}
+
+
 def binder_crossing(t1, u1, t2, u2):
     """Crossing temperature of two Binder-cumulant curves (the rule of
     analyze_gpu_fss.py:256-277: not-a-knot cubic splines on a 2000-point common
