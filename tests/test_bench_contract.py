@@ -1,0 +1,38 @@
+"""bench.py's command-line contract that holds without a GPU: more than one
+GPU is refused with a message, and the copy calibration and CPU baseline
+helpers report what the profiles and the JSON line use."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+
+def test_more_than_one_gpu_exits_with_a_message():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "exactly one GPU" in r.stderr
+    assert r.stdout == ""
+
+
+def test_cpu_baseline_reports_one_core_and_all_cores():
+    import bench
+    b = bench.cpu_baseline(0.2)
+    assert b["cores"] == 1 and b["kind"] == "port" and b["unit"] == "updates/ns" and b["value"] > 0
+    assert 1 <= b["all_cores"]["cores"] <= 16 and b["all_cores"]["value"] > 0
+    assert "metropolis.rs:8-27" in b["sample"]
+
+
+def test_traffic_file_matches_the_profile_summary():
+    import json
+    import bench
+    t = bench.load_traffic()
+    d = json.load(open(os.path.join(ROOT, "profiles", "ising_half_kernel_pt_L8-128_R32_traffic.json")))
+    assert t == d["hbm_bytes_per_launch"]
+    scaled = d["fetch_kib"] * 1024 * d["fetch_scale"] + d["write_kib"] * 1024 * d["write_scale"]
+    assert abs(scaled - t) <= 1.0
+    assert 0.99 < t / d["algorithmic_bytes_per_launch"] < 1.05
