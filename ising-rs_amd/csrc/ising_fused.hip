@@ -424,7 +424,8 @@ __global__ __launch_bounds__(kFT) void ising_fused_kernel(IsingArgs a) {
         __syncthreads();
         if ((int)threadIdx.x < nlanes) {
             const int gr = G.rep0 + rg * P + (int)threadIdx.x;
-            int64_t* acc = acc_of(a, t);
+            // accumulator copy by block, as in the two-launch kernel (copies are summed where read)
+            int64_t* acc = acc_of(a, t) + 2 * (int64_t)(b & (a.acc_copies - 1)) * a.total_reps;
             atomicAdd((unsigned long long*)&acc[2 * gr], (unsigned long long)(int64_t)sh.acc[threadIdx.x][0]);
             atomicAdd((unsigned long long*)&acc[2 * gr + 1], (unsigned long long)(int64_t)sh.acc[threadIdx.x][1]);
         }
