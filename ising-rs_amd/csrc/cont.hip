@@ -24,7 +24,7 @@
 namespace ising {
 
 constexpr int kCT = 256;        // threads per block, one site per thread
-constexpr int kCloseT = 1024;  // threads of the one-block closing kernel (one wave per replica at a time)
+constexpr int kCloseT = 256;  // threads of the one-block closing kernel (one replica or pair each at a time)
 
 __device__ __forceinline__ int find_group_c(const DevGroup* groups, int ngroups, int b) {
     int g = 0;
@@ -76,54 +76,61 @@ __device__ __forceinline__ void block_sum4(double v[4], double (*s_red)[kCT / 64
         }
 }
 
-// Closing step of a continuous sweep (one-block kernel after the colour-1 Metropolis pass):
-// ordered FP64 sums of the block partials, records, exchange, delta adaptation
-// during warm-up, sweep counter.
+// Per-replica FP64 sums (E, Mx, My, Mz) of the block partials of a measured
+// or exchange sweep, one wave per replica in a fixed order (lane l takes
+// tiles l, l + 64, ...; then a fixed shuffle tree), into dacc; launched just
+// before the one-block closing kernel, so the sums of many replicas proceed
+// on many CUs.  Other sweeps exit at once.
+__global__ __launch_bounds__(kCT) void cont_reduce_kernel(ContArgs a) {
+    const int32_t fl = a.ctl->flags;
+    if (!(fl & (F_MEAS | F_EXCH))) return;
+    const int lane = (int)threadIdx.x & 63;
+    const int gr = (int)blockIdx.x * (kCT / 64) + ((int)threadIdx.x >> 6);
+    if (gr >= a.total_reps) return;  // whole waves
+    int g = 0;
+    while (g + 1 < a.ngroups && gr >= a.groups[g + 1].rep0) ++g;
+    const int64_t tiles = a.groups[g].tiles;
+    const int64_t b0 = a.groups[g].blk0 + (int64_t)(gr - a.groups[g].rep0) * tiles;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    // unrolled so that a lane's loads are in flight together (the sums keep their order)
+#pragma unroll 8
+    for (int64_t b = lane; b < tiles; b += 64) {
+        const double* p0 = a.part0 + 4 * (b0 + b);
+        const double* p1 = a.part1 + 4 * (b0 + b);
+        v[0] += p1[0];
+        v[1] += p0[1] + p1[1];
+        v[2] += p0[2] + p1[2];
+        v[3] += p0[3] + p1[3];
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < 4; ++q) v[q] += __shfl_down(v[q], off);
+    if (lane == 0)
+        for (int q = 0; q < 4; ++q) a.dacc[4 * gr + q] = v[q];
+}
+
+// Closing step of a continuous sweep (one-block kernel after cont_reduce_kernel):
+// records from the per-replica sums, exchange, delta adaptation during
+// warm-up, sweep counter.
 __device__ void close_cont(const ContArgs& a, int64_t t, int32_t fl) {
     const bool meas = (fl & F_MEAS) != 0, exch = (fl & F_EXCH) != 0;
     DevCtl* ctl = a.ctl;
     const int tid = threadIdx.x;
-    const bool need = meas || exch;
     const int nrec = ctl->n_rec;
     const bool room = nrec < ctl->rec_cap;
-    if (need) {
-        // one wave per replica: lanes sum the block partials in a fixed
-        // assignment, then a fixed shuffle tree (deterministic FP64 order)
-        const int lane = tid & 63, wave = tid >> 6;
-        for (int gr = wave; gr < a.total_reps; gr += kCloseT / 64) {
-            int g = 0;
-            while (g + 1 < a.ngroups && gr >= a.groups[g + 1].rep0) ++g;
-            const int64_t tiles = a.groups[g].tiles;
-            const int64_t b0 = a.groups[g].blk0 + (int64_t)(gr - a.groups[g].rep0) * tiles;
-            double v[4] = {0.0, 0.0, 0.0, 0.0};
-            // unrolled so that a lane's loads are in flight together (the sums keep their order)
-#pragma unroll 8
-            for (int64_t b = lane; b < tiles; b += 64) {
-                const double* p0 = a.part0 + 4 * (b0 + b);
-                const double* p1 = a.part1 + 4 * (b0 + b);
-                v[0] += p1[0];
-                v[1] += p0[1] + p1[1];
-                v[2] += p0[2] + p1[2];
-                v[3] += p0[3] + p1[3];
-            }
-            for (int off = 32; off > 0; off >>= 1)
-                for (int k = 0; k < 4; ++k) v[k] += __shfl_down(v[k], off);
-            if (lane == 0) {
-                for (int k = 0; k < 4; ++k) a.dacc[4 * gr + k] = v[k];
-                if (meas && room) {
-                    ContRec rc;
-                    rc.e = v[0];
-                    rc.mx = v[1];
-                    rc.my = v[2];
-                    rc.mz = v[3];
-                    rc.tidx = a.r2t[gr];
-                    rc.pad = 0;
-                    a.rec[(int64_t)nrec * a.total_reps + gr] = rc;
-                }
-            }
+    if (meas && room) {
+        // the per-replica sums were written by cont_reduce_kernel just before
+        for (int gr = tid; gr < a.total_reps; gr += kCloseT) {
+            ContRec rc;
+            rc.e = a.dacc[4 * gr];
+            rc.mx = a.dacc[4 * gr + 1];
+            rc.my = a.dacc[4 * gr + 2];
+            rc.mz = a.dacc[4 * gr + 3];
+            rc.tidx = a.r2t[gr];
+            rc.pad = 0;
+            a.rec[(int64_t)nrec * a.total_reps + gr] = rc;
         }
-        __syncthreads();  // dacc complete before the exchange reads it
     }
+    __syncthreads();  // records read the permutation before the exchange rewrites it
     if (exch) {
         const int64_t round = ctl->round;
         for (int g = 0; g < a.ngroups; ++g) {
@@ -446,6 +453,10 @@ hipError_t launch_cont_metro(const ContArgs& a, int nblocks, int color, int clos
 }
 
 hipError_t launch_cont_close(const ContArgs& a, hipStream_t st) {
+    const int nb = (a.total_reps + kCT / 64 - 1) / (kCT / 64);
+    hipLaunchKernelGGL(cont_reduce_kernel, dim3(nb), dim3(kCT), 0, st, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(cont_close_kernel, dim3(1), dim3(kCloseT), 0, st, a);
     return hipGetLastError();
 }
