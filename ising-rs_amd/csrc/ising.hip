@@ -281,7 +281,8 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
     __shared__ uint32_t s_thr[kIsingTab];
     __shared__ int s_red[2];  // block sums of the bond sum and magnetisation
     __shared__ int s_waves;    // waves done with the block sums
-    __shared__ uint32_t s_qh[kTieCap], s_qm[kTieCap];
+    __shared__ __align__(8) uint32_t s_qh[kTieCap];
+    __shared__ uint32_t s_qm[kTieCap];
     __shared__ int s_qn;
     const int per = (int)(gridDim.x >> 3);
     const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
@@ -307,8 +308,25 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
     int tidx = 0;
     if (v.exch) {
         const int64_t po = (v.round & 1) * a.total_reps;
-        tidx = exch_tidx(a, G, r, v.round, a.r2t + po, a.t2r + po, acc_of(a, v.t - 1));
-        if (b == G.blk0) exch_write_group(a, G, v.round, acc_of(a, v.t - 1));
+        const int64_t* pacc = acc_of(a, v.t - 1);
+        if (G.R <= kThreads) {
+            // one round of loads (the group's permutation and bond sums, into the
+            // tie queue's LDS, unused until the barrier below) instead of the
+            // dependent chain replica -> temperature -> partner -> bond sums
+            int64_t* s_bond = reinterpret_cast<int64_t*>(s_qh);
+            int32_t* s_t2r = reinterpret_cast<int32_t*>(s_qm);
+            int i = 0;
+            if (threadIdx.x < kIsingTab) i = a.r2t[po + grep];
+            if ((int)threadIdx.x < G.R) {
+                s_bond[threadIdx.x] = acc_total(a, pacc, G.rep0 + (int)threadIdx.x, 0);
+                s_t2r[threadIdx.x] = a.t2r[po + G.rep0 + threadIdx.x];
+            }
+            __syncthreads();
+            if (threadIdx.x < kIsingTab) tidx = exch_tidx_gathered(a, G, i, v.round, s_t2r, s_bond);
+        } else {
+            tidx = exch_tidx(a, G, r, v.round, a.r2t + po, a.t2r + po, pacc);
+        }
+        if (b == G.blk0) exch_write_group(a, G, v.round, pacc);
     } else if (!a.thr_rep) {
         tidx = a.r2t[(int64_t)v.par * a.total_reps + grep];
     }

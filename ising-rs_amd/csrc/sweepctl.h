@@ -40,15 +40,21 @@ __device__ __forceinline__ int64_t acc_total(const IsingArgs& a, const int64_t* 
 // Metropolis decision of exchange round `round` for the temperature pair
 // (i, i + 1) of group G (parallel_tempering.rs:5-36, with Delta on the
 // committed bond sums and one Philox draw per pair and round).
-__device__ __forceinline__ bool exch_pair(const IsingArgs& a, const DevGroup& G, int i, int64_t round,
-                                          const int32_t* t2r_old, const int64_t* acc) {
-    const int ri = t2r_old[G.rep0 + i], rj = t2r_old[G.rep0 + i + 1];
-    const double Ei = -a.J * (double)acc_total(a, acc, G.rep0 + ri, 0);
-    const double Ej = -a.J * (double)acc_total(a, acc, G.rep0 + rj, 0);
+// Takes the bond sums of the replicas at temperatures i and i + 1.
+__device__ __forceinline__ bool exch_accept(const IsingArgs& a, const DevGroup& G, int i, int64_t round, int64_t bi,
+                                            int64_t bj) {
+    const double Ei = -a.J * (double)bi;
+    const double Ej = -a.J * (double)bj;
     const double delta = (a.betas[G.rep0 + i] - a.betas[G.rep0 + i + 1]) * (Ej - Ei);
     const u32x4 w = philox((uint32_t)i, (uint32_t)round, stream_id((uint32_t)G.L, 0u), tag(P_EXCH, MODEL_ISING), a.k0,
                            a.k1);
     return delta <= 0.0 || u01(w.x, w.y) < det_exp(-delta);
+}
+
+__device__ __forceinline__ bool exch_pair(const IsingArgs& a, const DevGroup& G, int i, int64_t round,
+                                          const int32_t* t2r_old, const int64_t* acc) {
+    const int ri = t2r_old[G.rep0 + i], rj = t2r_old[G.rep0 + i + 1];
+    return exch_accept(a, G, i, round, acc_total(a, acc, G.rep0 + ri, 0), acc_total(a, acc, G.rep0 + rj, 0));
 }
 
 // Temperature index of replica r of group G after exchange round `round`.
@@ -60,6 +66,19 @@ __device__ __forceinline__ int exch_tidx(const IsingArgs& a, const DevGroup& G, 
     const int i0 = par + ((i - par) & ~1);
     if (i0 + 1 >= G.R) return i;
     if (!exch_pair(a, G, i0, round, t2r_old, acc)) return i;
+    return i == i0 ? i0 + 1 : i0;
+}
+
+// The same from a block's copies of the group's permutation (temperature ->
+// replica, t2r[k] for k < G.R) and bond sums by replica (bond[r]), gathered
+// in one round of loads; i is the replica's temperature before the round.
+__device__ __forceinline__ int exch_tidx_gathered(const IsingArgs& a, const DevGroup& G, int i, int64_t round,
+                                                  const int32_t* t2r, const int64_t* bond) {
+    const int par = (int)(round & 1);
+    if (i < par) return i;
+    const int i0 = par + ((i - par) & ~1);
+    if (i0 + 1 >= G.R) return i;
+    if (!exch_accept(a, G, i0, round, bond[t2r[i0]], bond[t2r[i0 + 1]])) return i;
     return i == i0 ? i0 + 1 : i0;
 }
 
