@@ -68,7 +68,10 @@ def test_ground_state_preserved_and_measure_observables():
 
 
 @pytest.mark.parametrize("sizes,R,warmup,samples,ee,me", [([4, 8], 4, 10, 60, 10, 5), ([8, 16, 6], 8, 20, 100, 10, 1),
-                                                          ([32], 6, 5, 40, 3, 2), ([24, 40], 3, 4, 12, 3, 2)])
+                                                          ([32], 6, 5, 40, 3, 2), ([24, 40], 3, 4, 12, 3, 2),
+                                                          # more replicas per size than a block has threads:
+                                                          # the exchange closing reads the permutation per block
+                                                          ([4, 6], 260, 2, 12, 1, 2)])
 def test_pt_ensemble_records_are_bit_exact(sizes, R, warmup, samples, ee, me):
     seed = 42
     temps = o.linspace(4.40, 4.62, R)
