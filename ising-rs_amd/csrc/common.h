@@ -192,7 +192,7 @@ struct DevCtl {
     int32_t fl1;           // flags seen by colour-1 launches
     int32_t flp;           // flags of the previous sweep
     int32_t fdone;         // one-launch sweeps: blocks finished in the running launch
-    int32_t fpad;
+    int32_t cdone;         // continuous closing launches: blocks done with their sums
 };
 
 enum : int32_t { F_MEAS = 1, F_EXCH = 2, F_RENORM = 4, F_ADAPT = 8, F_WARM = 16 };

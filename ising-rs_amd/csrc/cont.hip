@@ -78,12 +78,9 @@ __device__ __forceinline__ void block_sum4(double v[4], double (*s_red)[kCT / 64
 
 // Per-replica FP64 sums (E, Mx, My, Mz) of the block partials of a measured
 // or exchange sweep, one wave per replica in a fixed order (lane l takes
-// tiles l, l + 64, ...; then a fixed shuffle tree), into dacc; launched just
-// before the one-block closing kernel, so the sums of many replicas proceed
-// on many CUs.  Other sweeps exit at once.
-__global__ __launch_bounds__(kCT) void cont_reduce_kernel(ContArgs a) {
-    const int32_t fl = a.ctl->flags;
-    if (!(fl & (F_MEAS | F_EXCH))) return;
+// tiles l, l + 64, ...; then a fixed shuffle tree), into dacc; the blocks of
+// the closing launch share the replicas, so the sums proceed on many CUs.
+__device__ __forceinline__ void reduce_replica(const ContArgs& a) {
     const int lane = (int)threadIdx.x & 63;
     const int gr = (int)blockIdx.x * (kCT / 64) + ((int)threadIdx.x >> 6);
     if (gr >= a.total_reps) return;  // whole waves
@@ -108,7 +105,7 @@ __global__ __launch_bounds__(kCT) void cont_reduce_kernel(ContArgs a) {
         for (int q = 0; q < 4; ++q) a.dacc[4 * gr + q] = v[q];
 }
 
-// Closing step of a continuous sweep (one-block kernel after cont_reduce_kernel):
+// Closing step of a continuous sweep (one block of cont_close_kernel, after the sums):
 // records from the per-replica sums, exchange, delta adaptation during
 // warm-up, sweep counter.
 __device__ void close_cont(const ContArgs& a, int64_t t, int32_t fl) {
@@ -118,7 +115,7 @@ __device__ void close_cont(const ContArgs& a, int64_t t, int32_t fl) {
     const int nrec = ctl->n_rec;
     const bool room = nrec < ctl->rec_cap;
     if (meas && room) {
-        // the per-replica sums were written by cont_reduce_kernel just before
+        // the per-replica sums were written by every block of this launch (reduce_replica)
         for (int gr = tid; gr < a.total_reps; gr += kCloseT) {
             ContRec rc;
             rc.e = a.dacc[4 * gr];
@@ -298,9 +295,30 @@ __global__ __launch_bounds__(kCT) void cont_metro_kernel(ContArgs a, int color, 
     }
 }
 
-// Closes a continuous sweep (one block, after the colour-1 Metropolis pass).
-__global__ __launch_bounds__(kCloseT) void cont_close_kernel(ContArgs a) {
-    close_cont(a, a.ctl->t, a.ctl->flags);
+// Closes a continuous sweep (after the colour-1 Metropolis pass).  A sweep
+// that measures or exchanges first sums the block partials per replica over
+// the whole grid; the block that finishes that last (a counter) then closes
+// the sweep.  Other sweeps are closed by block 0 alone.
+__global__ __launch_bounds__(kCT) void cont_close_kernel(ContArgs a) {
+    static_assert(kCloseT == kCT, "the closing step runs on one block of this launch");
+    __shared__ int s_last;
+    const int64_t t = a.ctl->t;
+    const int32_t fl = a.ctl->flags;
+    if (!(fl & (F_MEAS | F_EXCH))) {
+        if (blockIdx.x == 0) close_cont(a, t, fl);
+        return;
+    }
+    reduce_replica(a);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();  // this block's sums are visible before it counts as done
+        s_last = atomicAdd(&a.ctl->cdone, 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();  // and the last block reads every block's sums
+    close_cont(a, t, fl);
+    if (threadIdx.x == 0) a.ctl->cdone = 0;
 }
 
 // Overrelaxation pass over colour `color` (no random numbers, no renormalisation).
@@ -454,10 +472,7 @@ hipError_t launch_cont_metro(const ContArgs& a, int nblocks, int color, int clos
 
 hipError_t launch_cont_close(const ContArgs& a, hipStream_t st) {
     const int nb = (a.total_reps + kCT / 64 - 1) / (kCT / 64);
-    hipLaunchKernelGGL(cont_reduce_kernel, dim3(nb), dim3(kCT), 0, st, a);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(cont_close_kernel, dim3(1), dim3(kCloseT), 0, st, a);
+    hipLaunchKernelGGL(cont_close_kernel, dim3(nb), dim3(kCT), 0, st, a);
     return hipGetLastError();
 }
 
