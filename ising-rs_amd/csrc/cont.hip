@@ -322,32 +322,42 @@ __global__ __launch_bounds__(kCT) void cont_close_kernel(ContArgs a) {
 }
 
 // Overrelaxation pass over colour `color` (no random numbers, no renormalisation).
+// XY: two sites per thread (h and h + kCT / 2 of the block's tile), so each
+// thread has both sites' loads in flight (A/B on one device: 60.1 against
+// 57.8 Metropolis updates/ns; Heisenberg, with 21 loads per site, ran 0.6%
+// slower that way and keeps one site per thread).  The grid is the same tiles.
 template <int N>
-__global__ __launch_bounds__(kCT) void cont_or_kernel(ContArgs a, int color) {
+constexpr int kOrSites = N == 2 ? 2 : 1;
+
+template <int N>
+__global__ __launch_bounds__(kCT / kOrSites<N>) void cont_or_kernel(ContArgs a, int color) {
     const int g = find_group_c(a.groups, a.ngroups, blockIdx.x);
     const DevGroup G = a.groups[g];
     int r, tile;
     split_tiles(G, (int)blockIdx.x - G.blk0, &r, &tile);
     float* C = a.pool + G.off + (int64_t)(2 * r + color) * N * G.stride;
     const float* O = a.pool + G.off + (int64_t)(2 * r + (color ^ 1)) * N * G.stride;
-    const int64_t h = (int64_t)tile * kCT + threadIdx.x;
-    if (h >= G.n_half) return;
-    uint32_t nb[6];
-    neighbours(G, h, color, nb);
-    float hf[3];
-    load_field<N>(O, G.stride, nb, hf);
-    float h2 = 0.f, sh = 0.f, S[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-        hf[k] *= a.Jf;
-        S[k] = C[k * G.stride + h];
-        h2 += hf[k] * hf[k];
-        sh += S[k] * hf[k];
+    for (int j = 0; j < kOrSites<N>; ++j) {
+        const int64_t h = (int64_t)tile * kCT + j * (kCT / kOrSites<N>) + threadIdx.x;
+        if (h >= G.n_half) continue;
+        uint32_t nb[6];
+        neighbours(G, h, color, nb);
+        float hf[3];
+        load_field<N>(O, G.stride, nb, hf);
+        float h2 = 0.f, sh = 0.f, S[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            hf[k] *= a.Jf;
+            S[k] = C[k * G.stride + h];
+            h2 += hf[k] * hf[k];
+            sh += S[k] * hf[k];
+        }
+        if (h2 < 1e-30f) continue;
+        const float sc = 2.0f * sh / h2;
+#pragma unroll
+        for (int k = 0; k < N; ++k) C[k * G.stride + h] = sc * hf[k] - S[k];
     }
-    if (h2 < 1e-30f) return;
-    const float sc = 2.0f * sh / h2;
-#pragma unroll
-    for (int k = 0; k < N; ++k) C[k * G.stride + h] = sc * hf[k] - S[k];
 }
 
 // Random start: Heisenberg z = 2u-1, phi = 2 pi u' (uniform on S^2); XY theta = 2 pi u.
@@ -477,8 +487,8 @@ hipError_t launch_cont_close(const ContArgs& a, hipStream_t st) {
 }
 
 hipError_t launch_cont_or(const ContArgs& a, int nblocks, int color, hipStream_t st) {
-    if (a.ncomp == 2) hipLaunchKernelGGL(cont_or_kernel<2>, dim3(nblocks), dim3(kCT), 0, st, a, color);
-    else hipLaunchKernelGGL(cont_or_kernel<3>, dim3(nblocks), dim3(kCT), 0, st, a, color);
+    if (a.ncomp == 2) hipLaunchKernelGGL(cont_or_kernel<2>, dim3(nblocks), dim3(kCT / kOrSites<2>), 0, st, a, color);
+    else hipLaunchKernelGGL(cont_or_kernel<3>, dim3(nblocks), dim3(kCT / kOrSites<3>), 0, st, a, color);
     return hipGetLastError();
 }
 
