@@ -61,6 +61,7 @@ __device__ __forceinline__ void load_field(const float* O, int64_t stride, const
 }
 
 // Block sum of up to 4 doubles; result valid in thread 0.
+template <int NT = kCT>
 __device__ __forceinline__ void block_sum4(double v[4], double (*s_red)[kCT / 64]) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
@@ -71,7 +72,7 @@ __device__ __forceinline__ void block_sum4(double v[4], double (*s_red)[kCT / 64
     if (threadIdx.x == 0)
         for (int k = 0; k < 4; ++k) {
             double s = 0.0;
-            for (int i = 0; i < kCT / 64; ++i) s += s_red[k][i];
+            for (int i = 0; i < NT / 64; ++i) s += s_red[k][i];
             v[k] = s;
         }
 }
@@ -295,6 +296,126 @@ __global__ __launch_bounds__(kCT) void cont_metro_kernel(ContArgs a, int color, 
     }
 }
 
+// The same pass for XY with two sites per thread (h and h + kCT / 2 of the
+// block's 256-site tile), so each thread has both sites' loads in flight.  A
+// kernel of its own: sharing the loop in the template above changed the
+// Heisenberg instance's code and slowed it.
+template <int N>
+__global__ __launch_bounds__(kCT / 2) void cont_metro_pair_kernel(ContArgs a, int color, int closes) {
+    __shared__ double s_red[4][kCT / 64];
+    const int g = find_group_c(a.groups, a.ngroups, blockIdx.x);
+    const DevGroup G = a.groups[g];
+    int r, tile;
+    split_tiles(G, (int)blockIdx.x - G.blk0, &r, &tile);
+    const int grep = G.rep0 + r;
+    const int ti = G.rep0 + a.r2t[grep];
+    const float beta = a.inline_params ? a.beta_inline : a.betas_f[ti];
+    const float dlt = a.inline_params ? a.delta_inline : a.deltas[ti];
+    const int64_t t = a.ctl->t;
+    const int32_t fl = a.ctl->flags;
+    const bool need = (fl & (F_MEAS | F_EXCH)) != 0 && closes;
+    const bool renorm = (fl & F_RENORM) != 0;
+    const bool count = a.adapt_every > 0 && (fl & F_WARM) != 0;
+    const uint32_t hs = (uint32_t)(2 * t + color);
+    const uint32_t stream = stream_id((uint32_t)G.L, (uint32_t)r);
+
+    float* C = a.pool + G.off + (int64_t)(2 * r + color) * N * G.stride;
+    const float* O = a.pool + G.off + (int64_t)(2 * r + (color ^ 1)) * N * G.stride;
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+    int acc_n = 0;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int64_t h = (int64_t)tile * kCT + j * (kCT / 2) + threadIdx.x;
+        if (h < G.n_half) {
+            uint32_t nb[6];
+            neighbours(G, h, color, nb);
+            float nv[3][6];
+            load_nbrs<N>(O, G.stride, nb, nv);
+            float hf[3];
+            field_f32<N>(nv, hf);
+            float S[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < N; ++k) S[k] = C[k * G.stride + h];
+            const u32x4 w = philox((uint32_t)h, hs, stream, tag(P_CONT, a.model), a.k0, a.k1);
+            float nw[3] = {0.f, 0.f, 0.f};
+            if (N == 2) {
+                const float al = dlt * (2.0f * u24f(w.x) - 1.0f);
+                float sa, ca;
+                sincosf(al, &sa, &ca);
+                nw[0] = S[0] * ca - S[1] * sa;
+                nw[1] = S[1] * ca + S[0] * sa;
+            } else {
+                const float cd = cosf(dlt);
+                const float ct = cd + (1.0f - cd) * u24f(w.x);
+                const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+                float sp, cp;
+                sincosf(6.28318530717958647692f * u24f(w.y), &sp, &cp);
+                float tv0 = 0.f, tv1 = 0.f;
+                if (fabsf(S[0]) < 0.9f) tv0 = 1.f; else tv1 = 1.f;
+                const float ts = tv0 * S[0] + tv1 * S[1];
+                float u0 = tv0 - ts * S[0], u1 = tv1 - ts * S[1], u2 = -ts * S[2];
+                const float ui = 1.0f / sqrtf(u0 * u0 + u1 * u1 + u2 * u2);
+                u0 *= ui; u1 *= ui; u2 *= ui;
+                const float v0 = S[1] * u2 - S[2] * u1, v1 = S[2] * u0 - S[0] * u2, v2 = S[0] * u1 - S[1] * u0;
+                const float a0 = st * cp, a1 = st * sp;
+                nw[0] = a0 * u0 + a1 * v0 + ct * S[0];
+                nw[1] = a0 * u1 + a1 * v1 + ct * S[1];
+                nw[2] = a0 * u2 + a1 * v2 + ct * S[2];
+                const float ni = 1.0f / sqrtf(nw[0] * nw[0] + nw[1] * nw[1] + nw[2] * nw[2]);
+                nw[0] *= ni; nw[1] *= ni; nw[2] *= ni;
+            }
+            float dE = 0.f;
+#pragma unroll
+            for (int k = 0; k < N; ++k) dE += (nw[k] - S[k]) * hf[k];
+            dE *= -a.Jf;
+            const bool accept = dE <= 0.0f || u24f(w.z) < expf(-beta * dE);
+            if (accept) {
+#pragma unroll
+                for (int k = 0; k < N; ++k) S[k] = nw[k];
+                acc_n += 1;
+            }
+            if (renorm) {
+                float q = 0.f;
+#pragma unroll
+                for (int k = 0; k < N; ++k) q += S[k] * S[k];
+                const float qi = 1.0f / sqrtf(q);
+#pragma unroll
+                for (int k = 0; k < N; ++k) S[k] *= qi;
+            }
+            if (accept || renorm) {
+#pragma unroll
+                for (int k = 0; k < N; ++k) C[k * G.stride + h] = S[k];
+            }
+            if (need) {
+                // FP64 from the stored FP32 components (the field is re-summed in FP64)
+                double e = 0.0;
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    part[1 + k] += (double)S[k];
+                    if (color) {
+                        double f = 0.0;
+#pragma unroll
+                        for (int q = 0; q < 6; ++q) f += (double)nv[k][q];
+                        e += (double)S[k] * f;
+                    }
+                }
+                part[0] += color ? -a.J * e : 0.0;
+            }
+        }
+    }
+    if (count) {
+        const int wc = wave_sum(acc_n);
+        if ((threadIdx.x & 63) == 0 && wc) atomicAdd((unsigned long long*)&a.acc_cnt[grep], (unsigned long long)wc);
+    }
+    if (need) {
+        block_sum4<kCT / 2>(part, s_red);
+        if (threadIdx.x == 0) {
+            double* dst = (color ? a.part1 : a.part0) + 4 * (int64_t)blockIdx.x;
+            for (int k = 0; k < 4; ++k) dst[k] = part[k];
+        }
+    }
+}
+
 // Closes a continuous sweep (after the colour-1 Metropolis pass).  A sweep
 // that measures or exchanges first sums the block partials per replica over
 // the whole grid; the block that finishes that last (a counter) then closes
@@ -475,7 +596,7 @@ hipError_t launch_cont_helicity(const DevGroup* groups, int ngroups, const float
 }
 
 hipError_t launch_cont_metro(const ContArgs& a, int nblocks, int color, int closes, hipStream_t st) {
-    if (a.ncomp == 2) hipLaunchKernelGGL(cont_metro_kernel<2>, dim3(nblocks), dim3(kCT), 0, st, a, color, closes);
+    if (a.ncomp == 2) hipLaunchKernelGGL(cont_metro_pair_kernel<2>, dim3(nblocks), dim3(kCT / 2), 0, st, a, color, closes);
     else hipLaunchKernelGGL(cont_metro_kernel<3>, dim3(nblocks), dim3(kCT), 0, st, a, color, closes);
     return hipGetLastError();
 }
