@@ -32,47 +32,133 @@ def linspace(a, b, n):
     return [a + (b - a) * i / (n - 1) for i in range(n)]
 
 
-def cpu_baseline(seconds, model="ising"):
-    """The reference's single-threaded CPU algorithm (oracle restatement) on one
-    core, and as independent chains on every core of this process's CPU share
-    (capped at 16), each for `seconds` of wall time.  The C loops run with the
-    GIL released, one Python thread per chain."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_leg(model, seed, seconds):
+    """One single-threaded chain of the reference's CPU algorithm (oracle
+    restatement) run for about `seconds`: (updates, wall seconds, detail)."""
     import ctypes as C
-    import threading
 
     import oracle_lib as o
 
-    def one(seed, out, k):
-        wall = C.c_double()
-        if model.startswith("ising"):
-            upd = o.lib.o_bench_ref_metropolis(128, 1.0 / 4.5115, seed, seconds, C.byref(wall))
-        else:
-            upd = o.lib.o_bench_ref_heis(64, 1.0 / 1.443, 1.0, 2, seed, seconds, C.byref(wall)) // 3
-        out[k] = (upd, wall.value)
+    if model.startswith("ising"):
+        # config 1 exactly, repeated with seeds seed, seed + 1, ... until `seconds`:
+        # L=16, T=4.5115, hot start, 2000 warm-up + 2000 sample sweeps, E, |M|, M^2, M^4 every sweep
+        L, n = 16, 2000
+        b = np.zeros(n, np.int64)
+        m = np.zeros(n, np.int64)
+        runs, upd, obs = 0, 0, None
+        t0 = time.perf_counter()
+        while True:
+            o.lib.o_ref_ising_chain(0, L, seed + runs, 1.0 / 4.5115, 2000, n, b, m)
+            runs += 1
+            upd += 2 * n * L ** 3
+            if obs is None:
+                e, am = -b / L ** 3, np.abs(m) / L ** 3
+                obs = {"E": round(float(e.mean()), 5), "absM": round(float(am.mean()), 5),
+                       "M2": round(float((am ** 2).mean()), 6), "M4": round(float((am ** 4).mean()), 7)}
+            if time.perf_counter() - t0 >= seconds:
+                break
+        return upd, time.perf_counter() - t0, {"runs": runs, "first_run_means": obs}
+    wall = C.c_double()
+    if model == "xy":
+        steps = C.c_int64()
+        flipped = o.lib.o_bench_ref_xy_wolff(32, 1.0 / 2.2, seed, seconds, C.byref(wall), C.byref(steps))
+        return flipped, wall.value, {"wolff_steps": steps.value}
+    upd = o.lib.o_bench_ref_heis(32, 1.0 / 1.443, 1.0, 2, seed, seconds, C.byref(wall)) // 3
+    return upd, wall.value, {}
 
-    res = [None]
-    one(42, res, 0)
-    upd, wall = res[0]
+
+def cpu_baseline(seconds, model="ising"):
+    """The reference's single-threaded CPU algorithm (oracle restatement) on one
+    core, and as independent chains on every core of this process's CPU share
+    (capped at 16), each for about `seconds` of wall time.  The C loops run with
+    the GIL released, one Python thread per chain."""
+    import threading
+
+    upd, wall, detail = _cpu_leg(model, 42, seconds)
     ncores = max(1, min(16, len(os.sched_getaffinity(0))))
     many = [None] * ncores
-    threads = [threading.Thread(target=one, args=(42 + i, many, i)) for i in range(ncores)]
+
+    def run(k):
+        many[k] = _cpu_leg(model, 1000 * (k + 1), seconds)
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(ncores)]
     for th in threads:
         th.start()
     for th in threads:
         th.join()
-    all_upd = sum(u for u, _ in many)
-    all_wall = max(w for _, w in many)
+    all_upd = sum(u for u, _, _ in many)
+    all_wall = max(w for _, w, _ in many)
     if model.startswith("ising"):
-        what = ("oracle restatement of metropolis.rs:8-27 (random-site Metropolis), one L=128 replica at T=4.5115, "
-                "%d updates in %.1f s on 1 core" % (upd, wall))
+        what = ("config 1 on the reference's CPU path: oracle restatement of metropolis.rs:8-27 (random-site "
+                "Metropolis), L=16, T=4.5115, hot start, 2000 warm-up + 2000 sample sweeps measuring E, |M|, M^2, "
+                "M^4 every sweep (16.4M updates per run), %d runs (seeds 42..%d): %d updates in %.1f s on 1 core"
+                % (detail["runs"], 41 + detail["runs"], upd, wall))
+        unit = "updates/ns"
+    elif model == "xy":
+        what = ("oracle restatement of xy/wolff.rs:16-73 (Wolff, the reference's only XY CPU path), L=32, T=2.2, "
+                "%d steps flipping %d spins in %.1f s on 1 core" % (detail["wolff_steps"], upd, wall))
+        unit = "flipped spins/ns"
     else:
-        # Metropolis updates (each sweep = 1 Metropolis + 2 overrelaxation passes)
         what = ("oracle restatement of heisenberg/metropolis.rs:10-36 + overrelax.rs:15-48 (combined sweep with 2 "
-                "OR passes), one L=64 replica at T=1.443, %d Metropolis updates in %.1f s on 1 core" % (upd, wall))
-    return {"value": upd / (wall * 1e9), "unit": "updates/ns", "cores": 1, "kind": "port", "sample": what,
-            "all_cores": {"value": all_upd / (all_wall * 1e9), "cores": ncores,
-                          "sample": "%d independent chains (seeds 42..%d), one per core, %d updates in %.1f s"
-                                    % (ncores, 41 + ncores, all_upd, all_wall)}}
+                "OR passes), L=32, T=1.443, %d Metropolis updates in %.1f s on 1 core" % (upd, wall))
+        unit = "updates/ns"
+    out = {"value": upd / (wall * 1e9), "unit": unit, "cores": 1, "kind": "port", "sample": what,
+           "cpu_model": cpu_model(),
+           "all_cores": {"value": all_upd / (all_wall * 1e9), "cores": ncores,
+                         "sample": "%d independent chains of the same kind, one per core, %d updates in %.1f s"
+                                   % (ncores, all_upd, all_wall)}}
+    if detail.get("first_run_means"):
+        out["first_run_means"] = detail["first_run_means"]
+    return out
+
+
+def graph_length(steps, cap=64):
+    """Largest divisor of `steps` not above `cap`: the timed sweeps are whole graph replays."""
+    return max(g for g in range(1, min(cap, steps) + 1) if steps % g == 0)
+
+
+def beyond_mall(args, ising_hip, opts, layout):
+    """int8 Ising PT with a working set above the 256 MB Infinity Cache: L=256 x 32
+    replicas (512 MiB of spins), same ladder, cadence and thread-step layout;
+    steps and warm-up as the main run (at least 20 sweeps each)."""
+    L, steps = 256, max(20, args.steps // 10 * 10 if args.steps >= 20 else 20)
+    steps = min(steps, 600)
+    kw = dict(opts)
+    if layout is not None:
+        kw.update(ising_width=layout[0], ising_runs=layout[1], ising_spt=layout[2])
+    pt = ising_hip.PTEnsemble("ising", [L], R, linspace(4.40, 4.62, R), args.seed,
+                              record_capacity=steps // kw["measure_every"] + 8, **kw)
+    pt.warmup(max(20, args.warmup))
+    G = graph_length(steps)
+    pt.prepare_graph(G)
+    pt.sample(0)
+    pt.sync()
+    t0 = time.perf_counter()
+    pt.sample(steps)
+    ev_ms = pt.sync()
+    wall = time.perf_counter() - t0
+    pool = pt.device_bytes()
+    pt.close()
+    sites = L ** 3 * R
+    launch_s = ev_ms * 1e-3 / (2 * steps)
+    achieved = 3 * sites / 2 / launch_s / 1e9
+    return {"workload": "3D Ising PT, L=256 x 32 replicas, T=linspace(4.40,4.62,32), exchange every 10, measure "
+                        "every 5, int8 spins", "pool_bytes": pool, "steps": steps, "graph_sweeps": G,
+            "value": round(sites * steps / (wall * 1e9), 3), "unit": "updates/ns",
+            "device_updates_per_ns": round(sites * steps / (ev_ms * 1e6), 3),
+            "roofline": {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "alg_bytes_per_launch": 3 * sites // 2,
+                         "avg_launch_us": round(launch_s * 1e6, 3)}}
 
 
 def load_traffic():
@@ -143,8 +229,11 @@ def run_batched(args, ising_hip):
         cap = args.steps // opts["measure_every"] + 8
         members.append(ising_hip.PTEnsemble(model, BATCH_SIZES, R, linspace(tlo, thi, R), args.seed,
                                             record_capacity=cap, graph_sweeps=0, **opts))
-    batch = ising_hip.Batch(members, graph_sweeps=16)
-    batch.warmup(args.warmup)
+    G = graph_length(args.steps)
+    batch = ising_hip.Batch(members, graph_sweeps=G)
+    warm = max(args.warmup, G)  # the batch graph is captured by the first call of >= G sweeps
+    batch.warmup(warm)
+    batch.sample(0)
     batch.sync()
     t0 = time.perf_counter()
     batch.sample(args.steps)
@@ -164,7 +253,7 @@ def run_batched(args, ising_hip):
         "unit": "updates/ns",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": warm,
         "ms_per_step": wall * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
@@ -463,6 +552,8 @@ def main():
     ap.add_argument("--layout", default="auto", metavar="W,R,S",
                     help="int8 thread-step layout (run width, runs per step, steps per tile); "
                          "'auto' times all layouts during warm-up and keeps the fastest")
+    ap.add_argument("--no-beyond-mall", action="store_true",
+                    help="ising workload: skip the L=256 x 32 sub-result (working set above the Infinity Cache)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -508,6 +599,11 @@ def main():
         else:
             layout = tuple(int(v) for v in args.layout.split(","))
     pt.warmup(args.warmup)
+    # the timed sweeps replay one captured graph of G sweeps, G dividing --steps
+    # (production runs replay graphs too); captured here, outside the timed region
+    G = graph_length(args.steps)
+    pt.prepare_graph(G)
+    pt.sample(0)  # enters the sampling phase (a control-block round trip), outside the timed region
     pt.sync()
 
     t0 = time.perf_counter()
@@ -517,18 +613,28 @@ def main():
 
     recs = pt.read_ising() if model.startswith("ising") else pt.read_cont()
     assert recs[0].shape[0] == args.steps // me
+    pt.close()
 
     sites = sum(L ** 3 for L in sizes) * R
     passes = 1 + opts.get("n_overrelax", 0) if not model.startswith("ising") else 1
     updates = sites * args.steps                   # Metropolis updates (the reported metric)
     value = updates / (wall * 1e9)
     fused = model == "ising" and args.fused > 0
-    launches = (1 if fused else 2) * passes * args.steps  # colour passes, or whole sweeps when fused
+    cont = not model.startswith("ising")
+    # colour passes (or whole sweeps when fused); a continuous sweep adds one closing launch
+    launches = ((1 if fused else 2) * passes + (1 if cont else 0)) * args.steps
     avg_launch_s = ev_ms * 1e-3 / launches
-    alg_bytes_per_launch = int(bpu * sites / (1 if fused else 2))  # bytes per update x updates per launch
-    achieved = alg_bytes_per_launch / avg_launch_s / 1e9
-    kernel = {"ising": "ising_fused_kernel" if fused else "ising_half_kernel",
-              "ising_msc": "msc_half_kernel"}.get(model, "cont_metro_kernel+cont_or_kernel")
+    if cont:
+        # whole sweep: every pass moves bpu bytes per update; the closing launch moves ~nothing
+        alg_bytes_per_launch = None
+        alg_bytes_per_step = bpu * sites * passes
+        achieved = alg_bytes_per_step / (ev_ms * 1e-3 / args.steps) / 1e9
+    else:
+        alg_bytes_per_launch = int(bpu * sites / (1 if fused else 2))  # bytes per update x updates per launch
+        achieved = alg_bytes_per_launch / avg_launch_s / 1e9
+    kernel = {"ising": "ising_fused_kernel" if fused else "ising_half_kernel", "ising_msc": "msc_half_kernel",
+              "xy": "whole sweep: cont_or_kernel<2> x 4 + cont_metro_pair_kernel<2> x 2 + cont_close_kernel",
+              "heisenberg": "whole sweep: cont_or_kernel<3> x 4 + cont_metro_kernel<3> x 2 + cont_close_kernel"}[model]
     line = {
         "metric": "spin updates/ns",
         "value": round(value, 3),
@@ -548,9 +654,18 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": load_traffic() if args.workload == "ising" and not fused else None,
                      "kernel": kernel, "alg_bytes_per_launch": alg_bytes_per_launch,
-                     "avg_launch_us": round(avg_launch_s * 1e6, 3),
+                     "avg_launch_us": round(avg_launch_s * 1e6, 3), "launches": launches,
                      "device_updates_per_ns": round(updates / (ev_ms * 1e6), 3)},
     }
+    line["config"]["graph_sweeps"] = G
+    if cont:
+        line["roofline"]["alg_bytes_per_update"] = bpu
+        line["roofline"]["avg_step_us"] = round(ev_ms * 1e3 / args.steps, 3)
+        if model == "xy":
+            # stored as two FP32 components (24 B per update); one FP32 angle would be 12 B
+            line["roofline"]["frac_at_12B_angle"] = round(achieved / 2 / HBM_PEAK_GBS, 4)
+    if args.workload == "ising" and not fused and not args.no_beyond_mall:
+        line["config"]["beyond_mall"] = beyond_mall(args, ising_hip, opts, layout)
     if layout is not None:
         line["config"]["layout"] = {"run_width": layout[0], "runs_per_step": layout[1], "steps_per_tile": layout[2]}
     if passes > 1:

@@ -131,6 +131,32 @@ int ising_cont_overrelax_pass(ising_cont_t* lat, int color, float j);
 int ising_cont_sweeps(const ising_cont_t* lat, int64_t* out);
 
 /* ------------------------------------------------------------------------
+ * reduce_gpu::* (src/cuda/reduce_gpu.rs:8-178): totals of a lattice the
+ * caller holds in DEVICE memory, in the reference's layout (z-major; Ising
+ * int8 +-1, continuous spins component-interleaved float).  Each call
+ * allocates its partials, runs on the device's default stream (the caller
+ * orders any work on other streams that writes d_spins) and returns after
+ * copying the totals to the host.  Ising sums are exact (int64); continuous
+ * sums are FP64 in a fixed order (the reference adds FP32 block partials).
+ * Energies count the three forward bonds of each site, without a field.
+ * ------------------------------------------------------------------------ */
+/* load_reduce_kernels (reduce_gpu.rs:8-21): fails with ISING_E_HIP when no device is visible. */
+int ising_load_reduce_kernels(void);
+/* reduce_ising (reduce_gpu.rs:23-66): E total = -j * sum s_i (s_x+1 + s_y+1 + s_z+1), M total.  1 <= n <= 1024. */
+int ising_reduce_ising(const int8_t* d_spins, int32_t n, float j, double* e_total, double* m_total);
+/* measure_ising_gpu (reduce_gpu.rs:70-79): E per spin and |M| per spin. */
+int ising_measure_ising_gpu(const int8_t* d_spins, int32_t n, float j, double* e_per_spin, double* m_abs_per_spin);
+/* reduce_continuous_mag (reduce_gpu.rs:83-135): Mx, My, Mz totals over n_sites
+ * sites (mz = 0 for n_comp = 2).  0 <= n_sites <= 2^30. */
+int ising_reduce_continuous_mag(const float* d_spins, int64_t n_sites, int32_t n_comp, double* mx, double* my,
+                                double* mz);
+/* reduce_continuous_energy (reduce_gpu.rs:138-163): -j * sum S_i . S_fwd over an n^3 lattice. */
+int ising_reduce_continuous_energy(const float* d_spins, int32_t n, int32_t n_comp, float j, double* e_total);
+/* measure_continuous_gpu (reduce_gpu.rs:167-178): (E, Mx, My, Mz) totals. */
+int ising_measure_continuous_gpu(const float* d_spins, int32_t n, int32_t n_comp, float j, double* e, double* mx,
+                                 double* my, double* mz);
+
+/* ------------------------------------------------------------------------
  * Parallel-tempering ensemble: every lattice size and replica of a run in one
  * set of launches; exchange, measurement and the sweep counter on the device.
  * Replaces the host loop of gpu_fss (gpu_fss.rs:186-286 / 441-545).
@@ -331,6 +357,12 @@ int ising_fss_anneal(const int32_t* sizes, int32_t n_sizes, double j, double h, 
  * ordinary warm-up sweeps (9 * (sweeps + 1) of them: each layout's first
  * sweep is not timed).  Before sampling only.  Outputs may be NULL. */
 int ising_pt_autotune(ising_pt_t* pt, int32_t sweeps, int32_t* width, int32_t* runs, int32_t* spt);
+/* Sweeps per captured HIP graph (no counterpart in the reference): sets
+ * graph_sweeps and captures the graph now, without running a sweep, so that
+ * a later warmup/sample call of k sweeps replays it k / graph_sweeps times
+ * (and launches the remaining k % graph_sweeps sweeps directly).  0 turns
+ * graph replay off.  Results do not depend on it. */
+int ising_pt_prepare_graph(ising_pt_t* pt, int32_t graph_sweeps);
 int ising_pt_sweeps(const ising_pt_t* pt, int64_t* out);
 /* Bytes of spin storage on the device. */
 int ising_pt_device_bytes(const ising_pt_t* pt, int64_t* out);

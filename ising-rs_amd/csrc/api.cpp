@@ -32,6 +32,13 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+}  // namespace
+
+// Error reporting for entry points defined in other source files of the library.
+int ising::set_error(int code, const char* msg) { return fail(code, msg); }
+
+namespace {
+
 #define HIP_TRY(expr)                                                                         \
     do {                                                                                      \
         hipError_t e_ = (expr);                                                               \
@@ -131,6 +138,7 @@ struct ising_pt {
     // continuous models
     float* d_fpool = nullptr;
     float* d_deltas = nullptr;
+    float* d_omcd = nullptr;  // 1 - cos(delta) per temperature slot (Heisenberg cap)
     float* d_betas_f = nullptr;
     double* d_part0 = nullptr;
     double* d_part1 = nullptr;
@@ -197,6 +205,7 @@ struct ising_pt {
         (void)hipFree(d_rec);
         (void)hipFree(d_fpool);
         (void)hipFree(d_deltas);
+        (void)hipFree(d_omcd);
         (void)hipFree(d_betas_f);
         (void)hipFree(d_part0);
         (void)hipFree(d_part1);
@@ -264,6 +273,7 @@ struct ising_pt {
         a.model = model == ISING_MODEL_XY ? MODEL_XY : MODEL_HEIS;
         a.pool = d_fpool;
         a.deltas = d_deltas;
+        a.omcd = d_omcd;
         a.betas_f = d_betas_f;
         a.betas = d_betas;
         a.r2t = d_r2t;
@@ -284,6 +294,7 @@ struct ising_pt {
         a.inline_params = inline_cont ? 1 : 0;
         a.beta_inline = beta_inline;
         a.delta_inline = delta_inline;
+        a.omcd_inline = (float)(1.0 - std::cos((double)delta_inline));
         a.net = d_lnet;
         a.mh = d_mh;
         a.learn_H = learn_H;
@@ -295,25 +306,36 @@ struct ising_pt {
     // from the device control block, so one captured block of sweeps replays
     // unchanged.  Single-lattice handles pass per-call parameters by value and
     // always launch directly.
+    bool graph_capable() const { return opts.graph_sweeps > 0 && !inline_thr && !inline_cont; }
+
+    // Captures `opts.graph_sweeps` sweeps into graph_exec (nothing runs).
+    int capture_graph() {
+        const int G = opts.graph_sweeps;
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_sweeps(G);
+        hipGraph_t captured = nullptr;
+        const hipError_t ec = hipStreamEndCapture(st, &captured);
+        t_host -= G;  // capture enqueued nothing
+        if (rc) {
+            if (captured) (void)hipGraphDestroy(captured);
+            return rc;
+        }
+        HIP_TRY(ec);
+        graph = captured;
+        const hipError_t ei = hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_TRY(ei);
+        HIP_TRY(hipGraphUpload(graph_exec, st));  // the first replay then starts without the upload
+        return ISING_OK;
+    }
+
     int run_sweeps(int64_t n) {
         const int G = opts.graph_sweeps;
-        if (G > 0 && !inline_thr && !inline_cont && n >= G) {
+        if (graph_capable() && n >= G) {
             if (!graph_exec) {
-                hipGraph_t graph = nullptr;
-                HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                const int rc = enqueue_sweeps(G);
-                hipGraph_t captured = nullptr;
-                const hipError_t ec = hipStreamEndCapture(st, &captured);
-                t_host -= G;  // capture enqueued nothing
-                if (rc) {
-                    if (captured) (void)hipGraphDestroy(captured);
-                    return rc;
-                }
-                HIP_TRY(ec);
-                graph = captured;
-                const hipError_t ei = hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(graph);
-                HIP_TRY(ei);
+                const int rc = capture_graph();
+                if (rc) return rc;
             }
             const int64_t reps = n / G;
             for (int64_t i = 0; i < reps; ++i) HIP_TRY(hipGraphLaunch(graph_exec, st));
@@ -402,10 +424,18 @@ struct ising_pt {
         return ISING_OK;
     }
 
+    // The device's control block, once every enqueued sweep has run.  Every
+    // host read of device state goes through here, and fails if a sweep the
+    // host enqueued was not closed on the device or a launch left a block
+    // count behind.
     int read_ctl(DevCtl* out) {
         const int rc = sync_all();
         if (rc) return rc;
         ST_COPY(this, out, d_ctl, sizeof(DevCtl), hipMemcpyDeviceToHost);
+        if (out->t != t_host)
+            return fail(ISING_E_STATE, "device sweep counter " + std::to_string(out->t) + " differs from the " +
+                                           std::to_string(t_host) + " sweeps enqueued");
+        if (out->cdone != 0 || out->fdone != 0) return fail(ISING_E_STATE, "a sweep was left unclosed on the device");
         return ISING_OK;
     }
 
@@ -621,6 +651,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
                                      (size_t)o.record_capacity * p->total_reps));
     if (!ising && !msc) {
         HIP_TRY(hipMalloc(&p->d_deltas, sizeof(float) * p->total_reps));
+        HIP_TRY(hipMalloc(&p->d_omcd, sizeof(float) * p->total_reps));
         HIP_TRY(hipMalloc(&p->d_betas_f, sizeof(float) * p->total_reps));
         HIP_TRY(hipMalloc(&p->d_part0, sizeof(double) * 4 * (size_t)p->nblocks));
         HIP_TRY(hipMalloc(&p->d_part1, sizeof(double) * 4 * (size_t)p->nblocks));
@@ -631,7 +662,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
     std::vector<uint32_t> thr(kIsingTab * (size_t)p->total_reps);
     std::vector<int32_t> ident(2 * (size_t)p->total_reps);
     std::vector<double> betas(p->total_reps);
-    std::vector<float> betas_f(p->total_reps), deltas(p->total_reps);
+    std::vector<float> betas_f(p->total_reps), deltas(p->total_reps), omcd(p->total_reps);
     std::vector<uint32_t> mthr(3 * (size_t)p->total_reps);
     for (int g = 0; g < n_sizes; ++g)
         for (int i = 0; i < R; ++i) {
@@ -639,6 +670,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
             betas[g * R + i] = beta;
             betas_f[g * R + i] = (float)beta;
             deltas[g * R + i] = (float)o.delta;
+            omcd[g * R + i] = (float)(1.0 - std::cos((double)(float)o.delta));
             ident[g * R + i] = ident[p->total_reps + g * R + i] = i;
             ising_table(beta, o.j, o.h, &thr[kIsingTab * (size_t)(g * R + i)], p->dims);
             for (int k = 0; k < 3; ++k)  // multispin coding: T(a) for a = 4, 5, 6 aligned neighbours
@@ -685,6 +717,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
                                   (uint32_t)(seed >> 32), p->st));
     } else {
         ST_COPY(p, p->d_deltas, deltas.data(), sizeof(float) * deltas.size(), hipMemcpyHostToDevice);
+        ST_COPY(p, p->d_omcd, omcd.data(), sizeof(float) * omcd.size(), hipMemcpyHostToDevice);
         ST_COPY(p, p->d_betas_f, betas_f.data(), sizeof(float) * betas_f.size(), hipMemcpyHostToDevice);
         ST_FILL(p, p->d_fpool, 0, p->pool_bytes);
         ST_FILL(p, p->d_part0, 0, sizeof(double) * 4 * (size_t)p->nblocks);
@@ -1749,6 +1782,16 @@ int ising_pt_autotune(ising_pt_t* p, int32_t sweeps, int32_t* width, int32_t* ru
     return ISING_OK;
 }
 
+int ising_pt_prepare_graph(ising_pt_t* p, int32_t graph_sweeps) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    if (graph_sweeps < 0 || graph_sweeps > 100000) return fail(ISING_E_ARG, "graph_sweeps must be in 0..100000");
+    SYNC_ALL(p);
+    p->drop_graph();
+    p->opts.graph_sweeps = graph_sweeps;
+    if (p->graph_capable()) return p->capture_graph();
+    return ISING_OK;
+}
+
 int ising_pt_sweeps(const ising_pt_t* p, int64_t* out) {
     if (!p || !out) return fail(ISING_E_ARG, "null argument");
     *out = p->t_host;
@@ -2031,6 +2074,118 @@ int ising_cont_set_spins(ising_cont_t* l, const float* in) {
 int ising_cont_sweeps(const ising_cont_t* l, int64_t* out) {
     if (!l || !out) return fail(ISING_E_ARG, "null argument");
     *out = l->pt->t_host;
+    return ISING_OK;
+}
+
+// ---------------------------------------------------------------------------
+// reduce_gpu::* (src/cuda/reduce_gpu.rs:8-178): reductions of a caller's
+// device lattice in the reference's layout.  Like the reference, each call
+// allocates its partials, runs on the device's default stream and copies the
+// totals back synchronously.
+// ---------------------------------------------------------------------------
+
+}  // extern "C"
+
+namespace {
+
+int reduce_device_ready() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return fail(ISING_E_HIP, "no HIP device visible");
+    return ISING_OK;
+}
+
+// Totals of an int8 +-1 lattice: bond sum B (forward neighbours) and spin sum.
+int reduce_ising_totals(const int8_t* d_spins, int32_t n, int64_t* bm) {
+    if (!d_spins || !bm) return fail(ISING_E_ARG, "null argument");
+    if (n < 1 || n > 1024) return fail(ISING_E_ARG, "n must be in 1..1024");
+    int rc = reduce_device_ready();
+    if (rc) return rc;
+    const int nblk = reduce_blocks((int64_t)n * n * n);
+    int64_t* buf = nullptr;
+    HIP_TRY(hipMalloc(&buf, sizeof(int64_t) * (2 * (size_t)nblk + 2)));
+    hipError_t e = launch_reduce_ising(d_spins, n, buf + 2, buf, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(bm, buf, 2 * sizeof(int64_t), hipMemcpyDeviceToHost);
+    (void)hipFree(buf);
+    HIP_TRY(e);
+    return ISING_OK;
+}
+
+// FP64 totals of a continuous lattice: [0] = sum of S_i . S_fwd (when n > 0), [1..3] = Sx, Sy, Sz.
+int reduce_cont_totals(const float* d_spins, int64_t n_sites, int32_t n, int32_t n_comp, double* out) {
+    if (!d_spins || !out) return fail(ISING_E_ARG, "null argument");
+    if (n_comp != 2 && n_comp != 3) return fail(ISING_E_ARG, "n_comp must be 2 or 3");
+    if (n_sites < 0 || n_sites > (int64_t)1 << 30) return fail(ISING_E_ARG, "n_sites must be in 0..2^30");
+    int rc = reduce_device_ready();
+    if (rc) return rc;
+    if (n_sites == 0) {
+        for (int q = 0; q < 4; ++q) out[q] = 0.0;
+        return ISING_OK;
+    }
+    const int nblk = reduce_blocks(n_sites);
+    double* buf = nullptr;
+    HIP_TRY(hipMalloc(&buf, sizeof(double) * (4 * (size_t)nblk + 4)));
+    hipError_t e = launch_reduce_cont(d_spins, n_sites, n, n_comp, buf + 4, buf, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out, buf, 4 * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(buf);
+    HIP_TRY(e);
+    return ISING_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ising_load_reduce_kernels(void) { return reduce_device_ready(); }
+
+int ising_reduce_ising(const int8_t* d_spins, int32_t n, float j, double* e_total, double* m_total) {
+    int64_t bm[2];
+    const int rc = reduce_ising_totals(d_spins, n, bm);
+    if (rc) return rc;
+    if (e_total) *e_total = -(double)j * (double)bm[0];
+    if (m_total) *m_total = (double)bm[1];
+    return ISING_OK;
+}
+
+int ising_measure_ising_gpu(const int8_t* d_spins, int32_t n, float j, double* e_per_spin, double* m_abs_per_spin) {
+    double e = 0.0, m = 0.0;
+    const int rc = ising_reduce_ising(d_spins, n, j, &e, &m);
+    if (rc) return rc;
+    const double n3 = (double)n * n * n;
+    if (e_per_spin) *e_per_spin = e / n3;
+    if (m_abs_per_spin) *m_abs_per_spin = std::fabs(m) / n3;
+    return ISING_OK;
+}
+
+int ising_reduce_continuous_mag(const float* d_spins, int64_t n_sites, int32_t n_comp, double* mx, double* my,
+                                double* mz) {
+    double v[4];
+    const int rc = reduce_cont_totals(d_spins, n_sites, 0, n_comp, v);
+    if (rc) return rc;
+    if (mx) *mx = v[1];
+    if (my) *my = v[2];
+    if (mz) *mz = v[3];
+    return ISING_OK;
+}
+
+int ising_reduce_continuous_energy(const float* d_spins, int32_t n, int32_t n_comp, float j, double* e_total) {
+    if (n < 1 || n > 1024) return fail(ISING_E_ARG, "n must be in 1..1024");
+    double v[4];
+    const int rc = reduce_cont_totals(d_spins, (int64_t)n * n * n, n, n_comp, v);
+    if (rc) return rc;
+    if (e_total) *e_total = -(double)j * v[0];
+    return ISING_OK;
+}
+
+int ising_measure_continuous_gpu(const float* d_spins, int32_t n, int32_t n_comp, float j, double* e, double* mx,
+                                 double* my, double* mz) {
+    if (n < 1 || n > 1024) return fail(ISING_E_ARG, "n must be in 1..1024");
+    double v[4];
+    const int rc = reduce_cont_totals(d_spins, (int64_t)n * n * n, n, n_comp, v);
+    if (rc) return rc;
+    if (e) *e = -(double)j * v[0];
+    if (mx) *mx = v[1];
+    if (my) *my = v[2];
+    if (mz) *mz = v[3];
     return ISING_OK;
 }
 

@@ -17,14 +17,21 @@
 // Overrelaxation: S' = 2 (S.h) h / |h|^2 - S, skipped when |h|^2 < 1e-30.
 //
 // Measurement is fused into the Metropolis passes and summed in FP64: block
-// partials are written to a buffer and a one-block closing kernel after the
-// colour-1 launch adds them in a fixed order, so every run with the same seed gives the same bits.
+// partials are written to a buffer and the closing launch after the colour-1
+// pass adds them per replica in a fixed order (one wave per replica, spread over
+// the launch's blocks), so every run with the same seed gives the same bits; the
+// block of that launch that finishes last writes the records, decides the
+// exchanges and advances the sweep counter.
+//
+// Threads: one site per thread, except XY's Metropolis (cont_metro_pair_kernel)
+// and overrelaxation passes, which take two sites of the block's 256-site tile
+// per thread.
 #include "kernels.h"
 
 namespace ising {
 
-constexpr int kCT = 256;        // threads per block, one site per thread
-constexpr int kCloseT = 256;  // threads of the one-block closing kernel (one replica or pair each at a time)
+constexpr int kCT = 256;        // sites per block tile
+constexpr int kCloseT = 256;  // threads of the closing block (one replica or pair each at a time)
 
 __device__ __forceinline__ int find_group_c(const DevGroup* groups, int ngroups, int b) {
     int g = 0;
@@ -169,6 +176,7 @@ __device__ void close_cont(const ContArgs& a, int64_t t, int32_t fl) {
             double d = (double)a.deltas[ti] * f;
             d = d < 1e-3 ? 1e-3 : (d > 3.14159265358979 ? 3.14159265358979 : d);
             a.deltas[ti] = (float)d;
+            a.omcd[ti] = (float)(1.0 - cos((double)(float)d));
             a.acc_cnt[gr] = 0;
         }
     }
@@ -194,6 +202,7 @@ __global__ __launch_bounds__(kCT) void cont_metro_kernel(ContArgs a, int color, 
     const int ti = G.rep0 + a.r2t[grep];
     const float beta = a.inline_params ? a.beta_inline : a.betas_f[ti];
     const float dlt = a.inline_params ? a.delta_inline : a.deltas[ti];
+    const float omc = N == 3 ? (a.inline_params ? a.omcd_inline : a.omcd[ti]) : 0.f;
     const int64_t t = a.ctl->t;
     const int32_t fl = a.ctl->flags;
     const bool need = (fl & (F_MEAS | F_EXCH)) != 0 && closes;
@@ -226,9 +235,11 @@ __global__ __launch_bounds__(kCT) void cont_metro_kernel(ContArgs a, int color, 
             nw[0] = S[0] * ca - S[1] * sa;
             nw[1] = S[1] * ca + S[0] * sa;
         } else {
-            const float cd = cosf(dlt);
-            const float ct = cd + (1.0f - cd) * u24f(w.x);
-            const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+            // 1 - cos(theta) = (1 - cos d)(1 - u), exact in 1 - u: sin(theta) keeps
+            // its relative precision near the pole, where 1 - ct * ct would cancel
+            const float am = omc * (1.0f - u24f(w.x));
+            const float ct = 1.0f - am;
+            const float st = sqrtf(am * (2.0f - am));
             float sp, cp;
             sincosf(6.28318530717958647692f * u24f(w.y), &sp, &cp);
             float tv0 = 0.f, tv1 = 0.f;
@@ -311,6 +322,7 @@ __global__ __launch_bounds__(kCT / 2) void cont_metro_pair_kernel(ContArgs a, in
     const int ti = G.rep0 + a.r2t[grep];
     const float beta = a.inline_params ? a.beta_inline : a.betas_f[ti];
     const float dlt = a.inline_params ? a.delta_inline : a.deltas[ti];
+    const float omc = N == 3 ? (a.inline_params ? a.omcd_inline : a.omcd[ti]) : 0.f;
     const int64_t t = a.ctl->t;
     const int32_t fl = a.ctl->flags;
     const bool need = (fl & (F_MEAS | F_EXCH)) != 0 && closes;
@@ -345,9 +357,11 @@ __global__ __launch_bounds__(kCT / 2) void cont_metro_pair_kernel(ContArgs a, in
                 nw[0] = S[0] * ca - S[1] * sa;
                 nw[1] = S[1] * ca + S[0] * sa;
             } else {
-                const float cd = cosf(dlt);
-                const float ct = cd + (1.0f - cd) * u24f(w.x);
-                const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+                // 1 - cos(theta) = (1 - cos d)(1 - u), exact in 1 - u: sin(theta) keeps
+                // its relative precision near the pole, where 1 - ct * ct would cancel
+                const float am = omc * (1.0f - u24f(w.x));
+                const float ct = 1.0f - am;
+                const float st = sqrtf(am * (2.0f - am));
                 float sp, cp;
                 sincosf(6.28318530717958647692f * u24f(w.y), &sp, &cp);
                 float tv0 = 0.f, tv1 = 0.f;
@@ -418,26 +432,27 @@ __global__ __launch_bounds__(kCT / 2) void cont_metro_pair_kernel(ContArgs a, in
 
 // Closes a continuous sweep (after the colour-1 Metropolis pass).  A sweep
 // that measures or exchanges first sums the block partials per replica over
-// the whole grid; the block that finishes that last (a counter) then closes
-// the sweep.  Other sweeps are closed by block 0 alone.
+// the whole grid.  Every block then counts itself done (ctl->cdone), on every
+// sweep, and the block that counts last closes the sweep: by then every block
+// of the launch has read t and the flags (each block uses them before it
+// counts), so the close may rewrite them.  A close by a fixed block instead
+// would race with blocks dispatched after it, which would read the next
+// sweep's flags.
 __global__ __launch_bounds__(kCT) void cont_close_kernel(ContArgs a) {
     static_assert(kCloseT == kCT, "the closing step runs on one block of this launch");
     __shared__ int s_last;
     const int64_t t = a.ctl->t;
     const int32_t fl = a.ctl->flags;
-    if (!(fl & (F_MEAS | F_EXCH))) {
-        if (blockIdx.x == 0) close_cont(a, t, fl);
-        return;
-    }
-    reduce_replica(a);
+    const bool sums = (fl & (F_MEAS | F_EXCH)) != 0;
+    if (sums) reduce_replica(a);
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();  // this block's sums are visible before it counts as done
+        if (sums) __threadfence();  // this block's sums are visible before it counts as done
         s_last = atomicAdd(&a.ctl->cdone, 1) == (int)gridDim.x - 1;
     }
     __syncthreads();
     if (!s_last) return;
-    __threadfence();  // and the last block reads every block's sums
+    if (sums) __threadfence();  // and the last block reads every block's sums
     close_cont(a, t, fl);
     if (threadIdx.x == 0) a.ctl->cdone = 0;
 }

@@ -24,9 +24,11 @@
 // Measurement is fused: the colour-0 pass sums its new spins, the colour-1 pass
 // sums its new spins and s_i * (sum of its 6 neighbours), which counts every
 // bond exactly once (each bond joins the two colours).  Block partials go to
-// int64 accumulators by atomics; after the colour-1 launch a closing kernel writes
-// the records, performs the replica exchange and advances the sweep counter
-// in a one-block closing kernel.
+// int64 accumulators by atomics.  Sweep t is closed at the start of the
+// colour-0 launch of sweep t + 1 (sweepctl.h): its blocks decide the replica
+// exchanges from the committed sums, one block writes the records and the next
+// sweep's control fields.  A one-block closing kernel closes the last sweep of
+// each host call, so the host always reads complete state.
 #include "sweepctl.h"
 
 namespace ising {

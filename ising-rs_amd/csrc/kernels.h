@@ -59,6 +59,7 @@ struct ContArgs {
     uint32_t model;        // MODEL_XY or MODEL_HEIS (random-stream tag)
     float* pool;
     float* deltas;         // [rep0 + tidx] proposal size (radians), adapted during warm-up
+    float* omcd;           // [rep0 + tidx] 1 - cos(delta) of the FP32 delta, rounded from FP64 (Heisenberg cap)
     const float* betas_f;  // [rep0 + tidx] beta for the Metropolis test
     const double* betas;   // [rep0 + tidx] beta for the exchange test
     int32_t* r2t;
@@ -76,7 +77,7 @@ struct ContArgs {
     int32_t adapt_every;   // 0: proposal size fixed
     float adapt_target;    // target acceptance rate
     int32_t inline_params; // 1: beta/delta from the two fields below (single-lattice passes)
-    float beta_inline, delta_inline;
+    float beta_inline, delta_inline, omcd_inline;
     const uint16_t* net;   // learned proposal: BF16 parameters (null when off)
     int64_t* mh;           // learned proposal: [kMhSlots][8] moves tried (0) / accepted (1), spread by wave
     int32_t learn_H;       // hidden width of the learned proposal's MLP
@@ -161,6 +162,19 @@ struct ClusterArgs {
 hipError_t launch_cluster_update(const ClusterArgs& a, int nblocks, hipStream_t st);
 hipError_t launch_cont_fold(const DevGroup* groups, int ngroups, int total_reps, const double* part, double* out,
                             hipStream_t st);
+
+// Reductions of a caller's lattice in the reference's layout (reduce.hip):
+// partials per 4096-site tile into `part` (2 int64 / 4 doubles per tile), then
+// one block sums them in a fixed order into `out` (2 int64: bond sum, spin sum /
+// 4 doubles: bond dot sum, Sx, Sy, Sz).  n <= 1024 (n^3 < 2^32), n_sites < 2^32.
+// Sets the calling thread's ising_last_error() message; returns code.
+int set_error(int code, const char* msg);
+
+constexpr int kReduceTile = 4096;
+inline int reduce_blocks(int64_t sites) { return (int)((sites + kReduceTile - 1) / kReduceTile); }
+hipError_t launch_reduce_ising(const int8_t* spins, int32_t n, int64_t* part, int64_t* out, hipStream_t st);
+hipError_t launch_reduce_cont(const float* spins, int64_t n_sites, int32_t n, int32_t ncomp, double* part,
+                              double* out, hipStream_t st);
 
 constexpr int kIsingTile = 4096;  // half-sites per block step in Ising launches (256 threads x 16)
 constexpr int kIsingTab = 32;     // acceptance-table words per temperature

@@ -113,6 +113,7 @@ _SIGS = {
     "ising_pt_mh_stats": (C.c_int, [_vp, _i64p, _i64p]),
     "ising_pt_metropolis_pass": (C.c_int, [_vp, C.c_int32, C.c_int64]),
     "ising_pt_sweeps": (C.c_int, [_vp, _i64p]),
+    "ising_pt_prepare_graph": (C.c_int, [_vp, C.c_int32]),
     "ising_pt_device_bytes": (C.c_int, [_vp, _i64p]),
     "ising_pt_autotune": (C.c_int, [_vp, C.c_int32] + [C.POINTER(C.c_int32)] * 3),
     "ising_pt_helicity": (C.c_int, [_vp, _vp]),
@@ -135,6 +136,13 @@ _SIGS = {
     "ising_batch_warmup": (C.c_int, [_vp, C.c_int64]),
     "ising_batch_sample": (C.c_int, [_vp, C.c_int64]),
     "ising_batch_sync": (C.c_int, [_vp, _f64p]),
+    "ising_load_reduce_kernels": (C.c_int, []),
+    "ising_reduce_ising": (C.c_int, [_vp, C.c_int32, C.c_float, _f64p, _f64p]),
+    "ising_measure_ising_gpu": (C.c_int, [_vp, C.c_int32, C.c_float, _f64p, _f64p]),
+    "ising_reduce_continuous_mag": (C.c_int, [_vp, C.c_int64, C.c_int32, _f64p, _f64p, _f64p]),
+    "ising_reduce_continuous_energy": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_float, _f64p]),
+    "ising_measure_continuous_gpu": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_float, _f64p, _f64p, _f64p,
+                                               _f64p]),
     "ising_linspace_temperatures": (C.c_int, [C.c_double, C.c_double, C.c_int32, _vp]),
     "ising_replica_exchange": (C.c_int, [_vp, _vp, C.c_int32, _vp, _vp, _vp, C.c_int32, C.POINTER(C.c_int32)]),
     "ising_jackknife": (C.c_int, [_vp, _vp, C.c_int64, C.c_double, C.c_double, C.c_int32, _vp]),
@@ -531,6 +539,11 @@ class PTEnsemble:
         _check(lib.ising_pt_autotune(self._h, int(sweeps), C.byref(w), C.byref(r), C.byref(s)))
         return w.value, r.value, s.value
 
+    def prepare_graph(self, graph_sweeps):
+        """Capture `graph_sweeps` sweeps as a HIP graph now (nothing runs);
+        warmup/sample calls then replay it.  0 turns graph replay off."""
+        _check(lib.ising_pt_prepare_graph(self._h, int(graph_sweeps)))
+
     def sweeps(self):
         v = C.c_int64()
         _check(lib.ising_pt_sweeps(self._h, C.byref(v)))
@@ -641,6 +654,52 @@ def read_timeseries(path):
            "measure_every": int(me), "J": J, "temps": temps}
     out.update(fields)
     return out
+
+
+# reduce_gpu::* (src/cuda/reduce_gpu.rs:8-178).  `spins` is a lattice in DEVICE
+# memory in the reference's layout: an integer device address, or any object
+# with a data_ptr() method (a torch tensor on the GPU, for instance).
+def _dptr(spins):
+    return C.c_void_p(spins.data_ptr() if hasattr(spins, "data_ptr") else int(spins))
+
+
+def load_reduce_kernels():
+    _check(lib.ising_load_reduce_kernels())
+
+
+def reduce_ising(spins, n, j=1.0):
+    """(E total, M total) of an n^3 int8 +-1 device lattice (reduce_gpu.rs:23-66)."""
+    e, m = C.c_double(), C.c_double()
+    _check(lib.ising_reduce_ising(_dptr(spins), int(n), float(j), C.byref(e), C.byref(m)))
+    return e.value, m.value
+
+
+def measure_ising_gpu(spins, n, j=1.0):
+    """(E per spin, |M| per spin) (reduce_gpu.rs:70-79)."""
+    e, m = C.c_double(), C.c_double()
+    _check(lib.ising_measure_ising_gpu(_dptr(spins), int(n), float(j), C.byref(e), C.byref(m)))
+    return e.value, m.value
+
+
+def reduce_continuous_mag(spins, n_sites, n_comp):
+    """(Mx, My, Mz) totals of n_sites component-interleaved float spins (reduce_gpu.rs:83-135)."""
+    v = [C.c_double() for _ in range(3)]
+    _check(lib.ising_reduce_continuous_mag(_dptr(spins), int(n_sites), int(n_comp), *[C.byref(x) for x in v]))
+    return tuple(x.value for x in v)
+
+
+def reduce_continuous_energy(spins, n, n_comp, j=1.0):
+    """Total energy -j sum S_i . S_fwd of an n^3 lattice (reduce_gpu.rs:138-163)."""
+    e = C.c_double()
+    _check(lib.ising_reduce_continuous_energy(_dptr(spins), int(n), int(n_comp), float(j), C.byref(e)))
+    return e.value
+
+
+def measure_continuous_gpu(spins, n, n_comp, j=1.0):
+    """(E, Mx, My, Mz) totals (reduce_gpu.rs:167-178)."""
+    v = [C.c_double() for _ in range(4)]
+    _check(lib.ising_measure_continuous_gpu(_dptr(spins), int(n), int(n_comp), float(j), *[C.byref(x) for x in v]))
+    return tuple(x.value for x in v)
 
 
 def linspace_temperatures(t_min, t_max, n):

@@ -746,6 +746,31 @@ int64_t o_bench_ref_heis(int L, double beta, double delta, int n_or, uint64_t se
     return upd;
 }
 
+/* The reference's only XY CPU path, Wolff (xy/wolff.rs:16-73), from a random
+ * start: returns the spins flipped (cluster sizes summed) in `seconds`. */
+int64_t o_bench_ref_xy_wolff(int L, double beta, uint64_t seed, double seconds, double* wall, int64_t* steps) {
+    int64_t n = (int64_t)L * L * L, flipped = 0, k = 0;
+    double* sp = (double*)malloc(sizeof(double) * 2 * (size_t)n);
+    int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    uint8_t* mark = (uint8_t*)malloc((size_t)n);
+    o_rng rng;
+    o_rng_seed(&rng, seed);
+    for (int64_t i = 0; i < n; ++i) {
+        double phi = TWO_PI * o_rng_f64(&rng);
+        sp[2 * i] = cos(phi); sp[2 * i + 1] = sin(phi);
+    }
+    double t0 = now_s(), t1 = t0;
+    while (t1 - t0 < seconds) {
+        flipped += o_ref_xy_wolff_sweep(L, sp, beta, 1.0, &rng, stack, mark);
+        ++k;
+        t1 = now_s();
+    }
+    *wall = t1 - t0;
+    *steps = k;
+    free(sp); free(stack); free(mark);
+    return flipped;
+}
+
 /* Single-chain helpers for the statistical tests (one lattice, fixed beta,
  * measured every sweep).  dim = 2 uses the square lattice, dim = 3 the cube. */
 void o_ising_chain(int dim, int L, uint64_t seed, double beta, double J, double h, int64_t warmup,

@@ -150,6 +150,8 @@ int64_t o_ref_xy_wolff_sweep(int L, double* spins, double beta, double J, o_rng*
  * returns updates performed and wall seconds. */
 int64_t o_bench_ref_metropolis(int L, double beta, uint64_t seed, double seconds, double* wall);
 int64_t o_bench_ref_heis(int L, double beta, double delta, int n_or, uint64_t seed, double seconds, double* wall);
+/* xy/wolff.rs:16-73 timed: spins flipped in `seconds` (*steps = Wolff steps). */
+int64_t o_bench_ref_xy_wolff(int L, double beta, uint64_t seed, double seconds, double* wall, int64_t* steps);
 /* ---------------- Multispin-coded Ising (replicas across word bits) --------
  * Pinned by: the 2x2x2 brute-force Boltzmann distribution per replica, the
  * replica-independence of the bit lanes, and equality of the lazy bit-plane

@@ -84,6 +84,9 @@ lib.o_bench_ref_metropolis.argtypes = [C.c_int, C.c_double, C.c_uint64, C.c_doub
 lib.o_bench_ref_heis.restype = C.c_int64
 lib.o_bench_ref_heis.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, C.c_uint64, C.c_double,
                                  C.POINTER(C.c_double)]
+lib.o_bench_ref_xy_wolff.restype = C.c_int64
+lib.o_bench_ref_xy_wolff.argtypes = [C.c_int, C.c_double, C.c_uint64, C.c_double, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_int64)]
 
 
 def philox(ctr, key):

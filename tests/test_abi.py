@@ -74,6 +74,13 @@ def test_device_entry_points_fail_loudly_without_a_gpu():
     assert ei.value.code == -2
     with pytest.raises(ising_hip.IsingError):
         ising_hip.PTEnsemble("ising", [8], 4, np.linspace(4.4, 4.6, 4), 1)
+    # reduce_gpu::* checks for a device before it touches the (here fake) device address
+    for call in (ising_hip.load_reduce_kernels, lambda: ising_hip.reduce_ising(0x1000, 8),
+                 lambda: ising_hip.measure_continuous_gpu(0x1000, 8, 3),
+                 lambda: ising_hip.reduce_continuous_mag(0x1000, 100, 2)):
+        with pytest.raises(ising_hip.IsingError) as ei:
+            call()
+        assert ei.value.code == -2
 
 
 def test_invalid_arguments_are_rejected_before_touching_the_device():

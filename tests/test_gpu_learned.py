@@ -173,9 +173,12 @@ def plain_vs_learned(model, temps, L, hidden, train_steps, sweeps, seed=1):
     return out
 
 
-def test_training_raises_acceptance_and_keeps_equilibrium():
-    temps = np.linspace(2.18, 2.22, 4)
-    res = plain_vs_learned("xy", temps, 8, 16, 400, 20000)
+@pytest.mark.parametrize("model,temps", [("xy", np.linspace(2.18, 2.22, 4)), ("heisenberg", np.linspace(1.43, 1.46, 4))])
+def test_training_raises_acceptance_and_keeps_equilibrium(model, temps):
+    """Trained on the plain chain's configurations, the network raises the
+    learned moves' acceptance above the untrained network's, and the learned
+    chain's energies agree with the plain chain's per temperature."""
+    res = plain_vs_learned(model, temps, 8, 16, 400, 20000)
     _, _, (tu, au) = res["untrained"]
     _, _, (tl, al) = res["learned"]
     assert al / tl > 0.9 and al / tl > au / tu, (au / tu, al / tl)

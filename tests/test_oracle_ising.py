@@ -88,6 +88,28 @@ def test_3d_2x2x2_chain_matches_brute_force_boltzmann():
     assert abs(got - M2) < 4.5 * err
 
 
+@pytest.mark.parametrize("kind", ["metropolis", "wolff"])
+def test_reference_cpu_chains_match_brute_force_boltzmann(kind):
+    """The restatements of the reference's CPU algorithms (random-site
+    Metropolis, metropolis.rs:8-27, which the CPU baseline times; Wolff,
+    wolff.rs:27-65) sample the Boltzmann distribution of the 2x2x2 cube."""
+    L, T = 2, 3.0
+    sp = ((np.arange(256)[:, None] >> np.arange(8)[None, :]) & 1).astype(np.int8) * 2 - 1
+    Bs = np.array([o.ising_measure(L, s.copy())[0] for s in sp])
+    beta = 1.0 / T
+    w = np.exp(beta * (Bs - Bs.max()))
+    w /= w.sum()
+    b, m = o.ref_ising_chain(kind, L, 7, beta, 500, 300000)
+    for level in np.unique(Bs):
+        p_exact = w[Bs == level].sum()
+        got, err = jk_mean_err((b == level).astype(np.float64))
+        assert abs(got - p_exact) < 4.5 * err + 2e-4, (kind, level, got, p_exact, err)
+    for series, want in (((m / 8.0) ** 2, (w * (sp.sum(axis=1) / 8.0) ** 2).sum()),
+                         (np.abs(m) / 8.0, (w * np.abs(sp.sum(axis=1)) / 8.0).sum())):
+        got, err = jk_mean_err(series)
+        assert abs(got - want) < 4.5 * err, (kind, got, want, err)
+
+
 def test_ground_state_is_fixed_at_low_temperature():
     L = 4
     sp = np.ones(L ** 3, np.int8)
