@@ -155,6 +155,15 @@ int ising_reduce_continuous_energy(const float* d_spins, int32_t n, int32_t n_co
 /* measure_continuous_gpu (reduce_gpu.rs:167-178): (E, Mx, My, Mz) totals. */
 int ising_measure_continuous_gpu(const float* d_spins, int32_t n, int32_t n_comp, float j, double* e, double* mx,
                                  double* my, double* mz);
+/* No counterpart in the reference: ising_reduce_ising for callers that cannot
+ * allocate (code under HIP graph capture, tight loops, harnesses that forbid
+ * hipMalloc).  Allocates nothing: the partial sums live in a fixed 1 MiB
+ * device array, one per device, held until the process ends.  Writes
+ * d_out[0] = E total and d_out[1] = M total (caller-provided DEVICE memory, 2
+ * doubles) asynchronously on `stream` (a hipStream_t; NULL = the default
+ * stream).  n^3 <= 2^28 (n <= 645), else ISING_E_ARG.  Calls that overlap on
+ * one device race on the array: serialise them. */
+int ising_reduce_ising_noalloc(const int8_t* d_spins, int32_t n, float j, double* d_out, void* stream);
 
 /* ------------------------------------------------------------------------
  * Parallel-tempering ensemble: every lattice size and replica of a run in one

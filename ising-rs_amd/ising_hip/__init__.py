@@ -139,6 +139,7 @@ _SIGS = {
     "ising_load_reduce_kernels": (C.c_int, []),
     "ising_reduce_ising": (C.c_int, [_vp, C.c_int32, C.c_float, _f64p, _f64p]),
     "ising_measure_ising_gpu": (C.c_int, [_vp, C.c_int32, C.c_float, _f64p, _f64p]),
+    "ising_reduce_ising_noalloc": (C.c_int, [_vp, C.c_int32, C.c_float, _vp, _vp]),
     "ising_reduce_continuous_mag": (C.c_int, [_vp, C.c_int64, C.c_int32, _f64p, _f64p, _f64p]),
     "ising_reduce_continuous_energy": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_float, _f64p]),
     "ising_measure_continuous_gpu": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_float, _f64p, _f64p, _f64p,
