@@ -8,6 +8,9 @@
   (analysis/scripts/analyze_gpu_fss.py:256-277) -> T_c ~ 4.5115.
 * Heisenberg / XY: GPU cap/rotation Metropolis + overrelaxation against the
   restated CPU cap Metropolis + OR (heisenberg/) and XY Wolff (xy/wolff.rs).
+* E, |M|, chi and U4 per temperature (Ising, XY, Heisenberg at L = 8 and 16)
+  within 2 combined jackknife sigma of the CPU chains: at least 90% of the
+  z-scores within 2, none beyond 4; the reference's published L=16 summary.
 """
 import csv
 import math
@@ -97,6 +100,93 @@ def test_ising_pt_matches_published_reference_summary_L8():
             z.append((g - float(r[col])) / math.hypot(ge, float(r[col + "_err"])))
     z = np.array(z)
     # the published 20-block errors are themselves estimates: require agreement in distribution
+    assert np.mean(z ** 2) < 3.0 and np.max(np.abs(z)) < 6.0, (np.mean(z ** 2), np.max(np.abs(z)))
+    assert abs(np.mean(z)) < 4.0 / math.sqrt(len(z)) + 0.3
+
+
+def observables_z(gpu_e, gpu_m, cpu_e, cpu_m, beta, n3):
+    """z-scores of E, |M|, chi and U4 between two series (E and |M| per spin),
+    errors by the reference's 20-block jackknife (gpu_fss.rs:327-412) and, for
+    U4 = 1 - <m^4> / (3 <m^2>^2), the same blocking."""
+    jg = o.jackknife(gpu_e, gpu_m, beta, n3, 20)
+    jc = o.jackknife(cpu_e, cpu_m, beta, n3, 20)
+    z = [(jg[i] - jc[i]) / math.hypot(jg[i + 1], jc[i + 1]) for i in (0, 2, 10)]  # E, |M|, chi
+    (ug, uge), (uc, uce) = binder(gpu_m), binder(cpu_m)
+    z.append((ug - uc) / math.hypot(uge, uce))
+    return z
+
+
+def check_two_sigma(z, what):
+    """At least 90% of the z-scores within 2 and none beyond 4."""
+    a = np.abs(np.asarray(z))
+    assert np.all(np.isfinite(a)), (what, z)
+    assert np.mean(a < 2.0) >= 0.9 and a.max() < 4.0, (what, np.round(z, 2).tolist())
+
+
+def _parallel(fn, args):
+    from concurrent.futures import ThreadPoolExecutor  # the oracle's C loops release the GIL
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        return list(ex.map(lambda a: fn(*a), args))
+
+
+def test_ising_e_m_chi_u4_within_two_sigma_of_cpu_wolff():
+    """GPU PT (L = 8, 16; 8 temperatures in [4.40, 4.62]) against the restated
+    reference Wolff (wolff.rs:27-65) at each temperature."""
+    R = 8
+    temps = o.linspace(4.40, 4.62, R)
+    z = []
+    for L in (8, 16):
+        e, m = ising_pt_series(L, R, temps, 21 + L, 5000, 100000)
+        cpu = _parallel(lambda k: o.ref_ising_chain("wolff", L, 300 + 10 * L + k, 1.0 / temps[k], 2000, 60000),
+                        [(k,) for k in range(R)])
+        for k in range(R):
+            b, mg = cpu[k]
+            z += observables_z(e[k], m[k], -b / L ** 3, np.abs(mg) / L ** 3, 1.0 / temps[k], float(L ** 3))
+    check_two_sigma(z, "ising")
+
+
+@pytest.mark.parametrize("model", ["xy", "heisenberg"])
+def test_continuous_e_m_chi_u4_within_two_sigma_of_cpu_chains(model):
+    """GPU PT (L = 8, 16; 4 temperatures) against the restated reference CPU
+    chains: XY Wolff (xy/wolff.rs:16-73), Heisenberg cap Metropolis + 2
+    overrelaxation sweeps (heisenberg/metropolis.rs:10-92, overrelax.rs:15-48)."""
+    R = 4
+    temps = o.linspace(2.15, 2.25, R) if model == "xy" else o.linspace(1.40, 1.50, R)
+    kw = dict(delta=1.5, n_overrelax=2, exchange_every=10) if model == "xy" else \
+        dict(delta=1.0, n_overrelax=2, exchange_every=10, adapt_every=0)
+    z = []
+    for L in (8, 16):
+        e, m = cont_pt_series(model, L, R, temps, 31 + L, 3000, 60000, **kw)
+        if model == "xy":
+            cpu = _parallel(lambda k: o.ref_xy_chain(L, 500 + 10 * L + k, 1.0 / temps[k], 2000, 60000),
+                            [(k,) for k in range(R)])
+        else:
+            n = 40000 if L == 8 else 20000  # ~25 s of CPU per temperature at L = 16
+            cpu = _parallel(lambda k: o.ref_heis_chain(L, 700 + 10 * L + k, 1.0 / temps[k], 1.0, 2, 2000, n),
+                            [(k,) for k in range(R)])
+        for k in range(R):
+            ec, mc = cpu[k]
+            z += observables_z(e[k], m[k], ec / L ** 3, mc / L ** 3, 1.0 / temps[k], float(L ** 3))
+    check_two_sigma(z, model)
+
+
+def test_ising_pt_matches_published_reference_summary_L16():
+    """The reference's published L=16 summary (32 temperatures in [4.40, 4.62],
+    20-block jackknife errors) against GPU PT on the same ladder: E, |M|, M^2
+    and chi, in distribution (the published errors are estimates too)."""
+    rows = list(csv.DictReader(open(os.path.join(HERE, "golden", "published_gpu_fss_ising_N16_summary.csv"))))
+    temps = np.array([float(r["T"]) for r in rows])
+    R = len(temps)
+    assert R == 32 and abs(temps[0] - 4.40) < 1e-6 and abs(temps[-1] - 4.62) < 1e-6
+    L = 16
+    lt = o.linspace(4.40, 4.62, R)
+    e, m = ising_pt_series(L, R, lt, 6, 5000, 100000)
+    z = []
+    for k, r in enumerate(rows):
+        j = o.jackknife(e[k], m[k], 1.0 / lt[k], float(L ** 3), 20)
+        for col, i in (("E", 0), ("M", 2), ("M2", 4), ("chi", 10)):
+            z.append((j[i] - float(r[col])) / math.hypot(j[i + 1], float(r[col + "_err"])))
+    z = np.array(z)
     assert np.mean(z ** 2) < 3.0 and np.max(np.abs(z)) < 6.0, (np.mean(z ** 2), np.max(np.abs(z)))
     assert abs(np.mean(z)) < 4.0 / math.sqrt(len(z)) + 0.3
 
