@@ -285,50 +285,31 @@ def temperature_series(tidx, x):
     return np.take_along_axis(x, order, axis=2)  # [sweep, size, temperature]
 
 
-def run_learned(args, ising_hip):
-    """Learned proposal against plain Metropolis + overrelaxation:
-    XY, L = 32, 64 x 32 replicas, T = linspace(2.18, 2.22).  Both chains sample
-    the same distribution; the figure of merit is independent samples per
-    second, sweeps/s / (2 tau_int(E)) summed over (size, temperature), with
-    tau_int by Sokal's automatic window (c = 6)."""
-    sizes, Rr = [32, 64], R
-    temps = linspace(2.18, 2.22, Rr)
-    steps = args.steps
-    kw = dict(exchange_every=10, measure_every=1, n_overrelax=2, delta=1.0, adapt_every=10, adapt_target=0.5,
-              renorm_every=100, record_capacity=steps + 8)
-    out = {}
-    for mode in ("plain", "learned", "learned_or0", "cluster"):
-        nor = {"plain": 2, "learned": args.learned_or, "learned_or0": 0, "cluster": 2}[mode]
-        pt = ising_hip.PTEnsemble("xy", sizes, Rr, temps, args.seed, **dict(kw, n_overrelax=nor))
-        pt.warmup(args.warmup)
-        if mode == "cluster":
-            out[mode] = xy_cluster_leg(pt, sizes, Rr, temps, steps, ising_hip)
-            pt.close()
-            continue
-        train = None
-        if mode.startswith("learned"):
-            pt.learn_init(args.hidden, seed=7)
-            t0 = time.perf_counter()
-            loss = pt.learn_train(args.train_steps, batch=8192, lr=3e-3)
-            pt.sync()
-            train = {"steps": args.train_steps, "batch": 8192, "seconds": round(time.perf_counter() - t0, 3),
-                     "final_loss": loss}
-            pt.learn_enable(True)
-            pt.warmup(50)
+def binder_jk(m2, nb=20):
+    """U4 = 1 - <m^4> / (3 <m^2>^2) from a series of m^2, with a 20-block jackknife error."""
+    m2 = np.asarray(m2, np.float64)
+    bs = len(m2) // nb
+    x = m2[: bs * nb].reshape(nb, bs)
+    full = 1 - (x ** 2).mean() / (3 * x.mean() ** 2)
+    jk = np.array([1 - (np.delete(x, i, 0) ** 2).mean() / (3 * np.delete(x, i, 0).mean() ** 2) for i in range(nb)])
+    return full, float(np.sqrt((nb - 1) / nb * ((jk - full) ** 2).sum()))
+
+
+def chain_stats(pt, model, sizes, Rr, temps, steps, ising_hip):
+    """Samples `steps` sweeps (in chunks; XY helicity sums between chunks, outside
+    the timed sampling) and returns per-(size, temperature) statistics."""
+    chunk, wall, done = 10, 0.0, 0
+    hc = np.zeros((len(sizes), Rr))
+    hj2 = np.zeros((len(sizes), Rr))
+    nh = 0
+    while done < steps:
+        m = min(chunk, steps - done)
+        t0 = time.perf_counter()
+        pt.sample(m)
         pt.sync()
-        # sample in chunks; between chunks, the XY helicity sums per replica
-        # (outside the timed sampling)
-        chunk, wall, done = 10, 0.0, 0
-        hc = np.zeros((len(sizes), Rr))
-        hj2 = np.zeros((len(sizes), Rr))
-        nh = 0
-        while done < steps:
-            m = min(chunk, steps - done)
-            t0 = time.perf_counter()
-            pt.sample(m)
-            pt.sync()
-            wall += time.perf_counter() - t0
-            done += m
+        wall += time.perf_counter() - t0
+        done += m
+        if model == "xy":
             hel = pt.helicity()
             for g in range(len(sizes)):
                 r2t, _ = pt.get_perm(g)
@@ -336,52 +317,133 @@ def run_learned(args, ising_hip):
                 hc[g, r2t] += hg[:, :3].mean(axis=1)
                 hj2[g, r2t] += (hg[:, 3:] ** 2).mean(axis=1)
             nh += 1
-        ups = [(hc[g] / nh - np.array([1.0 / t for t in temps]) * hj2[g] / nh) / L ** 3 for g, L in enumerate(sizes)]
-        tidx, e, mx, my, mz = pt.read_cont()
-        es = temperature_series(tidx, e)
-        m2 = temperature_series(tidx, mx * mx + my * my)
-        taus_e, taus_m = [], []
-        for g in range(len(sizes)):
-            for k in range(Rr):
-                taus_e.append(ising_hip.tau_int(es[:, g, k])[0])
-                taus_m.append(ising_hip.tau_int(m2[:, g, k])[0])
-        taus_e, taus_m = np.array(taus_e), np.array(taus_m)
-        isps = float(np.sum(steps / (2.0 * np.maximum(taus_e, 0.5))) / wall)
-        tries, acc = pt.mh_stats() if mode.startswith("learned") else (0, 0)
-        m2s = temperature_series(tidx, (mx * mx + my * my))
-        u4 = [float(np.mean(1 - (m2s[:, g, :] ** 2).mean(axis=0) / (3 * m2s[:, g, :].mean(axis=0) ** 2)))
-              for g in range(len(sizes))]
-        out[mode] = {"sweeps_per_s": round(steps / wall, 2), "tau_int_E_mean": round(float(taus_e.mean()), 3),
-                     "helicity_modulus_mean": [round(float(u.mean()), 5) for u in ups],
-                     "binder_u4_mean": [round(v, 5) for v in u4],
-                     "tau_int_M2_mean": round(float(taus_m.mean()), 3), "independent_samples_per_s": round(isps, 1),
-                     "learned_acceptance": (round(acc / tries, 4) if tries else None), "training": train}
-        pt.close()
-    ratio = out["learned"]["independent_samples_per_s"] / out["plain"]["independent_samples_per_s"]
-    cratio = out["cluster"]["independent_samples_per_s"] / out["plain"]["independent_samples_per_s"]
-    sites = sum(L ** 3 for L in sizes) * Rr
+    tidx, e, mx, my, mz = pt.read_cont()
+    es = temperature_series(tidx, e)
+    m2s = temperature_series(tidx, mx * mx + my * my + mz * mz)
+    st = {"tau_E": [], "tau_M2": [], "E": [], "E_err": [], "U4": [], "U4_err": []}
+    for g, L in enumerate(sizes):
+        n3 = float(L ** 3)
+        for k in range(Rr):
+            ek, m2k = es[:, g, k] / n3, m2s[:, g, k] / n3 ** 2
+            st["tau_E"].append(ising_hip.tau_int(ek)[0])
+            st["tau_M2"].append(ising_hip.tau_int(m2k)[0])
+            jk = ising_hip.jackknife_observables(ek, np.sqrt(m2k), 1.0 / temps[k], n3, 20)
+            st["E"].append(jk[0])
+            st["E_err"].append(jk[1])
+            u, ue = binder_jk(m2k)
+            st["U4"].append(u)
+            st["U4_err"].append(ue)
+    st = {k: np.array(v) for k, v in st.items()}
+    st["wall"] = wall
+    if model == "xy":
+        st["helicity"] = [float(((hc[g] / nh - np.array([1.0 / t for t in temps]) * hj2[g] / nh) / L ** 3).mean())
+                          for g, L in enumerate(sizes)]
+    return st
+
+
+def summarize(st, steps, base=None):
+    """Independent samples/s of E and M^2 (steps / (2 tau) per (size, temperature),
+    summed, over the wall time) and, against the plain chain, the cost ratio at
+    equal error: (err_plain^2 t_plain) / (err^2 t), geometric mean over
+    (size, temperature), for E and U4."""
+    out = {"sweeps_per_s": round(steps / st["wall"], 2),
+           "tau_int_E_mean": round(float(st["tau_E"].mean()), 3),
+           "tau_int_M2_mean": round(float(st["tau_M2"].mean()), 3),
+           "independent_samples_per_s_E": round(float(np.sum(steps / (2 * np.maximum(st["tau_E"], 0.5))) / st["wall"]), 1),
+           "independent_samples_per_s_M2": round(float(np.sum(steps / (2 * np.maximum(st["tau_M2"], 0.5))) / st["wall"]), 1),
+           "E_mean": round(float(st["E"].mean()), 6), "E_err_mean": float("%.3g" % st["E_err"].mean()),
+           "U4_mean": round(float(st["U4"].mean()), 5), "U4_err_mean": float("%.3g" % st["U4_err"].mean())}
+    if "helicity" in st:
+        out["helicity_modulus_mean"] = [round(v, 5) for v in st["helicity"]]
+    if base is not None:
+        for q in ("E", "U4"):
+            r = (base[q + "_err"] ** 2 * base["wall"]) / (st[q + "_err"] ** 2 * st["wall"])
+            out["equal_error_speedup_" + q] = round(float(np.exp(np.mean(np.log(r)))), 4)
+        # agreement with the plain chain, per (size, temperature), in combined errors
+        for q in ("E", "U4"):
+            z = (st[q] - base[q]) / np.hypot(st[q + "_err"], base[q + "_err"])
+            out["z_vs_plain_" + q] = {"max_abs": round(float(np.abs(z).max()), 2),
+                                      "frac_within_2": round(float(np.mean(np.abs(z) < 2)), 3)}
+    return out
+
+
+LEARNED_MODELS = [
+    # model, sizes, temperatures, Metropolis step (radians)
+    ("xy", [32, 64], (2.18, 2.22), 1.0),
+    ("heisenberg", [32], (1.43, 1.46), 1.0),
+]
+
+
+def run_learned(args, ising_hip):
+    """Learned proposal against plain Metropolis + overrelaxation, and the
+    heat-bath ceiling: XY L = 32, 64 and Heisenberg L = 32, 32 replicas each.
+    Chains: plain (Metropolis + 2 overrelaxation passes per sweep), learned
+    (the trained network's Metropolis-Hastings pass + 2 overrelaxation passes),
+    heat bath (the exact single-site conditional + 2 overrelaxation passes: the
+    best any proposal that depends on the neighbours only can do per pass), and
+    for XY embedded Swendsen-Wang.  Same ensembles, temperatures and seed;
+    measured every sweep."""
+    steps = args.steps
+    res = {}
+    for model, sizes, (tlo, thi), dlt in LEARNED_MODELS:
+        temps = linspace(tlo, thi, R)
+        kw = dict(exchange_every=10, measure_every=1, n_overrelax=2, delta=dlt, adapt_every=10, adapt_target=0.5,
+                  renorm_every=100, record_capacity=steps + 8)
+        out, stats = {}, {}
+        for mode in ("plain", "learned", "heat_bath") + (("cluster",) if model == "xy" else ()):
+            nor = args.learned_or if mode == "learned" else 2
+            pt = ising_hip.PTEnsemble(model, sizes, R, temps, args.seed, **dict(kw, n_overrelax=nor))
+            pt.warmup(args.warmup)
+            if mode == "cluster":
+                out[mode] = xy_cluster_leg(pt, sizes, R, temps, steps, ising_hip)
+                pt.close()
+                continue
+            train = None
+            if mode == "learned":
+                pt.learn_init(args.hidden, seed=7)
+                t0 = time.perf_counter()
+                loss = pt.learn_train(args.train_steps, batch=8192, lr=3e-3)
+                pt.sync()
+                train = {"steps": args.train_steps, "batch": 8192, "seconds": round(time.perf_counter() - t0, 3),
+                         "final_loss": loss}
+                pt.learn_enable(True)
+                pt.warmup(50)
+            elif mode == "heat_bath":
+                pt.heat_bath(True)
+                pt.warmup(50)
+            pt.sync()
+            stats[mode] = chain_stats(pt, model, sizes, R, temps, steps, ising_hip)
+            out[mode] = summarize(stats[mode], steps, None if mode == "plain" else stats["plain"])
+            if mode == "learned":
+                tries, acc = pt.mh_stats()
+                out[mode]["learned_acceptance"] = round(acc / tries, 4) if tries else None
+                out[mode]["training"] = train
+            pt.close()
+        for mode in ("learned", "heat_bath"):
+            for q in ("E", "M2"):
+                out[mode + "_over_plain_samples_" + q] = round(out[mode]["independent_samples_per_s_" + q]
+                                                              / out["plain"]["independent_samples_per_s_" + q], 4)
+        out["sizes"], out["temperatures"] = sizes, [tlo, thi]
+        res[model] = out
+    value = sum(res[m]["learned"]["independent_samples_per_s_E"] for m in res)
     line = {
         "metric": "independent samples/s (learned proposal)",
-        "value": out["learned"]["independent_samples_per_s"],
+        "value": round(value, 1),
         "unit": "samples/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 / out["learned"]["sweeps_per_s"],
+        "ms_per_step": round(1e3 / res["xy"]["learned"]["sweeps_per_s"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 (MLP bf16 weights, f32 accumulate)",
         "data": "synthetic",
-        "config": {"workload": "learned proposal vs plain: XY L=32,64 x 32 replicas, T=linspace(2.18,2.22,32), "
-                               "plain = 1 Metropolis + 2 overrelaxation passes per sweep, learned = the learned "
-                               "proposal's pass + learned_overrelaxation_passes, measure every sweep, exchange every 10",
-                   "hidden": args.hidden, "sites": sites, "plain": out["plain"], "learned": out["learned"],
-                   "learned_or0": out["learned_or0"], "cluster": out["cluster"],
-                   "learned_overrelaxation_passes": args.learned_or, "learned_over_plain": round(ratio, 4),
-                   "learned_or0_over_plain": round(out["learned_or0"]["independent_samples_per_s"]
-                                                   / out["plain"]["independent_samples_per_s"], 4),
-                   "cluster_over_plain": round(cratio, 4)},
+        "config": {"workload": "learned proposal vs plain Metropolis + 2 overrelaxation passes and the heat-bath "
+                               "ceiling: XY L=32,64 x 32 replicas T=linspace(2.18,2.22,32); Heisenberg L=32 x 32 "
+                               "T=linspace(1.43,1.46,32); measured every sweep, exchange every 10; value = learned "
+                               "independent samples/s of E summed over both models",
+                   "hidden": args.hidden, "learned_overrelaxation_passes": args.learned_or, **res},
     }
     print(json.dumps(line), flush=True)
 

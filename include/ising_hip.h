@@ -266,10 +266,19 @@ int ising_pt_learn_eval(ising_pt_t* pt, int64_t n, const float* x, const float* 
 int ising_pt_learn_infer(ising_pt_t* pt, int64_t n, const float* x, float* out);
 /* 1: sweeps use the learned proposal for the Metropolis passes; 0: the cap/rotation proposal. */
 int ising_pt_learn_enable(ising_pt_t* pt, int32_t on);
+/* 1: sweeps use heat-bath passes (no counterpart in the reference): every
+ * site draws its spin from the exact single-site conditional exp(beta J s.h)
+ * -- Fisher on S^2, von Mises on S^1 (Best-Fisher rejection, at most 8
+ * attempts; when all fail the site keeps its spin) -- in place of the
+ * Metropolis passes; overrelaxation passes stay.  The learned proposal, which
+ * depends on the neighbours only, cannot mix faster per pass.  Turns the
+ * learned proposal off; 0 returns to the cap/rotation Metropolis passes. */
+int ising_pt_heat_bath(ising_pt_t* pt, int32_t on);
 /* Learned Metropolis-Hastings moves attempted / accepted since creation. */
 int ising_pt_mh_stats(ising_pt_t* pt, int64_t* tries, int64_t* accepts);
-/* One Metropolis pass over one colour at sweep index t, without measurement,
- * exchange or advancing the sweep counter (used by exactness tests). */
+/* One learned (or, with ising_pt_heat_bath on, heat-bath) pass over one
+ * colour at sweep index t, without measurement, exchange or advancing the
+ * sweep counter (used by exactness tests). */
 int ising_pt_metropolis_pass(ising_pt_t* pt, int32_t color, int64_t t);
 /* Sweeps applied so far (warm-up + sample). */
 /* XY spin-stiffness sums of every replica's current state (no counterpart in

@@ -148,6 +148,7 @@ struct ising_pt {
     // learned proposal
     int learn_H = 0, learn_P = 0;
     bool learned_on = false;
+    bool heatbath_on = false;  // continuous: heat-bath passes in place of the Metropolis passes
     int64_t learn_step = 0;
     float* d_lmaster = nullptr;
     uint16_t* d_lnet = nullptr;
@@ -385,7 +386,10 @@ struct ising_pt {
                     HIP_TRY(launch_cont_or(a, nblocks, 0, st));
                     HIP_TRY(launch_cont_or(a, nblocks, 1, st));
                 }
-                if (learned_on) {
+                if (heatbath_on) {
+                    HIP_TRY(launch_cont_heatbath(a, nblocks, 0, 1, -1, st));
+                    HIP_TRY(launch_cont_heatbath(a, nblocks, 1, 1, -1, st));
+                } else if (learned_on) {
                     HIP_TRY(launch_cont_learned(a, nblocks, 0, 1, -1, st));
                     HIP_TRY(launch_cont_learned(a, nblocks, 1, 1, -1, st));
                 } else {
@@ -1244,11 +1248,25 @@ int ising_pt_learn_enable(ising_pt_t* p, int32_t on) {
     } else if (!p) {
         return fail(ISING_E_ARG, "null handle");
     }
-    if ((bool)on != p->learned_on) {
+    if ((bool)on != p->learned_on || (on && p->heatbath_on)) {
         SYNC_ALL(p);
         p->drop_graph();
     }
     p->learned_on = on != 0;
+    if (on) p->heatbath_on = false;
+    return ISING_OK;
+}
+
+int ising_pt_heat_bath(ising_pt_t* p, int32_t on) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    if (p->ncomp == 0) return fail(ISING_E_STATE, "heat-bath passes exist for XY / Heisenberg ensembles");
+    if (p->inline_cont) return fail(ISING_E_STATE, "single-lattice handles take per-call parameters");
+    if ((bool)on != p->heatbath_on || (on && p->learned_on)) {
+        SYNC_ALL(p);
+        p->drop_graph();
+    }
+    p->heatbath_on = on != 0;
+    if (on) p->learned_on = false;
     return ISING_OK;
 }
 
@@ -1276,8 +1294,9 @@ int ising_pt_metropolis_pass(ising_pt_t* p, int32_t color, int64_t t) {
     if (p->ncomp == 0) return fail(ISING_E_STATE, "continuous ensembles only");
     if ((color != 0 && color != 1) || t < 0) return fail(ISING_E_ARG, "color 0/1 and t >= 0");
     ORDER(p);
-    if (p->learned_on) HIP_TRY(launch_cont_learned(p->cont_args(), p->nblocks, color, 0, t, p->st));
-    else return fail(ISING_E_STATE, "enable the learned proposal first");
+    if (p->heatbath_on) HIP_TRY(launch_cont_heatbath(p->cont_args(), p->nblocks, color, 0, t, p->st));
+    else if (p->learned_on) HIP_TRY(launch_cont_learned(p->cont_args(), p->nblocks, color, 0, t, p->st));
+    else return fail(ISING_E_STATE, "enable the learned proposal or heat-bath passes first");
     return ISING_OK;
 }
 

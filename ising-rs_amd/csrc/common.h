@@ -17,7 +17,8 @@ namespace ising {
 enum : uint32_t { MODEL_ISING = 0, MODEL_XY = 1, MODEL_HEIS = 2 };
 enum : uint32_t {
     P_INIT = 1, P_ISING = 2, P_TIE = 3, P_EXCH = 4, P_CONT = 5, P_MSC = 6,
-    P_BOND = 7, P_BTIE = 8, P_CDIR = 9, P_WSEED = 10  // cluster updates
+    P_BOND = 7, P_BTIE = 8, P_CDIR = 9, P_WSEED = 10,  // cluster updates
+    P_HEAT = 11                                        // heat-bath moves (continuous models)
 };
 
 __host__ __device__ constexpr uint32_t tag(uint32_t purpose, uint32_t model) { return (purpose << 8) | model; }

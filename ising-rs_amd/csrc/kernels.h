@@ -87,6 +87,8 @@ int learn_params(int H);  // parameter count of the MLP with hidden width H (8, 
 constexpr int kLearnBlock = 64;  // samples per training block
 hipError_t launch_cont_learned(const ContArgs& a, int nblocks, int color, int closes, int64_t t_override,
                                hipStream_t st);
+hipError_t launch_cont_heatbath(const ContArgs& a, int nblocks, int color, int closes, int64_t t_override,
+                                hipStream_t st);
 hipError_t launch_learn_grad(const ContArgs& a, int nblk, int64_t step, int mode, const float* xin, const float* cin,
                              int64_t n_in, float* gpart, double* lpart, hipStream_t st);
 hipError_t launch_learn_adam(float* master, uint16_t* net, float* m1, float* m2, const float* gpart, int nblk, int P,

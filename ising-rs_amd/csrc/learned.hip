@@ -334,6 +334,90 @@ __device__ __forceinline__ void mlp_block(const NetFrag<H>& F, const float x[kNI
     for (int r = 0; r < 3; ++r) o[r] = out[3 * t + r];
 }
 
+// Draws s' around the direction hh from one mixture component of concentration
+// kap, c' = s'.hh into *cn_out.  Heisenberg: c' = 1 + log(u + (1 - u) e^{-2 kappa}) / kappa
+// with u = w.y (the Fisher CDF inverted), azimuth 2 pi w.z.  XY: von Mises by
+// Best-Fisher rejection, attempts from Philox calls whose counter word 3 is tg
+// with the attempt index in bits 16 and up; returns false when all kVmAttempts
+// attempts failed (the site keeps its spin, with a probability that depends on
+// the neighbours only).
+template <int N>
+__device__ __forceinline__ bool draw_around(const float hh[3], float kap, const u32x4& w, uint32_t h, uint32_t hs,
+                                            uint32_t stream, uint32_t tg, uint32_t k0, uint32_t k1, float nw[3],
+                                            float* cn_out) {
+    float cn = 0.f;
+    bool have = false;
+    if (N == 3) {
+        // Fisher: c = 1 + log(u + (1 - u) e^{-2 kappa}) / kappa, azimuth uniform
+        const float u = u24f(w.y);
+        cn = 1.0f + __logf(u + (1.0f - u) * __expf(-2.0f * kap)) * __builtin_amdgcn_rcpf(kap);
+        cn = fminf(1.0f, fmaxf(-1.0f, cn));
+        float sp, cp;
+        sincosf(6.28318530717958647692f * u24f(w.z), &sp, &cp);
+        // orthonormal e1, e2 perpendicular to hh
+        float t0 = 0.f, t1 = 0.f;
+        if (fabsf(hh[0]) < 0.9f) t0 = 1.f; else t1 = 1.f;
+        const float th = t0 * hh[0] + t1 * hh[1];
+        float e0 = t0 - th * hh[0], e1 = t1 - th * hh[1], e2 = -th * hh[2];
+        const float ei = rsqrtf(e0 * e0 + e1 * e1 + e2 * e2);
+        e0 *= ei; e1 *= ei; e2 *= ei;
+        const float f0 = hh[1] * e2 - hh[2] * e1, f1 = hh[2] * e0 - hh[0] * e2, f2 = hh[0] * e1 - hh[1] * e0;
+        const float sn = sqrtf(fmaxf(0.0f, 1.0f - cn * cn));
+        nw[0] = cn * hh[0] + sn * (cp * e0 + sp * f0);
+        nw[1] = cn * hh[1] + sn * (cp * e1 + sp * f1);
+        nw[2] = cn * hh[2] + sn * (cp * e2 + sp * f2);
+        have = true;
+    } else {
+        // von Mises by Best-Fisher rejection.  The first four attempts
+        // draw from independent Philox calls and are evaluated together
+        // (no chain of dependent draws); the first accepted one wins, as
+        // if they had been made in turn.  Later attempts run one by one.
+        const float sr = 0.5f * __builtin_amdgcn_rcpf(kap);
+        const float rr = sr + sqrtf(1.0f + sr * sr);
+        auto try_vm = [&](const u32x4& v, float* W) -> bool {
+            const float zc = __cosf(3.14159265358979f * u24f(v.x));
+            *W = (1.0f + rr * zc) * __builtin_amdgcn_rcpf(rr + zc);
+            const float Y = kap * (rr - *W);
+            const float V = fmaxf(u24f(v.y), 1e-30f);
+            return Y * (2.0f - Y) - V >= 0.0f || __logf(Y) - __logf(V) + 1.0f - Y >= 0.0f;
+        };
+        constexpr int kBatch = 4;
+        u32x4 vb[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k)
+            vb[k] = philox(h, hs, stream, tg | ((uint32_t)(k + 1) << 16), k0, k1);
+        float Wsel = 0.f;
+        uint32_t zsel = 0u;
+#pragma unroll
+        for (int k = kBatch - 1; k >= 0; --k) {
+            float Wk;
+            if (try_vm(vb[k], &Wk)) {
+                Wsel = Wk;
+                zsel = vb[k].z;
+                have = true;
+            }
+        }
+#pragma unroll 1
+        for (int at = kBatch + 1; at <= kVmAttempts && !have; ++at) {
+            const u32x4 v = philox(h, hs, stream, tg | ((uint32_t)at << 16), k0, k1);
+            float Wk;
+            if (try_vm(v, &Wk)) {
+                Wsel = Wk;
+                zsel = v.z;
+                have = true;
+            }
+        }
+        if (have) {
+            cn = fminf(1.0f, fmaxf(-1.0f, Wsel));
+            const float sn = sqrtf(fmaxf(0.0f, 1.0f - cn * cn)) * (u24f(zsel) < 0.5f ? -1.0f : 1.0f);
+            nw[0] = cn * hh[0] - sn * hh[1];
+            nw[1] = sn * hh[0] + cn * hh[1];
+        }
+    }
+    *cn_out = cn;
+    return have;
+}
+
 // ---------------------------------------------------------------------------
 // Learned Metropolis-Hastings pass over one colour (replaces the cap/rotation
 // proposal when the ensemble's learned mode is on).  Same measurement fusion
@@ -394,76 +478,8 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int color
             const u32x4 w = philox((uint32_t)h, hs, stream, (uint32_t)((7u << 8) | a.model), a.k0, a.k1);
             const float kap = u24f(w.x) < m.w0 ? m.k0 : m.k1;
             float cn = 0.f;
-            bool have = false;
-            if (N == 3) {
-                // Fisher: c = 1 + log(u + (1 - u) e^{-2 kappa}) / kappa, azimuth uniform
-                const float u = u24f(w.y);
-                cn = 1.0f + __logf(u + (1.0f - u) * __expf(-2.0f * kap)) * __builtin_amdgcn_rcpf(kap);
-                cn = fminf(1.0f, fmaxf(-1.0f, cn));
-                float sp, cp;
-                sincosf(6.28318530717958647692f * u24f(w.z), &sp, &cp);
-                // orthonormal e1, e2 perpendicular to hh
-                float t0 = 0.f, t1 = 0.f;
-                if (fabsf(hh[0]) < 0.9f) t0 = 1.f; else t1 = 1.f;
-                const float th = t0 * hh[0] + t1 * hh[1];
-                float e0 = t0 - th * hh[0], e1 = t1 - th * hh[1], e2 = -th * hh[2];
-                const float ei = rsqrtf(e0 * e0 + e1 * e1 + e2 * e2);
-                e0 *= ei; e1 *= ei; e2 *= ei;
-                const float f0 = hh[1] * e2 - hh[2] * e1, f1 = hh[2] * e0 - hh[0] * e2, f2 = hh[0] * e1 - hh[1] * e0;
-                const float sn = sqrtf(fmaxf(0.0f, 1.0f - cn * cn));
-                nw[0] = cn * hh[0] + sn * (cp * e0 + sp * f0);
-                nw[1] = cn * hh[1] + sn * (cp * e1 + sp * f1);
-                nw[2] = cn * hh[2] + sn * (cp * e2 + sp * f2);
-                have = true;
-            } else {
-                // von Mises by Best-Fisher rejection.  The first four attempts
-                // draw from independent Philox calls and are evaluated together
-                // (no chain of dependent draws); the first accepted one wins, as
-                // if they had been made in turn.  Later attempts run one by one.
-                const float sr = 0.5f * __builtin_amdgcn_rcpf(kap);
-                const float rr = sr + sqrtf(1.0f + sr * sr);
-                auto try_vm = [&](const u32x4& v, float* W) -> bool {
-                    const float zc = __cosf(3.14159265358979f * u24f(v.x));
-                    *W = (1.0f + rr * zc) * __builtin_amdgcn_rcpf(rr + zc);
-                    const float Y = kap * (rr - *W);
-                    const float V = fmaxf(u24f(v.y), 1e-30f);
-                    return Y * (2.0f - Y) - V >= 0.0f || __logf(Y) - __logf(V) + 1.0f - Y >= 0.0f;
-                };
-                constexpr int kBatch = 4;
-                u32x4 vb[kBatch];
-#pragma unroll
-                for (int k = 0; k < kBatch; ++k)
-                    vb[k] = philox((uint32_t)h, hs, stream, (uint32_t)((7u << 8) | a.model | ((k + 1) << 16)), a.k0,
-                                   a.k1);
-                float Wsel = 0.f;
-                uint32_t zsel = 0u;
-#pragma unroll
-                for (int k = kBatch - 1; k >= 0; --k) {
-                    float Wk;
-                    if (try_vm(vb[k], &Wk)) {
-                        Wsel = Wk;
-                        zsel = vb[k].z;
-                        have = true;
-                    }
-                }
-#pragma unroll 1
-                for (int at = kBatch + 1; at <= kVmAttempts && !have; ++at) {
-                    const u32x4 v = philox((uint32_t)h, hs, stream, (uint32_t)((7u << 8) | a.model | (at << 16)),
-                                           a.k0, a.k1);
-                    float Wk;
-                    if (try_vm(v, &Wk)) {
-                        Wsel = Wk;
-                        zsel = v.z;
-                        have = true;
-                    }
-                }
-                if (have) {
-                    cn = fminf(1.0f, fmaxf(-1.0f, Wsel));
-                    const float sn = sqrtf(fmaxf(0.0f, 1.0f - cn * cn)) * (u24f(zsel) < 0.5f ? -1.0f : 1.0f);
-                    nw[0] = cn * hh[0] - sn * hh[1];
-                    nw[1] = sn * hh[0] + cn * hh[1];
-                }
-            }
+            const bool have = draw_around<N>(hh, kap, w, (uint32_t)h, hs, stream, (7u << 8) | a.model, a.k0, a.k1,
+                                             nw, &cn);
             if (have) {
                 float co = 0.f;
 #pragma unroll
@@ -526,6 +542,124 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int color
             for (int k = 0; k < 4; ++k) dst[k] = s_red[k][0] + s_red[k][1] + s_red[k][2] + s_red[k][3];
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Heat-bath pass over one colour: every site draws its new spin from the exact
+// single-site conditional p(s | neighbours) ~ exp(beta J s.h) -- Fisher(h^, kappa)
+// on S^2, von Mises(arg h, kappa) on S^1, kappa = beta |J| |h| -- with the
+// samplers of the learned proposal (draw_around), and keeps it (the
+// Metropolis-Hastings ratio of the exact conditional is 1).  The learned
+// proposal depends on the neighbours only, like this one, so by Peskun's
+// ordering no setting of its weights mixes faster per pass: this pass is the
+// ceiling of that family.  Same measurement fusion and block order as the
+// plain pass.  Draws: Philox(h, 2t + colour, stream, (11 << 8) | model).
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(256) void cont_heatbath_kernel(ContArgs a, int nblocks, int color, int closes,
+                                                            int64_t t_override) {
+    __shared__ double s_red[4][4];
+    const int per = (int)(gridDim.x >> 3);
+    const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);  // XCD-contiguous tiles
+    if (b >= nblocks) return;
+    int g = 0;
+    while (g + 1 < a.ngroups && b >= a.groups[g + 1].blk0) ++g;
+    const DevGroup G = a.groups[g];
+    int r, tile;
+    split_tiles(G, b - G.blk0, &r, &tile);
+    const int grep = G.rep0 + r;
+    const int ti = G.rep0 + a.r2t[grep];
+    const float beta = a.inline_params ? a.beta_inline : a.betas_f[ti];
+    const int64_t t = t_override >= 0 ? t_override : a.ctl->t;
+    const int32_t fl = t_override >= 0 ? 0 : a.ctl->flags;
+    const bool need = (fl & (F_MEAS | F_EXCH)) != 0 && closes;
+    const bool renorm = (fl & F_RENORM) != 0;
+    const uint32_t hs = (uint32_t)(2 * t + color);
+    const uint32_t stream = stream_id((uint32_t)G.L, (uint32_t)r);
+    const uint32_t tg = tag(P_HEAT, a.model);
+
+    float* C = a.pool + G.off + (int64_t)(2 * r + color) * N * G.stride;
+    const float* O = a.pool + G.off + (int64_t)(2 * r + (color ^ 1)) * N * G.stride;
+    const int64_t h = (int64_t)tile * 256 + threadIdx.x;
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+    if (h < G.n_half) {
+        uint32_t nb[6];
+        site_nbrs(G, (uint32_t)h, color, nb);
+        float nv[3][6] = {};
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) nv[k][q] = O[k * G.stride + nb[q]];
+        float S[3] = {0.f, 0.f, 0.f}, hf[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            S[k] = C[k * G.stride + h];
+            hf[k] = ((nv[k][0] + nv[k][1]) + (nv[k][2] + nv[k][3])) + (nv[k][4] + nv[k][5]);
+        }
+        float hn2 = 0.f;
+#pragma unroll
+        for (int k = 0; k < N; ++k) hn2 += hf[k] * hf[k];
+        bool moved = false;
+        if (hn2 > 1e-12f) {
+            // the conditional leans along sign(J) h
+            const float hi = __builtin_amdgcn_rsqf(hn2) * (a.Jf < 0.0f ? -1.0f : 1.0f);
+            const float hh[3] = {hf[0] * hi, hf[1] * hi, hf[2] * hi};
+            const float kap = beta * fabsf(a.Jf) * hn2 * fabsf(hi);
+            const u32x4 w = philox((uint32_t)h, hs, stream, tg, a.k0, a.k1);
+            float nw[3] = {S[0], S[1], S[2]}, cn = 0.f;
+            if (kap > 1e-6f && draw_around<N>(hh, kap, w, (uint32_t)h, hs, stream, tg, a.k0, a.k1, nw, &cn)) {
+#pragma unroll
+                for (int k = 0; k < N; ++k) S[k] = nw[k];
+                moved = true;
+            }
+        }
+        if (renorm) {
+            float q2 = 0.f;
+#pragma unroll
+            for (int k = 0; k < N; ++k) q2 += S[k] * S[k];
+            const float qi = rsqrtf(q2);
+#pragma unroll
+            for (int k = 0; k < N; ++k) S[k] *= qi;
+        }
+        if (moved || renorm) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) C[k * G.stride + h] = S[k];
+        }
+        if (need) {
+            double e = 0.0;
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                part[1 + k] = (double)S[k];
+                if (color) {
+                    double f = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) f += (double)nv[k][q];
+                    e += (double)S[k] * f;
+                }
+            }
+            part[0] = color ? -a.J * e : 0.0;
+        }
+    }
+    if (need) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) part[k] = wave_sum(part[k]);
+        if ((threadIdx.x & 63) == 0)
+            for (int k = 0; k < 4; ++k) s_red[k][threadIdx.x >> 6] = part[k];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double* dst = (color ? a.part1 : a.part0) + 4 * (int64_t)b;
+            for (int k = 0; k < 4; ++k) dst[k] = s_red[k][0] + s_red[k][1] + s_red[k][2] + s_red[k][3];
+        }
+    }
+}
+
+hipError_t launch_cont_heatbath(const ContArgs& a, int nblocks, int color, int closes, int64_t t_override,
+                                hipStream_t st) {
+    const dim3 grid((nblocks + 7) & ~7);
+    if (a.ncomp == 2)
+        hipLaunchKernelGGL(cont_heatbath_kernel<2>, grid, dim3(256), 0, st, a, nblocks, color, closes, t_override);
+    else hipLaunchKernelGGL(cont_heatbath_kernel<3>, grid, dim3(256), 0, st, a, nblocks, color, closes, t_override);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -111,6 +111,7 @@ _SIGS = {
     "ising_pt_learn_enable": (C.c_int, [_vp, C.c_int32]),
     "ising_pt_learn_infer": (C.c_int, [_vp, C.c_int64, _vp, _vp]),
     "ising_pt_mh_stats": (C.c_int, [_vp, _i64p, _i64p]),
+    "ising_pt_heat_bath": (C.c_int, [_vp, C.c_int32]),
     "ising_pt_metropolis_pass": (C.c_int, [_vp, C.c_int32, C.c_int64]),
     "ising_pt_sweeps": (C.c_int, [_vp, _i64p]),
     "ising_pt_prepare_graph": (C.c_int, [_vp, C.c_int32]),
@@ -481,6 +482,10 @@ class PTEnsemble:
 
     def learn_enable(self, on=True):
         _check(lib.ising_pt_learn_enable(self._h, 1 if on else 0))
+
+    def heat_bath(self, on=True):
+        """Heat-bath passes (the exact single-site conditional) in place of the Metropolis passes."""
+        _check(lib.ising_pt_heat_bath(self._h, 1 if on else 0))
 
     def mh_stats(self):
         a, b = C.c_int64(), C.c_int64()

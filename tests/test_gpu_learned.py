@@ -155,6 +155,77 @@ def test_learned_moves_sample_the_exact_conditional(model, nc, beta):
     assert abs(z.mean()) * math.sqrt(len(z)) < 5.0
 
 
+@pytest.mark.parametrize("model,nc,beta", [("heisenberg", 3, 0.6), ("xy", 2, 0.5), ("heisenberg", 3, 2.5),
+                                           ("xy", 2, 2.0)])
+def test_heat_bath_samples_the_exact_conditional(model, nc, beta):
+    """Colour 1 frozen; every heat-bath pass redraws each colour-0 site from
+    exp(beta s.h): the mean and second moment of c = s.h^ per site against the
+    closed forms (S^2: <c> = coth k - 1/k, <c^2> = 1 - 2 <c>/k; S^1:
+    <c> = I1(k)/I0(k), <c^2> = 1 - <c>/k; k = beta |h|)."""
+    L = 4
+    pt = ensemble(model, L=L, R=1, temps=[1.0 / beta], seed=9, delta=0.7)
+    pt.heat_bath(True)
+    sp0 = pt.get_spins(0, 0).astype(np.float64).reshape(-1, nc)
+    s = np.arange(L ** 3)
+    colour0 = s[((s % L + (s // L) % L + s // (L * L)) & 1) == 0]
+    hs = []
+    for site in colour0:
+        z, rem = divmod(site, L * L)
+        y, x = divmod(rem, L)
+        nbs = [((z * L + y) * L + (x + 1) % L), ((z * L + y) * L + (x - 1) % L), ((z * L + (y + 1) % L) * L + x),
+               ((z * L + (y - 1) % L) * L + x), ((((z + 1) % L) * L + y) * L + x), ((((z - 1) % L) * L + y) * L + x)]
+        hs.append(sp0[nbs].sum(axis=0))
+    hs = np.array(hs)
+    hn = np.linalg.norm(hs, axis=1)
+    n_pass = 4000
+    cs = np.zeros((n_pass, len(colour0)))
+    for t in range(n_pass):
+        pt.metropolis_pass(0, t)
+        S = pt.get_spins(0, 0).astype(np.float64).reshape(-1, nc)[colour0]
+        cs[t] = (S * hs).sum(axis=1) / hn
+    assert np.array_equal(pt.get_spins(0, 0).reshape(-1, nc)[np.setdiff1d(s, colour0)],
+                          sp0[np.setdiff1d(s, colour0)].astype(np.float32))
+    kap = beta * hn
+    m1 = ref.mean_cos(nc, kap)
+    m2 = 1.0 - (2.0 if nc == 3 else 1.0) * m1 / kap
+    nb = 20
+    for series, exact in ((cs, m1), (cs ** 2, m2)):
+        bm = series[: (n_pass // nb) * nb].reshape(nb, -1, len(colour0)).mean(axis=1)
+        err = bm.std(axis=0, ddof=1) / math.sqrt(nb)
+        z = (series.mean(axis=0) - exact) / err
+        assert np.all(np.abs(z) < 5.0), z
+        assert abs(z.mean()) * math.sqrt(len(z)) < 5.0
+    # passes redraw independently: consecutive draws of a site are uncorrelated
+    rho = np.mean([np.corrcoef(cs[:-1, i], cs[1:, i])[0, 1] for i in range(len(colour0))])
+    assert abs(rho) < 0.05, rho
+
+
+@pytest.mark.parametrize("model,temps", [("xy", np.linspace(2.18, 2.22, 4)), ("heisenberg", np.linspace(1.43, 1.46, 4))])
+def test_heat_bath_chain_keeps_equilibrium(model, temps):
+    """Heat-bath + overrelaxation sweeps against plain Metropolis +
+    overrelaxation: energies per temperature within 4.5 combined sigma."""
+    R, L, n = len(temps), 8, 20000
+    kw = dict(delta=1.0, n_overrelax=2, exchange_every=10, measure_every=1, record_capacity=n + 2)
+    res = {}
+    for mode in ("plain", "heat"):
+        pt = ising_hip.PTEnsemble(model, [L], R, np.asarray(temps), 3, **kw)
+        pt.warmup(500)
+        if mode == "heat":
+            pt.heat_bath(True)
+        pt.sample(n)
+        tidx, e, mx, my, mz = pt.read_cont()
+        res[mode] = (tidx[:, 0, :], e[:, 0, :] / L ** 3)
+    for k in range(R):
+        means = {}
+        for mode in res:
+            t, e = res[mode]
+            ek = e[t == k]
+            bm = ek[: len(ek) // 20 * 20].reshape(20, -1).mean(axis=1)
+            means[mode] = (bm.mean(), bm.std(ddof=1) / math.sqrt(20))
+        (a, ea), (b, eb) = means["plain"], means["heat"]
+        assert abs(a - b) < 4.5 * math.hypot(ea, eb), (k, a, b, ea, eb)
+
+
 def plain_vs_learned(model, temps, L, hidden, train_steps, sweeps, seed=1):
     R = len(temps)
     kw = dict(delta=1.0, n_overrelax=2, exchange_every=10, measure_every=1, record_capacity=sweeps + 2)
