@@ -657,9 +657,11 @@ def main():
     layout = None
     if model == "ising":
         if args.layout == "auto":
-            layout = pt.autotune(20)  # warm-up sweeps that also pick the thread-step layout
+            layout = pt.autotune(20)  # warm-up sweeps that also pick the thread-step layout of every size
         else:
             layout = tuple(int(v) for v in args.layout.split(","))
+    elif model in ("xy", "heisenberg"):
+        layout = pt.autotune(20)  # the overrelaxation pass's sites per thread
     pt.warmup(args.warmup)
     # the timed sweeps replay one captured graph of G sweeps, G dividing --steps
     # (production runs replay graphs too); captured here, outside the timed region
@@ -675,6 +677,7 @@ def main():
 
     recs = pt.read_ising() if model.startswith("ising") else pt.read_cont()
     assert recs[0].shape[0] == args.steps // me
+    per_size = [pt.get_layout(g) for g in range(len(sizes))] if model == "ising" else None
     pt.close()
 
     sites = sum(L ** 3 for L in sizes) * R
@@ -728,8 +731,11 @@ def main():
             line["roofline"]["frac_at_12B_angle"] = round(achieved / 2 / HBM_PEAK_GBS, 4)
     if args.workload == "ising" and not fused and not args.no_beyond_mall:
         line["config"]["beyond_mall"] = beyond_mall(args, ising_hip, opts, layout)
-    if layout is not None:
-        line["config"]["layout"] = {"run_width": layout[0], "runs_per_step": layout[1], "steps_per_tile": layout[2]}
+    if layout is not None and model == "ising":
+        line["config"]["layout"] = {"run_width": layout[0], "runs_per_step": layout[1], "steps_per_tile": layout[2],
+                                    "per_size": {L: list(lay) for L, lay in zip(sizes, per_size)}}
+    elif layout is not None:
+        line["config"]["overrelaxation_sites_per_thread"] = layout[0]
     if passes > 1:
         line["config"]["or_updates_per_ns"] = round(updates * (passes - 1) / (wall * 1e9), 3)
         line["config"]["combined_updates_per_ns"] = round(updates * passes / (wall * 1e9), 3)

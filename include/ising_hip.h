@@ -368,13 +368,26 @@ int ising_fss_anneal(const int32_t* sizes, int32_t n_sizes, double j, double h, 
                      int32_t t_steps, int64_t warmup_sweeps, int64_t sample_sweeps, uint64_t seed,
                      ising_observables_t* out);
 
-/* Autotuner (no counterpart in the reference): times each int8 Ising
- * thread-step layout (16-site runs, one or two per step, or 32-site runs, x
- * 1/2/4 steps per block tile) over `sweeps` warm-up sweeps of this ensemble
- * and keeps the fastest.  Results do not depend on the layout, so these are
- * ordinary warm-up sweeps (9 * (sweeps + 1) of them: each layout's first
- * sweep is not timed).  Before sampling only.  Outputs may be NULL. */
+/* Autotuner (no counterpart in the reference; the reference plans one,
+ * docs/plans/2026-03-08-gpu-phase2-performance.md:601-655).  int8 Ising
+ * (3D and 2D): times each thread-step layout (16-site runs, one or two per
+ * step, or 32-site runs, x 1/2/4 steps per block tile) for every size at once,
+ * then refines size by size, largest first, keeping a size's candidate when it
+ * is faster by more than 0.5%; the outputs are the largest size's layout
+ * (ising_pt_get_layout gives every size's).  XY / Heisenberg: times the
+ * overrelaxation pass with one and two sites per thread (*width = the choice,
+ * *runs = *spt = 0).  Results do not depend on any of these, so the timed
+ * sweeps are ordinary warm-up sweeps (each candidate's first sweep is not
+ * timed; about 50 * (sweeps + 1) sweeps for an int8 ensemble of five sizes).
+ * When the environment variable ISING_HIP_TUNE_CACHE names a file, a choice
+ * stored there for the same model, replica count, sizes and device
+ * architecture is applied without timing anything, and a new choice is
+ * appended.  Before sampling only; not for multispin-coded ensembles.
+ * Outputs may be NULL. */
 int ising_pt_autotune(ising_pt_t* pt, int32_t sweeps, int32_t* width, int32_t* runs, int32_t* spt);
+/* Thread-step layout of one size (int8 Ising) or the overrelaxation sites per
+ * thread in *width (XY / Heisenberg). */
+int ising_pt_get_layout(ising_pt_t* pt, int32_t size_index, int32_t* width, int32_t* runs, int32_t* spt);
 /* Sweeps per captured HIP graph (no counterpart in the reference): sets
  * graph_sweeps and captures the graph now, without running a sweep, so that
  * a later warmup/sample call of k sweeps replays it k / graph_sweeps times

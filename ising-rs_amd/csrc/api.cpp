@@ -10,9 +10,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -157,7 +159,9 @@ struct ising_pt {
     int learn_cap_blocks = 0;
     int64_t* d_mh = nullptr;  // [kMhSlots][8] learned-move counters
     float beta_inline = 0.f, delta_inline = 0.f, j_inline = 1.f;
-    int layout_width = 0, layout_runs = 0, layout_spt = 0;  // Ising thread-step layout in use
+    int layout_width = 0, layout_runs = 0, layout_spt = 0;  // Ising thread-step layout of the first size
+    std::vector<std::array<int, 3>> glayout;  // Ising thread-step layout per size: run width, runs, steps per tile
+    int or_spt = 0;  // continuous: sites per thread of the overrelaxation pass (0: the model's default)
     int fused_z = 0;         // one-launch sweeps: z planes per block asked for (0: two launches per sweep)
     bool fused_on = false;   // one-launch sweeps in use (every size qualifies)
     int fnblocks = 0, fused_lds = 0;
@@ -299,6 +303,7 @@ struct ising_pt {
         a.net = d_lnet;
         a.mh = d_mh;
         a.learn_H = learn_H;
+        a.or_spt = or_spt;
         return a;
     }
 
@@ -482,16 +487,20 @@ static int64_t acc_words(int total_reps) { return 2 * 2 * (int64_t)kMscAccCopies
 // run where L % 64 == 0; runs (1 or 2; 0 = 2) applies to 16-site runs; spt
 // (1..4; 0 = 1) is the number of thread steps per block tile.  Results do not
 // depend on any of them: only the grid changes.
-static int layout_ising(ising_pt* p, int width, int runs, int spt) {
-    if (width == 0) width = 16;
-    if (runs == 0) runs = 2;
-    if (spt == 0) spt = 1;
+// Per-size layouts: lw[g], lr[g], ls[g] as above for group g (results do not depend on them).
+static int layout_ising_each(ising_pt* p, const int* lw, const int* lr, const int* ls) {
+    p->glayout.resize(p->groups.size());
     int blk = 0;
-    for (DevGroup& G : p->groups) {
+    for (size_t gi = 0; gi < p->groups.size(); ++gi) {
+        DevGroup& G = p->groups[gi];
+        const int width = lw[gi] == 0 ? 16 : lw[gi], runs = lr[gi] == 0 ? 2 : lr[gi], spt = ls[gi] == 0 ? 1 : ls[gi];
         G.fast = G.dims == 2 ? 0 : ((G.Lh % 32) == 0 ? 3 : ((G.Lh % 16) == 0 ? 2 : ((G.Lh % 8) == 0 ? 1 : 0)));
         if (G.fast == 3 && width == 16) G.fast = 2;
         G.runs = G.fast == 2 ? runs : 1;
         G.spt = G.fast ? spt : 1;
+        p->glayout[gi][0] = width;
+        p->glayout[gi][1] = runs;
+        p->glayout[gi][2] = spt;
         G.tsz = (int32_t)((G.fast == 3 || G.runs == 2 ? 2 : 1) * kIsingTile * G.spt);
         G.tiles = (int32_t)((G.n_half + G.tsz - 1) / G.tsz);
         G.mag_T = G.sh_T = 0;
@@ -540,14 +549,20 @@ static int layout_ising(ising_pt* p, int width, int runs, int spt) {
         p->fused_lds = (int)lds;
         if (p->fused_on && p->d_pool && !p->d_pool2) HIP_TRY(hipMalloc(&p->d_pool2, p->pool_bytes));
     }
-    p->layout_width = width;
-    p->layout_runs = runs;
-    p->layout_spt = spt;
+    p->layout_width = p->glayout.empty() ? 16 : p->glayout[0][0];
+    p->layout_runs = p->glayout.empty() ? 2 : p->glayout[0][1];
+    p->layout_spt = p->glayout.empty() ? 1 : p->glayout[0][2];
     if (p->d_groups) {
         ST_COPY(p, p->d_groups, p->groups.data(), sizeof(DevGroup) * p->groups.size(), hipMemcpyHostToDevice);
         p->drop_graph();
     }
     return ISING_OK;
+}
+
+static int layout_ising(ising_pt* p, int width, int runs, int spt) {
+    const size_t n = p->groups.size();
+    std::vector<int> w(n, width), r(n, runs), t(n, spt);
+    return layout_ising_each(p, w.data(), r.data(), t.data());
 }
 
 int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, int R, const double* temps,
@@ -1760,44 +1775,244 @@ int ising_fss_anneal(const int32_t* sizes, int32_t n_sizes, double j, double h, 
     return ISING_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Device time of `sweeps` sweeps (after one untimed sweep: the first launch of a shape).
+int time_sweeps(ising_pt* p, int sweeps, double* ms_out) {
+    int rc = p->enqueue_sweeps(1);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(p->ev0, p->st));
+    rc = p->enqueue_sweeps(sweeps);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(p->ev1, p->st));
+    HIP_TRY(hipEventSynchronize(p->ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+    *ms_out = ms;
+    return ISING_OK;
+}
+
+// Tuning cache: lines "key | value" in the file named by ISING_HIP_TUNE_CACHE
+// (none when unset).  key = model, replicas, sizes and device architecture.
+std::string tune_key(const ising_pt* p) {
+    std::string arch = "unknown";
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) arch = prop.gcnArchName;
+    std::string k = "model=" + std::to_string(p->model) + " R=" + std::to_string(p->R) + " sizes=";
+    for (size_t i = 0; i < p->groups.size(); ++i) k += (i ? "," : "") + std::to_string(p->groups[i].L);
+    return k + " device=" + arch;
+}
+
+bool tune_cache_read(const ising_pt* p, std::string* value) {
+    const char* path = std::getenv("ISING_HIP_TUNE_CACHE");
+    if (!path || !*path) return false;
+    std::FILE* f = std::fopen(path, "r");
+    if (!f) return false;
+    const std::string key = tune_key(p) + " | ";
+    bool found = false;
+    char line[4096];
+    while (std::fgets(line, sizeof(line), f)) {
+        std::string l(line);
+        while (!l.empty() && (l.back() == '\n' || l.back() == '\r')) l.pop_back();
+        if (l.compare(0, key.size(), key) == 0) {
+            *value = l.substr(key.size());
+            found = true;  // the last entry wins
+        }
+    }
+    std::fclose(f);
+    return found;
+}
+
+void tune_cache_write(const ising_pt* p, const std::string& value) {
+    const char* path = std::getenv("ISING_HIP_TUNE_CACHE");
+    if (!path || !*path) return;
+    std::FILE* f = std::fopen(path, "a");
+    if (!f) return;
+    std::fprintf(f, "%s | %s\n", tune_key(p).c_str(), value.c_str());
+    std::fclose(f);
+}
+
+// Layout of group G for a candidate (width, runs, spt): what layout_ising_each makes of it.
+std::array<int, 3> effective_layout(const DevGroup& G, int w, int r, int s) {
+    int fast = G.dims == 2 ? 0 : ((G.Lh % 32) == 0 ? 3 : ((G.Lh % 16) == 0 ? 2 : ((G.Lh % 8) == 0 ? 1 : 0)));
+    if (fast == 3 && w == 16) fast = 2;
+    return {fast, fast == 2 ? r : 1, fast ? s : 1};
+}
+
+}  // namespace
+
+extern "C" {
+
 int ising_pt_autotune(ising_pt_t* p, int32_t sweeps, int32_t* width, int32_t* runs, int32_t* spt) {
     if (!p) return fail(ISING_E_ARG, "null handle");
     if (sweeps < 1 || sweeps > 100000) return fail(ISING_E_ARG, "sweeps per candidate must be in 1..100000");
-    if (p->model != ISING_MODEL_ISING && p->model != ISING_MODEL_ISING2D)
-        return fail(ISING_E_STATE, "the thread-step layout exists for int8 Ising ensembles only");
+    if (p->model == ISING_MODEL_ISING_MSC) return fail(ISING_E_STATE, "multispin-coded ensembles have no tunable layout");
     if (p->s_host >= 0) return fail(ISING_E_STATE, "autotune runs warm-up sweeps: call it before sampling");
-    if (p->inline_thr) return fail(ISING_E_STATE, "single-lattice handles are not tuned");
+    if (p->inline_thr || p->inline_cont) return fail(ISING_E_STATE, "single-lattice handles are not tuned");
     SYNC_ALL(p);
-    // (run width, runs per step) x steps per tile
-    static const int kL[3][2] = {{16, 1}, {16, 2}, {32, 1}}, kS[3] = {1, 2, 4};
-    double best = 1e300;
-    int bw = p->layout_width, br = p->layout_runs, bs = p->layout_spt;
-    for (int li = 0; li < 3; ++li)
-        for (int si = 0; si < 3; ++si) {
-            int rc = layout_ising(p, kL[li][0], kL[li][1], kS[si]);
+    const size_t ng = p->groups.size();
+    std::string cached;
+    const bool hit = tune_cache_read(p, &cached);
+    if (p->ncomp > 0) {
+        // continuous: sites per thread of the overrelaxation pass
+        int best_s = 0;
+        if (hit && std::sscanf(cached.c_str(), "or %d", &best_s) == 1 && (best_s == 1 || best_s == 2)) {
+            p->or_spt = best_s;
+        } else {
+            double best = 1e300;
+            for (int s2 = 1; s2 <= 2; ++s2) {
+                p->or_spt = s2;
+                double ms = 0.0;
+                const int rc = time_sweeps(p, sweeps, &ms);
+                if (rc) return rc;
+                if (ms < best) {
+                    best = ms;
+                    best_s = s2;
+                }
+            }
+            p->or_spt = best_s;
+            tune_cache_write(p, "or " + std::to_string(best_s));
+        }
+        p->drop_graph();
+        p->timed = false;
+        if (width) *width = p->or_spt;
+        if (runs) *runs = 0;
+        if (spt) *spt = 0;
+        return ISING_OK;
+    }
+    std::vector<int> cw(ng), cr(ng), cs(ng);
+    bool have = false;
+    if (hit) {
+        // "ising w,r,s w,r,s ..." one triple per size
+        const char* q = cached.c_str();
+        if (std::strncmp(q, "ising", 5) == 0) {
+            q += 5;
+            size_t g = 0;
+            int n = 0, w = 0, r = 0, s2 = 0;
+            while (g < ng && std::sscanf(q, " %d,%d,%d%n", &w, &r, &s2, &n) == 3) {
+                cw[g] = w;
+                cr[g] = r;
+                cs[g] = s2;
+                ++g;
+                q += n;
+            }
+            have = g == ng;
+        }
+    }
+    if (!have) {
+        // (run width, runs per step) x steps per tile, first for every size at once
+        static const int kL[3][2] = {{16, 1}, {16, 2}, {32, 1}}, kS[3] = {1, 2, 4};
+        double best = 1e300;
+        int bw = 16, br = 2, bs = 1;
+        for (int li = 0; li < 3; ++li)
+            for (int si = 0; si < 3; ++si) {
+                int rc = layout_ising(p, kL[li][0], kL[li][1], kS[si]);
+                if (rc) return rc;
+                double ms = 0.0;
+                rc = time_sweeps(p, sweeps, &ms);
+                if (rc) return rc;
+                if (ms < best) {
+                    best = ms;
+                    bw = kL[li][0];
+                    br = kL[li][1];
+                    bs = kS[si];
+                }
+            }
+        std::fill(cw.begin(), cw.end(), bw);
+        std::fill(cr.begin(), cr.end(), br);
+        std::fill(cs.begin(), cs.end(), bs);
+        // then size by size, largest first (coordinate descent): a size's
+        // block count changes how the launch's last round of blocks fills the device
+        const std::vector<int> uw = cw, ur = cr, us = cs;
+        if (ng > 1) {
+            int rc = layout_ising_each(p, cw.data(), cr.data(), cs.data());
             if (rc) return rc;
-            rc = p->enqueue_sweeps(1);  // first launch of a shape is not timed
+            double base = 0.0;
+            rc = time_sweeps(p, sweeps, &base);
             if (rc) return rc;
-            HIP_TRY(hipEventRecord(p->ev0, p->st));
-            rc = p->enqueue_sweeps(sweeps);
-            if (rc) return rc;
-            HIP_TRY(hipEventRecord(p->ev1, p->st));
-            HIP_TRY(hipEventSynchronize(p->ev1));
-            float ms = 0.f;
-            HIP_TRY(hipEventElapsedTime(&ms, p->ev0, p->ev1));
-            if (ms < best) {
-                best = ms;
-                bw = kL[li][0];
-                br = kL[li][1];
-                bs = kS[si];
+            std::vector<int> order(ng);
+            for (size_t i = 0; i < ng; ++i) order[i] = (int)i;
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return p->groups[x].L > p->groups[y].L; });
+            for (int g : order) {
+                std::vector<std::array<int, 3>> seen{effective_layout(p->groups[g], cw[g], cr[g], cs[g])};
+                for (int li = 0; li < 3; ++li)
+                    for (int si = 0; si < 3; ++si) {
+                        const std::array<int, 3> e = effective_layout(p->groups[g], kL[li][0], kL[li][1], kS[si]);
+                        if (std::find(seen.begin(), seen.end(), e) != seen.end()) continue;
+                        seen.push_back(e);
+                        const int ow = cw[g], orr = cr[g], os = cs[g];
+                        cw[g] = kL[li][0];
+                        cr[g] = kL[li][1];
+                        cs[g] = kS[si];
+                        rc = layout_ising_each(p, cw.data(), cr.data(), cs.data());
+                        if (rc) return rc;
+                        double ms = 0.0;
+                        rc = time_sweeps(p, sweeps, &ms);
+                        if (rc) return rc;
+                        if (ms < 0.995 * base) {
+                            base = ms;  // keep it
+                        } else {
+                            cw[g] = ow;
+                            cr[g] = orr;
+                            cs[g] = os;
+                        }
+                    }
+            }
+            // keep the per-size layouts only if they beat the common one again
+            // in interleaved timings (median of three each): single timings
+            // differ by about 1% from run to run
+            if (cw != uw || cr != ur || cs != us) {
+                double tu[3], tp[3];
+                for (int k = 0; k < 3; ++k) {
+                    rc = layout_ising_each(p, uw.data(), ur.data(), us.data());
+                    if (!rc) rc = time_sweeps(p, sweeps, &tu[k]);
+                    if (!rc) rc = layout_ising_each(p, cw.data(), cr.data(), cs.data());
+                    if (!rc) rc = time_sweeps(p, sweeps, &tp[k]);
+                    if (rc) return rc;
+                }
+                std::sort(tu, tu + 3);
+                std::sort(tp, tp + 3);
+                if (!(tp[1] < 0.995 * tu[1])) {
+                    cw = uw;
+                    cr = ur;
+                    cs = us;
+                }
             }
         }
-    const int rc = layout_ising(p, bw, br, bs);
+        std::string v = "ising";
+        for (size_t g = 0; g < ng; ++g)
+            v += " " + std::to_string(cw[g]) + "," + std::to_string(cr[g]) + "," + std::to_string(cs[g]);
+        tune_cache_write(p, v);
+    }
+    const int rc = layout_ising_each(p, cw.data(), cr.data(), cs.data());
     if (rc) return rc;
     p->timed = false;
-    if (width) *width = bw;
-    if (runs) *runs = br;
-    if (spt) *spt = bs;
+    // the largest size's layout
+    size_t gl = 0;
+    for (size_t g = 1; g < ng; ++g)
+        if (p->groups[g].L > p->groups[gl].L) gl = g;
+    if (width) *width = cw[gl];
+    if (runs) *runs = cr[gl];
+    if (spt) *spt = cs[gl];
+    return ISING_OK;
+}
+
+int ising_pt_get_layout(ising_pt_t* p, int32_t size_index, int32_t* width, int32_t* runs, int32_t* spt) {
+    if (!p) return fail(ISING_E_ARG, "null handle");
+    if (size_index < 0 || size_index >= (int)p->groups.size()) return fail(ISING_E_ARG, "size_index out of range");
+    if (p->ncomp > 0) {
+        if (width) *width = p->or_spt ? p->or_spt : (p->ncomp == 2 ? 2 : 1);
+        if (runs) *runs = 0;
+        if (spt) *spt = 0;
+        return ISING_OK;
+    }
+    if (p->glayout.empty()) return fail(ISING_E_STATE, "multispin-coded ensembles have no thread-step layout");
+    if (width) *width = p->glayout[size_index][0];
+    if (runs) *runs = p->glayout[size_index][1];
+    if (spt) *spt = p->glayout[size_index][2];
     return ISING_OK;
 }
 

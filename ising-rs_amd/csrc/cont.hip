@@ -478,15 +478,14 @@ __global__ __launch_bounds__(kCT) void cont_close_kernel(ContArgs a) {
 }
 
 // Overrelaxation pass over colour `color` (no random numbers, no renormalisation).
-// XY: two sites per thread (h and h + kCT / 2 of the block's tile), so each
-// thread has both sites' loads in flight (A/B on one device: 60.1 against
-// 57.8 Metropolis updates/ns; Heisenberg, with 21 loads per site, ran 0.6%
-// slower that way and keeps one site per thread).  The grid is the same tiles.
-template <int N>
-constexpr int kOrSites = N == 2 ? 2 : 1;
-
-template <int N>
-__global__ __launch_bounds__(kCT / kOrSites<N>) void cont_or_kernel(ContArgs a, int nblocks, int color) {
+// SPT sites per thread (h and h + kCT / SPT, ... of the block's 256-site tile), so
+// a thread has SPT sites' loads in flight; the grid is the same tiles for every S
+// and no result depends on it (each site's update reads only the other colour).
+// XY defaults to two (60.1 against 57.8 Metropolis updates/ns in an A/B run on
+// one device), Heisenberg, with 18 loads per site, to one; the autotuner times
+// both.
+template <int N, int SPT>
+__global__ __launch_bounds__(kCT / SPT) void cont_or_kernel(ContArgs a, int nblocks, int color) {
     const int b = xcd_block();
     if (b >= nblocks) return;
     const int g = find_group_c(a.groups, a.ngroups, b);
@@ -496,8 +495,8 @@ __global__ __launch_bounds__(kCT / kOrSites<N>) void cont_or_kernel(ContArgs a, 
     float* C = a.pool + G.off + (int64_t)(2 * r + color) * N * G.stride;
     const float* O = a.pool + G.off + (int64_t)(2 * r + (color ^ 1)) * N * G.stride;
 #pragma unroll
-    for (int j = 0; j < kOrSites<N>; ++j) {
-        const int64_t h = (int64_t)tile * kCT + j * (kCT / kOrSites<N>) + threadIdx.x;
+    for (int j = 0; j < SPT; ++j) {
+        const int64_t h = (int64_t)tile * kCT + j * (kCT / SPT) + threadIdx.x;
         if (h >= G.n_half) continue;
         uint32_t nb[6];
         neighbours(G, h, color, nb);
@@ -650,8 +649,11 @@ hipError_t launch_cont_close(const ContArgs& a, hipStream_t st) {
 
 hipError_t launch_cont_or(const ContArgs& a, int nblocks, int color, hipStream_t st) {
     const dim3 grid(xcd_grid(nblocks));
-    if (a.ncomp == 2) hipLaunchKernelGGL(cont_or_kernel<2>, grid, dim3(kCT / kOrSites<2>), 0, st, a, nblocks, color);
-    else hipLaunchKernelGGL(cont_or_kernel<3>, grid, dim3(kCT / kOrSites<3>), 0, st, a, nblocks, color);
+    const int spt = a.or_spt ? a.or_spt : (a.ncomp == 2 ? 2 : 1);
+    if (a.ncomp == 2 && spt == 2) hipLaunchKernelGGL((cont_or_kernel<2, 2>), grid, dim3(kCT / 2), 0, st, a, nblocks, color);
+    else if (a.ncomp == 2) hipLaunchKernelGGL((cont_or_kernel<2, 1>), grid, dim3(kCT), 0, st, a, nblocks, color);
+    else if (spt == 2) hipLaunchKernelGGL((cont_or_kernel<3, 2>), grid, dim3(kCT / 2), 0, st, a, nblocks, color);
+    else hipLaunchKernelGGL((cont_or_kernel<3, 1>), grid, dim3(kCT), 0, st, a, nblocks, color);
     return hipGetLastError();
 }
 

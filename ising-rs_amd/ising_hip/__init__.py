@@ -117,6 +117,7 @@ _SIGS = {
     "ising_pt_prepare_graph": (C.c_int, [_vp, C.c_int32]),
     "ising_pt_device_bytes": (C.c_int, [_vp, _i64p]),
     "ising_pt_autotune": (C.c_int, [_vp, C.c_int32] + [C.POINTER(C.c_int32)] * 3),
+    "ising_pt_get_layout": (C.c_int, [_vp, C.c_int32] + [C.POINTER(C.c_int32)] * 3),
     "ising_pt_helicity": (C.c_int, [_vp, _vp]),
     "ising_pt_anneal": (C.c_int, [_vp, _vp, C.c_int64, _vp, _vp]),
     "ising_pt_cluster": (C.c_int, [_vp, C.c_int32, C.c_int64, _vp, _vp, _vp]),
@@ -539,10 +540,17 @@ class PTEnsemble:
         return v.value
 
     def autotune(self, sweeps=20):
-        """Times the int8 thread-step layouts over warm-up sweeps and keeps the
-        fastest; returns (run width, runs per step, steps per tile)."""
+        """Times the int8 thread-step layouts (every size at once, then size by
+        size) or the overrelaxation sites per thread over warm-up sweeps and
+        keeps the fastest; returns the largest size's (run width, runs per step,
+        steps per tile), or (sites per thread, 0, 0) for XY / Heisenberg."""
         w, r, s = C.c_int32(), C.c_int32(), C.c_int32()
         _check(lib.ising_pt_autotune(self._h, int(sweeps), C.byref(w), C.byref(r), C.byref(s)))
+        return w.value, r.value, s.value
+
+    def get_layout(self, size_index=0):
+        w, r, s = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib.ising_pt_get_layout(self._h, int(size_index), C.byref(w), C.byref(r), C.byref(s)))
         return w.value, r.value, s.value
 
     def prepare_graph(self, graph_sweeps):

@@ -159,15 +159,70 @@ def test_autotune_keeps_the_chain():
     tuned = ising_hip.PTEnsemble("ising", [32, 64], 4, temps, 9, exchange_every=3, measure_every=2)
     width, runs, spt = tuned.autotune(3)
     assert (width, runs) in ((16, 1), (16, 2), (32, 1)) and spt in (1, 2, 4)
-    assert tuned.sweeps() == 9 * (3 + 1)  # nine layouts, one untimed sweep each
+    assert tuned.get_layout(1) == (width, runs, spt)  # the largest size's
+    n = tuned.sweeps()
+    assert n >= 9 * (3 + 1)  # nine layouts for both sizes, then size by size; one untimed sweep each
     plain = ising_hip.PTEnsemble("ising", [32, 64], 4, temps, 9, exchange_every=3, measure_every=2)
-    plain.warmup(36)
+    plain.warmup(n)
     for p in (tuned, plain):
         p.sample(10)
     for x, y in zip(tuned.read_ising(), plain.read_ising()):
         assert np.array_equal(x, y)
     with pytest.raises(ising_hip.IsingError):
         tuned.autotune(1)  # sampling has started
+
+
+def test_autotune_per_size_and_cache(tmp_path, monkeypatch):
+    """Per-size layouts keep the chain; the tuning cache applies a stored choice
+    without timing anything (no sweeps run) for the same model, replicas,
+    sizes and device."""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("ISING_HIP_TUNE_CACHE", str(cache))
+    sizes, R = [8, 12, 16, 32, 64], 4
+    temps = o.linspace(4.40, 4.62, R)
+    kw = dict(exchange_every=3, measure_every=2)
+    tuned = ising_hip.PTEnsemble("ising", sizes, R, temps, 5, **kw)
+    tuned.autotune(2)
+    layouts = [tuned.get_layout(g) for g in range(len(sizes))]
+    for w, r, s_ in layouts:
+        assert (w, r) in ((16, 1), (16, 2), (32, 1)) and s_ in (1, 2, 4)
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1 and "sizes=8,12,16,32,64" in lines[0] and "R=4" in lines[0]
+    n = tuned.sweeps()
+    again = ising_hip.PTEnsemble("ising", sizes, R, temps, 5, **kw)
+    assert again.autotune(2) == layouts[-1]
+    assert again.sweeps() == 0  # nothing timed
+    assert [again.get_layout(g) for g in range(len(sizes))] == layouts
+    plain = ising_hip.PTEnsemble("ising", sizes, R, temps, 5, **kw)
+    plain.warmup(n)
+    again.warmup(n)
+    for p in (tuned, plain, again):
+        p.sample(12)
+    ref = plain.read_ising()
+    for p in (tuned, again):
+        for x, y in zip(p.read_ising(), ref):
+            assert np.array_equal(x, y)
+    other = ising_hip.PTEnsemble("ising", [8, 16], R, temps, 5, **kw)
+    other.autotune(1)
+    assert other.sweeps() > 0 and len(cache.read_text().splitlines()) == 2  # another key
+
+
+@pytest.mark.parametrize("model", ["xy", "heisenberg"])
+def test_continuous_autotune_keeps_the_chain(model):
+    """The overrelaxation pass's sites per thread do not change any result."""
+    temps = np.linspace(2.15, 2.25, 4) if model == "xy" else np.linspace(1.40, 1.50, 4)
+    kw = dict(exchange_every=3, measure_every=2, n_overrelax=2, delta=1.0)
+    tuned = ising_hip.PTEnsemble(model, [8, 16], 4, temps, 6, **kw)
+    spt, _, _ = tuned.autotune(3)
+    assert spt in (1, 2) and tuned.get_layout(0)[0] == spt
+    plain = ising_hip.PTEnsemble(model, [8, 16], 4, temps, 6, **kw)
+    plain.warmup(tuned.sweeps())
+    for p in (tuned, plain):
+        p.sample(10)
+    for x, y in zip(tuned.read_cont(), plain.read_cont()):
+        assert np.array_equal(x, y)
+    for g in range(2):
+        assert np.array_equal(tuned.get_spins(g, 1), plain.get_spins(g, 1))
 
 
 def test_full_size_config_properties():
