@@ -614,6 +614,8 @@ def main():
     ap.add_argument("--layout", default="auto", metavar="W,R,S",
                     help="int8 thread-step layout (run width, runs per step, steps per tile); "
                          "'auto' times all layouts during warm-up and keeps the fastest")
+    ap.add_argument("--sizes", default=None, metavar="L,L,...",
+                    help="single-model workloads: lattice sizes in place of the default ladder (profiling runs)")
     ap.add_argument("--no-beyond-mall", action="store_true",
                     help="ising workload: skip the L=256 x 32 sub-result (working set above the Infinity Cache)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -646,6 +648,10 @@ def main():
         run_cluster(args, ising_hip)
         return
     model, sizes, (tlo, thi), bpu, dtype, opts, desc = WORKLOADS[args.workload]
+    if args.sizes:
+        sizes = [int(v) for v in args.sizes.split(",")]
+        desc = (desc.replace("L=8,16,32,64,128", "L=" + args.sizes).replace("L=8..128", "L=" + args.sizes)
+                .replace("L=16,32,64", "L=" + args.sizes))
     temps = linspace(tlo, thi, R)
     me = opts["measure_every"]
     if model == "ising" and args.layout != "auto":
@@ -717,7 +723,7 @@ def main():
                    "site_passes_per_step": passes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic() if args.workload == "ising" and not fused else None,
+                     "traffic": load_traffic() if args.workload == "ising" and not fused and not args.sizes else None,
                      "kernel": kernel, "alg_bytes_per_launch": alg_bytes_per_launch,
                      "avg_launch_us": round(avg_launch_s * 1e6, 3), "launches": launches,
                      "device_updates_per_ns": round(updates / (ev_ms * 1e6), 3)},
@@ -729,7 +735,7 @@ def main():
         if model == "xy":
             # stored as two FP32 components (24 B per update); one FP32 angle would be 12 B
             line["roofline"]["frac_at_12B_angle"] = round(achieved / 2 / HBM_PEAK_GBS, 4)
-    if args.workload == "ising" and not fused and not args.no_beyond_mall:
+    if args.workload == "ising" and not fused and not args.no_beyond_mall and not args.sizes:
         line["config"]["beyond_mall"] = beyond_mall(args, ising_hip, opts, layout)
     if layout is not None and model == "ising":
         line["config"]["layout"] = {"run_width": layout[0], "runs_per_step": layout[1], "steps_per_tile": layout[2],

@@ -435,7 +435,9 @@ int ising_jackknife(const double* e, const double* m, int64_t n, double beta, do
  * observables, each an array of N values (observable-major).  Iterates the
  * free energies until max |df| < tol or max_iter (f_out[K], f_out[0] = 0,
  * *iters = iterations used), then writes the reweighted means
- * out[t * n_obs + j] = <obs_j> at beta_targets[t].  K <= 1024, n_obs <= 8. */
+ * out[t * n_obs + j] = <obs_j> at beta_targets[t], with the per-sample
+ * denominators of the last iteration (computed from the free energies before
+ * its update), as the reference does.  K <= 1024, n_obs <= 8. */
 int ising_wham(int32_t K, const int64_t* n_k, const double* betas, const double* energies, int32_t n_obs,
                const double* obs, int32_t max_iter, double tol, double* f_out, int32_t* iters, int32_t n_targets,
                const double* beta_targets, double* out);

@@ -2867,9 +2867,13 @@ int ising_wham(int32_t K, const int64_t* n_k, const double* betas, const double*
         f = fn;
         if (diff < tol) break;
     }
-    // log D_n with the final f (reweighting.py:101 uses the denominator of the last iteration)
-    if (e == hipSuccess) e = hipMemcpyAsync(dF, f.data(), sizeof(double) * K, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = launch_wham_denom(dE, N, K, dlogN, dF, dB, dD, st);
+    // the weights use log D_n of the last iteration, computed from the free
+    // energies before its update, as reweighting.py:101 does (dD holds it);
+    // with no iteration at all, D_n of the initial f = 0
+    if (e == hipSuccess && it == 0) {
+        e = hipMemcpyAsync(dF, f.data(), sizeof(double) * K, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = launch_wham_denom(dE, N, K, dlogN, dF, dB, dD, st);
+    }
     if (e == hipSuccess && n_targets > 0) {
         e = hipMemcpyAsync(dT, beta_targets, sizeof(double) * n_targets, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = launch_wham_reduce(dE, N, dD, dT, n_targets, n_obs, dO, dP, nchunk, dOut, st);
