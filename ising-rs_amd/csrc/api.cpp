@@ -154,6 +154,7 @@ struct ising_pt {
     int64_t learn_step = 0;
     float* d_lmaster = nullptr;
     uint16_t* d_lnet = nullptr;
+    uint4* d_lfrag = nullptr;  // d_lnet as per-lane matrix-core fragments (rebuilt after every parameter change)
     float *d_lm1 = nullptr, *d_lm2 = nullptr, *d_lgpart = nullptr, *d_lgout = nullptr;
     double* d_llpart = nullptr;
     int learn_cap_blocks = 0;
@@ -218,6 +219,7 @@ struct ising_pt {
         (void)hipFree(d_cnt);
         (void)hipFree(d_lmaster);
         (void)hipFree(d_lnet);
+        (void)hipFree(d_lfrag);
         (void)hipFree(d_lm1);
         (void)hipFree(d_lm2);
         (void)hipFree(d_lgpart);
@@ -301,6 +303,7 @@ struct ising_pt {
         a.delta_inline = delta_inline;
         a.omcd_inline = (float)(1.0 - std::cos((double)delta_inline));
         a.net = d_lnet;
+        a.lfrag = d_lfrag;
         a.mh = d_mh;
         a.learn_H = learn_H;
         a.or_spt = or_spt;
@@ -675,7 +678,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
         HIP_TRY(hipMalloc(&p->d_part0, sizeof(double) * 4 * (size_t)p->nblocks));
         HIP_TRY(hipMalloc(&p->d_part1, sizeof(double) * 4 * (size_t)p->nblocks));
         HIP_TRY(hipMalloc(&p->d_dacc, sizeof(double) * 4 * (size_t)p->total_reps));
-        HIP_TRY(hipMalloc(&p->d_cnt, sizeof(int64_t) * (size_t)p->total_reps));
+        HIP_TRY(hipMalloc(&p->d_cnt, sizeof(int64_t) * kCntSlots * (size_t)p->total_reps));
     }
 
     std::vector<uint32_t> thr(kIsingTab * (size_t)p->total_reps);
@@ -742,7 +745,7 @@ int build_ensemble(ising_pt* p, int model, const int32_t* sizes, int n_sizes, in
         ST_FILL(p, p->d_part0, 0, sizeof(double) * 4 * (size_t)p->nblocks);
         ST_FILL(p, p->d_part1, 0, sizeof(double) * 4 * (size_t)p->nblocks);
         ST_FILL(p, p->d_dacc, 0, sizeof(double) * 4 * (size_t)p->total_reps);
-        ST_FILL(p, p->d_cnt, 0, sizeof(int64_t) * (size_t)p->total_reps);
+        ST_FILL(p, p->d_cnt, 0, sizeof(int64_t) * kCntSlots * (size_t)p->total_reps);
         HIP_TRY(launch_cont_init(p->d_groups, (int)p->groups.size(), p->d_fpool, p->ncomp,
                                  model == ISING_MODEL_XY ? MODEL_XY : MODEL_HEIS, p->nblocks, (uint32_t)seed,
                                  (uint32_t)(seed >> 32), p->st));
@@ -1101,11 +1104,12 @@ int ising_pt_learn_init(ising_pt_t* p, int32_t hidden, uint64_t seed) {
     if (P < 0) return fail(ISING_E_ARG, "hidden width must be 8, 16, 32 or 64");
     SYNC_ALL(p);
     const int cap = 4096;  // training blocks per step (64 samples each)
-    (void)hipFree(p->d_lmaster); (void)hipFree(p->d_lnet); (void)hipFree(p->d_lm1); (void)hipFree(p->d_lm2);
-    (void)hipFree(p->d_lgpart); (void)hipFree(p->d_lgout); (void)hipFree(p->d_llpart);
-    p->d_lmaster = nullptr; p->d_lnet = nullptr;
+    (void)hipFree(p->d_lmaster); (void)hipFree(p->d_lnet); (void)hipFree(p->d_lfrag); (void)hipFree(p->d_lm1);
+    (void)hipFree(p->d_lm2); (void)hipFree(p->d_lgpart); (void)hipFree(p->d_lgout); (void)hipFree(p->d_llpart);
+    p->d_lmaster = nullptr; p->d_lnet = nullptr; p->d_lfrag = nullptr;
     HIP_TRY(hipMalloc(&p->d_lmaster, sizeof(float) * P));
     HIP_TRY(hipMalloc(&p->d_lnet, sizeof(uint16_t) * P));
+    HIP_TRY(hipMalloc(&p->d_lfrag, learn_frag_bytes(hidden)));
     HIP_TRY(hipMalloc(&p->d_lm1, sizeof(float) * P));
     HIP_TRY(hipMalloc(&p->d_lm2, sizeof(float) * P));
     HIP_TRY(hipMalloc(&p->d_lgpart, sizeof(float) * (size_t)P * cap));
@@ -1145,6 +1149,7 @@ int ising_pt_learn_init(ising_pt_t* p, int32_t hidden, uint64_t seed) {
     ST_FILL(p, p->d_lm1, 0, sizeof(float) * P);
     ST_FILL(p, p->d_lm2, 0, sizeof(float) * P);
     HIP_TRY(launch_learn_pack(p->d_lmaster, p->d_lnet, P, p->st));
+    HIP_TRY(launch_learn_frag(p->d_lnet, hidden, p->d_lfrag, p->st));
     SYNC_ALL(p);
     p->drop_graph();  // captured launches carry the old network pointer
     return ISING_OK;
@@ -1176,6 +1181,7 @@ int ising_pt_learn_set_params(ising_pt_t* p, const float* in, int64_t n) {
     SYNC_ALL(p);
     ST_COPY(p, p->d_lmaster, in, sizeof(float) * n, hipMemcpyHostToDevice);
     HIP_TRY(launch_learn_pack(p->d_lmaster, p->d_lnet, (int)n, p->st));
+    HIP_TRY(launch_learn_frag(p->d_lnet, p->learn_H, p->d_lfrag, p->st));
     SYNC_ALL(p);
     return ISING_OK;
 }
@@ -1193,6 +1199,7 @@ int ising_pt_learn_train(ising_pt_t* p, int32_t steps, int32_t batch, double lr,
         HIP_TRY(launch_learn_adam(p->d_lmaster, p->d_lnet, p->d_lm1, p->d_lm2, p->d_lgpart, nblk, p->learn_P,
                                   1.0f / (float)(nblk * kLearnBlock), (float)lr, p->learn_step, nullptr, p->st));
     }
+    if (steps > 0) HIP_TRY(launch_learn_frag(p->d_lnet, p->learn_H, p->d_lfrag, p->st));
     if (mean_loss) {
         std::vector<double> lp(nblk);
         HIP_TRY(hipMemcpyAsync(lp.data(), p->d_llpart, sizeof(double) * nblk, hipMemcpyDeviceToHost, p->st));
@@ -1247,7 +1254,7 @@ int ising_pt_learn_infer(ising_pt_t* p, int64_t n, const float* x, float* out) {
     HIP_TRY(hipMalloc(&dx, sizeof(float) * 6 * n));
     hipError_t e = hipMalloc(&dout, sizeof(float) * 3 * n);
     if (e == hipSuccess) e = hipMemcpyAsync(dx, x, sizeof(float) * 6 * n, hipMemcpyHostToDevice, p->st);
-    if (e == hipSuccess) e = launch_learn_infer(p->d_lnet, p->learn_H, dx, n, dout, p->st);
+    if (e == hipSuccess) e = launch_learn_infer(p->d_lfrag, p->learn_H, dx, n, dout, p->st);
     if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, p->st);
     if (e == hipSuccess) e = hipStreamSynchronize(p->st);
     (void)hipFree(dx);

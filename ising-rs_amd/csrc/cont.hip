@@ -189,7 +189,10 @@ __device__ void close_cont(const ContArgs& a, int64_t t, int32_t fl) {
             while (g + 1 < a.ngroups && gr >= a.groups[g + 1].rep0) ++g;
             const DevGroup G = a.groups[g];
             const double sites = (double)G.n_half * 2.0 * (double)a.adapt_every;
-            const double rate = (double)__hip_atomic_load(&a.acc_cnt[gr], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) / sites;
+            int64_t* cnt = a.acc_cnt + (int64_t)gr * kCntSlots;
+            int64_t n = 0;
+            for (int k = 0; k < kCntSlots; ++k) n += __hip_atomic_load(&cnt[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const double rate = (double)n / sites;
             const int ti = G.rep0 + a.r2t[gr];
             double f = rate / (double)a.adapt_target;
             f = f < 0.5 ? 0.5 : (f > 2.0 ? 2.0 : f);
@@ -197,7 +200,7 @@ __device__ void close_cont(const ContArgs& a, int64_t t, int32_t fl) {
             d = d < 1e-3 ? 1e-3 : (d > 3.14159265358979 ? 3.14159265358979 : d);
             a.deltas[ti] = (float)d;
             a.omcd[ti] = (float)(1.0 - cos((double)(float)d));
-            a.acc_cnt[gr] = 0;
+            for (int k = 0; k < kCntSlots; ++k) cnt[k] = 0;
         }
     }
     if (tid == 0) {
@@ -316,7 +319,9 @@ __global__ __launch_bounds__(kCT) void cont_metro_kernel(ContArgs a, int nblocks
     }
     if (count) {
         const int wc = wave_sum(acc_n);
-        if ((threadIdx.x & 63) == 0 && wc) atomicAdd((unsigned long long*)&a.acc_cnt[grep], (unsigned long long)wc);
+        if ((threadIdx.x & 63) == 0 && wc)
+            atomicAdd((unsigned long long*)&a.acc_cnt[(int64_t)grep * kCntSlots + ((b * (kCT / 64) + (threadIdx.x >> 6)) & (kCntSlots - 1))],
+                      (unsigned long long)wc);
     }
     if (need) {
         block_sum4(part, s_red);
@@ -439,7 +444,9 @@ __global__ __launch_bounds__(kCT / 2) void cont_metro_pair_kernel(ContArgs a, in
     }
     if (count) {
         const int wc = wave_sum(acc_n);
-        if ((threadIdx.x & 63) == 0 && wc) atomicAdd((unsigned long long*)&a.acc_cnt[grep], (unsigned long long)wc);
+        if ((threadIdx.x & 63) == 0 && wc)
+            atomicAdd((unsigned long long*)&a.acc_cnt[(int64_t)grep * kCntSlots + ((b * (kCT / 128) + (threadIdx.x >> 6)) & (kCntSlots - 1))],
+                      (unsigned long long)wc);
     }
     if (need) {
         block_sum4<kCT / 2>(part, s_red);

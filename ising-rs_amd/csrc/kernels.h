@@ -51,6 +51,7 @@ hipError_t launch_msc_measure(const DevGroup* groups, int ngroups, const uint32_
                               int64_t* acc, hipStream_t st);
 
 constexpr int kMhSlots = 64;  // counter copies, so that waves do not contend on one address
+constexpr int kCntSlots = 64;  // acceptance counter copies per replica (same reason), summed at adaptation
 struct ContArgs {
     const DevGroup* groups;
     int32_t ngroups;
@@ -68,7 +69,7 @@ struct ContArgs {
     double* part0;         // [block][4] colour-0 partials (M)
     double* part1;         // [block][4] colour-1 partials (E, M)
     double* dacc;          // [global replica][4] E, Mx, My, Mz of the last measured sweep
-    int64_t* acc_cnt;      // [global replica] accepted Metropolis moves (warm-up adaptation)
+    int64_t* acc_cnt;      // [global replica][kCntSlots] accepted Metropolis moves (warm-up adaptation)
     ContRec* rec;
     uint32_t k0, k1;
     double J;
@@ -79,6 +80,7 @@ struct ContArgs {
     int32_t inline_params; // 1: beta/delta from the two fields below (single-lattice passes)
     float beta_inline, delta_inline, omcd_inline;
     const uint16_t* net;   // learned proposal: BF16 parameters (null when off)
+    const uint4* lfrag;    // the same network as per-lane matrix-core fragments (learn_frag_bytes(H))
     int64_t* mh;           // learned proposal: [kMhSlots][8] moves tried (0) / accepted (1), spread by wave
     int32_t learn_H;       // hidden width of the learned proposal's MLP
     int32_t or_spt;        // sites per thread of the overrelaxation pass (1 or 2; 0: the model's default)
@@ -95,7 +97,9 @@ hipError_t launch_learn_grad(const ContArgs& a, int nblk, int64_t step, int mode
 hipError_t launch_learn_adam(float* master, uint16_t* net, float* m1, float* m2, const float* gpart, int nblk, int P,
                              float inv_n, float lr, int64_t step, float* gout, hipStream_t st);
 hipError_t launch_learn_pack(const float* master, uint16_t* net, int P, hipStream_t st);
-hipError_t launch_learn_infer(const uint16_t* net, int H, const float* x, int64_t n, float* out, hipStream_t st);
+hipError_t launch_learn_infer(const uint4* frag, int H, const float* x, int64_t n, float* out, hipStream_t st);
+size_t learn_frag_bytes(int H);  // bytes of the fragment buffer of hidden width H (0 if unsupported)
+hipError_t launch_learn_frag(const uint16_t* net, int H, uint4* frag, hipStream_t st);  // fragments from BF16 parameters
 
 hipError_t launch_ising_close(const IsingArgs& a, hipStream_t st);
 hipError_t launch_cont_metro(const ContArgs& a, int nblocks, int color, int closes, hipStream_t st);

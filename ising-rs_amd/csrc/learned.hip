@@ -29,7 +29,13 @@ constexpr int kLearnThreads = 64;  // training: one wave per block, one sample p
 // v_cos_f32 / v_sin_f32 take revolutions (|error| <= 1.3e-7 on [0, 1) against FP64 on gfx950).
 __device__ __forceinline__ float cos_rev(float x) { return __builtin_amdgcn_cosf(x); }
 __device__ __forceinline__ float sin_rev(float x) { return __builtin_amdgcn_sinf(x); }
-constexpr int kVmAttempts = 8;
+// Natural log, square root and reciprocal square root straight from v_log_f32
+// (log2), v_sqrt_f32 and v_rsq_f32, for normal inputs: no denormal rescaling
+// and no correction steps.  Relative error ~1e-7; log_hw(0) = -inf.
+__device__ __forceinline__ float log_hw(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994531f; }
+__device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float rsq_hw(float x) { return __builtin_amdgcn_rsqf(x); }
+constexpr int kVmAttempts = 7;  // von Mises attempts per draw: 3 together, then 2 pairs
 constexpr int kNIn = 6;
 
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
@@ -47,7 +53,7 @@ __device__ __forceinline__ float i0e_f(float x) {
     }
     const float y = 3.75f * __builtin_amdgcn_rcpf(x);
     const float p = 0.39894228f + y * (0.01328592f + y * (0.00225319f + y * (-0.00157565f + y * (0.00916281f + y * (-0.02057706f + y * (0.02635537f + y * (-0.01647633f + y * 0.00392377f)))))));
-    return p * rsqrtf(x);
+    return p * rsq_hw(x);
 }
 __device__ __forceinline__ float i1e_f(float x) {
     if (x <= 3.75f) {
@@ -57,17 +63,17 @@ __device__ __forceinline__ float i1e_f(float x) {
     }
     const float y = 3.75f * __builtin_amdgcn_rcpf(x);
     const float p = 0.39894228f + y * (-0.03988024f + y * (-0.00362018f + y * (0.00163801f + y * (-0.01031555f + y * (0.02282967f + y * (-0.02895312f + y * (0.01787654f + y * -0.00420059f)))))));
-    return p * rsqrtf(x);
+    return p * rsq_hw(x);
 }
 
-// tanh from one exponential and one hardware reciprocal (v_rcp_f32, 1 ulp):
-// e = exp(-2|x|) in (0, 1] never overflows; absolute error ~1e-7.
+// tanh(x) = 1 - 2 / (1 + e^{2x}) from v_exp_f32 (2^y) and v_rcp_f32 (1 ulp):
+// e^{2x} = inf for large x gives 1, e^{2x} = 0 for very negative x gives -1;
+// absolute error <= 3e-7.
 __device__ __forceinline__ float tanh_f(float x) {
-    const float e = __expf(-2.0f * fabsf(x));
-    const float r = (1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e);
-    return copysignf(r, x);
+    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);  // 2 log2(e)
+    return fmaf(-2.0f, __builtin_amdgcn_rcpf(1.0f + e), 1.0f);
 }
-__device__ __forceinline__ float softplus_f(float x) { return x > 15.0f ? x : __logf(1.0f + __expf(x)); }
+__device__ __forceinline__ float softplus_f(float x) { return x > 15.0f ? x : log_hw(1.0f + __expf(x)); }
 __device__ __forceinline__ float sigmoid_f(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
 
 // log f_k(c) per component (see header).
@@ -103,20 +109,35 @@ struct MixLog {
     float k0, k1, a0, a1;
 };
 
+// log f_k(0), the sweep's form of log_comp(kap, 0) from the hardware log:
+//   XY          -kappa - log(2 pi) - log(e^-kappa I0(kappa))
+//   Heisenberg  log(kappa / (1 - e^{-2 kappa})) - kappa - log(2 pi); below
+//               kappa = 1/16 the series -kappa^2/6 + kappa^4/180 - log(4 pi)
+//               (error < 2e-11), which avoids the cancellation in 1 - e^{-2 kappa}.
+template <int N>
+__device__ __forceinline__ float log_comp0(float kap) {
+    if (N == 2) return -kap - 1.8378770664093453f - log_hw(i0e_f(kap));
+    if (kap < 0.0625f) {
+        const float k2 = kap * kap;
+        return k2 * (k2 * (1.0f / 180.0f) - (1.0f / 6.0f)) - 2.5310242469692907f;
+    }
+    return log_hw(kap) - log_hw(1.0f - __expf(-2.0f * kap)) - kap - 1.8378770664093453f;
+}
+
 template <int N>
 __device__ __forceinline__ MixLog mix_log(const Mix& m) {
     MixLog q;
     q.k0 = m.k0;
     q.k1 = m.k1;
-    q.a0 = __logf(m.w0) + log_comp<N>(m.k0, 0.0f);
-    q.a1 = __logf(1.0f - m.w0) + log_comp<N>(m.k1, 0.0f);
+    q.a0 = log_hw(m.w0) + log_comp0<N>(m.k0);
+    q.a1 = log_hw(1.0f - m.w0) + log_comp0<N>(m.k1);
     return q;
 }
 
 __device__ __forceinline__ float log_q_fast(const MixLog& q, float c) {
     const float l0 = q.a0 + q.k0 * c, l1 = q.a1 + q.k1 * c;
     const float mx = fmaxf(l0, l1);
-    return mx + __logf(__expf(l0 - mx) + __expf(l1 - mx));
+    return mx + log_hw(__expf(l0 - mx) + __expf(l1 - mx));
 }
 
 template <int N>
@@ -182,7 +203,7 @@ __device__ __forceinline__ float site_features(float beta, const float nv[3][6],
     float hn2 = 0.f;
 #pragma unroll
     for (int k = 0; k < N; ++k) hn2 += h[k] * h[k];
-    const float hn = sqrtf(hn2);
+    const float hn = sqrt_hw(hn2);
     const float inv = hn > 1e-20f ? __builtin_amdgcn_rcpf(hn) : 0.0f;
 #pragma unroll
     for (int k = 0; k < 3; ++k) hhat[k] = k < N ? h[k] * inv : 0.0f;
@@ -270,13 +291,48 @@ __device__ __forceinline__ void load_net(const uint16_t* __restrict__ w, NetFrag
     for (int r = 0; r < 4; ++r) F.b4[r] = kq + r < 3 ? bf2f(w[O::b4 + kq + r]) : 0.0f;
 }
 
+// The fragments as 16-byte words.  A network's fragments are built once per
+// parameter update (learn_frag_kernel): word k of lane l at frag[64 k + l], so
+// a wave loads each word in one coalesced 1 KiB read instead of gathering and
+// padding single BF16 weights.
+template <int H>
+struct NetWords {
+    static_assert(sizeof(NetFrag<H>) % 16 == 0, "fragments are whole 16-byte words");
+    static constexpr int K = (int)(sizeof(NetFrag<H>) / 16);
+    uint4 w[K];
+};
+
+template <int H>
+__device__ __forceinline__ void load_frag(const uint4* __restrict__ frag, NetFrag<H>& F) {
+    NetWords<H> W;
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < NetWords<H>::K; ++k) W.w[k] = frag[64 * k + l];
+    F = __builtin_bit_cast(NetFrag<H>, W);
+}
+
+template <int H>
+__global__ __launch_bounds__(64) void learn_frag_kernel(const uint16_t* __restrict__ net, uint4* __restrict__ frag) {
+    NetFrag<H> F;
+    load_net<H>(net, F);
+    const NetWords<H> W = __builtin_bit_cast(NetWords<H>, F);
+#pragma unroll
+    for (int k = 0; k < NetWords<H>::K; ++k) frag[64 * k + threadIdx.x] = W.w[k];
+}
+
 __device__ __forceinline__ short bf16_bits(float f) { return __builtin_bit_cast(short, (__bf16)f); }  // RNE
 
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+// Two floats to packed BF16 (nearest even) in one v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t pk_bf16(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((v2f){a, b}, v2bf));
+}
+
 __device__ __forceinline__ v4s tanh_bf16(v4f d) {
-    v4s r;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) r[q] = bf16_bits(tanh_f(d[q]));
-    return r;
+    const v2u p = {pk_bf16(tanh_f(d[0]), tanh_f(d[1])), pk_bf16(tanh_f(d[2]), tanh_f(d[3]))};
+    return __builtin_bit_cast(v4s, p);
 }
 
 // One tile of 16 sites: xb is this lane's input fragment (features 4 (lane / 16) .. + 3
@@ -340,10 +396,10 @@ __device__ __forceinline__ void mlp_block(const NetFrag<H>& F, const float x[kNI
 // Draws s' around the direction hh from one mixture component of concentration
 // kap, c' = s'.hh into *cn_out.  Heisenberg: c' = 1 + log(u + (1 - u) e^{-2 kappa}) / kappa
 // with u = w.y (the Fisher CDF inverted), azimuth 2 pi w.z.  XY: von Mises by
-// Best-Fisher rejection, attempts from Philox calls whose counter word 3 is tg
-// with the attempt index in bits 16 and up; returns false when all kVmAttempts
-// attempts failed (the site keeps its spin, with a probability that depends on
-// the neighbours only).
+// Best-Fisher rejection, the first attempt from words y, z of w, later ones
+// from Philox calls whose counter word 3 is tg with the call index in bits 16
+// and up; returns false when all kVmAttempts attempts failed (the site keeps
+// its spin, with a probability that depends on the neighbours only).
 template <int N>
 __device__ __forceinline__ bool draw_around(const float hh[3], float kap, const u32x4& w, uint32_t h, uint32_t hs,
                                             uint32_t stream, uint32_t tg, uint32_t k0, uint32_t k1, float nw[3],
@@ -353,7 +409,7 @@ __device__ __forceinline__ bool draw_around(const float hh[3], float kap, const 
     if (N == 3) {
         // Fisher: c = 1 + log(u + (1 - u) e^{-2 kappa}) / kappa, azimuth uniform
         const float u = u24f(w.y);
-        cn = 1.0f + __logf(u + (1.0f - u) * __expf(-2.0f * kap)) * __builtin_amdgcn_rcpf(kap);
+        cn = 1.0f + log_hw(u + (1.0f - u) * __expf(-2.0f * kap)) * __builtin_amdgcn_rcpf(kap);
         cn = fminf(1.0f, fmaxf(-1.0f, cn));
         const float sp = sin_rev(u24f(w.z)), cp = cos_rev(u24f(w.z));  // azimuth 2 pi u
         // orthonormal e1, e2 perpendicular to hh
@@ -361,49 +417,46 @@ __device__ __forceinline__ bool draw_around(const float hh[3], float kap, const 
         if (fabsf(hh[0]) < 0.9f) t0 = 1.f; else t1 = 1.f;
         const float th = t0 * hh[0] + t1 * hh[1];
         float e0 = t0 - th * hh[0], e1 = t1 - th * hh[1], e2 = -th * hh[2];
-        const float ei = rsqrtf(e0 * e0 + e1 * e1 + e2 * e2);
+        const float ei = rsq_hw(e0 * e0 + e1 * e1 + e2 * e2);
         e0 *= ei; e1 *= ei; e2 *= ei;
         const float f0 = hh[1] * e2 - hh[2] * e1, f1 = hh[2] * e0 - hh[0] * e2, f2 = hh[0] * e1 - hh[1] * e0;
-        const float sn = sqrtf(fmaxf(0.0f, 1.0f - cn * cn));
+        const float sn = sqrt_hw(fmaxf(0.0f, 1.0f - cn * cn));
         nw[0] = cn * hh[0] + sn * (cp * e0 + sp * f0);
         nw[1] = cn * hh[1] + sn * (cp * e1 + sp * f1);
         nw[2] = cn * hh[2] + sn * (cp * e2 + sp * f2);
         have = true;
     } else {
-        // von Mises by Best-Fisher rejection.  Two attempts per Philox call
-        // (words x, y and z, w: 24-bit uniforms for the angle and the test; the
-        // low bit of the first word gives the attempt's sign).  The first four
-        // attempts (two calls) are evaluated together, with no chain of
-        // dependent draws; the first accepted one wins, as if they had been
-        // made in turn.  Later attempts run pair by pair.
+        // von Mises by Best-Fisher rejection.  An attempt takes two words (24-bit
+        // uniforms for the angle and the test; the low bit of the first word
+        // gives the attempt's sign): attempt 0 words y, z of the caller's call
+        // (w), attempts 1 and 2 words x, y and z, w of one more call.  These
+        // three are evaluated together, with no chain of dependent draws, and
+        // the first accepted one wins, as if they had been made in turn.  Later
+        // attempts run pair by pair, one call per pair.
         const float sr = 0.5f * __builtin_amdgcn_rcpf(kap);
-        const float rr = sr + sqrtf(1.0f + sr * sr);
+        const float rr = sr + sqrt_hw(1.0f + sr * sr);
         auto try_vm = [&](uint32_t ua, uint32_t uv, float* W) -> bool {
             const float zc = cos_rev(0.5f * u24f(ua));  // cos(pi u)
             *W = (1.0f + rr * zc) * __builtin_amdgcn_rcpf(rr + zc);
             const float Y = kap * (rr - *W);
             const float V = fmaxf(u24f(uv), 1e-30f);
-            return Y * (2.0f - Y) - V >= 0.0f || __logf(Y) - __logf(V) + 1.0f - Y >= 0.0f;
+            return Y * (2.0f - Y) - V >= 0.0f || log_hw(Y) - log_hw(V) + 1.0f - Y >= 0.0f;
         };
-        constexpr int kCalls = 2;
-        u32x4 vb[kCalls];
-#pragma unroll
-        for (int k = 0; k < kCalls; ++k) vb[k] = philox(h, hs, stream, tg | ((uint32_t)(k + 1) << 16), k0, k1);
+        const u32x4 v1 = philox(h, hs, stream, tg | (1u << 16), k0, k1);
+        const uint32_t ua[3] = {w.y, v1.x, v1.z}, uv[3] = {w.z, v1.y, v1.w};
         float Wsel = 0.f;
         uint32_t zsel = 0u;
 #pragma unroll
-        for (int k = 2 * kCalls - 1; k >= 0; --k) {
-            const u32x4& v = vb[k >> 1];
-            const uint32_t ua = (k & 1) ? v.z : v.x, uv = (k & 1) ? v.w : v.y;
+        for (int k = 2; k >= 0; --k) {
             float Wk;
-            if (try_vm(ua, uv, &Wk)) {
+            if (try_vm(ua[k], uv[k], &Wk)) {
                 Wsel = Wk;
-                zsel = ua;
+                zsel = ua[k];
                 have = true;
             }
         }
 #pragma unroll 1
-        for (int at = kCalls + 1; 2 * at <= kVmAttempts && !have; ++at) {
+        for (int at = 2; 2 * at + 1 <= kVmAttempts && !have; ++at) {
             const u32x4 v = philox(h, hs, stream, tg | ((uint32_t)at << 16), k0, k1);
             float Wk;
             if (try_vm(v.x, v.y, &Wk)) {
@@ -418,7 +471,7 @@ __device__ __forceinline__ bool draw_around(const float hh[3], float kap, const 
         }
         if (have) {
             cn = fminf(1.0f, fmaxf(-1.0f, Wsel));
-            const float sn = sqrtf(fmaxf(0.0f, 1.0f - cn * cn)) * ((zsel & 1u) ? -1.0f : 1.0f);
+            const float sn = sqrt_hw(fmaxf(0.0f, 1.0f - cn * cn)) * ((zsel & 1u) ? -1.0f : 1.0f);
             nw[0] = cn * hh[0] - sn * hh[1];
             nw[1] = sn * hh[0] + cn * hh[1];
         }
@@ -445,7 +498,7 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int color
     const int grep = G.rep0 + r;
     const int ti = G.rep0 + a.r2t[grep];
     NetFrag<H> F;
-    load_net<H>(a.net, F);
+    load_frag<H>(a.lfrag, F);
     const float beta = a.betas_f[ti];
     const int64_t t = t_override >= 0 ? t_override : a.ctl->t;
     const int32_t fl = t_override >= 0 ? 0 : a.ctl->flags;
@@ -495,7 +548,7 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int color
                 for (int k = 0; k < N; ++k) co += S[k] * hh[k];
                 const MixLog ql = mix_log<N>(m);
                 const float lr = beta * a.Jf * hn * (cn - co) + log_q_fast(ql, co) - log_q_fast(ql, cn);
-                accept = lr >= 0.0f || __logf(fmaxf(u24f(w.w), 1e-30f)) < lr;
+                accept = lr >= 0.0f || log_hw(fmaxf(u24f(w.w), 1e-30f)) < lr;
                 tried = 1;
             }
         }
@@ -861,12 +914,12 @@ int learn_params(int H) {
 // Network outputs o[3] for explicit feature rows, evaluated exactly as the
 // fused sweep does (checks of the matrix-core path).
 template <int H>
-__global__ __launch_bounds__(256) void learn_infer_kernel(const uint16_t* __restrict__ net, const float* __restrict__ xin,
+__global__ __launch_bounds__(256) void learn_infer_kernel(const uint4* __restrict__ frag, const float* __restrict__ xin,
                                                           int64_t n, float* __restrict__ out) {
     __shared__ uint16_t s_feat[8 * 256];
     __shared__ float s_out[3 * 256];
     NetFrag<H> F;
-    load_net<H>(net, F);
+    load_frag<H>(frag, F);
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     float x[kNIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (i < n)
@@ -879,13 +932,34 @@ __global__ __launch_bounds__(256) void learn_infer_kernel(const uint16_t* __rest
         for (int r = 0; r < 3; ++r) out[i * 3 + r] = o[r];
 }
 
-hipError_t launch_learn_infer(const uint16_t* net, int H, const float* x, int64_t n, float* out, hipStream_t st) {
+hipError_t launch_learn_infer(const uint4* frag, int H, const float* x, int64_t n, float* out, hipStream_t st) {
     const int nb = (int)((n + 255) / 256);
     switch (H) {
-        case 8: hipLaunchKernelGGL(learn_infer_kernel<8>, dim3(nb), dim3(256), 0, st, net, x, n, out); break;
-        case 16: hipLaunchKernelGGL(learn_infer_kernel<16>, dim3(nb), dim3(256), 0, st, net, x, n, out); break;
-        case 32: hipLaunchKernelGGL(learn_infer_kernel<32>, dim3(nb), dim3(256), 0, st, net, x, n, out); break;
-        case 64: hipLaunchKernelGGL(learn_infer_kernel<64>, dim3(nb), dim3(256), 0, st, net, x, n, out); break;
+        case 8: hipLaunchKernelGGL(learn_infer_kernel<8>, dim3(nb), dim3(256), 0, st, frag, x, n, out); break;
+        case 16: hipLaunchKernelGGL(learn_infer_kernel<16>, dim3(nb), dim3(256), 0, st, frag, x, n, out); break;
+        case 32: hipLaunchKernelGGL(learn_infer_kernel<32>, dim3(nb), dim3(256), 0, st, frag, x, n, out); break;
+        case 64: hipLaunchKernelGGL(learn_infer_kernel<64>, dim3(nb), dim3(256), 0, st, frag, x, n, out); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+size_t learn_frag_bytes(int H) {
+    switch (H) {
+        case 8: return 64 * sizeof(NetFrag<8>);
+        case 16: return 64 * sizeof(NetFrag<16>);
+        case 32: return 64 * sizeof(NetFrag<32>);
+        case 64: return 64 * sizeof(NetFrag<64>);
+    }
+    return 0;
+}
+
+hipError_t launch_learn_frag(const uint16_t* net, int H, uint4* frag, hipStream_t st) {
+    switch (H) {
+        case 8: hipLaunchKernelGGL(learn_frag_kernel<8>, dim3(1), dim3(64), 0, st, net, frag); break;
+        case 16: hipLaunchKernelGGL(learn_frag_kernel<16>, dim3(1), dim3(64), 0, st, net, frag); break;
+        case 32: hipLaunchKernelGGL(learn_frag_kernel<32>, dim3(1), dim3(64), 0, st, net, frag); break;
+        case 64: hipLaunchKernelGGL(learn_frag_kernel<64>, dim3(1), dim3(64), 0, st, net, frag); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
