@@ -244,7 +244,7 @@ int ising_pt_set_spins_f32(ising_pt_t* pt, int32_t size_index, int32_t replica, 
 /* Current proposal size per temperature of one size (continuous models). */
 int ising_pt_get_deltas(ising_pt_t* pt, int32_t size_index, float* deltas);
 /* ---- Learned proposal (continuous models; no counterpart in the reference) ----
- * An MLP (6 rotation-invariant neighbour features -> H -> H -> H -> 3, tanh,
+ * An MLP (6 rotation-invariant neighbour features -> H -> H -> H -> 3, ReLU,
  * BF16 weights, FP32 accumulation, H in {8, 16, 32, 64}) gives a two-component
  * von Mises (XY) / Fisher (Heisenberg) mixture around the local field; moves
  * are accepted by exact Metropolis-Hastings including the proposal ratio.

@@ -1123,7 +1123,8 @@ int ising_pt_learn_init(ising_pt_t* p, int32_t hidden, uint64_t seed) {
     p->learn_P = P;
     p->learn_cap_blocks = cap;
     p->learn_step = 0;
-    // W ~ N(0, 1/fan_in) (output layer 0.1 of that), biases 0; Box-Muller on splitmix64
+    // hidden layers W ~ N(0, 2/fan_in) (He, for ReLU), output layer N(0, 0.01/fan_in), biases 0;
+    // Box-Muller on splitmix64
     std::vector<float> w(P, 0.0f);
     uint64_t state = seed ^ 0x5DEECE66DULL;
     auto next = [&]() {
@@ -1138,11 +1139,11 @@ int ising_pt_learn_init(ising_pt_t* p, int32_t hidden, uint64_t seed) {
     };
     const int H = hidden;
     int o = 0;
-    for (int i = 0; i < H * 6; ++i) w[o++] = (float)(gauss() / std::sqrt(6.0));
+    for (int i = 0; i < H * 6; ++i) w[o++] = (float)(gauss() * std::sqrt(2.0 / 6.0));
     o += H;
-    for (int i = 0; i < H * H; ++i) w[o++] = (float)(gauss() / std::sqrt((double)H));
+    for (int i = 0; i < H * H; ++i) w[o++] = (float)(gauss() * std::sqrt(2.0 / H));
     o += H;
-    for (int i = 0; i < H * H; ++i) w[o++] = (float)(gauss() / std::sqrt((double)H));
+    for (int i = 0; i < H * H; ++i) w[o++] = (float)(gauss() * std::sqrt(2.0 / H));
     o += H;
     for (int i = 0; i < 3 * H; ++i) w[o++] = (float)(0.1 * gauss() / std::sqrt((double)H));
     ST_COPY(p, p->d_lmaster, w.data(), sizeof(float) * P, hipMemcpyHostToDevice);

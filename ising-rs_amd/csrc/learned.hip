@@ -3,7 +3,7 @@
 // A small MLP maps rotation-invariant features of a site's six neighbours to a
 // two-component mixture around the local-field direction h^ = h/|h|:
 //   x = [beta, beta|h|, |h|/6, <q^2>, <q^3>, <q^4>],  q_j = S_j . h^
-//   6 -> H (tanh) -> H (tanh) -> H (tanh) -> 3:   kappa_k = softplus(o_k) + 1e-3,
+//   6 -> H (ReLU) -> H (ReLU) -> H (ReLU) -> 3:   kappa_k = softplus(o_k) + 1e-3,
 //   w_1 = sigmoid(o_2), w_2 = 1 - w_1
 // Components, in c = s . h^:  XY von Mises  f = e^{kappa c} / (2 pi I0(kappa)),
 //                             Heisenberg Fisher f = kappa e^{kappa c} / (4 pi sinh kappa).
@@ -66,13 +66,14 @@ __device__ __forceinline__ float i1e_f(float x) {
     return p * rsq_hw(x);
 }
 
-// tanh(x) = 1 - 2 / (1 + e^{2x}) from v_exp_f32 (2^y) and v_rcp_f32 (1 ulp):
-// e^{2x} = inf for large x gives 1, e^{2x} = 0 for very negative x gives -1;
-// absolute error <= 3e-7.
-__device__ __forceinline__ float tanh_f(float x) {
-    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);  // 2 log2(e)
-    return fmaf(-2.0f, __builtin_amdgcn_rcpf(1.0f + e), 1.0f);
-}
+// Hidden activation: ReLU (tanh would take an exponential, a reciprocal and
+// three more operations, for 48 activations per site at H = 16).  One
+// v_max_i32 on the bits: a float with the sign bit set is a negative int32, so
+// max(bits, 0) keeps z > 0 and gives +0 otherwise (fmaxf would add a NaN
+// canonicalisation).  Its derivative from the activation itself: 1 where a > 0,
+// else 0.
+__device__ __forceinline__ float act_f(float z) { return __int_as_float(max(__float_as_int(z), 0)); }
+__device__ __forceinline__ float act_grad(float a) { return a > 0.0f ? 1.0f : 0.0f; }
 __device__ __forceinline__ float softplus_f(float x) { return x > 15.0f ? x : log_hw(1.0f + __expf(x)); }
 __device__ __forceinline__ float sigmoid_f(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
 
@@ -166,7 +167,7 @@ __device__ __forceinline__ void mlp_forward(const uint16_t* __restrict__ w, cons
         float s = bf2f(w[F::b1 + j]);
 #pragma unroll
         for (int i = 0; i < kNIn; ++i) s += bf2f(w[F::W1 + j * kNIn + i]) * x[i];
-        a[j] = tanh_f(s);
+        a[j] = act_f(s);
         if (act) act[(0 * H + j) * act_stride] = a[j];
     }
 #pragma unroll
@@ -174,7 +175,7 @@ __device__ __forceinline__ void mlp_forward(const uint16_t* __restrict__ w, cons
         float s = bf2f(w[F::b2 + j]);
 #pragma unroll
         for (int i = 0; i < H; ++i) s += bf2f(w[F::W2 + j * H + i]) * a[i];
-        b[j] = tanh_f(s);
+        b[j] = act_f(s);
         if (act) act[(1 * H + j) * act_stride] = b[j];
     }
 #pragma unroll
@@ -182,7 +183,7 @@ __device__ __forceinline__ void mlp_forward(const uint16_t* __restrict__ w, cons
         float s = bf2f(w[F::b3 + j]);
 #pragma unroll
         for (int i = 0; i < H; ++i) s += bf2f(w[F::W3 + j * H + i]) * b[i];
-        a[j] = tanh_f(s);
+        a[j] = act_f(s);
         if (act) act[(2 * H + j) * act_stride] = a[j];
     }
 #pragma unroll
@@ -330,8 +331,8 @@ __device__ __forceinline__ uint32_t pk_bf16(float a, float b) {
     return __builtin_bit_cast(uint32_t, __builtin_convertvector((v2f){a, b}, v2bf));
 }
 
-__device__ __forceinline__ v4s tanh_bf16(v4f d) {
-    const v2u p = {pk_bf16(tanh_f(d[0]), tanh_f(d[1])), pk_bf16(tanh_f(d[2]), tanh_f(d[3]))};
+__device__ __forceinline__ v4s act_bf16(v4f d) {
+    const v2u p = {pk_bf16(act_f(d[0]), act_f(d[1])), pk_bf16(act_f(d[2]), act_f(d[3]))};
     return __builtin_bit_cast(v4s, p);
 }
 
@@ -343,20 +344,20 @@ __device__ __forceinline__ v4f mlp_tile(const NetFrag<H>& F, v4s xb) {
     const v4f zero = {0.f, 0.f, 0.f, 0.f};
     v4s a1[T], a2[T];
 #pragma unroll
-    for (int mt = 0; mt < T; ++mt) a1[mt] = tanh_bf16(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.w1[mt], xb, zero, 0, 0, 0));
+    for (int mt = 0; mt < T; ++mt) a1[mt] = act_bf16(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.w1[mt], xb, zero, 0, 0, 0));
 #pragma unroll
     for (int mt = 0; mt < T; ++mt) {
         v4f d = F.b2[mt];
 #pragma unroll
         for (int kt = 0; kt < T; ++kt) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.w2[mt][kt], a1[kt], d, 0, 0, 0);
-        a2[mt] = tanh_bf16(d);
+        a2[mt] = act_bf16(d);
     }
 #pragma unroll
     for (int mt = 0; mt < T; ++mt) {
         v4f d = F.b3[mt];
 #pragma unroll
         for (int kt = 0; kt < T; ++kt) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.w3[mt][kt], a2[kt], d, 0, 0, 0);
-        a1[mt] = tanh_bf16(d);
+        a1[mt] = act_bf16(d);
     }
     v4f d = F.b4;
 #pragma unroll
@@ -808,12 +809,12 @@ __global__ __launch_bounds__(kLearnThreads) void learn_grad_kernel(ContArgs a, i
         for (int u = 0; u < kLearnThreads; ++u) sacc += s_do[t][u];
         gb[F::b4 + t] = sacc;
     }
-    // d z3 = (W4^T do) (1 - a3^2)
+    // d z3 = (W4^T do) act'(a3)
     for (int i = 0; i < H; ++i) {
         float v = 0.f;
         for (int k = 0; k < 3; ++k) v += bf2f(s_w[F::W4 + k * H + i]) * s_do[k][t];
         const float ai = s_a[2][i][t];
-        s_d[i][t] = v * (1.0f - ai * ai);
+        s_d[i][t] = v * act_grad(ai);
     }
     __syncthreads();
     // layer 3 weights from d z3 and a2, then d z2 into s_a[2] (a3 no longer needed)
@@ -833,7 +834,7 @@ __global__ __launch_bounds__(kLearnThreads) void learn_grad_kernel(ContArgs a, i
         float v = 0.f;
         for (int j = 0; j < H; ++j) v += bf2f(s_w[F::W3 + j * H + i]) * s_d[j][t];
         const float ai = s_a[1][i][t];
-        s_a[2][i][t] = v * (1.0f - ai * ai);
+        s_a[2][i][t] = v * act_grad(ai);
     }
     __syncthreads();
     // layer 2 weights from d z2 (in s_a[2]) and a1; then d z1 into s_d
@@ -853,7 +854,7 @@ __global__ __launch_bounds__(kLearnThreads) void learn_grad_kernel(ContArgs a, i
         float v = 0.f;
         for (int j = 0; j < H; ++j) v += bf2f(s_w[F::W2 + j * H + i]) * s_a[2][j][t];
         const float ai = s_a[0][i][t];
-        s_d[i][t] = v * (1.0f - ai * ai);
+        s_d[i][t] = v * act_grad(ai);
     }
     __syncthreads();
     for (int e = t; e < H * kNIn; e += kLearnThreads) {

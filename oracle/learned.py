@@ -7,7 +7,7 @@ checked against:
 * features of a site from its 6 neighbours, in the frame of the local field
   h = sum_j S_j:  x = [beta, beta|h|, |h|/6, <q^2>, <q^3>, <q^4>],
   q_j = S_j . h/|h|  (rotation invariant, neighbour-order invariant);
-* MLP: 6 -> H (tanh) -> H (tanh) -> H (tanh) -> 3 outputs o;
+* MLP: 6 -> H (ReLU) -> H (ReLU) -> H (ReLU) -> 3 outputs o;
   weights W1 [H,6], W2 [H,H], W3 [H,H], W4 [3,H], biases b1..b4;
 * two-component mixture around h/|h|: kappa_k = softplus(o_k) + 1e-3,
   w_1 = sigmoid(o_2), w_2 = 1 - w_1; component densities in c = s . h/|h|:
@@ -36,18 +36,36 @@ def features(beta, nbrs):
 def init_params(H, seed):
     rng = np.random.default_rng(seed)
     p = {
-        "W1": rng.normal(0, 1 / np.sqrt(6), (H, 6)), "b1": np.zeros(H),
-        "W2": rng.normal(0, 1 / np.sqrt(H), (H, H)), "b2": np.zeros(H),
-        "W3": rng.normal(0, 1 / np.sqrt(H), (H, H)), "b3": np.zeros(H),
+        "W1": rng.normal(0, np.sqrt(2 / 6), (H, 6)), "b1": np.zeros(H),
+        "W2": rng.normal(0, np.sqrt(2 / H), (H, H)), "b2": np.zeros(H),
+        "W3": rng.normal(0, np.sqrt(2 / H), (H, H)), "b3": np.zeros(H),
         "W4": rng.normal(0, 0.1 / np.sqrt(H), (3, H)), "b4": np.zeros(3),
     }
     return p
 
 
+def relu(z):
+    return np.maximum(z, 0.0)
+
+
+def pre_activations(p, x):
+    """Hidden pre-activations z1, z2, z3 (the ReLU kinks are where they vanish)."""
+    z1 = x @ p["W1"].T + p["b1"]
+    z2 = relu(z1) @ p["W2"].T + p["b2"]
+    z3 = relu(z2) @ p["W3"].T + p["b3"]
+    return z1, z2, z3
+
+
+def kink_margin(p, x):
+    """Per sample, the smallest |pre-activation| over every hidden unit: central
+    differences of the loss with steps well below it never cross a ReLU kink."""
+    return np.min(np.abs(np.concatenate(pre_activations(p, x), axis=1)), axis=1)
+
+
 def forward(p, x):
-    a1 = np.tanh(x @ p["W1"].T + p["b1"])
-    a2 = np.tanh(a1 @ p["W2"].T + p["b2"])
-    a3 = np.tanh(a2 @ p["W3"].T + p["b3"])
+    a1 = relu(x @ p["W1"].T + p["b1"])
+    a2 = relu(a1 @ p["W2"].T + p["b2"])
+    a3 = relu(a2 @ p["W3"].T + p["b3"])
     o = a3 @ p["W4"].T + p["b4"]
     return o, (a1, a2, a3)
 

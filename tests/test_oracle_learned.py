@@ -1,7 +1,9 @@
 """Pins the FP64 restatement of the learned proposal (oracle/learned.py):
 component densities normalise, their mean cosines match numerical integrals,
 the Metropolis-Hastings ratio satisfies detailed balance, and the finite-
-difference gradient agrees with directional derivatives."""
+difference gradient agrees with directional derivatives (on samples whose
+hidden pre-activations all stay clear of the ReLU kinks, where the loss is
+smooth)."""
 import math
 import os
 import sys
@@ -66,6 +68,9 @@ def test_finite_difference_gradient_matches_directional_derivatives():
         s = hhat + 0.5 * rng.normal(size=hhat.shape)
         s /= np.linalg.norm(s, axis=1, keepdims=True)
         c = (s * hhat).sum(axis=1)
+        keep = ref.kink_margin(p, x) > 1e-3
+        assert keep.sum() > 48
+        x, c = x[keep], c[keep]
         g = ref.pack_params(ref.grad_fd(p, ncomp, x, c), H)
         flat = ref.pack_params(p, H)
         for _ in range(3):
