@@ -298,7 +298,7 @@ def binder_jk(m2, nb=20):
 def chain_stats(pt, model, sizes, Rr, temps, steps, ising_hip):
     """Samples `steps` sweeps (in chunks; XY helicity sums between chunks, outside
     the timed sampling) and returns per-(size, temperature) statistics."""
-    chunk, wall, done = 10, 0.0, 0
+    chunk, wall, dev_ms, done = 10, 0.0, 0.0, 0
     hc = np.zeros((len(sizes), Rr))
     hj2 = np.zeros((len(sizes), Rr))
     nh = 0
@@ -306,7 +306,7 @@ def chain_stats(pt, model, sizes, Rr, temps, steps, ising_hip):
         m = min(chunk, steps - done)
         t0 = time.perf_counter()
         pt.sample(m)
-        pt.sync()
+        dev_ms += pt.sync()
         wall += time.perf_counter() - t0
         done += m
         if model == "xy":
@@ -335,6 +335,7 @@ def chain_stats(pt, model, sizes, Rr, temps, steps, ising_hip):
             st["U4_err"].append(ue)
     st = {k: np.array(v) for k, v in st.items()}
     st["wall"] = wall
+    st["device_us_per_sweep"] = dev_ms * 1e3 / steps
     if model == "xy":
         st["helicity"] = [float(((hc[g] / nh - np.array([1.0 / t for t in temps]) * hj2[g] / nh) / L ** 3).mean())
                           for g, L in enumerate(sizes)]
@@ -347,6 +348,7 @@ def summarize(st, steps, base=None):
     equal error: (err_plain^2 t_plain) / (err^2 t), geometric mean over
     (size, temperature), for E and U4."""
     out = {"sweeps_per_s": round(steps / st["wall"], 2),
+           "device_us_per_sweep": round(st["device_us_per_sweep"], 2),
            "tau_int_E_mean": round(float(st["tau_E"].mean()), 3),
            "tau_int_M2_mean": round(float(st["tau_M2"].mean()), 3),
            "independent_samples_per_s_E": round(float(np.sum(steps / (2 * np.maximum(st["tau_E"], 0.5))) / st["wall"]), 1),
@@ -412,8 +414,20 @@ def run_learned(args, ising_hip):
                 pt.heat_bath(True)
                 pt.warmup(50)
             pt.sync()
+            pass_us = None
+            if mode in ("learned", "heat_bath"):
+                # one pass over one colour (no measurement), timed back to back
+                t_now = pt.sweeps()
+                pt.sync()
+                t0 = time.perf_counter()
+                for k in range(200):
+                    pt.metropolis_pass(k & 1, t_now + k // 2)
+                pt.sync()
+                pass_us = (time.perf_counter() - t0) * 1e6 / 200
             stats[mode] = chain_stats(pt, model, sizes, R, temps, steps, ising_hip)
             out[mode] = summarize(stats[mode], steps, None if mode == "plain" else stats["plain"])
+            if pass_us is not None:
+                out[mode]["pass_us"] = round(pass_us, 2)
             if mode == "learned":
                 tries, acc = pt.mh_stats()
                 out[mode]["learned_acceptance"] = round(acc / tries, 4) if tries else None

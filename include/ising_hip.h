@@ -268,7 +268,7 @@ int ising_pt_learn_infer(ising_pt_t* pt, int64_t n, const float* x, float* out);
 int ising_pt_learn_enable(ising_pt_t* pt, int32_t on);
 /* 1: sweeps use heat-bath passes (no counterpart in the reference): every
  * site draws its spin from the exact single-site conditional exp(beta J s.h)
- * -- Fisher on S^2, von Mises on S^1 (Best-Fisher rejection, at most 8
+ * -- Fisher on S^2, von Mises on S^1 (Best-Fisher rejection, at most 7
  * attempts; when all fail the site keeps its spin) -- in place of the
  * Metropolis passes; overrelaxation passes stay.  The learned proposal, which
  * depends on the neighbours only, cannot mix faster per pass.  Turns the
@@ -280,7 +280,6 @@ int ising_pt_mh_stats(ising_pt_t* pt, int64_t* tries, int64_t* accepts);
  * colour at sweep index t, without measurement, exchange or advancing the
  * sweep counter (used by exactness tests). */
 int ising_pt_metropolis_pass(ising_pt_t* pt, int32_t color, int64_t t);
-/* Sweeps applied so far (warm-up + sample). */
 /* XY spin-stiffness sums of every replica's current state (no counterpart in
  * the reference): out[6 * replica + k] = C_x, C_y, C_z, J_x, J_y, J_z with, over
  * the forward bonds (i, i + d), C_d = sum cos(theta_i - theta_j) and
@@ -394,6 +393,7 @@ int ising_pt_get_layout(ising_pt_t* pt, int32_t size_index, int32_t* width, int3
  * (and launches the remaining k % graph_sweeps sweeps directly).  0 turns
  * graph replay off.  Results do not depend on it. */
 int ising_pt_prepare_graph(ising_pt_t* pt, int32_t graph_sweeps);
+/* Sweeps applied so far (warm-up + sample). */
 int ising_pt_sweeps(const ising_pt_t* pt, int64_t* out);
 /* Bytes of spin storage on the device. */
 int ising_pt_device_bytes(const ising_pt_t* pt, int64_t* out);

@@ -93,6 +93,16 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+// Wave sums of a continuous pass's measurement partials (E, Mx, My, Mz) that
+// can be non-zero: E only in colour-1 passes, Mz only for Heisenberg (N = 3).
+// The others are +0.0 in every lane, which is also their sum.
+template <int N>
+__device__ __forceinline__ void wave_sum_parts(double v[4], int color) {
+    if (color) v[0] = wave_sum(v[0]);
+#pragma unroll
+    for (int k = 1; k <= N; ++k) v[k] = wave_sum(v[k]);
+}
+
 // ---------------------------------------------------------------------------
 // Ensemble descriptors shared by the host API and the kernels.
 // ---------------------------------------------------------------------------

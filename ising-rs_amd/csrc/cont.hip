@@ -87,11 +87,17 @@ __device__ __forceinline__ void load_field(const float* O, int64_t stride, const
     field_f32<N>(nv, hf);
 }
 
-// Block sum of up to 4 doubles; result valid in thread 0.
-template <int NT = kCT>
-__device__ __forceinline__ void block_sum4(double v[4], double (*s_red)[kCT / 64]) {
+// Block sum of up to 4 doubles; result valid in thread 0.  A Metropolis pass
+// passes its model's N and colour: only the partials that can be non-zero are
+// summed across the wave (wave_sum_parts).
+template <int NT = kCT, int N = 0>
+__device__ __forceinline__ void block_sum4(double v[4], double (*s_red)[kCT / 64], int color = 1) {
+    if (N > 0) {
+        wave_sum_parts<N>(v, color);
+    } else {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
+        for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
+    }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0)
         for (int k = 0; k < 4; ++k) s_red[k][wid] = v[k];
@@ -324,7 +330,7 @@ __global__ __launch_bounds__(kCT) void cont_metro_kernel(ContArgs a, int nblocks
                       (unsigned long long)wc);
     }
     if (need) {
-        block_sum4(part, s_red);
+        block_sum4<kCT, N>(part, s_red, color);
         if (threadIdx.x == 0) {
             double* dst = (color ? a.part1 : a.part0) + 4 * (int64_t)b;
             for (int k = 0; k < 4; ++k) dst[k] = part[k];
@@ -449,7 +455,7 @@ __global__ __launch_bounds__(kCT / 2) void cont_metro_pair_kernel(ContArgs a, in
                       (unsigned long long)wc);
     }
     if (need) {
-        block_sum4<kCT / 2>(part, s_red);
+        block_sum4<kCT / 2, N>(part, s_red, color);
         if (threadIdx.x == 0) {
             double* dst = (color ? a.part1 : a.part0) + 4 * (int64_t)b;
             for (int k = 0; k < 4; ++k) dst[k] = part[k];

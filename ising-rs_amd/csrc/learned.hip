@@ -487,15 +487,19 @@ __device__ __forceinline__ bool draw_around(const float hh[3], float kap, const 
 // as the plain pass.
 // ---------------------------------------------------------------------------
 template <int N, int H>
-__global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int color, int closes, int64_t t_override) {
+__global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int nblocks, int color, int closes,
+                                                           int64_t t_override) {
     __shared__ double s_red[4][4];
     __shared__ uint16_t s_feat[8 * 256];
     __shared__ float s_out[3 * 256];
+    const int per = (int)(gridDim.x >> 3);
+    const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);  // XCD-contiguous tiles
+    if (b >= nblocks) return;  // whole padding blocks, before any barrier
     int g = 0;
-    while (g + 1 < a.ngroups && (int)blockIdx.x >= a.groups[g + 1].blk0) ++g;
+    while (g + 1 < a.ngroups && b >= a.groups[g + 1].blk0) ++g;
     const DevGroup G = a.groups[g];
     int r, tile;
-    split_tiles(G, (int)blockIdx.x - G.blk0, &r, &tile);
+    split_tiles(G, b - G.blk0, &r, &tile);
     const int grep = G.rep0 + r;
     const int ti = G.rep0 + a.r2t[grep];
     NetFrag<H> F;
@@ -587,21 +591,21 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int color
     }
     // acceptance statistics (every pass)
     {
-        const int wa = wave_sum(acc_n), wt = wave_sum(tried);
+        // 0/1 per lane: a ballot and a scalar bit count each
+        const int wa = __popcll(__ballot(acc_n != 0)), wt = __popcll(__ballot(tried != 0));
         if ((threadIdx.x & 63) == 0 && wt) {
-            int64_t* slot = a.mh + 8 * ((blockIdx.x * 4 + (threadIdx.x >> 6)) & (kMhSlots - 1));
+            int64_t* slot = a.mh + 8 * ((b * 4 + (threadIdx.x >> 6)) & (kMhSlots - 1));
             atomicAdd((unsigned long long*)&slot[0], (unsigned long long)wt);
             atomicAdd((unsigned long long*)&slot[1], (unsigned long long)wa);
         }
     }
     if (need) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) part[k] = wave_sum(part[k]);
+        wave_sum_parts<N>(part, color);
         if ((threadIdx.x & 63) == 0)
             for (int k = 0; k < 4; ++k) s_red[k][threadIdx.x >> 6] = part[k];
         __syncthreads();
         if (threadIdx.x == 0) {
-            double* dst = (color ? a.part1 : a.part0) + 4 * (int64_t)blockIdx.x;
+            double* dst = (color ? a.part1 : a.part0) + 4 * (int64_t)b;
             for (int k = 0; k < 4; ++k) dst[k] = s_red[k][0] + s_red[k][1] + s_red[k][2] + s_red[k][3];
         }
     }
@@ -704,8 +708,7 @@ __global__ __launch_bounds__(256) void cont_heatbath_kernel(ContArgs a, int nblo
         }
     }
     if (need) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) part[k] = wave_sum(part[k]);
+        wave_sum_parts<N>(part, color);
         if ((threadIdx.x & 63) == 0)
             for (int k = 0; k < 4; ++k) s_red[k][threadIdx.x >> 6] = part[k];
         __syncthreads();
@@ -969,7 +972,8 @@ hipError_t launch_learn_frag(const uint16_t* net, int H, uint4* frag, hipStream_
 template <int N, int H>
 static hipError_t launch_learned_nh(const ContArgs& a, int nblocks, int color, int closes, int64_t t_override,
                                     hipStream_t st) {
-    hipLaunchKernelGGL((cont_learned_kernel<N, H>), dim3(nblocks), dim3(256), 0, st, a, color, closes, t_override);
+    const dim3 grid((nblocks + 7) & ~7);  // a multiple of the 8 XCDs
+    hipLaunchKernelGGL((cont_learned_kernel<N, H>), grid, dim3(256), 0, st, a, nblocks, color, closes, t_override);
     return hipGetLastError();
 }
 
