@@ -99,6 +99,26 @@ def test_matrix_core_inference_matches_the_bf16_forward(H):
     assert np.max(np.abs(got - full)) < 0.1 * scale
 
 
+def test_inference_follows_the_parameters_after_training():
+    """The fused pass reads the network as per-lane fragments built from the
+    BF16 parameters; after Adam steps (which rewrite the parameters on the
+    device) the inference path must use the updated network."""
+    H, nc = 16, 2
+    pt = ensemble("xy", L=8, R=4, temps=np.linspace(2.0, 2.4, 4))
+    pt.warmup(50)
+    pt.learn_init(H, seed=5)
+    x, _ = random_samples(nc, 512, 0.45, 31)
+    before = pt.learn_infer(x).astype(np.float64)
+    pt.learn_train(20, batch=1024, lr=3e-2)
+    after = pt.learn_infer(x).astype(np.float64)
+    p64 = ref.unpack_params(bf16_round(pt.learn_params()), H)
+    want = forward_bf16(p64, x.astype(np.float64))
+    scale = np.max(np.abs(want))
+    assert np.max(np.abs(after - before)) > 0.05 * scale  # training moved the outputs
+    err = np.abs(after - want)
+    assert np.median(err) < 1e-3 * scale and np.max(err) < 3e-2 * scale, (np.median(err), np.max(err), scale)
+
+
 def test_hidden64_gradients_on_a_parameter_subset():
     H, nc = 64, 3
     pt = ensemble("heisenberg")
