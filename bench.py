@@ -457,7 +457,9 @@ def run_learned(args, ising_hip):
                                "ceiling: XY L=32,64 x 32 replicas T=linspace(2.18,2.22,32); Heisenberg L=32 x 32 "
                                "T=linspace(1.43,1.46,32); measured every sweep, exchange every 10; value = learned "
                                "independent samples/s of E summed over both models",
-                   "hidden": args.hidden, "learned_overrelaxation_passes": args.learned_or, **res},
+                   "hidden": args.hidden, "network_parameters": 6 * args.hidden + args.hidden
+                   + 2 * (args.hidden * args.hidden + args.hidden) + 3 * args.hidden + 3,
+                   "learned_overrelaxation_passes": args.learned_or, **res},
     }
     print(json.dumps(line), flush=True)
 
@@ -617,7 +619,8 @@ def main():
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--workload", default="ising", choices=sorted(WORKLOADS) + ["batched", "learned", "kz", "coarsening", "fss",
                                                                   "cluster"])
-    ap.add_argument("--hidden", type=int, default=16, help="learned workload: MLP hidden width (8, 16, 32 or 64)")
+    ap.add_argument("--hidden", type=int, default=64,
+                    help="learned workload: MLP hidden width (8, 16, 32 or 64; 64 = 8963 parameters)")
     ap.add_argument("--train-steps", type=int, default=400, help="learned workload: Adam steps")
     ap.add_argument("--learned-or", type=int, default=2,
                     help="learned workload: overrelaxation passes per sweep of the learned chain (plain keeps 2)")

@@ -21,6 +21,9 @@
 // current PT configurations (loss = -log q(s | neighbours)); the minimiser is
 // the exact conditional (heat bath).  Per-block gradient partials are reduced
 // in a fixed order by the Adam kernel, so training is deterministic.
+#include <cstddef>
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace ising {
@@ -241,13 +244,37 @@ __device__ __forceinline__ float site_features(float beta, const float nv[3][6],
 // ---------------------------------------------------------------------------
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
+// A lane's operand fragments.  Layer 1 (K = 6 features + a constant 1, padded to
+// 16) uses v_mfma_f32_16x16x16_bf16: 4 BF16 per lane.  For H >= 32 the layers
+// that contract over the hidden width use v_mfma_f32_16x16x32_bf16 (gfx950),
+// KC = H / 32 products per 16 outputs, 8 BF16 per lane; for H <= 16 the
+// 16x16x16 form.
 template <int H>
 struct NetFrag {
     static constexpr int T = (H + 15) / 16;  // 16-wide tiles of the hidden width
-    v4s w1[T], w2[T][T], w3[T][T], w4[T];
+    static constexpr bool K32 = T >= 2;
+    static constexpr int KC = K32 ? T / 2 : 1;  // K chunks of the hidden-width contractions
+    using WK = std::conditional_t<K32, v8s, v4s>;
+    v4s w1[T];
+    WK w2[T][KC], w3[T][KC], w4[KC];
     v4f b2[T], b3[T], b4;
 };
+
+// Position p = 8 q + i of a 32-wide K chunk j (q = lane / 16, i = 0..7) stands
+// for this hidden neuron: i < 4 -> row 4 q + i of output tile 2 j, i >= 4 ->
+// row 4 q + i - 4 of tile 2 j + 1.  Those are exactly the values a lane holds
+// in its accumulators of the two tiles, so a layer's activations become the
+// next product's operand without moving between lanes; the weights are laid
+// out with the same order of K, which leaves the sum unchanged.
+__host__ __device__ constexpr int k32_neuron(int j, int q, int i) {
+    return 32 * j + (i < 4 ? 4 * q + i : 16 + 4 * q + (i - 4));
+}
 
 // This lane's operand fragments of every layer (zero padding past H and 3).
 template <int H>
@@ -265,15 +292,25 @@ __device__ __forceinline__ void load_net(const uint16_t* __restrict__ w, NetFrag
             if (j < H) v = k < kNIn ? w[O::W1 + j * kNIn + k] : (k == kNIn ? w[O::b1 + j] : (uint16_t)0);
             F.w1[mt][q] = (short)v;
         }
+        if constexpr (NetFrag<H>::K32) {
 #pragma unroll
-        for (int kt = 0; kt < T; ++kt)
+            for (int c = 0; c < NetFrag<H>::KC; ++c)
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int i = k32_neuron(c, l >> 4, q);
+                    const bool ok = j < H && i < H;
+                    F.w2[mt][c][q] = (short)(ok ? w[O::W2 + j * H + i] : (uint16_t)0);
+                    F.w3[mt][c][q] = (short)(ok ? w[O::W3 + j * H + i] : (uint16_t)0);
+                }
+        } else {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int i = 16 * kt + kq + q;
+                const int i = kq + q;
                 const bool ok = j < H && i < H;
-                F.w2[mt][kt][q] = (short)(ok ? w[O::W2 + j * H + i] : (uint16_t)0);
-                F.w3[mt][kt][q] = (short)(ok ? w[O::W3 + j * H + i] : (uint16_t)0);
+                F.w2[mt][0][q] = (short)(ok ? w[O::W2 + j * H + i] : (uint16_t)0);
+                F.w3[mt][0][q] = (short)(ok ? w[O::W3 + j * H + i] : (uint16_t)0);
             }
+        }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int n = 16 * mt + kq + r;
@@ -281,113 +318,168 @@ __device__ __forceinline__ void load_net(const uint16_t* __restrict__ w, NetFrag
             F.b3[mt][r] = n < H ? bf2f(w[O::b3 + n]) : 0.0f;
         }
     }
+    if constexpr (NetFrag<H>::K32) {
 #pragma unroll
-    for (int kt = 0; kt < T; ++kt)
+        for (int c = 0; c < NetFrag<H>::KC; ++c)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int i = k32_neuron(c, l >> 4, q);
+                F.w4[c][q] = (short)((col < 3 && i < H) ? w[O::W4 + col * H + i] : (uint16_t)0);
+            }
+    } else {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int i = 16 * kt + kq + q;
-            F.w4[kt][q] = (short)((col < 3 && i < H) ? w[O::W4 + col * H + i] : (uint16_t)0);
+            const int i = kq + q;
+            F.w4[0][q] = (short)((col < 3 && i < H) ? w[O::W4 + col * H + i] : (uint16_t)0);
         }
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) F.b4[r] = kq + r < 3 ? bf2f(w[O::b4 + kq + r]) : 0.0f;
 }
 
-// The fragments as 16-byte words.  A network's fragments are built once per
-// parameter update (learn_frag_kernel): word k of lane l at frag[64 k + l], so
-// a wave loads each word in one coalesced 1 KiB read instead of gathering and
-// padding single BF16 weights.
+// The fragments in device memory, as 8-byte units: unit u of lane l at byte
+// 8 (64 u + l) (every fragment is 8 bytes, a bias 16 = two units).  A
+// network's fragments are built once per parameter update (learn_frag_kernel);
+// a block copies the whole buffer into LDS, where a wave reads any unit of its
+// 64 lanes as one conflict-free 512-byte ds_read_b64.
 template <int H>
-struct NetWords {
+struct NetUnits {
     static_assert(sizeof(NetFrag<H>) % 16 == 0, "fragments are whole 16-byte words");
-    static constexpr int K = (int)(sizeof(NetFrag<H>) / 16);
-    uint4 w[K];
+    static constexpr int U = (int)(sizeof(NetFrag<H>) / 8);
+    static constexpr int BYTES = 64 * (int)sizeof(NetFrag<H>);  // the whole buffer (64 lanes)
+    uint2 u[U];
 };
 
 template <int H>
-__device__ __forceinline__ void load_frag(const uint4* __restrict__ frag, NetFrag<H>& F) {
-    NetWords<H> W;
-    const int l = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 0; k < NetWords<H>::K; ++k) W.w[k] = frag[64 * k + l];
-    F = __builtin_bit_cast(NetFrag<H>, W);
-}
-
-template <int H>
-__global__ __launch_bounds__(64) void learn_frag_kernel(const uint16_t* __restrict__ net, uint4* __restrict__ frag) {
+__global__ __launch_bounds__(64) void learn_frag_kernel(const uint16_t* __restrict__ net, uint2* __restrict__ frag) {
     NetFrag<H> F;
     load_net<H>(net, F);
-    const NetWords<H> W = __builtin_bit_cast(NetWords<H>, F);
+    const NetUnits<H> W = __builtin_bit_cast(NetUnits<H>, F);
 #pragma unroll
-    for (int k = 0; k < NetWords<H>::K; ++k) frag[64 * k + threadIdx.x] = W.w[k];
+    for (int u = 0; u < NetUnits<H>::U; ++u) frag[64 * u + threadIdx.x] = W.u[u];
+}
+
+// The block's LDS copy of the fragment buffer (every thread takes part; a
+// barrier must follow before any wave reads it).
+template <int H>
+__device__ __forceinline__ void stage_frag(const uint4* __restrict__ frag, uint4* s) {
+    for (int i = threadIdx.x; i < NetUnits<H>::BYTES / 16; i += blockDim.x) s[i] = frag[i];
+}
+
+// This lane's 8-byte unit at byte offset b of its NetFrag (b a multiple of 8).
+__device__ __forceinline__ v4s frag8(const char* s, int lane, int b) {
+    return *(const v4s*)(s + 8 * (64 * (b >> 3) + lane));
+}
+__device__ __forceinline__ v8s fragw(const char* s, int lane, int b, v8s) {
+    return __builtin_shufflevector(frag8(s, lane, b), frag8(s, lane, b + 8), 0, 1, 2, 3, 4, 5, 6, 7);
+}
+__device__ __forceinline__ v4s fragw(const char* s, int lane, int b, v4s) { return frag8(s, lane, b); }
+__device__ __forceinline__ v4f frag16(const char* s, int lane, int b) {
+    const v4s lo = frag8(s, lane, b), hi = frag8(s, lane, b + 8);
+    const v2u a = __builtin_bit_cast(v2u, lo), c = __builtin_bit_cast(v2u, hi);
+    return (v4f){__uint_as_float(a[0]), __uint_as_float(a[1]), __uint_as_float(c[0]), __uint_as_float(c[1])};
 }
 
 __device__ __forceinline__ short bf16_bits(float f) { return __builtin_bit_cast(short, (__bf16)f); }  // RNE
 
-typedef float v2f __attribute__((ext_vector_type(2)));
-typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
-typedef uint32_t v2u __attribute__((ext_vector_type(2)));
-// Two floats to packed BF16 (nearest even) in one v_cvt_pk_bf16_f32.
-__device__ __forceinline__ uint32_t pk_bf16(float a, float b) {
-    return __builtin_bit_cast(uint32_t, __builtin_convertvector((v2f){a, b}, v2bf));
+// BF16 activations of four accumulators: two v_cvt_pk_bf16_f32 (round to
+// nearest even), then ReLU as one v_pk_max_i16 per pair on the BF16 bits (a
+// negative BF16 is a negative int16).  Rounding keeps the sign, so this equals
+// rounding max(z, 0).
+typedef short v2s __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t relu_pk_bf16(float a, float b) {
+    const v2s p = __builtin_bit_cast(v2s, __builtin_convertvector((v2f){a, b}, v2bf));
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(p, (v2s){0, 0}));
 }
-
 __device__ __forceinline__ v4s act_bf16(v4f d) {
-    const v2u p = {pk_bf16(act_f(d[0]), act_f(d[1])), pk_bf16(act_f(d[2]), act_f(d[3]))};
+    const v2u p = {relu_pk_bf16(d[0], d[1]), relu_pk_bf16(d[2], d[3])};
     return __builtin_bit_cast(v4s, p);
 }
 
-// One tile of 16 sites: xb is this lane's input fragment (features 4 (lane / 16) .. + 3
-// of site lane % 16); returns the output accumulator (lanes 0-15: o_0..o_2 in rows 0-2).
+// Outputs o[3] for the site of every lane of the block's waves, from the
+// fragments staged in LDS (sf).  All threads of the block call it together.
+// feat: 8 halves per thread; out: 3 floats per thread.  A wave evaluates its
+// 64 sites as four 16-site tiles layer by layer: each weight fragment is read
+// from LDS once and applied to all four tiles, and only the activations of two
+// layers (4 tiles x H/16 fragments each) stay in registers.
 template <int H>
-__device__ __forceinline__ v4f mlp_tile(const NetFrag<H>& F, v4s xb) {
-    constexpr int T = NetFrag<H>::T;
-    const v4f zero = {0.f, 0.f, 0.f, 0.f};
-    v4s a1[T], a2[T];
-#pragma unroll
-    for (int mt = 0; mt < T; ++mt) a1[mt] = act_bf16(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.w1[mt], xb, zero, 0, 0, 0));
-#pragma unroll
-    for (int mt = 0; mt < T; ++mt) {
-        v4f d = F.b2[mt];
-#pragma unroll
-        for (int kt = 0; kt < T; ++kt) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.w2[mt][kt], a1[kt], d, 0, 0, 0);
-        a2[mt] = act_bf16(d);
-    }
-#pragma unroll
-    for (int mt = 0; mt < T; ++mt) {
-        v4f d = F.b3[mt];
-#pragma unroll
-        for (int kt = 0; kt < T; ++kt) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.w3[mt][kt], a2[kt], d, 0, 0, 0);
-        a1[mt] = act_bf16(d);
-    }
-    v4f d = F.b4;
-#pragma unroll
-    for (int kt = 0; kt < T; ++kt) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.w4[kt], a1[kt], d, 0, 0, 0);
-    return d;
-}
-
-// Outputs o[3] for the site of every lane of the block's waves.  All threads of
-// the block call it together.  feat: 8 halves per thread; out: 3 floats per thread.
-template <int H>
-__device__ __forceinline__ void mlp_block(const NetFrag<H>& F, const float x[kNIn], float o[3], uint16_t* feat,
+__device__ __forceinline__ void mlp_block(const char* sf, const float x[kNIn], float o[3], uint16_t* feat,
                                           float* out) {
+    using NF = NetFrag<H>;
+    constexpr int T = NF::T;
     const int t = threadIdx.x, l = t & 63, w0 = t - l;
 #pragma unroll
     for (int i = 0; i < kNIn; ++i) feat[8 * t + i] = (uint16_t)bf16_bits(x[i]);
     feat[8 * t + kNIn] = 0x3f80;  // 1.0
     feat[8 * t + kNIn + 1] = 0;
     __syncthreads();
+    // input fragments: features 4 (lane / 16) .. + 3 of site 16 st + lane % 16
+    v4s xb[4];
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
-        v4s xb = {0, 0, 0, 0};
+        xb[st] = (v4s){0, 0, 0, 0};
         if ((l >> 4) < 2) {
             const uint16_t* p = feat + 8 * (w0 + 16 * st + (l & 15)) + 4 * (l >> 4);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) xb[q] = (short)p[q];
+            for (int q = 0; q < 4; ++q) xb[st][q] = (short)p[q];
         }
-        const v4f d = mlp_tile<H>(F, xb);
-        if (l < 16)
+    }
+    const v4f zero = {0.f, 0.f, 0.f, 0.f};
+    v4s a[4][T], c[4][T];
 #pragma unroll
-            for (int r = 0; r < 3; ++r) out[3 * (w0 + 16 * st + l) + r] = d[r];
+    for (int mt = 0; mt < T; ++mt) {
+        const v4s A = frag8(sf, l, (int)offsetof(NF, w1) + 8 * mt);
+#pragma unroll
+        for (int st = 0; st < 4; ++st) a[st][mt] = act_bf16(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, xb[st], zero, 0, 0, 0));
+    }
+    // products over the hidden width: K = 16 (T = 1) or KC chunks of 32
+    using WK = typename NF::WK;
+    constexpr int KC = NF::KC;
+    auto contract = [](const WK (&A)[KC], const v4s (&in)[T], v4f d) -> v4f {
+        if constexpr (NF::K32) {
+#pragma unroll
+            for (int c = 0; c < KC; ++c) {
+                const v8s b = __builtin_shufflevector(in[2 * c], in[2 * c + 1], 0, 1, 2, 3, 4, 5, 6, 7);
+                d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, A[c]), __builtin_bit_cast(v8bf, b),
+                                                            d, 0, 0, 0);
+            }
+            return d;
+        } else {
+            return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[0], in[0], d, 0, 0, 0);
+        }
+    };
+    // hidden layers 2 and 3 (a -> c -> a)
+#pragma unroll
+    for (int layer = 0; layer < 2; ++layer) {
+        const int wo = layer == 0 ? (int)offsetof(NF, w2) : (int)offsetof(NF, w3);
+        const int bo = layer == 0 ? (int)offsetof(NF, b2) : (int)offsetof(NF, b3);
+#pragma unroll
+        for (int mt = 0; mt < T; ++mt) {
+            WK A[KC];
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) A[kc] = fragw(sf, l, wo + (int)sizeof(WK) * (mt * KC + kc), WK{});
+            const v4f bias = frag16(sf, l, bo + 16 * mt);
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+                const v4f d = contract(A, layer == 0 ? a[st] : c[st], bias);
+                if (layer == 0) c[st][mt] = act_bf16(d);
+                else a[st][mt] = act_bf16(d);
+            }
+        }
+    }
+    {
+        WK A[KC];
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) A[kc] = fragw(sf, l, (int)offsetof(NF, w4) + (int)sizeof(WK) * kc, WK{});
+        const v4f bias = frag16(sf, l, (int)offsetof(NF, b4));
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            const v4f d = contract(A, a[st], bias);
+            // lanes 0-15 hold o_0..o_2 of site 16 st + lane in rows 0-2
+            if (l < 16)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) out[3 * (w0 + 16 * st + l) + r] = d[r];
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -487,14 +579,8 @@ __device__ __forceinline__ bool draw_around(const float hh[3], float kap, const 
 // as the plain pass.
 // ---------------------------------------------------------------------------
 template <int N, int H>
-__global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int nblocks, int color, int closes,
-                                                           int64_t t_override) {
-    __shared__ double s_red[4][4];
-    __shared__ uint16_t s_feat[8 * 256];
-    __shared__ float s_out[3 * 256];
-    const int per = (int)(gridDim.x >> 3);
-    const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);  // XCD-contiguous tiles
-    if (b >= nblocks) return;  // whole padding blocks, before any barrier
+__device__ __forceinline__ void learned_tile(const ContArgs& a, int b, int color, int closes, int64_t t_override,
+                                             double (*s_red)[4], uint16_t* s_feat, float* s_out, const char* s_frag) {
     int g = 0;
     while (g + 1 < a.ngroups && b >= a.groups[g + 1].blk0) ++g;
     const DevGroup G = a.groups[g];
@@ -502,8 +588,6 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int nbloc
     split_tiles(G, b - G.blk0, &r, &tile);
     const int grep = G.rep0 + r;
     const int ti = G.rep0 + a.r2t[grep];
-    NetFrag<H> F;
-    load_frag<H>(a.lfrag, F);
     const float beta = a.betas_f[ti];
     const int64_t t = t_override >= 0 ? t_override : a.ctl->t;
     const int32_t fl = t_override >= 0 ? 0 : a.ctl->flags;
@@ -518,14 +602,15 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int nbloc
     double part[4] = {0.0, 0.0, 0.0, 0.0};
     int acc_n = 0, tried = 0;
     const bool live = h < G.n_half;
-    float nv[3][6] = {};
+    uint32_t nb[6] = {0u, 0u, 0u, 0u, 0u, 0u};
     float S[3] = {0.f, 0.f, 0.f};
     float x[kNIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, hh[3] = {0.f, 0.f, 0.f};
     float hn = 0.f;
     if (live) {
-        // neighbours (same addressing as the plain pass)
-        uint32_t nb[6];
+        // neighbours (same addressing as the plain pass); their values are not
+        // kept across the network (registers), a measured sweep reads them again
         site_nbrs(G, (uint32_t)h, color, nb);
+        float nv[3][6] = {};
 #pragma unroll
         for (int k = 0; k < N; ++k)
 #pragma unroll
@@ -536,7 +621,7 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int nbloc
     }
     // the proposal's parameters for every site of the block, on the matrix cores
     float o[3];
-    mlp_block<H>(F, x, o, s_feat, s_out);
+    mlp_block<H>(s_frag, x, o, s_feat, s_out);
     if (live) {
         bool accept = false;
         float nw[3] = {S[0], S[1], S[2]};
@@ -582,7 +667,7 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int nbloc
                 if (color) {
                     double f = 0.0;
 #pragma unroll
-                    for (int q = 0; q < 6; ++q) f += (double)nv[k][q];
+                    for (int q = 0; q < 6; ++q) f += (double)O[k * G.stride + nb[q]];
                     e += (double)S[k] * f;
                 }
             }
@@ -609,6 +694,24 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int nbloc
             for (int k = 0; k < 4; ++k) dst[k] = s_red[k][0] + s_red[k][1] + s_red[k][2] + s_red[k][3];
         }
     }
+}
+
+// The pass's kernel: one tile per block, in the XCD-contiguous order, with the
+// network's fragments copied into LDS per block.  (Four tiles per block, sharing
+// one copy, measured slower at H = 64: the loop raised the kernel from 116 to
+// 144 VGPRs, 3 waves per SIMD instead of 4; XY pass 163 against 148 us.)
+template <int N, int H>
+__global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int nblocks, int color, int closes,
+                                                           int64_t t_override) {
+    __shared__ double s_red[4][4];
+    __shared__ uint16_t s_feat[8 * 256];
+    __shared__ float s_out[3 * 256];
+    __shared__ uint4 s_frag[NetUnits<H>::BYTES / 16];
+    const int per = (int)(gridDim.x >> 3);
+    const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);  // XCD-contiguous tiles
+    if (b >= nblocks) return;  // whole padding blocks, before any barrier
+    stage_frag<H>(a.lfrag, s_frag);  // read after the first barrier in mlp_block
+    learned_tile<N, H>(a, b, color, closes, t_override, s_red, s_feat, s_out, (const char*)s_frag);
 }
 
 // ---------------------------------------------------------------------------
@@ -922,15 +1025,15 @@ __global__ __launch_bounds__(256) void learn_infer_kernel(const uint4* __restric
                                                           int64_t n, float* __restrict__ out) {
     __shared__ uint16_t s_feat[8 * 256];
     __shared__ float s_out[3 * 256];
-    NetFrag<H> F;
-    load_frag<H>(frag, F);
+    __shared__ uint4 s_frag[NetUnits<H>::BYTES / 16];
+    stage_frag<H>(frag, s_frag);  // read after the barrier in mlp_block
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     float x[kNIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (i < n)
 #pragma unroll
         for (int k = 0; k < kNIn; ++k) x[k] = xin[i * kNIn + k];
     float o[3];
-    mlp_block<H>(F, x, o, s_feat, s_out);
+    mlp_block<H>((const char*)s_frag, x, o, s_feat, s_out);
     if (i < n)
 #pragma unroll
         for (int r = 0; r < 3; ++r) out[i * 3 + r] = o[r];
@@ -950,20 +1053,21 @@ hipError_t launch_learn_infer(const uint4* frag, int H, const float* x, int64_t 
 
 size_t learn_frag_bytes(int H) {
     switch (H) {
-        case 8: return 64 * sizeof(NetFrag<8>);
-        case 16: return 64 * sizeof(NetFrag<16>);
-        case 32: return 64 * sizeof(NetFrag<32>);
-        case 64: return 64 * sizeof(NetFrag<64>);
+        case 8: return NetUnits<8>::BYTES;
+        case 16: return NetUnits<16>::BYTES;
+        case 32: return NetUnits<32>::BYTES;
+        case 64: return NetUnits<64>::BYTES;
     }
     return 0;
 }
 
 hipError_t launch_learn_frag(const uint16_t* net, int H, uint4* frag, hipStream_t st) {
+    uint2* f = (uint2*)frag;
     switch (H) {
-        case 8: hipLaunchKernelGGL(learn_frag_kernel<8>, dim3(1), dim3(64), 0, st, net, frag); break;
-        case 16: hipLaunchKernelGGL(learn_frag_kernel<16>, dim3(1), dim3(64), 0, st, net, frag); break;
-        case 32: hipLaunchKernelGGL(learn_frag_kernel<32>, dim3(1), dim3(64), 0, st, net, frag); break;
-        case 64: hipLaunchKernelGGL(learn_frag_kernel<64>, dim3(1), dim3(64), 0, st, net, frag); break;
+        case 8: hipLaunchKernelGGL(learn_frag_kernel<8>, dim3(1), dim3(64), 0, st, net, f); break;
+        case 16: hipLaunchKernelGGL(learn_frag_kernel<16>, dim3(1), dim3(64), 0, st, net, f); break;
+        case 32: hipLaunchKernelGGL(learn_frag_kernel<32>, dim3(1), dim3(64), 0, st, net, f); break;
+        case 64: hipLaunchKernelGGL(learn_frag_kernel<64>, dim3(1), dim3(64), 0, st, net, f); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
