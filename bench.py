@@ -3,8 +3,10 @@
 Workload (the reference's 3D Ising parallel-tempering run at full size): L in
 {8,16,32,64,128}, 32 replicas per L on T = linspace(4.40, 4.62, 32), int8
 spins from a random +-1 start, Philox per (seed, replica, sweep, site),
-exchange every 10 sweeps, measure every 5.  One step = one PT sweep of every
-lattice (76,693,504 spin updates), measurement and exchange included.
+exchange every 10 sweeps, measure every 5.  One step = one exchange interval,
+the period at which that schedule repeats: 10 sweeps of every lattice
+(766,935,040 spin updates), the 5th and 10th measured, the 10th closed by a
+replica-exchange round.
 
 Prints one JSON line (metric, value, configuration) with a `roofline` object for the
 dominant kernel (ising_half_kernel) and a `cpu_baseline` from the oracle's
@@ -122,23 +124,27 @@ def cpu_baseline(seconds, model="ising"):
     return out
 
 
-def graph_length(steps, cap=64):
-    """Largest divisor of `steps` not above `cap`: the timed sweeps are whole graph replays."""
-    return max(g for g in range(1, min(cap, steps) + 1) if steps % g == 0)
+def graph_length(sweeps, cap=200):
+    """Largest divisor of `sweeps` not above `cap`: the timed sweeps are whole graph replays."""
+    return max(g for g in range(1, min(cap, sweeps) + 1) if sweeps % g == 0)
+
+
+# Parallel-tempering workloads time whole exchange intervals: one step is
+# `exchange_every` sweeps (10 at every configuration here).
+SWEEPS_PER_STEP = 10
 
 
 def beyond_mall(args, ising_hip, opts, layout):
     """int8 Ising PT with a working set above the 256 MB Infinity Cache: L=256 x 32
     replicas (512 MiB of spins), same ladder, cadence and thread-step layout;
-    steps and warm-up as the main run (at least 20 sweeps each)."""
-    L, steps = 256, max(20, args.steps // 10 * 10 if args.steps >= 20 else 20)
-    steps = min(steps, 600)
+    the main run's sweeps, between 200 and 600."""
+    L, steps = 256, min(max(200, args.steps * SWEEPS_PER_STEP), 600)
     kw = dict(opts)
     if layout is not None:
         kw.update(ising_width=layout[0], ising_runs=layout[1], ising_spt=layout[2])
     pt = ising_hip.PTEnsemble("ising", [L], R, linspace(4.40, 4.62, R), args.seed,
                               record_capacity=steps // kw["measure_every"] + 8, **kw)
-    pt.warmup(max(20, args.warmup))
+    pt.warmup(max(20, args.warmup * SWEEPS_PER_STEP))
     G = graph_length(steps)
     pt.prepare_graph(G)
     pt.sample(0)
@@ -153,8 +159,8 @@ def beyond_mall(args, ising_hip, opts, layout):
     launch_s = ev_ms * 1e-3 / (2 * steps)
     achieved = 3 * sites / 2 / launch_s / 1e9
     return {"workload": "3D Ising PT, L=256 x 32 replicas, T=linspace(4.40,4.62,32), exchange every 10, measure "
-                        "every 5, int8 spins", "pool_bytes": pool, "steps": steps, "graph_sweeps": G,
-            "value": round(sites * steps / (wall * 1e9), 3), "unit": "updates/ns",
+                        "every 5, int8 spins", "pool_bytes": pool, "graph_sweeps": G,
+            "sweeps": steps, "value": round(sites * steps / (wall * 1e9), 3), "unit": "updates/ns",
             "device_updates_per_ns": round(sites * steps / (ev_ms * 1e6), 3),
             "roofline": {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4), "alg_bytes_per_launch": 3 * sites // 2,
@@ -221,39 +227,41 @@ BATCH_MODELS = [
 def run_batched(args, ising_hip):
     """Ising + XY + Heisenberg, L = 8..128 (9 sizes) x 32 replicas each, advanced
     together by one captured graph per block of sweeps (a branch per model).
-    One step = one PT sweep of all 3 x 109,041,664 sites (one Metropolis pass
-    per site; the continuous models add 2 overrelaxation passes)."""
+    One step = one exchange interval: 10 PT sweeps of all 3 x 109,041,664
+    sites (one Metropolis pass per site and sweep; the continuous models add 2
+    overrelaxation passes)."""
     sites = sum(L ** 3 for L in BATCH_SIZES) * R
+    sweeps = args.steps * SWEEPS_PER_STEP
     members = []
     for model, (tlo, thi), _, opts in BATCH_MODELS:
-        cap = args.steps // opts["measure_every"] + 8
+        cap = sweeps // opts["measure_every"] + 8
         members.append(ising_hip.PTEnsemble(model, BATCH_SIZES, R, linspace(tlo, thi, R), args.seed,
                                             record_capacity=cap, graph_sweeps=0, **opts))
-    G = graph_length(args.steps)
+    G = graph_length(sweeps, cap=50)
     batch = ising_hip.Batch(members, graph_sweeps=G)
-    warm = max(args.warmup, G)  # the batch graph is captured by the first call of >= G sweeps
+    warm = max(args.warmup * SWEEPS_PER_STEP, G)  # the batch graph is captured by the first call of >= G sweeps
     batch.warmup(warm)
     batch.sample(0)
     batch.sync()
     t0 = time.perf_counter()
-    batch.sample(args.steps)
+    batch.sample(sweeps)
     ev_ms = batch.sync()
     wall = time.perf_counter() - t0
     for p, (model, _, _, opts) in zip(members, BATCH_MODELS):
         recs = p.read_ising() if model == "ising" else p.read_cont()
-        assert recs[0].shape[0] == args.steps // opts["measure_every"]
-    updates = 3 * sites * args.steps
+        assert recs[0].shape[0] == sweeps // opts["measure_every"]
+    updates = 3 * sites * sweeps
     passes = sum(1 + o.get("n_overrelax", 0) for _, _, _, o in BATCH_MODELS)
-    alg_bytes_per_step = sum(bpu * sites * (1 + o.get("n_overrelax", 0)) for _, _, bpu, o in BATCH_MODELS)
-    step_s = ev_ms * 1e-3 / args.steps
-    achieved = alg_bytes_per_step / step_s / 1e9
+    alg_bytes_per_sweep = sum(bpu * sites * (1 + o.get("n_overrelax", 0)) for _, _, bpu, o in BATCH_MODELS)
+    sweep_s = ev_ms * 1e-3 / sweeps
+    achieved = alg_bytes_per_sweep / sweep_s / 1e9
     line = {
         "metric": "spin updates/ns",
         "value": round(updates / (wall * 1e9), 3),
         "unit": "updates/ns",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": warm,
+        "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
@@ -264,12 +272,14 @@ def run_batched(args, ising_hip):
                                "T=1.40-1.50, L=8,12,16,24,32,48,64,96,128 x 32 replicas each, exchange every 10, "
                                "measure every 5, continuous models 1 Metropolis + 2 overrelaxation passes",
                    "sizes": BATCH_SIZES, "replicas": R, "models": [m for m, _, _, _ in BATCH_MODELS],
-                   "updates_per_step": 3 * sites, "site_passes_per_step": passes * sites,
-                   "combined_updates_per_ns": round(passes * sites * args.steps / (wall * 1e9), 3)},
+                   "sweeps_per_step": SWEEPS_PER_STEP, "sweeps": sweeps, "warmup_sweeps": warm,
+                   "updates_per_step": 3 * sites * SWEEPS_PER_STEP,
+                   "site_passes_per_sweep": passes * sites, "graph_sweeps": G,
+                   "combined_updates_per_ns": round(passes * sites * sweeps / (wall * 1e9), 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "whole sweep graph (all kernels of the three models, concurrent branches)",
-                     "alg_bytes_per_step": int(alg_bytes_per_step), "avg_step_us": round(step_s * 1e6, 3),
+                     "alg_bytes_per_sweep": int(alg_bytes_per_sweep), "avg_sweep_us": round(sweep_s * 1e6, 3),
                      "device_updates_per_ns": round(updates / (ev_ms * 1e6), 3)},
     }
     if args.cpu_seconds > 0:
@@ -613,8 +623,11 @@ def run_driver(args, ising_hip):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3000)
-    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps: exchange intervals of 10 sweeps for the PT workloads (default 300), "
+                         "sweeps for the learned, cluster and driver workloads (default 3000)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed warm-up steps, same unit (default 30, or 300 sweeps)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--workload", default="ising", choices=sorted(WORKLOADS) + ["batched", "learned", "kz", "coarsening", "fss",
@@ -638,6 +651,11 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (PMC byte-counter calibration)")
     args = ap.parse_args()
+    pt_workload = args.workload in WORKLOADS or args.workload == "batched"
+    if args.steps is None:
+        args.steps = 300 if pt_workload else 3000
+    if args.warmup is None:
+        args.warmup = 30 if pt_workload else 300
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
@@ -676,7 +694,9 @@ def main():
         opts = dict(opts, ising_width=w, ising_runs=r, ising_spt=s)
     if model == "ising" and args.fused:
         opts = dict(opts, ising_fused=args.fused)
-    pt = ising_hip.PTEnsemble(model, sizes, R, temps, args.seed, record_capacity=args.steps // me + 8, **opts)
+    sweeps = args.steps * SWEEPS_PER_STEP
+    assert opts["exchange_every"] == SWEEPS_PER_STEP
+    pt = ising_hip.PTEnsemble(model, sizes, R, temps, args.seed, record_capacity=sweeps // me + 8, **opts)
     layout = None
     if model == "ising":
         if args.layout == "auto":
@@ -685,38 +705,39 @@ def main():
             layout = tuple(int(v) for v in args.layout.split(","))
     elif model in ("xy", "heisenberg"):
         layout = pt.autotune(20)  # the overrelaxation pass's sites per thread
-    pt.warmup(args.warmup)
-    # the timed sweeps replay one captured graph of G sweeps, G dividing --steps
-    # (production runs replay graphs too); captured here, outside the timed region
-    G = graph_length(args.steps)
+    pt.warmup(args.warmup * SWEEPS_PER_STEP)
+    # the timed sweeps replay one captured graph of G sweeps, G dividing the
+    # timed sweeps (production runs replay graphs too); captured here, outside
+    # the timed region
+    G = graph_length(sweeps)
     pt.prepare_graph(G)
     pt.sample(0)  # enters the sampling phase (a control-block round trip), outside the timed region
     pt.sync()
 
     t0 = time.perf_counter()
-    pt.sample(args.steps)
+    pt.sample(sweeps)
     ev_ms = pt.sync()
     wall = time.perf_counter() - t0
 
     recs = pt.read_ising() if model.startswith("ising") else pt.read_cont()
-    assert recs[0].shape[0] == args.steps // me
+    assert recs[0].shape[0] == sweeps // me
     per_size = [pt.get_layout(g) for g in range(len(sizes))] if model == "ising" else None
     pt.close()
 
     sites = sum(L ** 3 for L in sizes) * R
     passes = 1 + opts.get("n_overrelax", 0) if not model.startswith("ising") else 1
-    updates = sites * args.steps                   # Metropolis updates (the reported metric)
+    updates = sites * sweeps                       # Metropolis updates (the reported metric)
     value = updates / (wall * 1e9)
     fused = model == "ising" and args.fused > 0
     cont = not model.startswith("ising")
     # colour passes (or whole sweeps when fused); a continuous sweep adds one closing launch
-    launches = ((1 if fused else 2) * passes + (1 if cont else 0)) * args.steps
+    launches = ((1 if fused else 2) * passes + (1 if cont else 0)) * sweeps
     avg_launch_s = ev_ms * 1e-3 / launches
     if cont:
         # whole sweep: every pass moves bpu bytes per update; the closing launch moves ~nothing
         alg_bytes_per_launch = None
-        alg_bytes_per_step = bpu * sites * passes
-        achieved = alg_bytes_per_step / (ev_ms * 1e-3 / args.steps) / 1e9
+        alg_bytes_per_sweep = bpu * sites * passes
+        achieved = alg_bytes_per_sweep / (ev_ms * 1e-3 / sweeps) / 1e9
     else:
         alg_bytes_per_launch = int(bpu * sites / (1 if fused else 2))  # bytes per update x updates per launch
         achieved = alg_bytes_per_launch / avg_launch_s / 1e9
@@ -736,8 +757,10 @@ def main():
         "vs_baseline": None,
         "dtype": dtype,
         "data": "synthetic",
-        "config": {"workload": desc, "sizes": sizes, "replicas": R, "updates_per_step": sites,
-                   "site_passes_per_step": passes},
+        "config": {"workload": desc, "sizes": sizes, "replicas": R,
+                   "step": "one exchange interval: %d sweeps of every lattice" % SWEEPS_PER_STEP,
+                   "sweeps_per_step": SWEEPS_PER_STEP, "sweeps": sweeps, "warmup_sweeps": args.warmup * SWEEPS_PER_STEP,
+                   "updates_per_step": sites * SWEEPS_PER_STEP, "site_passes_per_sweep": passes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": load_traffic() if args.workload == "ising" and not fused and not args.sizes else None,
@@ -748,7 +771,7 @@ def main():
     line["config"]["graph_sweeps"] = G
     if cont:
         line["roofline"]["alg_bytes_per_update"] = bpu
-        line["roofline"]["avg_step_us"] = round(ev_ms * 1e3 / args.steps, 3)
+        line["roofline"]["avg_sweep_us"] = round(ev_ms * 1e3 / sweeps, 3)
         if model == "xy":
             # stored as two FP32 components (24 B per update); one FP32 angle would be 12 B
             line["roofline"]["frac_at_12B_angle"] = round(achieved / 2 / HBM_PEAK_GBS, 4)

@@ -85,6 +85,20 @@ __device__ __forceinline__ double det_exp(double x) {
     return __builtin_ldexp(p, (int)kd);
 }
 
+// Sum of a 32-bit integer over the 64 lanes of a wave, valid in lane 63 only
+// (every lane must be active): pairs, quads, half rows and rows of 16 lanes by
+// DPP lane permutations, then the four rows by DPP row broadcasts.  Six
+// VALU adds, no LDS traffic.
+__device__ __forceinline__ int wave_total_last(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);   // quad_perm [1,0,3,2]
+    v += __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);   // quad_perm [2,3,0,1]
+    v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xf, 0xf, false);  // row_half_mirror
+    v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xf, 0xf, false);  // row_mirror
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1 and 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 // Wave-wide sums (64 lanes) by xor shuffles.
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {

@@ -257,7 +257,7 @@ __device__ __forceinline__ int fast_tile(const IsingArgs& a, const DevGroup& G, 
         if (bal) {
             int base = 0;
             if ((threadIdx.x & 63) == 0) base = atomicAdd(qn, __popcll(bal));
-            base = __shfl(base, 0);
+            base = __builtin_amdgcn_readfirstlane(base);  // lane 0's count
             if (tie) {
                 const uint32_t pos = (uint32_t)base + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
                                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
@@ -405,8 +405,8 @@ __global__ __launch_bounds__(kThreads) void ising_half_kernel(IsingArgs a, int c
     // finishes last (workgroup-scope acquire-release counter) issues the
     // atomics, so waves without tied sites need not wait for the tie pass.
     if (need) {
-        const int wb = wave_sum(bsum), wm = wave_sum(msum);
-        if (lane == 0) {
+        const int wb = wave_total_last(bsum), wm = wave_total_last(msum);
+        if (lane == 63) {
             atomicAdd(&s_red[0], wb);
             atomicAdd(&s_red[1], wm);
             const int done = __hip_atomic_fetch_add(&s_waves, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);

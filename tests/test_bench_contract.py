@@ -36,3 +36,19 @@ def test_traffic_file_matches_the_profile_summary():
     scaled = d["fetch_kib"] * 1024 * d["fetch_scale"] + d["write_kib"] * 1024 * d["write_scale"]
     assert abs(scaled - t) <= 1.0
     assert 0.99 < t / d["algorithmic_bytes_per_launch"] < 1.05
+
+
+def test_a_step_is_one_exchange_interval_of_whole_graph_replays():
+    import bench
+    # every parallel-tempering workload exchanges once per step
+    for name, (_, _, _, _, _, opts, _) in bench.WORKLOADS.items():
+        assert opts["exchange_every"] == bench.SWEEPS_PER_STEP, name
+        assert bench.SWEEPS_PER_STEP % opts["measure_every"] == 0, name
+    for _, _, _, opts in bench.BATCH_MODELS:
+        assert opts["exchange_every"] == bench.SWEEPS_PER_STEP
+    # the timed sweeps are whole replays of one captured graph
+    for steps in (1, 5, 20, 37, 300):
+        sweeps = steps * bench.SWEEPS_PER_STEP
+        g = bench.graph_length(sweeps)
+        assert sweeps % g == 0 and 1 <= g <= 200
+    assert bench.graph_length(200) == 200 and bench.graph_length(3000) == 200
