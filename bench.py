@@ -134,6 +134,12 @@ def graph_length(sweeps, cap=200):
 SWEEPS_PER_STEP = 10
 
 
+def warmup_sweeps(warmup_steps, G):
+    """Warm-up sweeps: the requested steps rounded up to whole replays of the
+    timed graph of G sweeps, at least one replay."""
+    return max(1, -(-warmup_steps * SWEEPS_PER_STEP // G)) * G
+
+
 def beyond_mall(args, ising_hip, opts, layout):
     """int8 Ising PT with a working set above the 256 MB Infinity Cache: L=256 x 32
     replicas (512 MiB of spins), same ladder, cadence and thread-step layout;
@@ -144,9 +150,9 @@ def beyond_mall(args, ising_hip, opts, layout):
         kw.update(ising_width=layout[0], ising_runs=layout[1], ising_spt=layout[2])
     pt = ising_hip.PTEnsemble("ising", [L], R, linspace(4.40, 4.62, R), args.seed,
                               record_capacity=steps // kw["measure_every"] + 8, **kw)
-    pt.warmup(max(20, args.warmup * SWEEPS_PER_STEP))
     G = graph_length(steps)
     pt.prepare_graph(G)
+    pt.warmup(warmup_sweeps(args.warmup, G))
     pt.sample(0)
     pt.sync()
     t0 = time.perf_counter()
@@ -238,7 +244,9 @@ def run_batched(args, ising_hip):
         members.append(ising_hip.PTEnsemble(model, BATCH_SIZES, R, linspace(tlo, thi, R), args.seed,
                                             record_capacity=cap, graph_sweeps=0, **opts))
     G = graph_length(sweeps, cap=50)
-    batch = ising_hip.Batch(members, graph_sweeps=G)
+    mode = {"branches": ising_hip.Batch.BRANCHES, "worklist": ising_hip.Batch.WORKLIST,
+            "persistent": ising_hip.Batch.PERSISTENT}[args.batch_mode]
+    batch = ising_hip.Batch(members, graph_sweeps=G, mode=mode)
     warm = max(args.warmup * SWEEPS_PER_STEP, G)  # the batch graph is captured by the first call of >= G sweeps
     batch.warmup(warm)
     batch.sample(0)
@@ -272,13 +280,17 @@ def run_batched(args, ising_hip):
                                "T=1.40-1.50, L=8,12,16,24,32,48,64,96,128 x 32 replicas each, exchange every 10, "
                                "measure every 5, continuous models 1 Metropolis + 2 overrelaxation passes",
                    "sizes": BATCH_SIZES, "replicas": R, "models": [m for m, _, _, _ in BATCH_MODELS],
+                   "batch_mode": args.batch_mode,
                    "sweeps_per_step": SWEEPS_PER_STEP, "sweeps": sweeps, "warmup_sweeps": warm,
                    "updates_per_step": 3 * sites * SWEEPS_PER_STEP,
                    "site_passes_per_sweep": passes * sites, "graph_sweeps": G,
                    "combined_updates_per_ns": round(passes * sites * sweeps / (wall * 1e9), 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "whole sweep graph (all kernels of the three models, concurrent branches)",
+                     "kernel": {"branches": "whole sweep graph (each model's kernels, concurrent branches)",
+                                "worklist": "whole sweep: batch_pass_kernel x 7 (work list of all models' blocks)",
+                                "persistent": "whole sweep: batch_pass_persistent_kernel x 7 (persistent grid)"}[
+                                    args.batch_mode],
                      "alg_bytes_per_sweep": int(alg_bytes_per_sweep), "avg_sweep_us": round(sweep_s * 1e6, 3),
                      "device_updates_per_ns": round(updates / (ev_ms * 1e6), 3)},
     }
@@ -646,6 +658,9 @@ def main():
                          "'auto' times all layouts during warm-up and keeps the fastest")
     ap.add_argument("--sizes", default=None, metavar="L,L,...",
                     help="single-model workloads: lattice sizes in place of the default ladder (profiling runs)")
+    ap.add_argument("--batch-mode", default="branches", choices=["branches", "worklist", "persistent"],
+                    help="batched workload: each member's own launches on a graph branch per member, one "
+                         "work-list launch per pass for all members, or that work list on a persistent grid")
     ap.add_argument("--no-beyond-mall", action="store_true",
                     help="ising workload: skip the L=256 x 32 sub-result (working set above the Infinity Cache)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -705,12 +720,14 @@ def main():
             layout = tuple(int(v) for v in args.layout.split(","))
     elif model in ("xy", "heisenberg"):
         layout = pt.autotune(20)  # the overrelaxation pass's sites per thread
-    pt.warmup(args.warmup * SWEEPS_PER_STEP)
-    # the timed sweeps replay one captured graph of G sweeps, G dividing the
-    # timed sweeps (production runs replay graphs too); captured here, outside
-    # the timed region
+    # the timed sweeps replay one captured graph of G sweeps (production runs
+    # replay graphs too), captured here; the warm-up sweeps are whole replays of
+    # the same graph, at least one, since the first replay of a newly
+    # instantiated graph runs ~1% slower than the later ones
     G = graph_length(sweeps)
+    warm_sweeps = warmup_sweeps(args.warmup, G)
     pt.prepare_graph(G)
+    pt.warmup(warm_sweeps)
     pt.sample(0)  # enters the sampling phase (a control-block round trip), outside the timed region
     pt.sync()
 
@@ -759,7 +776,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "sizes": sizes, "replicas": R,
                    "step": "one exchange interval: %d sweeps of every lattice" % SWEEPS_PER_STEP,
-                   "sweeps_per_step": SWEEPS_PER_STEP, "sweeps": sweeps, "warmup_sweeps": args.warmup * SWEEPS_PER_STEP,
+                   "sweeps_per_step": SWEEPS_PER_STEP, "sweeps": sweeps, "warmup_sweeps": warm_sweeps,
                    "updates_per_step": sites * SWEEPS_PER_STEP, "site_passes_per_sweep": passes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -413,6 +413,23 @@ int ising_batch_warmup(ising_batch_t* b, int64_t sweeps);
 int ising_batch_sample(ising_batch_t* b, int64_t sweeps);
 int ising_batch_sync(ising_batch_t* b, double* last_ms);
 
+/* How a batch launches its members' passes.  No counterpart in the reference.
+ *   ISING_BATCH_BRANCHES (default): each member's own launches, on a graph
+ *     branch per member.
+ *   ISING_BATCH_WORKLIST: one launch per pass for all members: a work list of
+ *     (member, size, replica, tile) blocks; a sweep is 2 n_overrelax + 3
+ *     launches (overrelaxation passes, Metropolis colour 0 with the Ising
+ *     colour-0 pass, colour 1 likewise, the continuous closing step).  At
+ *     most one int8 Ising ensemble (two-launch sweeps) and two continuous
+ *     ensembles (XY, Heisenberg, Metropolis passes) with equal n_overrelax.
+ *   ISING_BATCH_PERSISTENT: the same work list walked by a persistent grid
+ *     (the blocks the device holds at once).
+ * Lattices, records and permutations equal the branch mode's bit for bit. */
+#define ISING_BATCH_BRANCHES 0
+#define ISING_BATCH_WORKLIST 1
+#define ISING_BATCH_PERSISTENT 2
+int ising_batch_set_mode(ising_batch_t* b, int32_t mode);
+
 /* ------------------------------------------------------------------------
  * Host-side statistics and PT helpers (no GPU needed).
  * ------------------------------------------------------------------------ */

@@ -2446,6 +2446,8 @@ struct ising_batch {
     hipGraphExec_t exec = nullptr;
     int G = 32;
     bool timed = false;
+    int mode = ISING_BATCH_BRANCHES;  // ISING_BATCH_*
+    int resident = 0;                 // persistent work list: blocks the device holds at once
     ~ising_batch() {
         if (st) (void)hipStreamSynchronize(st);
         for (ising_pt* p : m) p->ext_st = nullptr;
@@ -2471,6 +2473,52 @@ struct ising_batch {
         return ISING_OK;
     }
 
+    // Work-list mode: every pass of every member in one launch of the work-list
+    // kernel on the batch stream (batch.hip); n sweeps, then the Ising
+    // member's closing kernel.
+    int enqueue_worklist(int64_t n) {
+        ising_pt* pi = nullptr;
+        std::vector<ising_pt*> pc;
+        for (ising_pt* p : m) {
+            if (p->model == ISING_MODEL_ISING) pi = p;
+            else pc.push_back(p);
+        }
+        BatchPass base{};
+        if (pi) base.ia = pi->ising_args();
+        for (size_t i = 0; i < pc.size(); ++i) base.ca[i] = pc[i]->cont_args();
+        const int nor = pc.empty() ? 0 : pc[0]->opts.n_overrelax;
+        const int maxb = mode == ISING_BATCH_PERSISTENT ? resident : 0;
+        auto pass = [&](int kind, int color) -> hipError_t {
+            BatchPass bp = base;
+            bp.color = color;
+            auto add = [&](int k, int member, int count, int tiles) {
+                bp.seg[bp.nseg] = WorkSeg{k, member, count, 0, 0, tiles};
+                ++bp.nseg;
+            };
+            if (kind == W_METRO && pi) add(W_ISING, 0, pi->nblocks, pi->nblocks);
+            for (size_t i = 0; i < pc.size(); ++i) {
+                const int tiles = pc[i]->nblocks;
+                if (kind == W_CLOSE)
+                    add(kind, (int)i, (pc[i]->total_reps + kContClosePerBlock - 1) / kContClosePerBlock, 0);
+                else  // an XY item holds two tiles
+                    add(kind, (int)i, pc[i]->model == ISING_MODEL_XY ? (tiles + 1) / 2 : tiles, tiles);
+            }
+            return launch_batch_pass(bp, maxb, st);
+        };
+        for (int64_t i = 0; i < n; ++i) {
+            for (int k = 0; k < nor; ++k) {
+                HIP_TRY(pass(W_OR, 0));
+                HIP_TRY(pass(W_OR, 1));
+            }
+            HIP_TRY(pass(W_METRO, 0));
+            HIP_TRY(pass(W_METRO, 1));
+            if (!pc.empty()) HIP_TRY(pass(W_CLOSE, 0));
+        }
+        if (n > 0 && pi) HIP_TRY(launch_ising_close(pi->ising_args(), st));
+        for (ising_pt* p : m) p->t_host += n;
+        return ISING_OK;
+    }
+
     int run(int64_t n) {
         int rc = members_after_batch();
         if (rc) return rc;
@@ -2488,7 +2536,13 @@ struct ising_batch {
                 HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
                 hipError_t e = hipEventRecord(fork, st);
                 int rc2 = ISING_OK;
-                for (size_t i = 0; i < m.size() && e == hipSuccess && rc2 == ISING_OK; ++i) {
+                if (mode != ISING_BATCH_BRANCHES) {
+                    rc2 = enqueue_worklist(G);
+                    if (rc2 == ISING_OK)
+                        for (ising_pt* p : m) p->t_host -= G;  // capture enqueued nothing
+                }
+                for (size_t i = 0; mode == ISING_BATCH_BRANCHES && i < m.size() && e == hipSuccess && rc2 == ISING_OK;
+                     ++i) {
                     e = hipStreamWaitEvent(m[i]->st, fork, 0);
                     if (e != hipSuccess) break;
                     rc2 = m[i]->enqueue_sweeps(G);
@@ -2515,7 +2569,10 @@ struct ising_batch {
             for (ising_pt* p : m) p->t_host += reps * G;
             n -= reps * G;
         }
-        if (n > 0) {
+        if (n > 0 && mode != ISING_BATCH_BRANCHES) {
+            rc = enqueue_worklist(n);
+            if (rc) return rc;
+        } else if (n > 0) {
             rc = members_after_batch();
             if (rc) return rc;
             for (ising_pt* p : m) {
@@ -2568,6 +2625,41 @@ int ising_batch_create(ising_batch_t** out, ising_pt_t* const* members, int32_t 
 }
 
 void ising_batch_destroy(ising_batch_t* b) { delete b; }
+
+int ising_batch_set_mode(ising_batch_t* b, int32_t mode) {
+    if (!b) return fail(ISING_E_ARG, "null handle");
+    if (mode != ISING_BATCH_BRANCHES && mode != ISING_BATCH_WORKLIST && mode != ISING_BATCH_PERSISTENT)
+        return fail(ISING_E_ARG, "mode must be ISING_BATCH_BRANCHES, ISING_BATCH_WORKLIST or ISING_BATCH_PERSISTENT");
+    if (mode != ISING_BATCH_BRANCHES) {
+        int n_ising = 0, n_cont = 0, nor = -1;
+        for (ising_pt* p : b->m) {
+            if (p->model == ISING_MODEL_ISING) {
+                if (p->fused_on) return fail(ISING_E_ARG, "work-list batches need two-launch Ising sweeps (ising_fused = 0)");
+                ++n_ising;
+            } else if (p->model == ISING_MODEL_XY || p->model == ISING_MODEL_HEISENBERG) {
+                if (p->learned_on || p->heatbath_on)
+                    return fail(ISING_E_ARG, "work-list batches run Metropolis passes (no learned or heat-bath passes)");
+                if (nor >= 0 && p->opts.n_overrelax != nor)
+                    return fail(ISING_E_ARG, "the continuous members of a work-list batch need the same n_overrelax");
+                nor = p->opts.n_overrelax;
+                ++n_cont;
+            } else {
+                return fail(ISING_E_ARG, "work-list batches hold int8 Ising, XY and Heisenberg ensembles");
+            }
+        }
+        if (n_ising > 1 || n_cont > kBatchCont)
+            return fail(ISING_E_ARG, "a work-list batch holds at most one Ising and two continuous ensembles");
+        if (mode == ISING_BATCH_PERSISTENT) {
+            b->resident = batch_pass_resident_blocks();
+            if (b->resident < 8) return fail(ISING_E_HIP, "cannot size the persistent grid on this device");
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(b->st));
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+    b->mode = mode;
+    return ISING_OK;
+}
 
 int ising_batch_warmup(ising_batch_t* b, int64_t sweeps) {
     if (!b || sweeps < 0) return fail(ISING_E_ARG, "null handle or negative sweeps");

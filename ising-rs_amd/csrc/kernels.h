@@ -101,6 +101,31 @@ hipError_t launch_learn_infer(const uint4* frag, int H, const float* x, int64_t 
 size_t learn_frag_bytes(int H);  // bytes of the fragment buffer of hidden width H (0 if unsupported)
 hipError_t launch_learn_frag(const uint16_t* net, int H, uint4* frag, hipStream_t st);  // fragments from BF16 parameters
 
+// Work-list launch of several ensembles (batch.hip): the work of one pass of
+// every member, as segments of a list of blocks.
+enum WorkKind : int32_t { W_ISING = 0, W_OR = 1, W_METRO = 2, W_CLOSE = 3 };
+struct WorkSeg {
+    int32_t kind;    // WorkKind
+    int32_t member;  // continuous member (index into BatchPass::ca) of W_OR, W_METRO, W_CLOSE
+    int32_t count;   // work items: the blocks of the member's own launch of this pass
+    int32_t chunk;   // items of the segment per XCD, ceil(count / 8) (set by launch_batch_pass)
+    int32_t kfirst;  // position of the segment's chunk in every XCD's list (set by launch_batch_pass)
+    int32_t tiles;   // the member's tiles (blocks of its own launch); an XY item holds two tiles
+};
+constexpr int kBatchSegs = 3;  // segments per launch: the Ising member and two continuous members
+constexpr int kBatchCont = 2;  // continuous members of a work-list batch
+struct BatchPass {
+    IsingArgs ia;             // the int8 Ising member (when a segment is W_ISING)
+    ContArgs ca[kBatchCont];  // the continuous members
+    int32_t color;
+    int32_t nseg;
+    int32_t per_xcd;          // items in every XCD's list (set by launch_batch_pass)
+    int32_t pad;
+    WorkSeg seg[kBatchSegs];
+};
+hipError_t launch_batch_pass(BatchPass bp, int max_blocks, hipStream_t st);  // max_blocks 0: one block per item
+int batch_pass_resident_blocks();  // blocks of batch_pass_kernel the current device holds at once (0 on error)
+
 hipError_t launch_ising_close(const IsingArgs& a, hipStream_t st);
 hipError_t launch_cont_metro(const ContArgs& a, int nblocks, int color, int closes, hipStream_t st);
 hipError_t launch_cont_close(const ContArgs& a, hipStream_t st);
@@ -112,6 +137,7 @@ hipError_t launch_cont_helicity(const DevGroup* groups, int ngroups, const float
 hipError_t launch_cont_measure(const DevGroup* groups, int ngroups, const float* pool, int ncomp, double J,
                                double* part, int nblocks, hipStream_t st);
 constexpr int kContTile = 256;  // half-sites per block in continuous launches
+constexpr int kContClosePerBlock = 4;  // replicas per block of a continuous closing launch (a wave each)
 
 hipError_t launch_wham_denom(const double* E, int64_t n, int K, const double* logN, const double* f,
                              const double* beta, double* logD, hipStream_t st);

@@ -138,6 +138,7 @@ _SIGS = {
     "ising_batch_warmup": (C.c_int, [_vp, C.c_int64]),
     "ising_batch_sample": (C.c_int, [_vp, C.c_int64]),
     "ising_batch_sync": (C.c_int, [_vp, _f64p]),
+    "ising_batch_set_mode": (C.c_int, [_vp, C.c_int32]),
     "ising_load_reduce_kernels": (C.c_int, []),
     "ising_reduce_ising": (C.c_int, [_vp, C.c_int32, C.c_float, _f64p, _f64p]),
     "ising_measure_ising_gpu": (C.c_int, [_vp, C.c_int32, C.c_float, _f64p, _f64p]),
@@ -572,12 +573,22 @@ class PTEnsemble:
 class Batch:
     """Several PTEnsembles advanced together by one captured graph per block of sweeps."""
 
-    def __init__(self, members, graph_sweeps=32):
+    BRANCHES, WORKLIST, PERSISTENT = 0, 1, 2
+
+    def __init__(self, members, graph_sweeps=32, mode=0):
+        """mode: BRANCHES (each member's own launches on a graph branch per
+        member), WORKLIST (one launch per pass for all members, over a work list
+        of their blocks) or PERSISTENT (that work list walked by a persistent grid)."""
         self.members = list(members)
         arr = (_vp * len(self.members))(*[m._h for m in self.members])
         self._arr = arr
         self._h = _vp()
         _check(lib.ising_batch_create(C.byref(self._h), arr, len(self.members), int(graph_sweeps)))
+        if mode:
+            self.set_mode(mode)
+
+    def set_mode(self, mode):
+        _check(lib.ising_batch_set_mode(self._h, int(mode)))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -52,3 +52,10 @@ def test_a_step_is_one_exchange_interval_of_whole_graph_replays():
         g = bench.graph_length(sweeps)
         assert sweeps % g == 0 and 1 <= g <= 200
     assert bench.graph_length(200) == 200 and bench.graph_length(3000) == 200
+
+
+def test_warmup_is_whole_replays_of_the_timed_graph():
+    import bench
+    for w, G, want in ((5, 200, 200), (0, 200, 200), (30, 200, 400), (20, 200, 200), (21, 200, 400), (3, 10, 30)):
+        got = bench.warmup_sweeps(w, G)
+        assert got == want and got % G == 0 and got >= w * bench.SWEEPS_PER_STEP

@@ -82,3 +82,62 @@ def test_batch_rejects_shared_members():
         ising_hip.Batch([a])
     batch.close()
     ising_hip.Batch([a]).close()  # free again once the first batch is gone
+
+
+def make_worklist_members(seed):
+    kw = dict(exchange_every=4, measure_every=2, graph_sweeps=0)
+    ckw = dict(kw, n_overrelax=2, renorm_every=7)
+    return [
+        ising_hip.PTEnsemble("ising", [8, 16, 32], 6, o.linspace(4.4, 4.62, 6), seed, **kw),
+        ising_hip.PTEnsemble("xy", [8, 12], 6, np.linspace(2.1, 2.3, 6), seed + 1, delta=1.5, adapt_every=3, **ckw),
+        ising_hip.PTEnsemble("heisenberg", [8, 6], 6, np.linspace(1.40, 1.46, 6), seed + 2, delta=1.0, adapt_every=3,
+                             **ckw),
+    ]
+
+
+@pytest.mark.parametrize("mode", [ising_hip.Batch.WORKLIST, ising_hip.Batch.PERSISTENT])
+def test_worklist_batch_equals_branch_batch(mode):
+    """One launch per pass over a work list of every member's blocks (and the
+    same list on a persistent grid) gives the branch batch's lattices,
+    permutations, step sizes and records bit for bit (every block runs its
+    member's own body, XY's with two sites per thread in half blocks)."""
+    ref, wl = make_worklist_members(31), make_worklist_members(31)
+    b_ref = ising_hip.Batch(ref, graph_sweeps=3)
+    b_wl = ising_hip.Batch(wl, graph_sweeps=3, mode=mode)
+    for b in (b_ref, b_wl):
+        b.warmup(10)   # step-size adaptation in warm-up
+    outs = []
+    for b, ms in ((b_ref, ref), (b_wl, wl)):
+        recs = []
+        for n in (6, 1, 8):  # graph replays and direct launches
+            b.sample(n)
+            b.sync()
+            recs.append([ms[0].read_ising(), ms[1].read_cont(), ms[2].read_cont()])
+        outs.append(recs)
+    for ra, rb in zip(*outs):
+        for xa, xb in zip(ra, rb):
+            for u, v in zip(xa, xb):
+                assert np.array_equal(u, v)
+    for p, q in zip(ref, wl):
+        assert p.sweeps() == q.sweeps() == 10 + 15
+        for g in range(len(p.sizes)):
+            assert np.array_equal(p.get_perm(g)[0], q.get_perm(g)[0])
+            for r in range(6):
+                assert np.array_equal(p.get_spins(g, r), q.get_spins(g, r))
+    for i in (1, 2):
+        assert np.array_equal(ref[i].get_deltas(0), wl[i].get_deltas(0))
+
+
+def test_worklist_batch_rejects_other_members():
+    temps = o.linspace(4.4, 4.62, 32)
+    msc = ising_hip.PTEnsemble("ising_msc", [8], 32, temps, 1)
+    b = ising_hip.Batch([msc])
+    with pytest.raises(ising_hip.IsingError):
+        b.set_mode(ising_hip.Batch.WORKLIST)
+    b.close()
+    xy = ising_hip.PTEnsemble("xy", [8], 2, np.linspace(2.1, 2.3, 2), 1, n_overrelax=2)
+    heis = ising_hip.PTEnsemble("heisenberg", [8], 2, np.linspace(1.4, 1.5, 2), 1, n_overrelax=1)
+    b = ising_hip.Batch([xy, heis])
+    with pytest.raises(ising_hip.IsingError):
+        b.set_mode(ising_hip.Batch.WORKLIST)  # unequal overrelaxation counts
+    b.close()

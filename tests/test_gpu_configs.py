@@ -1,7 +1,8 @@
 """The batched and continuous workloads at their full sizes on the GPU.
 
 Batched uneven PT: Ising + XY + Heisenberg, L = 8, 12, 16, 24, 32, 48, 64, 96,
-128 x 32 replicas each (1.85 GB of spins), advanced by one batch graph.  At
+128 x 32 replicas each (1.85 GB of spins), advanced by one batch graph, either
+of each member's own launches or of one work-list launch per pass.  At
 this size the oracle cannot replay the continuous members, so they are checked
 through size-independent properties: every record equals a standalone FP64
 reduction of the downloaded lattice, spins stay unit vectors, permutations stay
@@ -71,7 +72,8 @@ def check_cont_member(pt, nc, temps, seed, model, sizes, replicas, exchange_ever
         assert np.array_equal(t2r[r2t], np.arange(R))
 
 
-def test_batched_uneven_three_models_full_size():
+@pytest.mark.parametrize("mode", [ising_hip.Batch.BRANCHES, ising_hip.Batch.WORKLIST])
+def test_batched_uneven_three_models_full_size(mode):
     seed = 2024
     t_ising = o.linspace(4.40, 4.62, R)
     t_xy = np.linspace(2.15, 2.25, R)
@@ -82,7 +84,7 @@ def test_batched_uneven_three_models_full_size():
                ising_hip.PTEnsemble("xy", SIZES, R, t_xy, seed, delta=1.5, **ckw),
                ising_hip.PTEnsemble("heisenberg", SIZES, R, t_heis, seed, delta=1.0, **ckw)]
     assert sum(p.device_bytes() for p in members) > 1.8e9
-    batch = ising_hip.Batch(members, graph_sweeps=5)
+    batch = ising_hip.Batch(members, graph_sweeps=5, mode=mode)
     warm, samp = 5, 20
     batch.warmup(warm)
     batch.sample(samp)
