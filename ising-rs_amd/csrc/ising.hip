@@ -86,10 +86,8 @@ __global__ __launch_bounds__(kThreads) void ising_close_kernel(IsingArgs a) {
     const int64_t* acc = acc_of(a, tl);
     if (pfl & F_MEAS) write_records(a, acc, a.r2t + (round & 1) * a.total_reps);
     if (pfl & F_EXCH) {
-        for (int g = 0; g < a.ngroups; ++g) {
-            exch_write_group(a, a.groups[g], round, acc);
-            __syncthreads();
-        }
+        exch_write_all(a, round, acc);
+        __syncthreads();
         if (a.mplanes) msc_planes(a, (int)((round + 1) & 1));
     }
     __syncthreads();

@@ -121,6 +121,52 @@ __device__ __forceinline__ void exch_write_group(const IsingArgs& a, const DevGr
     }
 }
 
+// exch_write_group for every group at once (one block; the closing kernel):
+// the same permutations, counts and tables, each phase over all groups, so
+// the block waits at three barriers instead of three per group.
+__device__ __forceinline__ void exch_write_all(const IsingArgs& a, int64_t round, const int64_t* acc) {
+    const int po = (int)(round & 1);
+    const int32_t* r2t_o = a.r2t + (int64_t)po * a.total_reps;
+    const int32_t* t2r_o = a.t2r + (int64_t)po * a.total_reps;
+    int32_t* r2t_n = a.r2t + (int64_t)(po ^ 1) * a.total_reps;
+    int32_t* t2r_n = a.t2r + (int64_t)(po ^ 1) * a.total_reps;
+    for (int k = threadIdx.x; k < a.total_reps; k += blockDim.x) {
+        r2t_n[k] = r2t_o[k];
+        t2r_n[k] = t2r_o[k];
+    }
+    __syncthreads();
+    int base = 0;  // pairs of the groups before g
+    int64_t tries = 0;
+    for (int g = 0; g < a.ngroups; ++g) {
+        const DevGroup G = a.groups[g];
+        const int npairs = (G.R - po) / 2;
+        for (int q = (int)threadIdx.x - base; q < npairs; q += blockDim.x) {
+            if (q < 0) continue;
+            const int i = po + 2 * q;
+            if (exch_pair(a, G, i, round, t2r_o, acc)) {
+                const int ri = t2r_o[G.rep0 + i], rj = t2r_o[G.rep0 + i + 1];
+                t2r_n[G.rep0 + i] = rj;
+                t2r_n[G.rep0 + i + 1] = ri;
+                r2t_n[G.rep0 + ri] = i + 1;
+                r2t_n[G.rep0 + rj] = i;
+                atomicAdd((unsigned long long*)&a.ctl->exch_accepts, 1ull);
+            }
+        }
+        base = (base + npairs) % (int)blockDim.x;
+        tries += npairs;
+    }
+    if (threadIdx.x == 0) atomicAdd((unsigned long long*)&a.ctl->exch_tries, (unsigned long long)tries);
+    if (a.thr_rep) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < a.total_reps * kIsingTab; k += blockDim.x) {
+            const int gr = k / kIsingTab, e = k - gr * kIsingTab;
+            int g = 0;
+            while (g + 1 < a.ngroups && gr >= a.groups[g + 1].rep0) ++g;
+            a.thr_rep[(int64_t)gr * kIsingTab + e] = a.thr[(int64_t)(a.groups[g].rep0 + r2t_n[gr]) * kIsingTab + e];
+        }
+    }
+}
+
 // Measurement records of sweep `ts` (one per replica, in global replica
 // order, temperature index from the permutation in use during that sweep).
 // All threads of one block; returns nothing, updates n_rec / overflow.
