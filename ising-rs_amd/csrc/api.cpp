@@ -2480,6 +2480,10 @@ struct ising_batch {
         ising_pt* pi = nullptr;
         std::vector<ising_pt*> pc;
         for (ising_pt* p : m) {
+            // a member switched to one-launch, learned or heat-bath sweeps after the
+            // mode was set would silently run the work list's passes instead
+            if (p->fused_on || p->learned_on || p->heatbath_on)
+                return fail(ISING_E_STATE, "a work-list batch member runs one-launch, learned or heat-bath sweeps");
             if (p->model == ISING_MODEL_ISING) pi = p;
             else pc.push_back(p);
         }

@@ -141,3 +141,9 @@ def test_worklist_batch_rejects_other_members():
     with pytest.raises(ising_hip.IsingError):
         b.set_mode(ising_hip.Batch.WORKLIST)  # unequal overrelaxation counts
     b.close()
+    b = ising_hip.Batch([xy], mode=ising_hip.Batch.WORKLIST)
+    b.sample(2)
+    xy.heat_bath(True)  # a member switched to passes the work list does not run
+    with pytest.raises(ising_hip.IsingError):
+        b.sample(2)
+    b.close()
