@@ -137,7 +137,7 @@ __device__ __forceinline__ void reduce_replica(const ContArgs& a, const int lb) 
 // Closing step of a continuous sweep (one block of cont_close_kernel, after the sums):
 // records from the per-replica sums, exchange, delta adaptation during
 // warm-up, sweep counter.
-__device__ void close_cont(const ContArgs& a, int64_t t, int32_t fl) {
+static __device__ void close_cont(const ContArgs& a, int64_t t, int32_t fl) {
     const bool meas = (fl & F_MEAS) != 0, exch = (fl & F_EXCH) != 0;
     DevCtl* ctl = a.ctl;
     const int tid = threadIdx.x;
