@@ -398,37 +398,38 @@ LEARNED_MODELS = [
 ]
 
 
-def run_learned(args, ising_hip):
+def learned_compare(steps, warmup, seed, hidden, train_steps, learned_or, cluster=True):
     """Learned proposal against plain Metropolis + overrelaxation, and the
     heat-bath ceiling: XY L = 32, 64 and Heisenberg L = 32, 32 replicas each.
     Chains: plain (Metropolis + 2 overrelaxation passes per sweep), learned
-    (the trained network's Metropolis-Hastings pass + 2 overrelaxation passes),
-    heat bath (the exact single-site conditional + 2 overrelaxation passes: the
-    best any proposal that depends on the neighbours only can do per pass), and
-    for XY embedded Swendsen-Wang.  Same ensembles, temperatures and seed;
-    measured every sweep."""
-    steps = args.steps
+    (the trained network's Metropolis-Hastings pass + `learned_or`
+    overrelaxation passes), heat bath (the exact single-site conditional + 2
+    overrelaxation passes: the best any proposal that depends on the
+    neighbours only can do per pass), and with `cluster` for XY embedded
+    Swendsen-Wang.  Same ensembles, temperatures and seed; measured every
+    sweep; `warmup` sweeps, then `steps` sampled sweeps per chain."""
+    import ising_hip
     res = {}
     for model, sizes, (tlo, thi), dlt in LEARNED_MODELS:
         temps = linspace(tlo, thi, R)
         kw = dict(exchange_every=10, measure_every=1, n_overrelax=2, delta=dlt, adapt_every=10, adapt_target=0.5,
                   renorm_every=100, record_capacity=steps + 8)
         out, stats = {}, {}
-        for mode in ("plain", "learned", "heat_bath") + (("cluster",) if model == "xy" else ()):
-            nor = args.learned_or if mode == "learned" else 2
-            pt = ising_hip.PTEnsemble(model, sizes, R, temps, args.seed, **dict(kw, n_overrelax=nor))
-            pt.warmup(args.warmup)
+        for mode in ("plain", "learned", "heat_bath") + (("cluster",) if model == "xy" and cluster else ()):
+            nor = learned_or if mode == "learned" else 2
+            pt = ising_hip.PTEnsemble(model, sizes, R, temps, seed, **dict(kw, n_overrelax=nor))
+            pt.warmup(warmup)
             if mode == "cluster":
                 out[mode] = xy_cluster_leg(pt, sizes, R, temps, steps, ising_hip)
                 pt.close()
                 continue
             train = None
             if mode == "learned":
-                pt.learn_init(args.hidden, seed=7)
+                pt.learn_init(hidden, seed=7)
                 t0 = time.perf_counter()
-                loss = pt.learn_train(args.train_steps, batch=8192, lr=3e-3)
+                loss = pt.learn_train(train_steps, batch=8192, lr=3e-3)
                 pt.sync()
-                train = {"steps": args.train_steps, "batch": 8192, "seconds": round(time.perf_counter() - t0, 3),
+                train = {"steps": train_steps, "batch": 8192, "seconds": round(time.perf_counter() - t0, 3),
                          "final_loss": loss}
                 pt.learn_enable(True)
                 pt.warmup(50)
@@ -461,6 +462,36 @@ def run_learned(args, ising_hip):
                                                               / out["plain"]["independent_samples_per_s_" + q], 4)
         out["sizes"], out["temperatures"] = sizes, [tlo, thi]
         res[model] = out
+    return res
+
+
+def learned_digest(res, steps, warmup, hidden, train_steps):
+    """The learned-vs-plain comparison in brief, for the headline line: per
+    model, the ratios of independent samples/s (E, M^2) and the equal-error
+    speed-ups (E, U4) of the learned and heat-bath chains over the plain one."""
+    out = {"chains": "plain Metropolis + 2 overrelaxation passes vs learned MLP proposal (exact Metropolis-"
+                     "Hastings) vs exact heat bath, each + 2 overrelaxation passes; 32 replicas, measured every "
+                     "sweep, exchange every 10",
+           "sweeps": steps, "warmup_sweeps": warmup, "hidden": hidden, "train_steps": train_steps}
+    for model, r in res.items():
+        d = {"sizes": r["sizes"], "temperatures": r["temperatures"]}
+        for mode in ("learned", "heat_bath"):
+            for q in ("E", "M2"):
+                d[mode + "_over_plain_samples_" + q] = r[mode + "_over_plain_samples_" + q]
+            for q in ("E", "U4"):
+                d[mode + "_equal_error_speedup_" + q] = r[mode]["equal_error_speedup_" + q]
+                d[mode + "_z_vs_plain_max_" + q] = r[mode]["z_vs_plain_" + q]["max_abs"]
+        d["U4_err_mean"] = {m: r[m]["U4_err_mean"] for m in ("plain", "learned", "heat_bath")}
+        d["learned_acceptance"] = r["learned"]["learned_acceptance"]
+        out[model] = d
+    return out
+
+
+def run_learned(args, ising_hip):
+    """The learned-proposal workload (see learned_compare) as a bench line whose
+    value is the learned chains' independent samples/s of E."""
+    steps = args.steps
+    res = learned_compare(steps, args.warmup, args.seed, args.hidden, args.train_steps, args.learned_or)
     value = sum(res[m]["learned"]["independent_samples_per_s_E"] for m in res)
     line = {
         "metric": "independent samples/s (learned proposal)",
@@ -644,8 +675,10 @@ def main():
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--workload", default="ising", choices=sorted(WORKLOADS) + ["batched", "learned", "kz", "coarsening", "fss",
                                                                   "cluster"])
-    ap.add_argument("--hidden", type=int, default=64,
-                    help="learned workload: MLP hidden width (8, 16, 32 or 64; 64 = 8963 parameters)")
+    ap.add_argument("--hidden", type=int, default=16,
+                    help="learned proposal: MLP hidden width (8, 16, 32 or 64; 16 = 707 parameters, 64 = 8963; "
+                         "16 gives the most independent samples/s: the wider networks reach the same acceptance "
+                         "at a costlier pass)")
     ap.add_argument("--train-steps", type=int, default=400, help="learned workload: Adam steps")
     ap.add_argument("--learned-or", type=int, default=2,
                     help="learned workload: overrelaxation passes per sweep of the learned chain (plain keeps 2)")
@@ -663,6 +696,9 @@ def main():
                          "work-list launch per pass for all members, or that work list on a persistent grid")
     ap.add_argument("--no-beyond-mall", action="store_true",
                     help="ising workload: skip the L=256 x 32 sub-result (working set above the Infinity Cache)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="ising workload: skip the learned-vs-plain sub-result (XY L=32,64 and Heisenberg L=32 "
+                         "x 32 replicas, 1000 + 4000 sweeps per chain)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -794,6 +830,11 @@ def main():
             line["roofline"]["frac_at_12B_angle"] = round(achieved / 2 / HBM_PEAK_GBS, 4)
     if args.workload == "ising" and not fused and not args.no_beyond_mall and not args.sizes:
         line["config"]["beyond_mall"] = beyond_mall(args, ising_hip, opts, layout)
+    if args.workload == "ising" and not fused and not args.no_learned and not args.sizes:
+        lsteps, lwarm = 4000, 1000
+        line["config"]["learned_vs_plain"] = learned_digest(
+            learned_compare(lsteps, lwarm, args.seed, args.hidden, args.train_steps, args.learned_or, cluster=False),
+            lsteps, lwarm, args.hidden, args.train_steps)
     if layout is not None and model == "ising":
         line["config"]["layout"] = {"run_width": layout[0], "runs_per_step": layout[1], "steps_per_tile": layout[2],
                                     "per_size": {L: list(lay) for L, lay in zip(sizes, per_size)}}

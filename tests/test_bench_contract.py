@@ -59,3 +59,26 @@ def test_warmup_is_whole_replays_of_the_timed_graph():
     for w, G, want in ((5, 200, 200), (0, 200, 200), (30, 200, 400), (20, 200, 200), (21, 200, 400), (3, 10, 30)):
         got = bench.warmup_sweeps(w, G)
         assert got == want and got % G == 0 and got >= w * bench.SWEEPS_PER_STEP
+
+
+def test_learned_digest_keeps_the_ratios_of_each_model():
+    import bench
+    chain = {"equal_error_speedup_E": 1.2, "equal_error_speedup_U4": 0.9, "U4_err_mean": 0.004,
+             "z_vs_plain_E": {"max_abs": 1.5, "frac_within_2": 1.0},
+             "z_vs_plain_U4": {"max_abs": 2.5, "frac_within_2": 0.9}}
+    res = {}
+    for model in ("xy", "heisenberg"):
+        r = {"sizes": [32], "temperatures": [1.0, 2.0], "plain": {"U4_err_mean": 0.005},
+             "learned": dict(chain, learned_acceptance=0.99), "heat_bath": dict(chain)}
+        for mode in ("learned", "heat_bath"):
+            for q in ("E", "M2"):
+                r[mode + "_over_plain_samples_" + q] = 1.25
+        res[model] = r
+    d = bench.learned_digest(res, 4000, 1000, 64, 400)
+    assert d["sweeps"] == 4000 and d["warmup_sweeps"] == 1000 and d["hidden"] == 64
+    for model in ("xy", "heisenberg"):
+        m = d[model]
+        assert m["learned_over_plain_samples_E"] == 1.25 and m["heat_bath_over_plain_samples_M2"] == 1.25
+        assert m["learned_equal_error_speedup_U4"] == 0.9 and m["learned_z_vs_plain_max_U4"] == 2.5
+        assert m["U4_err_mean"] == {"plain": 0.005, "learned": 0.004, "heat_bath": 0.004}
+        assert m["learned_acceptance"] == 0.99
