@@ -493,6 +493,18 @@ def run_learned(args, ising_hip):
     steps = args.steps
     res = learned_compare(steps, args.warmup, args.seed, args.hidden, args.train_steps, args.learned_or)
     value = sum(res[m]["learned"]["independent_samples_per_s_E"] for m in res)
+    # the learned pass (cont_learned_kernel, one colour, no measurement) against
+    # the HBM roofline at the plain pass's bytes per update, and its network's
+    # matrix-core work: 6 -> H -> H -> H -> 3, 9H + 2H^2 multiply-adds per site
+    bpu = {"xy": 24, "heisenberg": 36}
+    per_model = {}
+    for m, sizes, _, _ in LEARNED_MODELS:
+        sites = sum(L ** 3 for L in sizes) * R // 2
+        us = res[m]["learned"]["pass_us"]
+        per_model[m] = {"sites_per_pass": sites, "pass_us": us,
+                        "achieved_GBs": round(bpu[m] * sites / (us * 1e3), 1),
+                        "frac": round(bpu[m] * sites / (us * 1e3) / HBM_PEAK_GBS, 4),
+                        "mlp_tflops": round(2 * (9 * args.hidden + 2 * args.hidden ** 2) * sites / (us * 1e6), 2)}
     line = {
         "metric": "independent samples/s (learned proposal)",
         "value": round(value, 1),
@@ -513,6 +525,10 @@ def run_learned(args, ising_hip):
                    "hidden": args.hidden, "network_parameters": 6 * args.hidden + args.hidden
                    + 2 * (args.hidden * args.hidden + args.hidden) + 3 * args.hidden + 3,
                    "learned_overrelaxation_passes": args.learned_or, **res},
+        "roofline": {"bound": "hbm", "achieved": per_model["xy"]["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": per_model["xy"]["frac"], "traffic": None,
+                     "kernel": "cont_learned_kernel (XY learned pass, one colour, 24 B per update)",
+                     "per_model": per_model, "bf16_mfma_peak_tflops": 2500.0},
     }
     print(json.dumps(line), flush=True)
 
