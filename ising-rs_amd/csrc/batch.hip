@@ -3,17 +3,19 @@
 // launches of a batched parallel-tempering run, config 4).
 //
 // A sweep of a batch holding an int8 Ising ensemble and up to two continuous
-// ensembles (XY, Heisenberg) with n_or overrelaxation passes is 2 n_or + 3
-// launches, each over the concatenated work of every member for that pass:
+// ensembles (XY, Heisenberg) with n_or overrelaxation passes is 2 n_or + 5
+// launches, each over the concatenated work of the members for that pass:
 //
 //   n_or x (OR colour 0, OR colour 1)   continuous members' overrelaxation tiles  (batch_or_kernel)
-//   Metropolis colour 0                 Ising colour-0 tiles + continuous
-//                                       Metropolis tiles                         (batch_pass_kernel)
-//   Metropolis colour 1                 the same for colour 1
+//   Ising colour 0                      Ising colour-0 tiles                      (batch_pass_kernel<0, 0, true>)
+//   Metropolis colour 0                 continuous Metropolis tiles               (batch_pass_kernel<N0, N1, false>)
+//   Ising colour 1, Metropolis colour 1 the same for colour 1
 //   close                               continuous members' closing blocks       (batch_close_kernel)
 //
-// instead of 2 + 2 (2 n_or + 3) launches on one stream per member.  The work
-// list of a launch is a short table of segments in the kernel arguments, one
+// instead of 2 + 2 (2 n_or + 3) launches on one stream per member.  The Ising
+// pass has launches of its own: a kernel holding the Ising body has its
+// registers (71 VGPRs), which would set the occupancy of every continuous
+// Metropolis item beside it.  The work list of a launch is a short table of segments in the kernel arguments, one
 // per member, item j of a segment being block j of that member's own launch
 // of the pass.  A block runs the member's own block body (ising_pass.h,
 // cont_pass.h), so every result equals the per-member launches'.
@@ -67,7 +69,12 @@ namespace ising {
 template <int KIND, int NM>
 __device__ __forceinline__ void cont_item(const ContArgs& a, const int lb, const int color, const int tiles) {
     if (NM == 2) {
-        const int q = (int)(threadIdx.x >> 7), tid = (int)(threadIdx.x & 127), tile = 2 * lb + q;
+        // the half is uniform in a wave; read as a scalar, it keeps the tile and
+        // everything derived from it (size, replica, base pointers) in scalar
+        // registers, as in the member's own kernel (per lane: 51 VGPRs instead
+        // of 34 in the overrelaxation pass, and 24% slower XY sweeps)
+        const int q = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 7));
+        const int tid = (int)(threadIdx.x & 127), tile = 2 * lb + q;
         if (KIND == W_OR) {
             if (tile < tiles) cont_or_block<2, 2>(a, tile, color, tid);
         } else {
@@ -86,16 +93,17 @@ __device__ __forceinline__ void cont_member_item(const BatchPass& bp, const int 
     else if (N1) cont_item<KIND, N1 ? N1 : 2>(bp.ca[1], lb, bp.color, tiles);
 }
 
-// A Metropolis launch: the Ising member's colour pass and the continuous
-// members' Metropolis passes (N0, N1: components of continuous members 0 and
-// 1, 0 when absent).  At most 80 VGPRs, six waves per SIMD (the Ising body's
-// occupancy in its own launch).
-template <int N0, int N1>
+// A Metropolis launch: the Ising member's colour pass (ISING) or the
+// continuous members' Metropolis passes (N0, N1: components of continuous
+// members 0 and 1, 0 when absent).  At most 80 VGPRs, six waves per SIMD (the
+// Ising body's occupancy in its own launch).  The instance without the Ising
+// body keeps the continuous bodies' registers (45-50 VGPRs against 71).
+template <int N0, int N1, bool ISING>
 __global__ __launch_bounds__(kThreads, 6) void batch_pass_kernel(BatchPass bp) {
     const int x = (int)(blockIdx.x & 7), k = (int)(blockIdx.x >> 3);
     WL_ITEM(bp, x, k, s, lb);
     if (lb >= bp.seg[s].count) return;
-    if (bp.seg[s].kind == W_ISING) {
+    if (ISING && bp.seg[s].kind == W_ISING) {
         WL_ISING_GROUP(bp, lb, G)
         ising_half_block(bp.ia, lb, bp.color, G);
     } else {
@@ -145,23 +153,25 @@ __global__ __launch_bounds__(kThreads) void batch_close_kernel(BatchPass bp) {
 }
 
 template <int N0, int N1>
-void launch_batch_kinds(const BatchPass& bp, int kind, int max_blocks, hipStream_t st) {
+void launch_batch_kinds(const BatchPass& bp, int kind, bool ising, int max_blocks, hipStream_t st) {
     if (max_blocks > 0) {
         const int slots = max_blocks / 8 > 0 ? max_blocks / 8 : 1;
         hipLaunchKernelGGL((batch_pass_persistent_kernel<N0, N1>), dim3(8 * (bp.per_xcd < slots ? bp.per_xcd : slots)),
                            dim3(kThreads), 0, st, bp);
     } else if (kind == W_OR) {
         hipLaunchKernelGGL((batch_or_kernel<N0, N1>), dim3(8 * bp.per_xcd), dim3(kThreads), 0, st, bp);
+    } else if (ising) {
+        hipLaunchKernelGGL((batch_pass_kernel<N0, N1, true>), dim3(8 * bp.per_xcd), dim3(kThreads), 0, st, bp);
     } else {
-        hipLaunchKernelGGL((batch_pass_kernel<N0, N1>), dim3(8 * bp.per_xcd), dim3(kThreads), 0, st, bp);
+        hipLaunchKernelGGL((batch_pass_kernel<N0, N1, false>), dim3(8 * bp.per_xcd), dim3(kThreads), 0, st, bp);
     }
 }
 
 template <int N0>
-void launch_batch_n1(const BatchPass& bp, int n1, int kind, int max_blocks, hipStream_t st) {
-    if (n1 == 2) launch_batch_kinds<N0, 2>(bp, kind, max_blocks, st);
-    else if (n1 == 3) launch_batch_kinds<N0, 3>(bp, kind, max_blocks, st);
-    else launch_batch_kinds<N0, 0>(bp, kind, max_blocks, st);
+void launch_batch_n1(const BatchPass& bp, int n1, int kind, bool ising, int max_blocks, hipStream_t st) {
+    if (n1 == 2) launch_batch_kinds<N0, 2>(bp, kind, ising, max_blocks, st);
+    else if (n1 == 3) launch_batch_kinds<N0, 3>(bp, kind, ising, max_blocks, st);
+    else launch_batch_kinds<N0, 0>(bp, kind, ising, max_blocks, st);
 }
 
 hipError_t launch_batch_pass(BatchPass bp, int max_blocks, hipStream_t st) {
@@ -175,15 +185,22 @@ hipError_t launch_batch_pass(BatchPass bp, int max_blocks, hipStream_t st) {
         total += bp.seg[s].count;
     }
     if (total <= 0) return hipSuccess;
-    const int kind = bp.seg[bp.nseg - 1].kind;  // the Ising segment, if any, comes first
+    // the segments of a launch share one kind, except that a Metropolis
+    // launch may hold the Ising segment (always first) beside continuous ones
+    const int kind = bp.seg[bp.nseg - 1].kind;
+    const bool ising = bp.seg[0].kind == W_ISING;
     if (kind == W_CLOSE) {
         hipLaunchKernelGGL(batch_close_kernel, dim3(total), dim3(kThreads), 0, st, bp);
         return hipGetLastError();
     }
+    if (kind == W_ISING && max_blocks <= 0) {  // the Ising member alone: no continuous bodies
+        hipLaunchKernelGGL((batch_pass_kernel<0, 0, true>), dim3(8 * bp.per_xcd), dim3(kThreads), 0, st, bp);
+        return hipGetLastError();
+    }
     const int n0 = bp.ca[0].ncomp, n1 = bp.ca[1].ncomp;  // 0 for an absent member
-    if (n0 == 2) launch_batch_n1<2>(bp, n1, kind, max_blocks, st);
-    else if (n0 == 3) launch_batch_n1<3>(bp, n1, kind, max_blocks, st);
-    else launch_batch_n1<0>(bp, n1, kind, max_blocks, st);
+    if (n0 == 2) launch_batch_n1<2>(bp, n1, kind, ising, max_blocks, st);
+    else if (n0 == 3) launch_batch_n1<3>(bp, n1, kind, ising, max_blocks, st);
+    else launch_batch_n1<0>(bp, n1, kind, ising, max_blocks, st);
     return hipGetLastError();
 }
 
