@@ -416,13 +416,13 @@ int ising_batch_sync(ising_batch_t* b, double* last_ms);
 /* How a batch launches its members' passes.  No counterpart in the reference.
  *   ISING_BATCH_BRANCHES (default): each member's own launches, on a graph
  *     branch per member.
- *   ISING_BATCH_WORKLIST: one launch per pass for all continuous members: a
- *     work list of (member, size, replica, tile) blocks; a sweep is
- *     2 n_overrelax + 5 launches (overrelaxation passes, the Ising colour-0
- *     pass, the continuous Metropolis colour 0, both again for colour 1, the
- *     continuous closing step).  At most one int8 Ising ensemble (two-launch
- *     sweeps) and two continuous ensembles (XY, Heisenberg, Metropolis
- *     passes) with equal n_overrelax.
+ *   ISING_BATCH_WORKLIST: launches over work lists of (member, size,
+ *     replica, tile) blocks: one per overrelaxation pass for both continuous
+ *     members, one per colour pass of each model's Metropolis pass, one for
+ *     the continuous closing step (2 n_overrelax + 7 launches per sweep with
+ *     all three models).  At most one int8 Ising ensemble (two-launch sweeps)
+ *     and two continuous ensembles (XY, Heisenberg, Metropolis passes) with
+ *     equal n_overrelax.
  *   ISING_BATCH_PERSISTENT: the same work list walked by a persistent grid
  *     (the blocks the device holds at once).
  * Lattices, records and permutations equal the branch mode's bit for bit. */

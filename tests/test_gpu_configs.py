@@ -2,7 +2,8 @@
 
 Batched uneven PT: Ising + XY + Heisenberg, L = 8, 12, 16, 24, 32, 48, 64, 96,
 128 x 32 replicas each (1.85 GB of spins), advanced by one batch graph, either
-of each member's own launches or of one work-list launch per pass.  At
+of each member's own launches or of work-list launches over the members'
+blocks.  At
 this size the oracle cannot replay the continuous members, so they are checked
 through size-independent properties: every record equals a standalone FP64
 reduction of the downloaded lattice, spins stay unit vectors, permutations stay
