@@ -2491,7 +2491,7 @@ struct ising_batch {
         if (pi) base.ia = pi->ising_args();
         for (size_t i = 0; i < pc.size(); ++i) base.ca[i] = pc[i]->cont_args();
         const int nor = pc.empty() ? 0 : pc[0]->opts.n_overrelax;
-        const int maxb = mode == ISING_BATCH_PERSISTENT ? resident : 0;
+        const bool persistent = mode == ISING_BATCH_PERSISTENT;
         // Launches of their own: kind W_ISING, the Ising member's colour pass (in
         // a launch with the continuous Metropolis tiles, the Ising body's
         // registers set the occupancy of every item: 5.26 against 4.92 ms per
@@ -2509,7 +2509,7 @@ struct ising_batch {
             };
             if (kind == W_ISING) {
                 add(W_ISING, 0, pi->nblocks, pi->nblocks);
-                return launch_batch_pass(bp, maxb, st);
+                return launch_batch_pass(bp, persistent, st);
             }
             for (size_t i = 0; i < pc.size(); ++i) {
                 if (only >= 0 && (int)i != only) {
@@ -2522,7 +2522,7 @@ struct ising_batch {
                 else  // an XY item holds two tiles
                     add(kind, (int)i, pc[i]->model == ISING_MODEL_XY ? (tiles + 1) / 2 : tiles, tiles);
             }
-            return launch_batch_pass(bp, maxb, st);
+            return launch_batch_pass(bp, persistent, st);
         };
         for (int64_t i = 0; i < n; ++i) {
             for (int k = 0; k < nor; ++k) {

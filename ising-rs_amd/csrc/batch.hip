@@ -32,9 +32,10 @@
 // still walks contiguous tiles of a member, whose z-neighbour planes stay in
 // its L2.  With one segment this is the per-member launches' block order.
 //
-// Persistent grids (ISING_BATCH_PERSISTENT): at most the blocks the device
-// holds at once; block k of XCD x takes the XCD's items k, k + (the XCD's
-// blocks), ..., with a barrier between items, whose LDS the bodies reuse.
+// Persistent grids (ISING_BATCH_PERSISTENT): the same launches with at most
+// the blocks the device holds at once of each kernel; block k of XCD x takes
+// the XCD's items k, k + (the XCD's blocks), ..., with a barrier between
+// items, whose LDS the bodies reuse.
 #include "cont_pass.h"
 #include "ising_pass.h"
 
@@ -115,21 +116,26 @@ __global__ __launch_bounds__(kThreads, 6) void batch_pass_kernel(BatchPass bp) {
     }
 }
 
-// The same lists walked by a persistent grid (every pass but the close).
-template <int N0, int N1>
-__global__ __launch_bounds__(kThreads) void batch_pass_persistent_kernel(BatchPass bp) {
+// The same lists walked by a persistent grid (every pass but the close), one
+// kernel per kind of pass, holding only that kind's bodies: with every body
+// in one kernel it needed 142 VGPRs (3 waves per SIMD) for passes whose own
+// kernels use 34-73.  The continuous kernels keep 40-63 VGPRs; the Ising body
+// in the item loop needs 128 (four waves per SIMD; capped at 80 it spilled).
+template <int N0, int N1, int KIND>
+__global__ __launch_bounds__(kThreads, KIND == W_ISING ? 4 : 6) void batch_persistent_kernel(BatchPass bp) {
     const int x = (int)(blockIdx.x & 7), slot = (int)(blockIdx.x >> 3), nslot = (int)(gridDim.x >> 3);
+#pragma nounroll
     for (int k = slot; k < bp.per_xcd; k += nslot) {
-        if (k != slot) __syncthreads();
+        // a barrier between items whose bodies reuse LDS (the overrelaxation
+        // bodies use none: without it their launches took 0.7 of the time)
+        if (KIND != W_OR && k != slot) __syncthreads();
         WL_ITEM(bp, x, k, s, lb);
         if (lb >= bp.seg[s].count) continue;
-        if (bp.seg[s].kind == W_ISING) {
+        if (KIND == W_ISING) {
             WL_ISING_GROUP(bp, lb, G)
             ising_half_block(bp.ia, lb, bp.color, G);
-        } else if (bp.seg[s].kind == W_OR) {
-            cont_member_item<W_OR, N0, N1>(bp, bp.seg[s].member, lb, bp.seg[s].tiles);
         } else {
-            cont_member_item<W_METRO, N0, N1>(bp, bp.seg[s].member, lb, bp.seg[s].tiles);
+            cont_member_item<KIND, N0, N1>(bp, bp.seg[s].member, lb, bp.seg[s].tiles);
         }
     }
 }
@@ -156,12 +162,36 @@ __global__ __launch_bounds__(kThreads) void batch_close_kernel(BatchPass bp) {
     else cont_close_block(bp.ca[1], lb, bp.seg[s].count);
 }
 
+// Blocks of a persistent kernel the device holds at once (0 if unknown).
+template <int N0, int N1, int KIND>
+int persistent_blocks() {
+    static int blocks = -1;
+    if (blocks < 0) {
+        int dev = 0, per_cu = 0;
+        hipDeviceProp_t prop;
+        blocks = hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+                         hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &per_cu, batch_persistent_kernel<N0, N1, KIND>, kThreads, 0) == hipSuccess
+                     ? per_cu * prop.multiProcessorCount
+                     : 0;
+    }
+    return blocks;
+}
+
+template <int N0, int N1, int KIND>
+void launch_persistent(const BatchPass& bp, hipStream_t st) {
+    const int resident = persistent_blocks<N0, N1, KIND>();
+    const int slots = resident / 8 > 0 ? resident / 8 : 1;
+    hipLaunchKernelGGL((batch_persistent_kernel<N0, N1, KIND>), dim3(8 * (bp.per_xcd < slots ? bp.per_xcd : slots)),
+                       dim3(kThreads), 0, st, bp);
+}
+
 template <int N0, int N1>
-void launch_batch_kinds(const BatchPass& bp, int kind, bool ising, int max_blocks, hipStream_t st) {
-    if (max_blocks > 0) {
-        const int slots = max_blocks / 8 > 0 ? max_blocks / 8 : 1;
-        hipLaunchKernelGGL((batch_pass_persistent_kernel<N0, N1>), dim3(8 * (bp.per_xcd < slots ? bp.per_xcd : slots)),
-                           dim3(kThreads), 0, st, bp);
+void launch_batch_kinds(const BatchPass& bp, int kind, bool ising, bool persistent, hipStream_t st) {
+    if (persistent && kind == W_OR) {
+        launch_persistent<N0, N1, W_OR>(bp, st);
+    } else if (persistent) {
+        launch_persistent<N0, N1, W_METRO>(bp, st);
     } else if (kind == W_OR) {
         hipLaunchKernelGGL((batch_or_kernel<N0, N1>), dim3(8 * bp.per_xcd), dim3(kThreads), 0, st, bp);
     } else if (ising) {
@@ -172,13 +202,13 @@ void launch_batch_kinds(const BatchPass& bp, int kind, bool ising, int max_block
 }
 
 template <int N0>
-void launch_batch_n1(const BatchPass& bp, int n1, int kind, bool ising, int max_blocks, hipStream_t st) {
-    if (n1 == 2) launch_batch_kinds<N0, 2>(bp, kind, ising, max_blocks, st);
-    else if (n1 == 3) launch_batch_kinds<N0, 3>(bp, kind, ising, max_blocks, st);
-    else launch_batch_kinds<N0, 0>(bp, kind, ising, max_blocks, st);
+void launch_batch_n1(const BatchPass& bp, int n1, int kind, bool ising, bool persistent, hipStream_t st) {
+    if (n1 == 2) launch_batch_kinds<N0, 2>(bp, kind, ising, persistent, st);
+    else if (n1 == 3) launch_batch_kinds<N0, 3>(bp, kind, ising, persistent, st);
+    else launch_batch_kinds<N0, 0>(bp, kind, ising, persistent, st);
 }
 
-hipError_t launch_batch_pass(BatchPass bp, int max_blocks, hipStream_t st) {
+hipError_t launch_batch_pass(BatchPass bp, bool persistent, hipStream_t st) {
     // chunks per XCD and their prefix sums
     bp.per_xcd = 0;
     int total = 0;
@@ -197,25 +227,23 @@ hipError_t launch_batch_pass(BatchPass bp, int max_blocks, hipStream_t st) {
         hipLaunchKernelGGL(batch_close_kernel, dim3(total), dim3(kThreads), 0, st, bp);
         return hipGetLastError();
     }
-    if (kind == W_ISING && max_blocks <= 0) {  // the Ising member alone: no continuous bodies
-        hipLaunchKernelGGL((batch_pass_kernel<0, 0, true>), dim3(8 * bp.per_xcd), dim3(kThreads), 0, st, bp);
+    if (kind == W_ISING) {  // the Ising member alone: no continuous bodies
+        if (persistent) launch_persistent<0, 0, W_ISING>(bp, st);
+        else hipLaunchKernelGGL((batch_pass_kernel<0, 0, true>), dim3(8 * bp.per_xcd), dim3(kThreads), 0, st, bp);
         return hipGetLastError();
     }
+    if (persistent && ising) return hipErrorInvalidValue;  // the Ising segment comes in launches of its own
     const int n0 = bp.ca[0].ncomp, n1 = bp.ca[1].ncomp;  // 0 for an absent member
-    if (n0 == 2) launch_batch_n1<2>(bp, n1, kind, ising, max_blocks, st);
-    else if (n0 == 3) launch_batch_n1<3>(bp, n1, kind, ising, max_blocks, st);
-    else launch_batch_n1<0>(bp, n1, kind, ising, max_blocks, st);
+    if (n0 == 2) launch_batch_n1<2>(bp, n1, kind, ising, persistent, st);
+    else if (n0 == 3) launch_batch_n1<3>(bp, n1, kind, ising, persistent, st);
+    else launch_batch_n1<0>(bp, n1, kind, ising, persistent, st);
     return hipGetLastError();
 }
 
 int batch_pass_resident_blocks() {
-    int dev = 0, per_cu = 0;
-    hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_pass_persistent_kernel<2, 3>, kThreads, 0) !=
-        hipSuccess)
-        return 0;
-    return per_cu * prop.multiProcessorCount;
+    // the smallest of the persistent kernels' capacities (the Ising kernel's)
+    const int a = persistent_blocks<0, 0, W_ISING>(), b = persistent_blocks<2, 3, W_METRO>();
+    return a < b ? a : b;
 }
 
 }  // namespace ising

@@ -123,8 +123,8 @@ struct BatchPass {
     int32_t pad;
     WorkSeg seg[kBatchSegs];
 };
-hipError_t launch_batch_pass(BatchPass bp, int max_blocks, hipStream_t st);  // max_blocks 0: one block per item
-int batch_pass_resident_blocks();  // blocks of batch_pass_kernel the current device holds at once (0 on error)
+hipError_t launch_batch_pass(BatchPass bp, bool persistent, hipStream_t st);  // persistent: grids the device holds at once
+int batch_pass_resident_blocks();  // least blocks of a persistent batch kernel the current device holds at once (0 on error)
 
 hipError_t launch_ising_close(const IsingArgs& a, hipStream_t st);
 hipError_t launch_cont_metro(const ContArgs& a, int nblocks, int color, int closes, hipStream_t st);
