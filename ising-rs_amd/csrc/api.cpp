@@ -163,6 +163,10 @@ struct ising_pt {
     int layout_width = 0, layout_runs = 0, layout_spt = 0;  // Ising thread-step layout of the first size
     std::vector<std::array<int, 3>> glayout;  // Ising thread-step layout per size: run width, runs, steps per tile
     int or_spt = 0;  // continuous: sites per thread of the overrelaxation pass (0: the model's default)
+    // continuous: 64-bit offsets in every pass (ISING_HIP_OFFSETS64 set at
+    // creation; otherwise only lattices whose half-lattices reach 2^30 sites
+    // use them), so tests can run that path at sizes the oracle replays
+    bool offsets64 = std::getenv("ISING_HIP_OFFSETS64") != nullptr;
     int fused_z = 0;         // one-launch sweeps: z planes per block asked for (0: two launches per sweep)
     bool fused_on = false;   // one-launch sweeps in use (every size qualifies)
     int fnblocks = 0, fused_lds = 0;
@@ -307,6 +311,9 @@ struct ising_pt {
         a.mh = d_mh;
         a.learn_H = learn_H;
         a.or_spt = or_spt;
+        a.off32 = offsets64 ? 0 : 1;
+        for (const DevGroup& g : groups)
+            if (g.stride >= (int64_t(1) << 30)) a.off32 = 0;
         return a;
     }
 

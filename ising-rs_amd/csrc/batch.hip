@@ -71,7 +71,7 @@ namespace ising {
 // An XY item is two tiles (2 lb, 2 lb + 1), one per half of the block, with
 // the member's own thread shape: two sites per thread in 128-thread halves.
 // A Heisenberg item is one tile, one site per thread.
-template <int KIND, int NM>
+template <int KIND, int NM, bool O32>
 __device__ __forceinline__ void cont_item(const ContArgs& a, const int lb, const int color, const int tiles) {
     if (NM == 2) {
         // the half is uniform in a wave; read as a scalar, it keeps the tile and
@@ -81,38 +81,39 @@ __device__ __forceinline__ void cont_item(const ContArgs& a, const int lb, const
         const int q = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 7));
         const int tid = (int)(threadIdx.x & 127), tile = 2 * lb + q;
         if (KIND == W_OR) {
-            if (tile < tiles) cont_or_block<2, 2>(a, tile, color, tid);
+            if (tile < tiles) cont_or_block<2, 2, O32>(a, tile, color, tid);
         } else {
-            cont_metro_pair_block<2>(a, tile, color, 1, tile < tiles, q, tid);
+            cont_metro_pair_block<2, O32>(a, tile, color, 1, tile < tiles, q, tid);
         }
     } else if (KIND == W_OR) {
-        cont_or_block<NM, 1>(a, lb, color);
+        cont_or_block<NM, 1, O32>(a, lb, color);
     } else {
-        cont_metro_block<NM>(a, lb, color, 1);
+        cont_metro_block<NM, O32>(a, lb, color, 1);
     }
 }
 
-template <int KIND, int N0, int N1>
+template <int KIND, int N0, int N1, bool O32>
 __device__ __forceinline__ void cont_member_item(const BatchPass& bp, const int member, const int lb, const int tiles) {
-    if (N0 && member == 0) cont_item<KIND, N0 ? N0 : 2>(bp.ca[0], lb, bp.color, tiles);
-    else if (N1) cont_item<KIND, N1 ? N1 : 2>(bp.ca[1], lb, bp.color, tiles);
+    if (N0 && member == 0) cont_item<KIND, N0 ? N0 : 2, O32>(bp.ca[0], lb, bp.color, tiles);
+    else if (N1) cont_item<KIND, N1 ? N1 : 2, O32>(bp.ca[1], lb, bp.color, tiles);
 }
 
-// A Metropolis launch: the Ising member's colour pass (ISING) or the
-// continuous members' Metropolis passes (N0, N1: components of continuous
-// members 0 and 1, 0 when absent).  At most 80 VGPRs, six waves per SIMD (the
-// Ising body's occupancy in its own launch).  The instance without the Ising
-// body keeps the continuous bodies' registers (45-50 VGPRs against 71).
-template <int N0, int N1, bool ISING>
+// A Metropolis launch: the Ising member's colour pass (ISING; no continuous
+// member) or the continuous members' Metropolis passes (N0, N1: components of
+// continuous members 0 and 1, 0 when absent; O32: 32-bit offsets, see
+// comp_at).  At most 80 VGPRs, six waves per SIMD (the Ising body's occupancy
+// in its own launch); the continuous instances keep their bodies' registers
+// (45-50 VGPRs against 71 with the Ising body beside them).
+template <int N0, int N1, bool ISING, bool O32>
 __global__ __launch_bounds__(kThreads, 6) void batch_pass_kernel(BatchPass bp) {
     const int x = (int)(blockIdx.x & 7), k = (int)(blockIdx.x >> 3);
     WL_ITEM(bp, x, k, s, lb);
     if (lb >= bp.seg[s].count) return;
-    if (ISING && bp.seg[s].kind == W_ISING) {
+    if (ISING) {
         WL_ISING_GROUP(bp, lb, G)
         ising_half_block(bp.ia, lb, bp.color, G);
     } else {
-        cont_member_item<W_METRO, N0, N1>(bp, bp.seg[s].member, lb, bp.seg[s].tiles);
+        cont_member_item<W_METRO, N0, N1, O32>(bp, bp.seg[s].member, lb, bp.seg[s].tiles);
     }
 }
 
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(kThreads, 6) void batch_pass_kernel(BatchPass bp) {
 // in one kernel it needed 142 VGPRs (3 waves per SIMD) for passes whose own
 // kernels use 34-73.  The continuous kernels keep 40-63 VGPRs; the Ising body
 // in the item loop needs 128 (four waves per SIMD; capped at 80 it spilled).
-template <int N0, int N1, int KIND>
+template <int N0, int N1, int KIND, bool O32>
 __global__ __launch_bounds__(kThreads, KIND == W_ISING ? 4 : 6) void batch_persistent_kernel(BatchPass bp) {
     const int x = (int)(blockIdx.x & 7), slot = (int)(blockIdx.x >> 3), nslot = (int)(gridDim.x >> 3);
 #pragma nounroll
@@ -135,19 +136,19 @@ __global__ __launch_bounds__(kThreads, KIND == W_ISING ? 4 : 6) void batch_persi
             WL_ISING_GROUP(bp, lb, G)
             ising_half_block(bp.ia, lb, bp.color, G);
         } else {
-            cont_member_item<KIND, N0, N1>(bp, bp.seg[s].member, lb, bp.seg[s].tiles);
+            cont_member_item<KIND, N0, N1, O32>(bp, bp.seg[s].member, lb, bp.seg[s].tiles);
         }
     }
 }
 
 // An overrelaxation launch: a kernel of its own, with the registers (and
 // occupancy) of the overrelaxation bodies.
-template <int N0, int N1>
+template <int N0, int N1, bool O32>
 __global__ __launch_bounds__(kThreads) void batch_or_kernel(BatchPass bp) {
     const int x = (int)(blockIdx.x & 7), k = (int)(blockIdx.x >> 3);
     WL_ITEM(bp, x, k, s, lb);
     if (lb >= bp.seg[s].count) return;
-    cont_member_item<W_OR, N0, N1>(bp, bp.seg[s].member, lb, bp.seg[s].tiles);
+    cont_member_item<W_OR, N0, N1, O32>(bp, bp.seg[s].member, lb, bp.seg[s].tiles);
 }
 
 // The continuous members' closing blocks of one sweep: block i of the launch
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(kThreads) void batch_close_kernel(BatchPass bp) {
 }
 
 // Blocks of a persistent kernel the device holds at once (0 if unknown).
-template <int N0, int N1, int KIND>
+template <int N0, int N1, int KIND, bool O32>
 int persistent_blocks() {
     static int blocks = -1;
     if (blocks < 0) {
@@ -171,41 +172,46 @@ int persistent_blocks() {
         hipDeviceProp_t prop;
         blocks = hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
                          hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                             &per_cu, batch_persistent_kernel<N0, N1, KIND>, kThreads, 0) == hipSuccess
+                             &per_cu, batch_persistent_kernel<N0, N1, KIND, O32>, kThreads, 0) == hipSuccess
                      ? per_cu * prop.multiProcessorCount
                      : 0;
     }
     return blocks;
 }
 
-template <int N0, int N1, int KIND>
+template <int N0, int N1, int KIND, bool O32>
 void launch_persistent(const BatchPass& bp, hipStream_t st) {
-    const int resident = persistent_blocks<N0, N1, KIND>();
+    const int resident = persistent_blocks<N0, N1, KIND, O32>();
     const int slots = resident / 8 > 0 ? resident / 8 : 1;
-    hipLaunchKernelGGL((batch_persistent_kernel<N0, N1, KIND>), dim3(8 * (bp.per_xcd < slots ? bp.per_xcd : slots)),
-                       dim3(kThreads), 0, st, bp);
+    hipLaunchKernelGGL((batch_persistent_kernel<N0, N1, KIND, O32>),
+                       dim3(8 * (bp.per_xcd < slots ? bp.per_xcd : slots)), dim3(kThreads), 0, st, bp);
 }
 
-template <int N0, int N1>
-void launch_batch_kinds(const BatchPass& bp, int kind, bool ising, bool persistent, hipStream_t st) {
+// A continuous pass (W_OR or W_METRO) of continuous members with N0, N1 components.
+template <int N0, int N1, bool O32>
+void launch_cont_kind(const BatchPass& bp, int kind, bool persistent, hipStream_t st) {
     if (persistent && kind == W_OR) {
-        launch_persistent<N0, N1, W_OR>(bp, st);
+        launch_persistent<N0, N1, W_OR, O32>(bp, st);
     } else if (persistent) {
-        launch_persistent<N0, N1, W_METRO>(bp, st);
+        launch_persistent<N0, N1, W_METRO, O32>(bp, st);
     } else if (kind == W_OR) {
-        hipLaunchKernelGGL((batch_or_kernel<N0, N1>), dim3(8 * bp.per_xcd), dim3(kThreads), 0, st, bp);
-    } else if (ising) {
-        hipLaunchKernelGGL((batch_pass_kernel<N0, N1, true>), dim3(8 * bp.per_xcd), dim3(kThreads), 0, st, bp);
+        hipLaunchKernelGGL((batch_or_kernel<N0, N1, O32>), dim3(8 * bp.per_xcd), dim3(kThreads), 0, st, bp);
     } else {
-        hipLaunchKernelGGL((batch_pass_kernel<N0, N1, false>), dim3(8 * bp.per_xcd), dim3(kThreads), 0, st, bp);
+        hipLaunchKernelGGL((batch_pass_kernel<N0, N1, false, O32>), dim3(8 * bp.per_xcd), dim3(kThreads), 0, st, bp);
     }
 }
 
+template <int N0, int N1>
+void launch_cont_o32(const BatchPass& bp, int kind, bool persistent, bool o32, hipStream_t st) {
+    if (o32) launch_cont_kind<N0, N1, true>(bp, kind, persistent, st);
+    else launch_cont_kind<N0, N1, false>(bp, kind, persistent, st);
+}
+
 template <int N0>
-void launch_batch_n1(const BatchPass& bp, int n1, int kind, bool ising, bool persistent, hipStream_t st) {
-    if (n1 == 2) launch_batch_kinds<N0, 2>(bp, kind, ising, persistent, st);
-    else if (n1 == 3) launch_batch_kinds<N0, 3>(bp, kind, ising, persistent, st);
-    else launch_batch_kinds<N0, 0>(bp, kind, ising, persistent, st);
+void launch_cont_n1(const BatchPass& bp, int n1, int kind, bool persistent, bool o32, hipStream_t st) {
+    if (n1 == 2) launch_cont_o32<N0, 2>(bp, kind, persistent, o32, st);
+    else if (n1 == 3) launch_cont_o32<N0, 3>(bp, kind, persistent, o32, st);
+    else launch_cont_o32<N0, 0>(bp, kind, persistent, o32, st);
 }
 
 hipError_t launch_batch_pass(BatchPass bp, bool persistent, hipStream_t st) {
@@ -219,30 +225,30 @@ hipError_t launch_batch_pass(BatchPass bp, bool persistent, hipStream_t st) {
         total += bp.seg[s].count;
     }
     if (total <= 0) return hipSuccess;
-    // the segments of a launch share one kind, except that a Metropolis
-    // launch may hold the Ising segment (always first) beside continuous ones
-    const int kind = bp.seg[bp.nseg - 1].kind;
-    const bool ising = bp.seg[0].kind == W_ISING;
+    // the segments of a launch share one kind
+    const int kind = bp.seg[0].kind;
+    for (int s = 1; s < bp.nseg; ++s)
+        if (bp.seg[s].kind != kind) return hipErrorInvalidValue;
     if (kind == W_CLOSE) {
         hipLaunchKernelGGL(batch_close_kernel, dim3(total), dim3(kThreads), 0, st, bp);
         return hipGetLastError();
     }
-    if (kind == W_ISING) {  // the Ising member alone: no continuous bodies
-        if (persistent) launch_persistent<0, 0, W_ISING>(bp, st);
-        else hipLaunchKernelGGL((batch_pass_kernel<0, 0, true>), dim3(8 * bp.per_xcd), dim3(kThreads), 0, st, bp);
+    if (kind == W_ISING) {
+        if (persistent) launch_persistent<0, 0, W_ISING, false>(bp, st);
+        else hipLaunchKernelGGL((batch_pass_kernel<0, 0, true, false>), dim3(8 * bp.per_xcd), dim3(kThreads), 0, st, bp);
         return hipGetLastError();
     }
-    if (persistent && ising) return hipErrorInvalidValue;  // the Ising segment comes in launches of its own
     const int n0 = bp.ca[0].ncomp, n1 = bp.ca[1].ncomp;  // 0 for an absent member
-    if (n0 == 2) launch_batch_n1<2>(bp, n1, kind, ising, persistent, st);
-    else if (n0 == 3) launch_batch_n1<3>(bp, n1, kind, ising, persistent, st);
-    else launch_batch_n1<0>(bp, n1, kind, ising, persistent, st);
+    const bool o32 = (n0 == 0 || bp.ca[0].off32) && (n1 == 0 || bp.ca[1].off32);
+    if (n0 == 2) launch_cont_n1<2>(bp, n1, kind, persistent, o32, st);
+    else if (n0 == 3) launch_cont_n1<3>(bp, n1, kind, persistent, o32, st);
+    else launch_cont_n1<0>(bp, n1, kind, persistent, o32, st);
     return hipGetLastError();
 }
 
 int batch_pass_resident_blocks() {
     // the smallest of the persistent kernels' capacities (the Ising kernel's)
-    const int a = persistent_blocks<0, 0, W_ISING>(), b = persistent_blocks<2, 3, W_METRO>();
+    const int a = persistent_blocks<0, 0, W_ISING, false>(), b = persistent_blocks<2, 3, W_METRO, true>();
     return a < b ? a : b;
 }
 

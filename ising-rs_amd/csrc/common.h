@@ -172,12 +172,15 @@ __device__ __forceinline__ void site_nbrs(const DevGroup& G, uint32_t h, int col
     const uint32_t ym = y == 0u ? L - 1u : y - 1u, yp = y + 1u == L ? 0u : y + 1u;
     const uint32_t zm = z == 0u ? L - 1u : z - 1u, zp = z + 1u == L ? 0u : z + 1u;
     const uint32_t xo = p ? (xh + 1u == Lh ? 0u : xh + 1u) : (xh == 0u ? Lh - 1u : xh - 1u);
-    nb[0] = row * Lh + xh;
-    nb[1] = row * Lh + xo;
-    nb[2] = (z * L + ym) * Lh + xh;
-    nb[3] = (z * L + yp) * Lh + xh;
-    nb[4] = (zm * L + y) * Lh + xh;
-    nb[5] = (zp * L + y) * Lh + xh;
+    // 24-bit products (one full-rate instruction each): every factor is below
+    // 2^24 for L <= 2046 (rows < L^2 < 2^22)
+    const uint32_t rb = __umul24(row, Lh);
+    nb[0] = rb + xh;
+    nb[1] = rb + xo;
+    nb[2] = __umul24(__umul24(z, L) + ym, Lh) + xh;
+    nb[3] = __umul24(__umul24(z, L) + yp, Lh) + xh;
+    nb[4] = __umul24(__umul24(zm, L) + y, Lh) + xh;
+    nb[5] = __umul24(__umul24(zp, L) + y, Lh) + xh;
 }
 
 // Replica (or replica word) and tile of a block's local index.

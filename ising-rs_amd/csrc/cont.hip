@@ -30,31 +30,32 @@
 
 namespace ising {
 
-// Metropolis pass over colour `color` of every replica.
-template <int N>
+// Metropolis pass over colour `color` of every replica (O32: 32-bit offsets,
+// see comp_at).
+template <int N, bool O32>
 __global__ __launch_bounds__(kCT) void cont_metro_kernel(ContArgs a, int nblocks, int color, int closes) {
     const int b = xcd_block();
     if (b >= nblocks) return;
-    cont_metro_block<N>(a, b, color, closes);
+    cont_metro_block<N, O32>(a, b, color, closes);
 }
 
 // The same pass for XY with two sites per thread (see cont_metro_pair_block).
-template <int N>
+template <int N, bool O32>
 __global__ __launch_bounds__(kCT / 2) void cont_metro_pair_kernel(ContArgs a, int nblocks, int color, int closes) {
     const int b = xcd_block();
     if (b >= nblocks) return;
-    cont_metro_pair_block<N>(a, b, color, closes, true, 0, (int)threadIdx.x);
+    cont_metro_pair_block<N, O32>(a, b, color, closes, true, 0, (int)threadIdx.x);
 }
 
 // Closes a continuous sweep; one launch of kCT-thread blocks (see cont_close_block).
 __global__ __launch_bounds__(kCT) void cont_close_kernel(ContArgs a) { cont_close_block(a, (int)blockIdx.x, (int)gridDim.x); }
 
 // Overrelaxation pass over colour `color`, SPT sites per thread (see cont_or_block).
-template <int N, int SPT>
+template <int N, int SPT, bool O32>
 __global__ __launch_bounds__(kCT / SPT) void cont_or_kernel(ContArgs a, int nblocks, int color) {
     const int b = xcd_block();
     if (b >= nblocks) return;
-    cont_or_block<N, SPT>(a, b, color);
+    cont_or_block<N, SPT, O32>(a, b, color);
 }
 
 // Random start: Heisenberg z = 2u-1, phi = 2 pi u' (uniform on S^2); XY theta = 2 pi u.
@@ -173,11 +174,17 @@ hipError_t launch_cont_helicity(const DevGroup* groups, int ngroups, const float
 
 static inline int xcd_grid(int nblocks) { return (nblocks + 7) & ~7; }
 
-hipError_t launch_cont_metro(const ContArgs& a, int nblocks, int color, int closes, hipStream_t st) {
+template <bool O32>
+static void launch_metro(const ContArgs& a, int nblocks, int color, int closes, hipStream_t st) {
     const dim3 grid(xcd_grid(nblocks));
     if (a.ncomp == 2)
-        hipLaunchKernelGGL(cont_metro_pair_kernel<2>, grid, dim3(kCT / 2), 0, st, a, nblocks, color, closes);
-    else hipLaunchKernelGGL(cont_metro_kernel<3>, grid, dim3(kCT), 0, st, a, nblocks, color, closes);
+        hipLaunchKernelGGL((cont_metro_pair_kernel<2, O32>), grid, dim3(kCT / 2), 0, st, a, nblocks, color, closes);
+    else hipLaunchKernelGGL((cont_metro_kernel<3, O32>), grid, dim3(kCT), 0, st, a, nblocks, color, closes);
+}
+
+hipError_t launch_cont_metro(const ContArgs& a, int nblocks, int color, int closes, hipStream_t st) {
+    if (a.off32) launch_metro<true>(a, nblocks, color, closes, st);
+    else launch_metro<false>(a, nblocks, color, closes, st);
     return hipGetLastError();
 }
 
@@ -188,13 +195,20 @@ hipError_t launch_cont_close(const ContArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_cont_or(const ContArgs& a, int nblocks, int color, hipStream_t st) {
+template <bool O32>
+static void launch_or(const ContArgs& a, int nblocks, int color, hipStream_t st) {
     const dim3 grid(xcd_grid(nblocks));
     const int spt = a.or_spt ? a.or_spt : (a.ncomp == 2 ? 2 : 1);
-    if (a.ncomp == 2 && spt == 2) hipLaunchKernelGGL((cont_or_kernel<2, 2>), grid, dim3(kCT / 2), 0, st, a, nblocks, color);
-    else if (a.ncomp == 2) hipLaunchKernelGGL((cont_or_kernel<2, 1>), grid, dim3(kCT), 0, st, a, nblocks, color);
-    else if (spt == 2) hipLaunchKernelGGL((cont_or_kernel<3, 2>), grid, dim3(kCT / 2), 0, st, a, nblocks, color);
-    else hipLaunchKernelGGL((cont_or_kernel<3, 1>), grid, dim3(kCT), 0, st, a, nblocks, color);
+    if (a.ncomp == 2 && spt == 2)
+        hipLaunchKernelGGL((cont_or_kernel<2, 2, O32>), grid, dim3(kCT / 2), 0, st, a, nblocks, color);
+    else if (a.ncomp == 2) hipLaunchKernelGGL((cont_or_kernel<2, 1, O32>), grid, dim3(kCT), 0, st, a, nblocks, color);
+    else if (spt == 2) hipLaunchKernelGGL((cont_or_kernel<3, 2, O32>), grid, dim3(kCT / 2), 0, st, a, nblocks, color);
+    else hipLaunchKernelGGL((cont_or_kernel<3, 1, O32>), grid, dim3(kCT), 0, st, a, nblocks, color);
+}
+
+hipError_t launch_cont_or(const ContArgs& a, int nblocks, int color, hipStream_t st) {
+    if (a.off32) launch_or<true>(a, nblocks, color, st);
+    else launch_or<false>(a, nblocks, color, st);
     return hipGetLastError();
 }
 

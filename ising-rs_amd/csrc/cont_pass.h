@@ -3,6 +3,7 @@
 // cont.hip and of the work-list launch of several ensembles.  Storage,
 // proposals and measurement are described in cont.hip.
 #pragma once
+#include <type_traits>
 #include "kernels.h"
 
 namespace ising {
@@ -40,13 +41,25 @@ __device__ __forceinline__ void neighbours(const DevGroup& G, int64_t h, int col
     site_nbrs(G, (uint32_t)h, color, nb);
 }
 
-template <int N>
+// Component k of half-site h of a colour array C (components `stride` apart).
+// O32: every half-lattice of the ensemble holds fewer than 2^30 sites, so the
+// byte offset of h fits in 32 bits and a load is the component's scalar base
+// plus a 32-bit vector offset (no 64-bit address arithmetic per load).
+template <bool O32, class T>
+__device__ __forceinline__ T* comp_at(T* C, int64_t stride, int k, uint32_t h) {
+    if (O32) {
+        typedef typename std::conditional<std::is_const<T>::value, const char, char>::type B;
+        return reinterpret_cast<T*>(reinterpret_cast<B*>(C + k * stride) + (h << 2));
+    }
+    return C + k * stride + h;
+}
+
+template <int N, bool O32>
 __device__ __forceinline__ void load_nbrs(const float* O, int64_t stride, const uint32_t nb[6], float nv[3][6]) {
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        const float* Ok = O + k * stride;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) nv[k][q] = Ok[nb[q]];
+        for (int q = 0; q < 6; ++q) nv[k][q] = *comp_at<O32>(O, stride, k, nb[q]);
     }
 }
 
@@ -57,10 +70,10 @@ __device__ __forceinline__ void field_f32(const float nv[3][6], float hf[3]) {
     for (int k = 0; k < N; ++k) hf[k] = ((nv[k][0] + nv[k][1]) + (nv[k][2] + nv[k][3])) + (nv[k][4] + nv[k][5]);
 }
 
-template <int N>
+template <int N, bool O32>
 __device__ __forceinline__ void load_field(const float* O, int64_t stride, const uint32_t nb[6], float hf[3]) {
     float nv[3][6];
-    load_nbrs<N>(O, stride, nb, nv);
+    load_nbrs<N, O32>(O, stride, nb, nv);
     field_f32<N>(nv, hf);
 }
 
@@ -216,7 +229,7 @@ static __device__ void close_cont(const ContArgs& a, int64_t t, int32_t fl) {
 
 // Metropolis pass over colour `color`: work block b (in the XCD-aware order,
 // b < the pass's work blocks), one site per thread.
-template <int N>
+template <int N, bool O32>
 __device__ __forceinline__ void cont_metro_block(const ContArgs& a, const int b, const int color, const int closes) {
     __shared__ double s_red[4][kCT / 64];
     const int g = find_group_c(a.groups, a.ngroups, b);
@@ -245,12 +258,12 @@ __device__ __forceinline__ void cont_metro_block(const ContArgs& a, const int b,
         uint32_t nb[6];
         neighbours(G, h, color, nb);
         float nv[3][6];
-        load_nbrs<N>(O, G.stride, nb, nv);
+        load_nbrs<N, O32>(O, G.stride, nb, nv);
         float hf[3];
         field_f32<N>(nv, hf);
         float S[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-        for (int k = 0; k < N; ++k) S[k] = C[k * G.stride + h];
+        for (int k = 0; k < N; ++k) S[k] = *comp_at<O32>(C, G.stride, k, (uint32_t)h);
         const u32x4 w = philox((uint32_t)h, hs, stream, tag(P_CONT, a.model), a.k0, a.k1);
         float nw[3] = {0.f, 0.f, 0.f};
         if (N == 2) {
@@ -299,7 +312,7 @@ __device__ __forceinline__ void cont_metro_block(const ContArgs& a, const int b,
         }
         if (accept || renorm) {
 #pragma unroll
-            for (int k = 0; k < N; ++k) C[k * G.stride + h] = S[k];
+            for (int k = 0; k < N; ++k) *comp_at<O32>(C, G.stride, k, (uint32_t)h) = S[k];
         }
         if (need) {
             // FP64 from the stored FP32 components (the field is re-summed in FP64)
@@ -341,7 +354,7 @@ __device__ __forceinline__ void cont_metro_block(const ContArgs& a, const int b,
 // thread index in the sub-block).  A block of kCT threads runs two tiles, one
 // per sub-block; `valid` false for a sub-block without a tile, which still
 // takes part in the block's barrier.
-template <int N>
+template <int N, bool O32>
 __device__ __forceinline__ void cont_metro_pair_block(const ContArgs& a, const int b, const int color, const int closes,
                                                       const bool valid, const int q, const int tid) {
     __shared__ double s_red[4][kCT / 64];
@@ -374,12 +387,12 @@ __device__ __forceinline__ void cont_metro_pair_block(const ContArgs& a, const i
             uint32_t nb[6];
             neighbours(G, h, color, nb);
             float nv[3][6];
-            load_nbrs<N>(O, G.stride, nb, nv);
+            load_nbrs<N, O32>(O, G.stride, nb, nv);
             float hf[3];
             field_f32<N>(nv, hf);
             float S[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-            for (int k = 0; k < N; ++k) S[k] = C[k * G.stride + h];
+            for (int k = 0; k < N; ++k) S[k] = *comp_at<O32>(C, G.stride, k, (uint32_t)h);
             const u32x4 w = philox((uint32_t)h, hs, stream, tag(P_CONT, a.model), a.k0, a.k1);
             float nw[3] = {0.f, 0.f, 0.f};
             if (N == 2) {
@@ -428,7 +441,7 @@ __device__ __forceinline__ void cont_metro_pair_block(const ContArgs& a, const i
             }
             if (accept || renorm) {
 #pragma unroll
-                for (int k = 0; k < N; ++k) C[k * G.stride + h] = S[k];
+                for (int k = 0; k < N; ++k) *comp_at<O32>(C, G.stride, k, (uint32_t)h) = S[k];
             }
             if (need) {
                 // FP64 from the stored FP32 components (the field is re-summed in FP64)
@@ -498,7 +511,7 @@ __device__ __forceinline__ void cont_close_block(const ContArgs& a, const int lb
 // XY defaults to two (60.1 against 57.8 Metropolis updates/ns in an A/B run on
 // one device), Heisenberg, with 18 loads per site, to one; the autotuner times
 // both.
-template <int N, int SPT>
+template <int N, int SPT, bool O32>
 __device__ __forceinline__ void cont_or_block(const ContArgs& a, const int b, const int color,
                                               const int tid = (int)threadIdx.x) {
     const int g = find_group_c(a.groups, a.ngroups, b);
@@ -514,19 +527,19 @@ __device__ __forceinline__ void cont_or_block(const ContArgs& a, const int b, co
         uint32_t nb[6];
         neighbours(G, h, color, nb);
         float hf[3];
-        load_field<N>(O, G.stride, nb, hf);
+        load_field<N, O32>(O, G.stride, nb, hf);
         float h2 = 0.f, sh = 0.f, S[3] = {0.f, 0.f, 0.f};
 #pragma unroll
         for (int k = 0; k < N; ++k) {
             hf[k] *= a.Jf;
-            S[k] = C[k * G.stride + h];
+            S[k] = *comp_at<O32>(C, G.stride, k, (uint32_t)h);
             h2 += hf[k] * hf[k];
             sh += S[k] * hf[k];
         }
         if (h2 < 1e-30f) continue;
         const float sc = 2.0f * sh * __builtin_amdgcn_rcpf(h2);
 #pragma unroll
-        for (int k = 0; k < N; ++k) C[k * G.stride + h] = sc * hf[k] - S[k];
+        for (int k = 0; k < N; ++k) *comp_at<O32>(C, G.stride, k, (uint32_t)h) = sc * hf[k] - S[k];
     }
 }
 

@@ -84,6 +84,7 @@ struct ContArgs {
     int64_t* mh;           // learned proposal: [kMhSlots][8] moves tried (0) / accepted (1), spread by wave
     int32_t learn_H;       // hidden width of the learned proposal's MLP
     int32_t or_spt;        // sites per thread of the overrelaxation pass (1 or 2; 0: the model's default)
+    int32_t off32;         // 1: every group's half-lattice stride < 2^30 (32-bit byte offsets in the passes)
 };
 
 int learn_params(int H);  // parameter count of the MLP with hidden width H (8, 16, 32, 64), -1 otherwise

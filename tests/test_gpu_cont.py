@@ -248,3 +248,42 @@ def test_pt_multiblock_close_between_unmeasured_sweeps(nc):
     assert accepted > 0
     tries, acc = pt.exchange_stats()
     assert acc >= accepted and tries >= 19 * len(sizes) * ((R - 1) // 2)
+
+
+def _ensembles_with_offsets(nc, wide):
+    """A continuous ensemble whose passes use 32-bit (default at these sizes)
+    or 64-bit offsets (ISING_HIP_OFFSETS64 at creation: the path of lattices
+    whose half-lattices reach 2^30 sites)."""
+    import os
+    if wide:
+        os.environ["ISING_HIP_OFFSETS64"] = "1"
+    try:
+        return ising_hip.PTEnsemble(MODEL[nc], [8, 12, 16], 6, np.linspace(1.4, 2.3, 6), 21, exchange_every=3,
+                                    measure_every=2, n_overrelax=2, delta=1.0, adapt_every=4, renorm_every=5,
+                                    graph_sweeps=0)
+    finally:
+        os.environ.pop("ISING_HIP_OFFSETS64", None)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_64bit_offset_passes_equal_32bit_ones(nc):
+    """The passes with 64-bit addressing (kept for half-lattices of 2^30 sites
+    or more) give the 32-bit-offset passes' lattices and records bit for bit,
+    per ensemble and in a work-list batch."""
+    a, b = _ensembles_with_offsets(nc, False), _ensembles_with_offsets(nc, True)
+    for p in (a, b):
+        p.warmup(9)
+        p.sample(11)
+    for x, y in zip(a.read_cont(), b.read_cont()):
+        assert np.array_equal(x, y)
+    for g in range(3):
+        for r in range(6):
+            assert np.array_equal(a.get_spins(g, r), b.get_spins(g, r))
+    ba = ising_hip.Batch([a], graph_sweeps=2, mode=ising_hip.Batch.WORKLIST)
+    bb = ising_hip.Batch([b], graph_sweeps=2, mode=ising_hip.Batch.WORKLIST)
+    for bt in (ba, bb):
+        bt.sample(5)
+        bt.sync()
+    for x, y in zip(a.read_cont(), b.read_cont()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a.get_spins(2, 5), b.get_spins(2, 5))
