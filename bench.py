@@ -291,7 +291,7 @@ def run_batched(args, ising_hip):
                                 "worklist": "whole sweep: batch_or_kernel x 4 (both continuous models) + "
                                             "batch_pass_kernel x 6 (Ising, XY, Heisenberg colour passes) + "
                                             "batch_close_kernel, over work lists of the models' blocks",
-                                "persistent": "whole sweep: batch_pass_persistent_kernel x 10 + batch_close_kernel "
+                                "persistent": "whole sweep: batch_persistent_kernel x 10 + batch_close_kernel "
                                               "(the same work lists on a persistent grid)"}[
                                     args.batch_mode],
                      "alg_bytes_per_sweep": int(alg_bytes_per_sweep), "avg_sweep_us": round(sweep_s * 1e6, 3),

@@ -2511,9 +2511,8 @@ struct ising_batch {
             BatchPass bp = base;
             bp.color = color;
             auto add = [&](int k, int member, int count, int tiles) {
-                bp.seg[bp.nseg] = WorkSeg{k, member, count, 0, 0, tiles};
-                ++bp.nseg;
-            };
+                if (bp.nseg < kBatchSegs) bp.seg[bp.nseg++] = WorkSeg{k, member, count, 0, 0, tiles};
+            };  // at most two continuous members (ising_batch_set_mode)
             if (kind == W_ISING) {
                 add(W_ISING, 0, pi->nblocks, pi->nblocks);
                 return launch_batch_pass(bp, persistent, st);

@@ -113,7 +113,7 @@ struct WorkSeg {
     int32_t kfirst;  // position of the segment's chunk in every XCD's list (set by launch_batch_pass)
     int32_t tiles;   // the member's tiles (blocks of its own launch); an XY item holds two tiles
 };
-constexpr int kBatchSegs = 3;  // segments per launch: the Ising member and two continuous members
+constexpr int kBatchSegs = 2;  // segments per launch, all of one kind: the Ising member, or the continuous members
 constexpr int kBatchCont = 2;  // continuous members of a work-list batch
 struct BatchPass {
     IsingArgs ia;             // the int8 Ising member (when a segment is W_ISING)
