@@ -201,6 +201,35 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_calibration_bytes": nbytes, "dispatches": 3}), flush=True)
 
 
+def stream_calibration():
+    """Achievable HBM/Infinity-Cache rate of the int8 colour pass's access
+    pattern without its arithmetic: an in-place byte add (two streams read,
+    one written, torch's elementwise kernel) and a device copy, at one colour
+    of the config-2 ensemble (38,346,752 bytes per stream), of the L = 256 x 32
+    ensemble (268,435,456) and twice that, timed with events over 200 passes."""
+    import torch
+    out = []
+    for n in (38346752, 268435456, 536870912):
+        a = torch.randint(0, 2, (n,), dtype=torch.uint8, device="cuda:0")
+        b = torch.randint(0, 2, (n,), dtype=torch.uint8, device="cuda:0")
+        c = torch.empty_like(a)
+        row = {"bytes_per_stream": n}
+        for name, fn, streams in (("add_read2_write1", lambda: a.add_(b), 3), ("copy", lambda: c.copy_(a), 2)):
+            for _ in range(20):
+                fn()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(200):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) * 1e3 / 200
+            row[name] = {"us": round(us, 2), "TB_s": round(streams * n / (us * 1e6), 3)}
+        out.append(row)
+        del a, b, c
+    print(json.dumps({"stream_calibration": out}), flush=True)
+
+
 WORKLOADS = {
     # name: (model, sizes, temps, bytes/update, dtype, options, description)
     "ising": ("ising", SIZES, (4.40, 4.62), 3, "u8", dict(exchange_every=10, measure_every=5),
@@ -718,6 +747,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true",
                     help="ising workload: skip the learned-vs-plain sub-result (XY L=32,64 and Heisenberg L=32 "
                          "x 32 replicas, 1000 + 4000 sweeps per chain)")
+    ap.add_argument("--stream-calibration", action="store_true",
+                    help="only time a read-two / write-one byte stream and a copy at the int8 pass's sizes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -728,6 +759,9 @@ def main():
         args.warmup = 30 if pt_workload else 300
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.stream_calibration:
+        stream_calibration()
         return
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the library has no multi-GPU path (an ensemble would be "
