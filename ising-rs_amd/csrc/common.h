@@ -9,6 +9,7 @@
 //   c2 = stream id (L << 16) | replica
 //   c3 = (purpose << 8) | model
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -181,6 +182,19 @@ __device__ __forceinline__ void site_nbrs(const DevGroup& G, uint32_t h, int col
     nb[3] = __umul24(__umul24(z, L) + yp, Lh) + xh;
     nb[4] = __umul24(__umul24(zm, L) + y, Lh) + xh;
     nb[5] = __umul24(__umul24(zp, L) + y, Lh) + xh;
+}
+
+// Component k of half-site h of a colour array C (components `stride` apart).
+// O32: every half-lattice of the ensemble holds fewer than 2^30 sites, so the
+// byte offset of h fits in 32 bits and a load is the component's scalar base
+// plus a 32-bit vector offset (no 64-bit address arithmetic per load).
+template <bool O32, class T>
+__device__ __forceinline__ T* comp_at(T* C, int64_t stride, int k, uint32_t h) {
+    if (O32) {
+        typedef typename std::conditional<std::is_const<T>::value, const char, char>::type B;
+        return reinterpret_cast<T*>(reinterpret_cast<B*>(C + k * stride) + (h << 2));
+    }
+    return C + k * stride + h;
 }
 
 // Replica (or replica word) and tile of a block's local index.

@@ -3,7 +3,6 @@
 // cont.hip and of the work-list launch of several ensembles.  Storage,
 // proposals and measurement are described in cont.hip.
 #pragma once
-#include <type_traits>
 #include "kernels.h"
 
 namespace ising {
@@ -39,19 +38,6 @@ constexpr float kInv2Pi = 0.15915494309189533577f;
 
 __device__ __forceinline__ void neighbours(const DevGroup& G, int64_t h, int color, uint32_t nb[6]) {
     site_nbrs(G, (uint32_t)h, color, nb);
-}
-
-// Component k of half-site h of a colour array C (components `stride` apart).
-// O32: every half-lattice of the ensemble holds fewer than 2^30 sites, so the
-// byte offset of h fits in 32 bits and a load is the component's scalar base
-// plus a 32-bit vector offset (no 64-bit address arithmetic per load).
-template <bool O32, class T>
-__device__ __forceinline__ T* comp_at(T* C, int64_t stride, int k, uint32_t h) {
-    if (O32) {
-        typedef typename std::conditional<std::is_const<T>::value, const char, char>::type B;
-        return reinterpret_cast<T*>(reinterpret_cast<B*>(C + k * stride) + (h << 2));
-    }
-    return C + k * stride + h;
 }
 
 template <int N, bool O32>

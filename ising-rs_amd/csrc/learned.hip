@@ -578,7 +578,7 @@ __device__ __forceinline__ bool draw_around(const float hh[3], float kap, const 
 // proposal when the ensemble's learned mode is on).  Same measurement fusion
 // as the plain pass.
 // ---------------------------------------------------------------------------
-template <int N, int H>
+template <int N, int H, bool O32>
 __device__ __forceinline__ void learned_tile(const ContArgs& a, int b, int color, int closes, int64_t t_override,
                                              double (*s_red)[4], uint16_t* s_feat, float* s_out, const char* s_frag) {
     int g = 0;
@@ -614,9 +614,9 @@ __device__ __forceinline__ void learned_tile(const ContArgs& a, int b, int color
 #pragma unroll
         for (int k = 0; k < N; ++k)
 #pragma unroll
-            for (int q = 0; q < 6; ++q) nv[k][q] = O[k * G.stride + nb[q]];
+            for (int q = 0; q < 6; ++q) nv[k][q] = *comp_at<O32>(O, G.stride, k, nb[q]);
 #pragma unroll
-        for (int k = 0; k < N; ++k) S[k] = C[k * G.stride + h];
+        for (int k = 0; k < N; ++k) S[k] = *comp_at<O32>(C, G.stride, k, (uint32_t)h);
         hn = site_features<N>(beta, nv, x, hh);
     }
     // the proposal's parameters for every site of the block, on the matrix cores
@@ -657,7 +657,7 @@ __device__ __forceinline__ void learned_tile(const ContArgs& a, int b, int color
         }
         if (accept || renorm) {
 #pragma unroll
-            for (int k = 0; k < N; ++k) C[k * G.stride + h] = S[k];
+            for (int k = 0; k < N; ++k) *comp_at<O32>(C, G.stride, k, (uint32_t)h) = S[k];
         }
         if (need) {
             double e = 0.0;
@@ -667,7 +667,7 @@ __device__ __forceinline__ void learned_tile(const ContArgs& a, int b, int color
                 if (color) {
                     double f = 0.0;
 #pragma unroll
-                    for (int q = 0; q < 6; ++q) f += (double)O[k * G.stride + nb[q]];
+                    for (int q = 0; q < 6; ++q) f += (double)*comp_at<O32>(O, G.stride, k, nb[q]);
                     e += (double)S[k] * f;
                 }
             }
@@ -700,7 +700,7 @@ __device__ __forceinline__ void learned_tile(const ContArgs& a, int b, int color
 // network's fragments copied into LDS per block.  (Four tiles per block, sharing
 // one copy, measured slower at H = 64: the loop raised the kernel from 116 to
 // 144 VGPRs, 3 waves per SIMD instead of 4; XY pass 163 against 148 us.)
-template <int N, int H>
+template <int N, int H, bool O32>
 __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int nblocks, int color, int closes,
                                                            int64_t t_override) {
     __shared__ double s_red[4][4];
@@ -711,7 +711,7 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int nbloc
     const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);  // XCD-contiguous tiles
     if (b >= nblocks) return;  // whole padding blocks, before any barrier
     stage_frag<H>(a.lfrag, s_frag);  // read after the first barrier in mlp_block
-    learned_tile<N, H>(a, b, color, closes, t_override, s_red, s_feat, s_out, (const char*)s_frag);
+    learned_tile<N, H, O32>(a, b, color, closes, t_override, s_red, s_feat, s_out, (const char*)s_frag);
 }
 
 // ---------------------------------------------------------------------------
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(256) void cont_learned_kernel(ContArgs a, int nbloc
 // ceiling of that family.  Same measurement fusion and block order as the
 // plain pass.  Draws: Philox(h, 2t + colour, stream, (11 << 8) | model).
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, bool O32>
 __global__ __launch_bounds__(256) void cont_heatbath_kernel(ContArgs a, int nblocks, int color, int closes,
                                                             int64_t t_override) {
     __shared__ double s_red[4][4];
@@ -759,11 +759,11 @@ __global__ __launch_bounds__(256) void cont_heatbath_kernel(ContArgs a, int nblo
 #pragma unroll
         for (int k = 0; k < N; ++k)
 #pragma unroll
-            for (int q = 0; q < 6; ++q) nv[k][q] = O[k * G.stride + nb[q]];
+            for (int q = 0; q < 6; ++q) nv[k][q] = *comp_at<O32>(O, G.stride, k, nb[q]);
         float S[3] = {0.f, 0.f, 0.f}, hf[3] = {0.f, 0.f, 0.f};
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-            S[k] = C[k * G.stride + h];
+            S[k] = *comp_at<O32>(C, G.stride, k, (uint32_t)h);
             hf[k] = ((nv[k][0] + nv[k][1]) + (nv[k][2] + nv[k][3])) + (nv[k][4] + nv[k][5]);
         }
         float hn2 = 0.f;
@@ -793,7 +793,7 @@ __global__ __launch_bounds__(256) void cont_heatbath_kernel(ContArgs a, int nblo
         }
         if (moved || renorm) {
 #pragma unroll
-            for (int k = 0; k < N; ++k) C[k * G.stride + h] = S[k];
+            for (int k = 0; k < N; ++k) *comp_at<O32>(C, G.stride, k, (uint32_t)h) = S[k];
         }
         if (need) {
             double e = 0.0;
@@ -822,12 +822,22 @@ __global__ __launch_bounds__(256) void cont_heatbath_kernel(ContArgs a, int nblo
     }
 }
 
-hipError_t launch_cont_heatbath(const ContArgs& a, int nblocks, int color, int closes, int64_t t_override,
-                                hipStream_t st) {
+template <bool O32>
+static void launch_heatbath(const ContArgs& a, int nblocks, int color, int closes, int64_t t_override,
+                            hipStream_t st) {
     const dim3 grid((nblocks + 7) & ~7);
     if (a.ncomp == 2)
-        hipLaunchKernelGGL(cont_heatbath_kernel<2>, grid, dim3(256), 0, st, a, nblocks, color, closes, t_override);
-    else hipLaunchKernelGGL(cont_heatbath_kernel<3>, grid, dim3(256), 0, st, a, nblocks, color, closes, t_override);
+        hipLaunchKernelGGL((cont_heatbath_kernel<2, O32>), grid, dim3(256), 0, st, a, nblocks, color, closes,
+                           t_override);
+    else
+        hipLaunchKernelGGL((cont_heatbath_kernel<3, O32>), grid, dim3(256), 0, st, a, nblocks, color, closes,
+                           t_override);
+}
+
+hipError_t launch_cont_heatbath(const ContArgs& a, int nblocks, int color, int closes, int64_t t_override,
+                                hipStream_t st) {
+    if (a.off32) launch_heatbath<true>(a, nblocks, color, closes, t_override, st);
+    else launch_heatbath<false>(a, nblocks, color, closes, t_override, st);
     return hipGetLastError();
 }
 
@@ -1077,7 +1087,12 @@ template <int N, int H>
 static hipError_t launch_learned_nh(const ContArgs& a, int nblocks, int color, int closes, int64_t t_override,
                                     hipStream_t st) {
     const dim3 grid((nblocks + 7) & ~7);  // a multiple of the 8 XCDs
-    hipLaunchKernelGGL((cont_learned_kernel<N, H>), grid, dim3(256), 0, st, a, nblocks, color, closes, t_override);
+    if (a.off32)
+        hipLaunchKernelGGL((cont_learned_kernel<N, H, true>), grid, dim3(256), 0, st, a, nblocks, color, closes,
+                           t_override);
+    else
+        hipLaunchKernelGGL((cont_learned_kernel<N, H, false>), grid, dim3(256), 0, st, a, nblocks, color, closes,
+                           t_override);
     return hipGetLastError();
 }
 

@@ -287,3 +287,16 @@ def test_64bit_offset_passes_equal_32bit_ones(nc):
     for x, y in zip(a.read_cont(), b.read_cont()):
         assert np.array_equal(x, y)
     assert np.array_equal(a.get_spins(2, 5), b.get_spins(2, 5))
+    ba.close()
+    bb.close()
+    for p in (a, b):  # heat-bath passes, then the (untrained) learned proposal
+        p.heat_bath(True)
+        p.sample(4)
+        p.heat_bath(False)
+        p.learn_init(16, seed=3)
+        p.learn_enable(True)
+        p.sample(4)
+    for x, y in zip(a.read_cont(), b.read_cont()):
+        assert np.array_equal(x, y)
+    for g in range(3):
+        assert np.array_equal(a.get_spins(g, 0), b.get_spins(g, 0))
