@@ -147,3 +147,37 @@ def test_worklist_batch_rejects_other_members():
     with pytest.raises(ising_hip.IsingError):
         b.sample(2)
     b.close()
+
+
+@pytest.mark.parametrize("mode", [ising_hip.Batch.WORKLIST, ising_hip.Batch.PERSISTENT])
+@pytest.mark.parametrize("which", ["ising", "xy", "heisenberg_xy"])
+def test_worklist_batch_of_member_subsets(mode, which):
+    """Work lists with the Ising member alone (its colour passes and closing
+    kernel only), one continuous member, or both continuous members in the
+    other order, against the branch batch."""
+    def members(seed):
+        kw = dict(exchange_every=4, measure_every=2, graph_sweeps=0)
+        ckw = dict(kw, n_overrelax=1, renorm_every=7)
+        out = []
+        if which == "ising":
+            out.append(ising_hip.PTEnsemble("ising", [8, 16], 6, o.linspace(4.4, 4.62, 6), seed, **kw))
+        if which.startswith("heisenberg"):
+            out.append(ising_hip.PTEnsemble("heisenberg", [8], 6, np.linspace(1.40, 1.46, 6), seed + 2, delta=1.0,
+                                            **ckw))
+        if which.endswith("xy"):
+            out.append(ising_hip.PTEnsemble("xy", [8, 12], 6, np.linspace(2.1, 2.3, 6), seed + 1, delta=1.5, **ckw))
+        return out
+    ref, wl = members(5), members(5)
+    b_ref = ising_hip.Batch(ref, graph_sweeps=3)
+    b_wl = ising_hip.Batch(wl, graph_sweeps=3, mode=mode)
+    for b in (b_ref, b_wl):
+        b.warmup(5)
+        b.sample(7)
+        b.sync()
+    for p, q in zip(ref, wl):
+        ra = p.read_ising() if which == "ising" else p.read_cont()
+        rb = q.read_ising() if which == "ising" else q.read_cont()
+        for u, v in zip(ra, rb):
+            assert np.array_equal(u, v)
+        assert p.sweeps() == q.sweeps() == 12
+        assert np.array_equal(p.get_spins(0, 3), q.get_spins(0, 3))
