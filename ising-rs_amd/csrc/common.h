@@ -190,6 +190,7 @@ __device__ __forceinline__ void site_nbrs(const DevGroup& G, uint32_t h, int col
 // plus a 32-bit vector offset (no 64-bit address arithmetic per load).
 template <bool O32, class T>
 __device__ __forceinline__ T* comp_at(T* C, int64_t stride, int k, uint32_t h) {
+    static_assert(sizeof(T) == 4, "4-byte elements");
     if (O32) {
         typedef typename std::conditional<std::is_const<T>::value, const char, char>::type B;
         return reinterpret_cast<T*>(reinterpret_cast<B*>(C + k * stride) + (h << 2));
