@@ -317,11 +317,11 @@ def run_batched(args, ising_hip):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": {"branches": "whole sweep graph (each model's kernels, concurrent branches)",
-                                "worklist": "whole sweep: batch_or_kernel x 4 (both continuous models) + "
-                                            "batch_pass_kernel x 6 (Ising, XY, Heisenberg colour passes) + "
+                                "worklist": "whole sweep: batch_or_kernel x 4 + batch_pass_kernel x 4 (Ising "
+                                            "colour passes; XY and Heisenberg Metropolis passes together) + "
                                             "batch_close_kernel, over work lists of the models' blocks",
-                                "persistent": "whole sweep: batch_persistent_kernel x 10 + batch_close_kernel "
-                                              "(the same work lists on a persistent grid)"}[
+                                "persistent": "whole sweep: batch_persistent_kernel x 8 + batch_close_kernel "
+                                              "(the same work lists on persistent grids)"}[
                                     args.batch_mode],
                      "alg_bytes_per_sweep": int(alg_bytes_per_sweep), "avg_sweep_us": round(sweep_s * 1e6, 3),
                      "device_updates_per_ns": round(updates / (ev_ms * 1e6), 3)},

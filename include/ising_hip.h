@@ -417,12 +417,12 @@ int ising_batch_sync(ising_batch_t* b, double* last_ms);
  *   ISING_BATCH_BRANCHES (default): each member's own launches, on a graph
  *     branch per member.
  *   ISING_BATCH_WORKLIST: launches over work lists of (member, size,
- *     replica, tile) blocks: one per overrelaxation pass for both continuous
- *     members, one per colour pass of each model's Metropolis pass, one for
- *     the continuous closing step (2 n_overrelax + 7 launches per sweep with
- *     all three models).  At most one int8 Ising ensemble (two-launch sweeps)
- *     and two continuous ensembles (XY, Heisenberg, Metropolis passes) with
- *     equal n_overrelax.
+ *     replica, tile) blocks: one per pass for the continuous members
+ *     (overrelaxation, Metropolis, the closing step) and one per colour pass
+ *     of the Ising member (2 n_overrelax + 5 launches per sweep with all
+ *     three models).  At most one int8 Ising ensemble (two-launch sweeps) and
+ *     two continuous ensembles (XY, Heisenberg, Metropolis passes) with equal
+ *     n_overrelax.
  *   ISING_BATCH_PERSISTENT: the same work list walked by a persistent grid
  *     (the blocks the device holds at once).
  * Lattices, records and permutations equal the branch mode's bit for bit. */

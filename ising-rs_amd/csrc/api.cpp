@@ -2499,15 +2499,11 @@ struct ising_batch {
         for (size_t i = 0; i < pc.size(); ++i) base.ca[i] = pc[i]->cont_args();
         const int nor = pc.empty() ? 0 : pc[0]->opts.n_overrelax;
         const bool persistent = mode == ISING_BATCH_PERSISTENT;
-        // Launches of their own: kind W_ISING, the Ising member's colour pass (in
+        // Kind W_ISING: the Ising member's colour pass, in launches of its own (in
         // a launch with the continuous Metropolis tiles, the Ising body's
         // registers set the occupancy of every item: 5.26 against 4.92 ms per
-        // config-4 sweep), and `only` >= 0, the Metropolis pass of continuous
-        // member `only` (in a kernel holding both members' Metropolis bodies,
-        // the compiler lays them out one after the other and the second one
-        // reads its block's parameters through vector loads: XY's pass took 505
-        // against 318 us).
-        auto pass = [&](int kind, int color, int only) -> hipError_t {
+        // config-4 sweep).
+        auto pass = [&](int kind, int color) -> hipError_t {
             BatchPass bp = base;
             bp.color = color;
             auto add = [&](int k, int member, int count, int tiles) {
@@ -2518,10 +2514,6 @@ struct ising_batch {
                 return launch_batch_pass(bp, persistent, st);
             }
             for (size_t i = 0; i < pc.size(); ++i) {
-                if (only >= 0 && (int)i != only) {
-                    bp.ca[i] = ContArgs{};  // absent from this launch's kernel
-                    continue;
-                }
                 const int tiles = pc[i]->nblocks;
                 if (kind == W_CLOSE)
                     add(kind, (int)i, (pc[i]->total_reps + kContClosePerBlock - 1) / kContClosePerBlock, 0);
@@ -2532,14 +2524,14 @@ struct ising_batch {
         };
         for (int64_t i = 0; i < n; ++i) {
             for (int k = 0; k < nor; ++k) {
-                HIP_TRY(pass(W_OR, 0, -1));
-                HIP_TRY(pass(W_OR, 1, -1));
+                HIP_TRY(pass(W_OR, 0));
+                HIP_TRY(pass(W_OR, 1));
             }
             for (int c = 0; c < 2; ++c) {
-                if (pi) HIP_TRY(pass(W_ISING, c, -1));
-                for (size_t j = 0; j < pc.size(); ++j) HIP_TRY(pass(W_METRO, c, (int)j));
+                if (pi) HIP_TRY(pass(W_ISING, c));
+                if (!pc.empty()) HIP_TRY(pass(W_METRO, c));
             }
-            if (!pc.empty()) HIP_TRY(pass(W_CLOSE, 0, -1));
+            if (!pc.empty()) HIP_TRY(pass(W_CLOSE, 0));
         }
         if (n > 0 && pi) HIP_TRY(launch_ising_close(pi->ising_args(), st));
         for (ising_pt* p : m) p->t_host += n;

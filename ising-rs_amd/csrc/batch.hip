@@ -3,26 +3,26 @@
 // per-model launches of a batched parallel-tempering run, config 4).
 //
 // A sweep of a batch holding an int8 Ising ensemble and up to two continuous
-// ensembles (XY, Heisenberg) with n_or overrelaxation passes is 2 n_or + 7
+// ensembles (XY, Heisenberg) with n_or overrelaxation passes is 2 n_or + 5
 // launches, each over a work list of the members' blocks for that pass:
 //
-//   n_or x (OR colour 0, OR colour 1)   both continuous members' overrelaxation tiles (batch_or_kernel)
-//   Ising colour 0                      Ising colour-0 tiles                 (batch_pass_kernel<0, 0, true>)
-//   XY, Heisenberg Metropolis colour 0  each member's Metropolis tiles       (batch_pass_kernel<N, 0 or 0, N, false>)
-//   the same three for colour 1
+//   n_or x (OR colour 0, OR colour 1)   continuous members' overrelaxation tiles (batch_or_kernel)
+//   Ising colour 0                      Ising colour-0 tiles                (batch_pass_kernel<0, 0, true>)
+//   Metropolis colour 0                 continuous members' Metropolis tiles (batch_pass_kernel<N0, N1, false>)
+//   Ising colour 1, Metropolis colour 1 the same for colour 1
 //   close                               continuous members' closing blocks  (batch_close_kernel)
 //
-// instead of 2 + 2 (2 n_or + 3) launches on one stream per member.  The
-// Metropolis passes have launches of their own: a kernel holding the Ising
-// body has its registers (71 VGPRs), which would set the occupancy of every
-// item beside it, and in a kernel holding two Metropolis bodies the compiler
-// lays them out one after the other and the second reads its block's
-// parameters through vector loads (XY's pass: 505 against 318 us).  The
-// overrelaxation bodies share a launch without either cost.  The work list of
-// a launch is a short table of segments in the kernel arguments, one per
-// member, item j of a segment being block j of that member's own launch of
-// the pass.  A block runs the member's own block body (ising_pass.h,
-// cont_pass.h), so every result equals the per-member launches'.
+// instead of 2 + 2 (2 n_or + 3) launches on one stream per member.  The Ising
+// pass has launches of its own: a kernel holding the Ising body has its
+// registers (71 VGPRs), which would set the occupancy of every continuous item
+// beside it.  The work list of a launch is a short table of segments in the
+// kernel arguments, one per member, item j of a segment being block j of that
+// member's own launch of the pass.  A block runs the member's own block body
+// (ising_pass.h, cont_pass.h), so every result equals the per-member
+// launches'.  The bodies read their block's parameters (size descriptor,
+// replica tables, sweep counter) with ld_uniform: as the second of two member
+// branches, or inside a persistent grid's item loop, the compiler otherwise
+// loaded them with vector loads (XY's Metropolis pass 505 against 318 us).
 //
 // Items and XCDs: blocks are dealt to the 8 XCDs round-robin (block i runs on
 // XCD i % 8).  Each segment is cut into 8 contiguous chunks and XCD x takes

@@ -10,9 +10,35 @@ namespace ising {
 constexpr int kCT = 256;        // sites per block tile
 constexpr int kCloseT = 256;  // threads of the closing block (one replica or pair each at a time)
 
+// A value the whole block reads at one address and no block of the running
+// launch writes (size descriptors, the ensemble's tables, the sweep counter),
+// read through the constant address space: a scalar load also where the
+// compiler cannot rule out an earlier store of the kernel to it -- inside a
+// persistent grid's item loop, or in the second of two members' bodies laid
+// out one after the other -- and issued vector loads (then readfirstlane),
+// with the Metropolis items' parameters at several times the latency.
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T* p) {
+    return *(const __attribute__((address_space(4))) T*)p;
+}
+
+// A size descriptor by scalar loads, word by word.
+__device__ __forceinline__ DevGroup ld_group(const DevGroup* p) {
+    static_assert(sizeof(DevGroup) % 4 == 0, "whole words");
+    uint32_t w[sizeof(DevGroup) / 4];
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(DevGroup) / 4); ++i) w[i] = ld_uniform(reinterpret_cast<const uint32_t*>(p) + i);
+    DevGroup G;
+    __builtin_memcpy(&G, w, sizeof(G));
+    return G;
+}
+
+// The size group of block b: a count over every group, with no loop exit that
+// depends on the data, so the index stays uniform (scalar loads of its
+// descriptor) also inside a persistent grid's item loop.
 __device__ __forceinline__ int find_group_c(const DevGroup* groups, int ngroups, int b) {
     int g = 0;
-    while (g + 1 < ngroups && b >= groups[g + 1].blk0) ++g;
+    for (int i = 1; i < ngroups; ++i) g += b >= ld_uniform(&groups[i].blk0) ? 1 : 0;
     return g;
 }
 
@@ -219,16 +245,16 @@ template <int N, bool O32>
 __device__ __forceinline__ void cont_metro_block(const ContArgs& a, const int b, const int color, const int closes) {
     __shared__ double s_red[4][kCT / 64];
     const int g = find_group_c(a.groups, a.ngroups, b);
-    const DevGroup G = a.groups[g];
+    const DevGroup G = ld_group(&a.groups[g]);
     int r, tile;
     split_tiles(G, b - G.blk0, &r, &tile);
     const int grep = G.rep0 + r;
-    const int ti = G.rep0 + a.r2t[grep];
-    const float beta = a.inline_params ? a.beta_inline : a.betas_f[ti];
-    const float dlt = a.inline_params ? a.delta_inline : a.deltas[ti];
-    const float omc = N == 3 ? (a.inline_params ? a.omcd_inline : a.omcd[ti]) : 0.f;
-    const int64_t t = a.ctl->t;
-    const int32_t fl = a.ctl->flags;
+    const int ti = G.rep0 + ld_uniform(&a.r2t[grep]);
+    const float beta = a.inline_params ? a.beta_inline : ld_uniform(&a.betas_f[ti]);
+    const float dlt = a.inline_params ? a.delta_inline : ld_uniform(&a.deltas[ti]);
+    const float omc = N == 3 ? (a.inline_params ? a.omcd_inline : ld_uniform(&a.omcd[ti])) : 0.f;
+    const int64_t t = ld_uniform(&a.ctl->t);
+    const int32_t fl = ld_uniform(&a.ctl->flags);
     const bool need = (fl & (F_MEAS | F_EXCH)) != 0 && closes;
     const bool renorm = (fl & F_RENORM) != 0;
     const bool count = a.adapt_every > 0 && (fl & F_WARM) != 0;
@@ -346,16 +372,16 @@ __device__ __forceinline__ void cont_metro_pair_block(const ContArgs& a, const i
     __shared__ double s_red[4][kCT / 64];
     const int bb = valid ? b : 0;  // an absent tile reads tile 0's parameters and updates nothing
     const int g = find_group_c(a.groups, a.ngroups, bb);
-    const DevGroup G = a.groups[g];
+    const DevGroup G = ld_group(&a.groups[g]);
     int r, tile;
     split_tiles(G, bb - G.blk0, &r, &tile);
     const int grep = G.rep0 + r;
-    const int ti = G.rep0 + a.r2t[grep];
-    const float beta = a.inline_params ? a.beta_inline : a.betas_f[ti];
-    const float dlt = a.inline_params ? a.delta_inline : a.deltas[ti];
-    const float omc = N == 3 ? (a.inline_params ? a.omcd_inline : a.omcd[ti]) : 0.f;
-    const int64_t t = a.ctl->t;
-    const int32_t fl = a.ctl->flags;
+    const int ti = G.rep0 + ld_uniform(&a.r2t[grep]);
+    const float beta = a.inline_params ? a.beta_inline : ld_uniform(&a.betas_f[ti]);
+    const float dlt = a.inline_params ? a.delta_inline : ld_uniform(&a.deltas[ti]);
+    const float omc = N == 3 ? (a.inline_params ? a.omcd_inline : ld_uniform(&a.omcd[ti])) : 0.f;
+    const int64_t t = ld_uniform(&a.ctl->t);
+    const int32_t fl = ld_uniform(&a.ctl->flags);
     const bool need = (fl & (F_MEAS | F_EXCH)) != 0 && closes;  // the same for the whole block
     const bool renorm = (fl & F_RENORM) != 0;
     const bool count = a.adapt_every > 0 && (fl & F_WARM) != 0;
@@ -501,7 +527,7 @@ template <int N, int SPT, bool O32>
 __device__ __forceinline__ void cont_or_block(const ContArgs& a, const int b, const int color,
                                               const int tid = (int)threadIdx.x) {
     const int g = find_group_c(a.groups, a.ngroups, b);
-    const DevGroup G = a.groups[g];
+    const DevGroup G = ld_group(&a.groups[g]);
     int r, tile;
     split_tiles(G, b - G.blk0, &r, &tile);
     float* C = a.pool + G.off + (int64_t)(2 * r + color) * N * G.stride;

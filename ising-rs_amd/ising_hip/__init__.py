@@ -578,9 +578,8 @@ class Batch:
     def __init__(self, members, graph_sweeps=32, mode=0):
         """mode: BRANCHES (each member's own launches on a graph branch per
         member), WORKLIST (launches over work lists of the members' blocks: one
-        per overrelaxation pass for both continuous members, one per colour
-        pass of each model's Metropolis pass) or PERSISTENT (those work lists
-        walked by a persistent grid)."""
+        per pass for the continuous members, one per colour pass for the Ising
+        member) or PERSISTENT (those work lists walked by persistent grids)."""
         self.members = list(members)
         arr = (_vp * len(self.members))(*[m._h for m in self.members])
         self._arr = arr
