@@ -42,7 +42,7 @@ struct IsingArgs {
 };
 
 constexpr int kMscAccCopies = 16;
-constexpr int kIsingAccCopies = 4;  // int8 Ising accumulator copies (a power of two, <= kMscAccCopies)
+constexpr int kIsingAccCopies = 2;  // int8 Ising accumulator copies (a power of two, <= kMscAccCopies; 2 measured fastest of 1, 2, 4, 8)
 constexpr int kMscTile = 256;  // site words per block in multispin-coded launches
 hipError_t launch_msc_half(const IsingArgs& a, int nblocks, int color, hipStream_t st);
 hipError_t launch_msc_init(const DevGroup* groups, int ngroups, uint32_t* pool, int nblocks, uint32_t k0, uint32_t k1,
